@@ -1,0 +1,152 @@
+/*
+ * rxmesh_amd: C-ABI of the MI355X (gfx950) patch-based static mesh query
+ * engine. Plain pointers and sizes only; device pointers are caller-owned
+ * unless stated; every function returns an rxm_status and never exits.
+ *
+ * Layout conventions
+ *   - Element ids on input are the caller's ("global") ids: vertex = index in
+ *     the vertex array, face = index in fv, edge = first-appearance order over
+ *     half-edges (f, i) -> (fv[3f+i], fv[3f+(i+1)%3]).
+ *   - Device attributes and query results use LINEAR ids: the owned elements
+ *     of patch 0, then patch 1, ... (the reference's linear_id,
+ *     include/rxmesh/context.h:299-331). rxm_mesh_get_array() exports the
+ *     linear<->global maps; rxm_to_linear / rxm_to_global permute on device.
+ *   - Multi-component vertex attributes are SoA over linear ids: component c
+ *     of linear vertex l is at data[c * num_vertices + l] (the reference's SoA
+ *     layout, include/rxmesh/attribute.h:406-472).
+ *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
+ *
+ * Thread compatibility: a mesh may be used from one host thread at a time.
+ */
+#ifndef RXMESH_AMD_H
+#define RXMESH_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int rxm_status;
+enum {
+    RXM_OK = 0,
+    RXM_ERR_INVALID_ARG = 1,
+    RXM_ERR_NON_MANIFOLD = 2,
+    RXM_ERR_PATCH_TOO_BIG = 3,
+    RXM_ERR_IO = 4,
+    RXM_ERR_DEVICE = 5,
+    RXM_ERR_UNSUPPORTED = 6,
+    RXM_ERR_OUT_OF_MEMORY = 7
+};
+
+/* Query operations; values follow the reference enum Op
+ * (include/rxmesh/types.h:113-129). */
+enum {
+    RXM_OP_VV = 3, RXM_OP_VE = 4, RXM_OP_VF = 5, RXM_OP_FV = 6,
+    RXM_OP_FE = 7, RXM_OP_FF = 8, RXM_OP_EV = 9, RXM_OP_EF = 11
+};
+
+/* rxm_mesh_create flags */
+enum { RXM_HOST_ONLY = 1 };
+
+/* vertex normal weighting */
+enum { RXM_NORMALS_MAX = 0, RXM_NORMALS_AREA = 1 };
+
+typedef struct rxm_mesh rxm_mesh;
+
+typedef struct rxm_mesh_info_t {
+    uint32_t num_vertices, num_edges, num_faces, num_patches;
+    uint32_t patch_size;
+    uint32_t max_vertices_per_patch, max_edges_per_patch, max_faces_per_patch; /* incl. ribbon */
+    uint32_t max_owned_vertices, max_owned_edges, max_owned_faces;
+    uint32_t is_closed, is_edge_manifold;
+    uint32_t max_valence, max_edge_incident_faces, max_face_adjacent_faces;
+    uint32_t num_ribbon_faces;
+    uint32_t on_device;
+    uint64_t device_bytes;
+} rxm_mesh_info_t;
+
+/* Arrays exportable with rxm_mesh_get_array (host copies, uint32 unless noted). */
+enum {
+    RXM_ARR_EV = 0,            /* 2E global vertex ids, first-seen orientation */
+    RXM_ARR_FACE_PATCH = 1,    /* F */
+    RXM_ARR_VERTEX_PATCH = 2,  /* V owner patch */
+    RXM_ARR_EDGE_PATCH = 3,    /* E owner patch */
+    RXM_ARR_PATCHES_VAL = 4,   /* F faces grouped by patch */
+    RXM_ARR_PATCHES_OFFSET = 5,/* P inclusive ends */
+    RXM_ARR_RIBBON_VAL = 6,    /* ribbon faces per patch, discovery order */
+    RXM_ARR_RIBBON_OFFSET = 7, /* P inclusive ends */
+    RXM_ARR_V_L2G = 8, RXM_ARR_E_L2G = 9, RXM_ARR_F_L2G = 10,   /* linear -> global */
+    RXM_ARR_V_G2L = 11, RXM_ARR_E_G2L = 12, RXM_ARR_F_G2L = 13, /* global -> linear */
+    RXM_ARR_PATCH_COUNTS = 14, /* 6P: nv, ne, nf, nvo, neo, nfo per patch */
+    RXM_ARR_PATCH_OFFSETS = 15,/* 3(P+1): v_off, e_off, f_off slab offsets */
+    RXM_ARR_LTOG_V = 16, RXM_ARR_LTOG_E = 17, RXM_ARR_LTOG_F = 18, /* slabs */
+    RXM_ARR_EV_LOCAL = 19,     /* e-slab u32: lo16 local(max gid), hi16 local(min gid) */
+    RXM_ARR_FE_LOCAL = 20      /* 3 * f-slab uint16: (local edge << 1) | dir */
+};
+
+/* Build the patch mesh from F x 3 triangle indices and upload it.
+ * Replaces RXMeshStatic(fv, patcher_file, patch_size, ...)
+ * (include/rxmesh/rxmesh_static.h:61-100) -> RXMesh::init / build
+ * (include/rxmesh/rxmesh.cpp:89-225, 290-448).
+ * num_vertices = 0 infers max index + 1. patch_file (may be NULL) is a
+ * reference patch archive (cereal PortableBinary, patcher.h:154-182) whose
+ * layout is replayed instead of running the patcher. */
+rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_vertices,
+                           uint32_t patch_size, const char* patch_file, uint32_t flags,
+                           rxm_mesh** out);
+/* Same, with an explicit face -> patch assignment (faces grouped by patch in
+ * ascending face order). */
+rxm_status rxm_mesh_create_with_patches(const uint32_t* fv, uint32_t num_faces,
+                                        uint32_t num_vertices, const uint32_t* face_patch,
+                                        uint32_t num_patches, uint32_t flags, rxm_mesh** out);
+/* Frees host and device state (reference ~RXMesh, rxmesh.cpp:227-288). */
+rxm_status rxm_mesh_destroy(rxm_mesh* mesh);
+rxm_status rxm_mesh_info(const rxm_mesh* mesh, rxm_mesh_info_t* info);
+/* Size in elements of an exportable array. */
+rxm_status rxm_mesh_array_size(const rxm_mesh* mesh, int which, uint64_t* count);
+/* Copy an exportable array into host memory of `bytes` bytes. */
+rxm_status rxm_mesh_get_array(const rxm_mesh* mesh, int which, void* dst, uint64_t bytes);
+/* Write the patch layout as a reference-compatible patch archive
+ * (Patcher::save, include/rxmesh/patcher/patcher.h:154-159). */
+rxm_status rxm_mesh_save_patches(const rxm_mesh* mesh, const char* path);
+
+/* Number of entries of a query over all owned source elements. */
+rxm_status rxm_query_nnz(rxm_mesh* mesh, int op, int64_t* nnz);
+/* Materialise a query as CSR over linear ids: row_ptr[n_src + 1], col[nnz].
+ * Replaces for_each<Op, N>(lambda, oriented) / Query::dispatch<Op>
+ * (include/rxmesh/rxmesh_static.h:524-566, include/rxmesh/query.inl:108-156)
+ * driven by the reference test lambda output(id, i) = iter[i]
+ * (tests/RXMesh_test/query_kernel.cuh:13-45). Row order within a row:
+ * ascending source-local order (unoriented), or the canonical CCW one-ring
+ * (oriented, VV/VE only, manifold meshes). */
+rxm_status rxm_query_csr(rxm_mesh* mesh, int op, int oriented, int32_t* row_ptr,
+                         int32_t* col, void* stream);
+
+/* Vertex normals, unnormalised. mode RXM_NORMALS_MAX restates the reference
+ * VertexNormal app (apps/VertexNormal/vertex_normal_kernel.cuh:10-42). */
+rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float* normals,
+                              void* stream);
+
+/* `iters` Jacobi steps of uniform Laplacian smoothing
+ * x <- x + lambda (mean of VV neighbours - x); `out` receives the result,
+ * `tmp` is a caller workspace of 3V floats (may be NULL when iters == 1).
+ * No single reference function; nearest are apps/Smoothing/manual.h:85-103
+ * and the uniform branch of apps/MCF/mcf_kernels.cuh:114-213. */
+rxm_status rxm_laplacian_smooth(rxm_mesh* mesh, const float* pos, float* out, float* tmp,
+                                float lambda, int iters, void* stream);
+
+/* Permute a vertex/edge/face attribute (kind 0/1/2) of ncomp SoA components
+ * between global and linear order on device (reference: attribute copy by
+ * ltog in RXMeshStatic::add_vertex_attribute, rxmesh_static.inl:147-189, and
+ * map_to_global). */
+rxm_status rxm_to_linear(rxm_mesh* mesh, int kind, int ncomp, const float* global_data,
+                         float* linear_data, void* stream);
+rxm_status rxm_to_global(rxm_mesh* mesh, int kind, int ncomp, const float* linear_data,
+                         float* global_data, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RXMESH_AMD_H */

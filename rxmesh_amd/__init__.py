@@ -1,0 +1,208 @@
+"""rxmesh_amd: MI355X-native patch-based static mesh query engine.
+
+Python host side over the C-ABI (include/rxmesh_amd.h). PyTorch provides
+device memory and streams; all compute runs in the HIP kernels of
+librxmesh_amd.so. There is no CPU fallback: if the library is missing the
+import of the native handle fails loudly.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_lib", "librxmesh_amd.so")
+
+OP = {"VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7, "FF": 8, "EV": 9, "EF": 11}
+STATUS = {0: "ok", 1: "invalid argument", 2: "non-manifold mesh", 3: "patch too big for LDS/16-bit ids",
+          4: "I/O error", 5: "device error", 6: "unsupported", 7: "out of memory"}
+ARR = {"ev": 0, "face_patch": 1, "vertex_patch": 2, "edge_patch": 3, "patches_val": 4,
+       "patches_offset": 5, "ribbon_val": 6, "ribbon_offset": 7, "v_l2g": 8, "e_l2g": 9,
+       "f_l2g": 10, "v_g2l": 11, "e_g2l": 12, "f_g2l": 13, "patch_counts": 14,
+       "patch_offsets": 15, "ltog_v": 16, "ltog_e": 17, "ltog_f": 18, "ev_local": 19,
+       "fe_local": 20}
+RXM_HOST_ONLY = 1
+
+
+class RXMError(RuntimeError):
+    def __init__(self, code, what):
+        super().__init__("%s failed: %s (status %d)" % (what, STATUS.get(code, "?"), code))
+        self.code = code
+
+
+class MeshInfo(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_uint32) for n in [
+        "num_vertices", "num_edges", "num_faces", "num_patches", "patch_size",
+        "max_vertices_per_patch", "max_edges_per_patch", "max_faces_per_patch",
+        "max_owned_vertices", "max_owned_edges", "max_owned_faces", "is_closed",
+        "is_edge_manifold", "max_valence", "max_edge_incident_faces",
+        "max_face_adjacent_faces", "num_ribbon_faces", "on_device"]] + [("device_bytes", ctypes.c_uint64)]
+
+
+_lib = None
+
+# exported symbols of include/rxmesh_amd.h with their argtypes
+_vp, _i, _u32, _f = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_float
+_SIGS = {
+    "rxm_mesh_create": [_vp, _u32, _u32, _u32, ctypes.c_char_p, _u32, ctypes.POINTER(_vp)],
+    "rxm_mesh_create_with_patches": [_vp, _u32, _u32, _vp, _u32, _u32, ctypes.POINTER(_vp)],
+    "rxm_mesh_destroy": [_vp],
+    "rxm_mesh_info": [_vp, ctypes.POINTER(MeshInfo)],
+    "rxm_mesh_array_size": [_vp, _i, ctypes.POINTER(ctypes.c_uint64)],
+    "rxm_mesh_get_array": [_vp, _i, _vp, ctypes.c_uint64],
+    "rxm_mesh_save_patches": [_vp, ctypes.c_char_p],
+    "rxm_query_nnz": [_vp, _i, ctypes.POINTER(ctypes.c_int64)],
+    "rxm_query_csr": [_vp, _i, _i, _vp, _vp, _vp],
+    "rxm_vertex_normals": [_vp, _i, _vp, _vp, _vp],
+    "rxm_laplacian_smooth": [_vp, _vp, _vp, _vp, _f, _i, _vp],
+    "rxm_to_linear": [_vp, _i, _i, _vp, _vp, _vp],
+    "rxm_to_global": [_vp, _i, _i, _vp, _vp, _vp],
+}
+
+
+def lib():
+    """Load librxmesh_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("librxmesh_amd.so not built: run `python -m rxmesh_amd.build` "
+                              "(no CPU fallback exists)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, argt in _SIGS.items():
+            fn = getattr(L, name)
+            fn.argtypes = argt
+            fn.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _check(rc, what):
+    if rc != 0:
+        raise RXMError(rc, what)
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+    if hasattr(stream, "cuda_stream"):
+        return ctypes.c_void_p(stream.cuda_stream)
+    return ctypes.c_void_p(stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr())
+
+
+class Mesh:
+    """One static patch mesh (host build + device slabs).
+
+    Parameters mirror RXMeshStatic(fv, patcher_file, patch_size)
+    (include/rxmesh/rxmesh_static.h:61-100)."""
+
+    def __init__(self, fv, num_vertices=0, patch_size=512, patch_file=None, face_patch=None,
+                 host_only=False):
+        fv = np.ascontiguousarray(fv, dtype=np.uint32).reshape(-1, 3)
+        self._fv = fv
+        h = ctypes.c_void_p()
+        flags = RXM_HOST_ONLY if host_only else 0
+        if face_patch is not None:
+            fp = np.ascontiguousarray(face_patch, dtype=np.uint32)
+            P = int(fp.max()) + 1
+            rc = lib().rxm_mesh_create_with_patches(fv.ctypes.data, fv.shape[0], num_vertices,
+                                                    fp.ctypes.data, P, flags, ctypes.byref(h))
+        else:
+            pf = patch_file.encode() if patch_file else None
+            rc = lib().rxm_mesh_create(fv.ctypes.data, fv.shape[0], num_vertices, patch_size, pf, flags,
+                                       ctypes.byref(h))
+        _check(rc, "rxm_mesh_create")
+        self._h = h
+        self.info = self._info()
+        self.nV = self.info["num_vertices"]
+        self.nE = self.info["num_edges"]
+        self.nF = self.info["num_faces"]
+        self.P = self.info["num_patches"]
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            _check(lib().rxm_mesh_destroy(self._h), "rxm_mesh_destroy")
+        self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _info(self):
+        i = MeshInfo()
+        _check(lib().rxm_mesh_info(self._h, ctypes.byref(i)), "rxm_mesh_info")
+        return {n: getattr(i, n) for n, _ in MeshInfo._fields_}
+
+    def array(self, name):
+        which = ARR[name]
+        n = ctypes.c_uint64()
+        _check(lib().rxm_mesh_array_size(self._h, which, ctypes.byref(n)), "rxm_mesh_array_size")
+        dt = np.uint16 if name == "fe_local" else np.uint32
+        out = np.zeros(max(n.value, 1), dt)
+        _check(lib().rxm_mesh_get_array(self._h, which, out.ctypes.data, out.nbytes), "rxm_mesh_get_array")
+        return out[: n.value]
+
+    def save_patches(self, path):
+        _check(lib().rxm_mesh_save_patches(self._h, path.encode()), "rxm_mesh_save_patches")
+
+    # -- queries --------------------------------------------------------
+    def query_nnz(self, op):
+        n = ctypes.c_int64()
+        _check(lib().rxm_query_nnz(self._h, OP[op], ctypes.byref(n)), "rxm_query_nnz")
+        return n.value
+
+    def num_src(self, op):
+        return {"V": self.nV, "E": self.nE, "F": self.nF}[op[0]]
+
+    def query(self, op, oriented=False, row_ptr=None, col=None, stream=None):
+        """CSR (row_ptr, col) of `op` over linear ids, as int32 device tensors."""
+        import torch
+        if row_ptr is None:
+            row_ptr = torch.empty(self.num_src(op) + 1, dtype=torch.int32, device="cuda")
+        if col is None:
+            col = torch.empty(max(self.query_nnz(op), 1), dtype=torch.int32, device="cuda")
+        _check(lib().rxm_query_csr(self._h, OP[op], int(oriented), _ptr(row_ptr), _ptr(col), _stream(stream)),
+               "rxm_query_csr")
+        return row_ptr, col
+
+    # -- geometry -------------------------------------------------------
+    def vertex_normals(self, pos, mode="max", out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(pos)
+        _check(lib().rxm_vertex_normals(self._h, 0 if mode == "max" else 1, _ptr(pos), _ptr(out),
+                                        _stream(stream)), "rxm_vertex_normals")
+        return out
+
+    def smooth(self, pos, lam=0.5, iters=1, out=None, tmp=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(pos)
+        if tmp is None and iters > 1:
+            tmp = torch.empty_like(pos)
+        _check(lib().rxm_laplacian_smooth(self._h, _ptr(pos), _ptr(out), _ptr(tmp) if tmp is not None else None,
+                                          float(lam), int(iters), _stream(stream)), "rxm_laplacian_smooth")
+        return out
+
+    def to_linear(self, x, kind=0, out=None, stream=None):
+        """x: (ncomp, n) float32 device tensor in global order -> linear order."""
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        ncomp = x.shape[0] if x.dim() == 2 else 1
+        _check(lib().rxm_to_linear(self._h, kind, ncomp, _ptr(x), _ptr(out), _stream(stream)), "rxm_to_linear")
+        return out
+
+    def to_global(self, x, kind=0, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        ncomp = x.shape[0] if x.dim() == 2 else 1
+        _check(lib().rxm_to_global(self._h, kind, ncomp, _ptr(x), _ptr(out), _stream(stream)), "rxm_to_global")
+        return out

@@ -1,0 +1,62 @@
+"""Builds librxmesh_amd.so (host connectivity build + gfx950 kernels + C-ABI)
+in-tree with hipcc. Run ``python -m rxmesh_amd.build``."""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIBDIR = os.path.join(HERE, "_lib")
+LIB = os.path.join(LIBDIR, "librxmesh_amd.so")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+ARCH = "gfx950"
+# packed-FP32 instructions are disabled for every HIP translation unit
+NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+
+SOURCES_HIP = ["capi.hip", "batch.hip", "cpp_api.hip"]
+SOURCES_CPP = ["host_build.cpp"]
+
+
+def _run(cmd, verbose):
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.check_call(cmd)
+
+
+def build(verbose=False, jobs=None):
+    os.makedirs(LIBDIR, exist_ok=True)
+    objdir = os.path.join(LIBDIR, "obj")
+    os.makedirs(objdir, exist_ok=True)
+    common = ["-O3", "-std=c++17", "-fPIC", "-I", os.path.join(ROOT, "include"), "-I", CSRC,
+              "-Wall", "-Wno-unused-function", "-Wno-unused-variable"]
+    procs, objs = [], []
+    for s in SOURCES_CPP:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        o = os.path.join(objdir, s + ".o")
+        objs.append(o)
+        procs.append(subprocess.Popen(["g++", *common, "-c", src, "-o", o]))
+    for s in SOURCES_HIP:
+        src = os.path.join(CSRC, s)
+        if not os.path.exists(src):
+            continue
+        o = os.path.join(objdir, s + ".o")
+        objs.append(o)
+        cmd = [HIPCC, *common, f"--offload-arch={ARCH}", *NO_PK_FP32, "-c", src, "-o", o]
+        if verbose:
+            print(" ".join(cmd), flush=True)
+        procs.append(subprocess.Popen(cmd))
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError("rxmesh_amd: compilation failed")
+    tmp = LIB + ".tmp"
+    _run([HIPCC, "-shared", "-fPIC", f"--offload-arch={ARCH}", *objs, "-o", tmp, "-lpthread"], verbose)
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose="-v" in sys.argv))

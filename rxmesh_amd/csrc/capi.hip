@@ -1,0 +1,451 @@
+// C-ABI implementation: mesh lifetime, device upload, kernel launches.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/rxmesh_amd.h"
+#include "host_build.h"
+#include "k_geometry.hpp"
+#include "k_query.hpp"
+
+using namespace rxm;
+
+struct rxm_mesh
+{
+    HostMesh              h;
+    bool                  on_device = false;
+    int                   device    = -1;
+    DevMesh               dm{};
+    std::vector<void*>    allocs;
+    uint64_t              device_bytes = 0;
+    std::map<int, uint32_t*>            d_base;  // per-op patch entry bases (P+1)
+    std::map<int, std::vector<uint32_t>> h_base;
+};
+
+namespace {
+
+#define RXM_HIP(call)                                 \
+    do {                                              \
+        hipError_t e_ = (call);                       \
+        if (e_ != hipSuccess) return RXM_ERR_DEVICE;  \
+    } while (0)
+
+template <typename T>
+int upload(rxm_mesh* m, const std::vector<T>& v, const T*& dst)
+{
+    T*     p     = nullptr;
+    size_t bytes = std::max<size_t>(v.size() * sizeof(T), 16);
+    if (hipMalloc(&p, bytes) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+    m->allocs.push_back(p);
+    m->device_bytes += bytes;
+    if (!v.empty() && hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess)
+        return RXM_ERR_DEVICE;
+    dst = p;
+    return RXM_OK;
+}
+
+int upload_mesh(rxm_mesh* m)
+{
+    const HostMesh& h = m->h;
+    RXM_HIP(hipGetDevice(&m->device));
+    std::vector<PatchDesc> pd(h.P);
+    for (uint32_t p = 0; p < h.P; ++p)
+        pd[p] = PatchDesc{h.nv[p], h.ne[p], h.nf[p], h.nvo[p], h.neo[p], h.nfo[p],
+                          h.v_off[p], h.e_off[p], h.f_off[p], h.vp[p], h.ep[p], h.fp[p]};
+    DevMesh& d = m->dm;
+    d.nV = h.nV; d.nE = h.nE; d.nF = h.nF; d.P = h.P;
+    d.max_nv = h.max_nv; d.max_ne = h.max_ne; d.max_nf = h.max_nf;
+    int rc = RXM_OK;
+    if ((rc = upload(m, pd, d.patch))) return rc;
+    if ((rc = upload(m, h.ev_local, d.ev))) return rc;
+    if ((rc = upload(m, h.fe_local, d.fe))) return rc;
+    if ((rc = upload(m, h.v_lin, d.v_lin))) return rc;
+    if ((rc = upload(m, h.e_lin, d.e_lin))) return rc;
+    if ((rc = upload(m, h.f_lin, d.f_lin))) return rc;
+    if ((rc = upload(m, h.v_own, d.v_own))) return rc;
+    if ((rc = upload(m, h.e_own, d.e_own))) return rc;
+    if ((rc = upload(m, h.f_own, d.f_own))) return rc;
+    if ((rc = upload(m, h.ltog_v, d.ltog_v))) return rc;
+    if ((rc = upload(m, h.ltog_e, d.ltog_e))) return rc;
+    if ((rc = upload(m, h.ltog_f, d.ltog_f))) return rc;
+    if ((rc = upload(m, h.v_l2g, d.v_l2g))) return rc;
+    if ((rc = upload(m, h.e_l2g, d.e_l2g))) return rc;
+    if ((rc = upload(m, h.f_l2g, d.f_l2g))) return rc;
+    if ((rc = upload(m, h.v_g2l, d.v_g2l))) return rc;
+    if ((rc = upload(m, h.e_g2l, d.e_g2l))) return rc;
+    if ((rc = upload(m, h.f_g2l, d.f_g2l))) return rc;
+    m->on_device = true;
+    return RXM_OK;
+}
+
+int finish_create(rxm_mesh* m, uint32_t flags, rxm_mesh** out)
+{
+    int rc = build_patches(m->h);
+    if (rc == RXM_OK && !(flags & RXM_HOST_ONLY)) rc = upload_mesh(m);
+    if (rc != RXM_OK) {
+        rxm_mesh_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return RXM_OK;
+}
+
+// Per-patch entry base of a query (static mesh: counts are fixed), P + 1.
+const std::vector<uint32_t>& op_base(rxm_mesh* m, int op)
+{
+    auto it = m->h_base.find(op);
+    if (it != m->h_base.end()) return it->second;
+    const HostMesh& h = m->h;
+    std::vector<uint32_t> b(h.P + 1, 0);
+    std::vector<uint32_t> vval(h.nV, 0);
+    if (op == RXM_OP_VV || op == RXM_OP_VE)
+        for (uint32_t e = 0; e < h.nE; ++e) {
+            vval[h.ev[2ull * e]]++;
+            vval[h.ev[2ull * e + 1]]++;
+        }
+    for (uint32_t p = 0; p < h.P; ++p) {
+        uint64_t c = 0;
+        switch (op) {
+            case RXM_OP_VV:
+            case RXM_OP_VE:
+                for (uint32_t l = 0; l < h.nvo[p]; ++l) c += vval[h.ltog_v[h.v_off[p] + l]];
+                break;
+            case RXM_OP_VF:
+                for (uint32_t l = 0; l < h.nvo[p]; ++l) {
+                    uint32_t g = h.ltog_v[h.v_off[p] + l];
+                    c += h.vf_off[g + 1] - h.vf_off[g];
+                }
+                break;
+            case RXM_OP_EF:
+                for (uint32_t l = 0; l < h.neo[p]; ++l) {
+                    uint32_t g = h.ltog_e[h.e_off[p] + l];
+                    c += h.ef_off[g + 1] - h.ef_off[g];
+                }
+                break;
+            case RXM_OP_FF:
+                for (uint32_t l = 0; l < h.nfo[p]; ++l) {
+                    uint32_t g = h.ltog_f[h.f_off[p] + l];
+                    for (int i = 0; i < 3; ++i) {
+                        uint32_t e = h.fe_g[3ull * g + i];
+                        c += h.ef_off[e + 1] - h.ef_off[e] - 1;
+                    }
+                }
+                break;
+            case RXM_OP_EV: c = 2ull * h.neo[p]; break;
+            default: c = 3ull * h.nfo[p]; break;
+        }
+        b[p + 1] = b[p] + static_cast<uint32_t>(c);
+    }
+    return m->h_base[op] = std::move(b);
+}
+
+int device_base(rxm_mesh* m, int op, const uint32_t** out)
+{
+    auto it = m->d_base.find(op);
+    if (it == m->d_base.end()) {
+        const auto& b = op_base(m, op);
+        const uint32_t* p = nullptr;
+        int rc = upload(m, b, p);
+        if (rc) return rc;
+        it = m->d_base.emplace(op, const_cast<uint32_t*>(p)).first;
+    }
+    *out = it->second;
+    return RXM_OK;
+}
+
+// Launch a one-workgroup-per-patch kernel with `lds` bytes of dynamic LDS.
+template <typename K, typename... Args>
+int launch_patches(K kernel, uint32_t threads, uint32_t lds, void* stream, const rxm_mesh* m, Args... args)
+{
+    if (lds > 160u * 1024u) return RXM_ERR_PATCH_TOO_BIG;
+    if (lds > 64u * 1024u)
+        RXM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
+    hipLaunchKernelGGL(kernel, dim3(m->h.P), dim3(threads), lds, static_cast<hipStream_t>(stream), args...);
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+bool valid_op(int op)
+{
+    return op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF || op == RXM_OP_FV ||
+           op == RXM_OP_FE || op == RXM_OP_FF || op == RXM_OP_EV || op == RXM_OP_EF;
+}
+
+}  // namespace
+
+extern "C" {
+
+rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_vertices,
+                           uint32_t patch_size, const char* patch_file, uint32_t flags, rxm_mesh** out)
+{
+    if (!out || !fv || num_faces == 0) return RXM_ERR_INVALID_ARG;
+    *out = nullptr;
+    rxm_mesh* m = new (std::nothrow) rxm_mesh();
+    if (!m) return RXM_ERR_OUT_OF_MEMORY;
+    int rc = build_supporting(m->h, fv, num_faces, num_vertices);
+    if (rc == RXM_OK) {
+        if (patch_file && patch_file[0]) {
+            std::vector<uint32_t> fp, pv, po;
+            uint32_t              P = 0, nV = 0, nE = 0, nF = 0;
+            rc = read_patch_file(patch_file, fp, pv, po, P, nV, nE, nF);
+            if (rc == RXM_OK && (nF != m->h.nF || nE != m->h.nE || nV != m->h.nV)) rc = RXM_ERR_INVALID_ARG;
+            if (rc == RXM_OK) rc = set_patches(m->h, fp.data(), P, pv.data(), po.data());
+            m->h.patch_size = patch_size;
+        } else {
+            rc = run_patcher(m->h, patch_size);
+        }
+    }
+    if (rc != RXM_OK) {
+        delete m;
+        return rc;
+    }
+    return finish_create(m, flags, out);
+}
+
+rxm_status rxm_mesh_create_with_patches(const uint32_t* fv, uint32_t num_faces, uint32_t num_vertices,
+                                        const uint32_t* face_patch, uint32_t num_patches, uint32_t flags,
+                                        rxm_mesh** out)
+{
+    if (!out || !fv || !face_patch || num_faces == 0) return RXM_ERR_INVALID_ARG;
+    *out = nullptr;
+    rxm_mesh* m = new (std::nothrow) rxm_mesh();
+    if (!m) return RXM_ERR_OUT_OF_MEMORY;
+    int rc = build_supporting(m->h, fv, num_faces, num_vertices);
+    if (rc == RXM_OK) rc = set_patches(m->h, face_patch, num_patches, nullptr, nullptr);
+    if (rc != RXM_OK) {
+        delete m;
+        return rc;
+    }
+    m->h.patch_size = 0;
+    return finish_create(m, flags, out);
+}
+
+rxm_status rxm_mesh_destroy(rxm_mesh* m)
+{
+    if (!m) return RXM_ERR_INVALID_ARG;
+    int rc = RXM_OK;
+    for (void* p : m->allocs)
+        if (hipFree(p) != hipSuccess) rc = RXM_ERR_DEVICE;
+    delete m;
+    return rc;
+}
+
+rxm_status rxm_mesh_info(const rxm_mesh* m, rxm_mesh_info_t* info)
+{
+    if (!m || !info) return RXM_ERR_INVALID_ARG;
+    const HostMesh& h = m->h;
+    std::memset(info, 0, sizeof(*info));
+    info->num_vertices = h.nV;
+    info->num_edges = h.nE;
+    info->num_faces = h.nF;
+    info->num_patches = h.P;
+    info->patch_size = h.patch_size;
+    info->max_vertices_per_patch = h.max_nv;
+    info->max_edges_per_patch = h.max_ne;
+    info->max_faces_per_patch = h.max_nf;
+    info->max_owned_vertices = h.max_nvo;
+    info->max_owned_edges = h.max_neo;
+    info->max_owned_faces = h.max_nfo;
+    info->is_closed = h.closed;
+    info->is_edge_manifold = h.manifold;
+    info->max_valence = h.max_valence;
+    info->max_edge_incident_faces = h.max_edge_faces;
+    info->max_face_adjacent_faces = h.max_face_adj;
+    info->num_ribbon_faces = static_cast<uint32_t>(h.ribbon_val.size());
+    info->on_device = m->on_device;
+    info->device_bytes = m->device_bytes;
+    return RXM_OK;
+}
+
+static int array_ref(const rxm_mesh* m, int which, const void** ptr, uint64_t* count, uint32_t* esize)
+{
+    const HostMesh& h = m->h;
+    static thread_local std::vector<uint32_t> tmp;
+    *esize = 4;
+    auto set = [&](const std::vector<uint32_t>& v) {
+        *ptr   = v.data();
+        *count = v.size();
+    };
+    switch (which) {
+        case RXM_ARR_EV: set(h.ev); break;
+        case RXM_ARR_FACE_PATCH: set(h.face_patch); break;
+        case RXM_ARR_VERTEX_PATCH: set(h.vertex_patch); break;
+        case RXM_ARR_EDGE_PATCH: set(h.edge_patch); break;
+        case RXM_ARR_PATCHES_VAL: set(h.patches_val); break;
+        case RXM_ARR_PATCHES_OFFSET: set(h.patches_offset); break;
+        case RXM_ARR_RIBBON_VAL: set(h.ribbon_val); break;
+        case RXM_ARR_RIBBON_OFFSET: set(h.ribbon_offset); break;
+        case RXM_ARR_V_L2G: set(h.v_l2g); break;
+        case RXM_ARR_E_L2G: set(h.e_l2g); break;
+        case RXM_ARR_F_L2G: set(h.f_l2g); break;
+        case RXM_ARR_V_G2L: set(h.v_g2l); break;
+        case RXM_ARR_E_G2L: set(h.e_g2l); break;
+        case RXM_ARR_F_G2L: set(h.f_g2l); break;
+        case RXM_ARR_PATCH_COUNTS:
+            tmp.assign(6ull * h.P, 0);
+            for (uint32_t p = 0; p < h.P; ++p) {
+                tmp[6 * p] = h.nv[p]; tmp[6 * p + 1] = h.ne[p]; tmp[6 * p + 2] = h.nf[p];
+                tmp[6 * p + 3] = h.nvo[p]; tmp[6 * p + 4] = h.neo[p]; tmp[6 * p + 5] = h.nfo[p];
+            }
+            set(tmp);
+            break;
+        case RXM_ARR_PATCH_OFFSETS:
+            tmp.clear();
+            tmp.insert(tmp.end(), h.v_off.begin(), h.v_off.end());
+            tmp.insert(tmp.end(), h.e_off.begin(), h.e_off.end());
+            tmp.insert(tmp.end(), h.f_off.begin(), h.f_off.end());
+            set(tmp);
+            break;
+        case RXM_ARR_LTOG_V: set(h.ltog_v); break;
+        case RXM_ARR_LTOG_E: set(h.ltog_e); break;
+        case RXM_ARR_LTOG_F: set(h.ltog_f); break;
+        case RXM_ARR_EV_LOCAL: set(h.ev_local); break;
+        case RXM_ARR_FE_LOCAL:
+            *ptr   = h.fe_local.data();
+            *count = h.fe_local.size();
+            *esize = 2;
+            break;
+        default: return RXM_ERR_INVALID_ARG;
+    }
+    return RXM_OK;
+}
+
+rxm_status rxm_mesh_array_size(const rxm_mesh* m, int which, uint64_t* count)
+{
+    if (!m || !count) return RXM_ERR_INVALID_ARG;
+    const void* p;
+    uint32_t    es;
+    return array_ref(m, which, &p, count, &es);
+}
+
+rxm_status rxm_mesh_get_array(const rxm_mesh* m, int which, void* dst, uint64_t bytes)
+{
+    if (!m || !dst) return RXM_ERR_INVALID_ARG;
+    const void* p;
+    uint64_t    n;
+    uint32_t    es;
+    int         rc = array_ref(m, which, &p, &n, &es);
+    if (rc) return rc;
+    if (bytes < n * es) return RXM_ERR_INVALID_ARG;
+    std::memcpy(dst, p, n * es);
+    return RXM_OK;
+}
+
+rxm_status rxm_mesh_save_patches(const rxm_mesh* m, const char* path)
+{
+    if (!m || !path) return RXM_ERR_INVALID_ARG;
+    return write_patch_file(path, m->h);
+}
+
+rxm_status rxm_query_nnz(rxm_mesh* m, int op, int64_t* nnz)
+{
+    if (!m || !nnz || !valid_op(op)) return RXM_ERR_INVALID_ARG;
+    *nnz = op_base(m, op).back();
+    return RXM_OK;
+}
+
+rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, int32_t* col, void* stream)
+{
+    if (!m || !row_ptr || !col || !valid_op(op)) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    if (oriented && op != RXM_OP_VV && op != RXM_OP_VE) return RXM_ERR_UNSUPPORTED;
+    if (oriented && !m->h.manifold) return RXM_ERR_NON_MANIFOLD;
+    const HostMesh& h = m->h;
+    const uint32_t* base = nullptr;
+    int rc = device_base(m, op, &base);
+    if (rc) return rc;
+    Bump b(nullptr);
+    switch (op) {
+        case RXM_OP_VV:
+            plan_vx(b, h.max_nv, h.max_ne, h.max_nf, oriented != 0);
+            return launch_patches(k_query_vx<true>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
+                                  col, oriented);
+        case RXM_OP_VE:
+            plan_vx(b, h.max_nv, h.max_ne, h.max_nf, oriented != 0);
+            return launch_patches(k_query_vx<false>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
+                                  col, oriented);
+        case RXM_OP_VF:
+            plan_vf(b, h.max_nv, h.max_ne, h.max_nf);
+            return launch_patches(k_query_vf, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+        case RXM_OP_EF:
+            plan_ef(b, h.max_ne, h.max_nf, h.max_nfo);
+            return launch_patches(k_query_ef<false>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
+                                  col);
+        case RXM_OP_FF:
+            plan_ef(b, h.max_ne, h.max_nf, h.max_nfo);
+            return launch_patches(k_query_ef<true>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
+                                  col);
+        case RXM_OP_EV:
+            plan_fx(b, h.max_ne, h.max_nf);
+            return launch_patches(k_query_fixed<9>, kQueryThreads, b.bytes(), stream, m, m->dm, row_ptr, col);
+        case RXM_OP_FV:
+            plan_fx(b, h.max_ne, h.max_nf);
+            return launch_patches(k_query_fixed<6>, kQueryThreads, b.bytes(), stream, m, m->dm, row_ptr, col);
+        case RXM_OP_FE:
+            plan_fx(b, h.max_ne, h.max_nf);
+            return launch_patches(k_query_fixed<7>, kQueryThreads, b.bytes(), stream, m, m->dm, row_ptr, col);
+        default: return RXM_ERR_INVALID_ARG;
+    }
+}
+
+rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* normals, void* stream)
+{
+    if (!m || !pos || !normals || (mode != RXM_NORMALS_MAX && mode != RXM_NORMALS_AREA)) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    Bump b(nullptr);
+    plan_vn(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
+    return launch_patches(k_vertex_normals, kGeomThreads, b.bytes(), stream, m, m->dm, mode, pos, normals);
+}
+
+rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float* tmp, float lambda, int iters,
+                                void* stream)
+{
+    if (!m || !pos || !out || iters < 1 || pos == out) return RXM_ERR_INVALID_ARG;
+    if (iters > 1 && (!tmp || tmp == out || tmp == pos)) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    Bump b(nullptr);
+    plan_sm(b, m->h.max_nv, m->h.max_ne);
+    // ping-pong so that the last step lands in `out`
+    const float* src = pos;
+    for (int it = 0; it < iters; ++it) {
+        float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
+        int rc = launch_patches(k_smooth_uniform, kGeomThreads, b.bytes(), stream, m, m->dm, lambda, src, dst);
+        if (rc) return rc;
+        src = dst;
+    }
+    return RXM_OK;
+}
+
+static rxm_status permute(rxm_mesh* m, int kind, int ncomp, const float* src, float* dst, bool to_linear,
+                          void* stream)
+{
+    if (!m || !src || !dst || src == dst || kind < 0 || kind > 2 || ncomp < 1) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const DevMesh& d = m->dm;
+    const uint32_t n = (kind == 0) ? d.nV : (kind == 1) ? d.nE : d.nF;
+    // to linear: dst[l] = src[l2g[l]]; to global: dst[g] = src[g2l[g]]
+    const uint32_t* map = to_linear ? ((kind == 0) ? d.v_l2g : (kind == 1) ? d.e_l2g : d.f_l2g)
+                                    : ((kind == 0) ? d.v_g2l : (kind == 1) ? d.e_g2l : d.f_g2l);
+    hipLaunchKernelGGL(k_gather_rows, dim3((n + 255) / 256), dim3(256), 0, static_cast<hipStream_t>(stream), n,
+                       static_cast<uint32_t>(ncomp), map, src, dst);
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+rxm_status rxm_to_linear(rxm_mesh* m, int kind, int ncomp, const float* g, float* l, void* stream)
+{
+    return permute(m, kind, ncomp, g, l, true, stream);
+}
+
+rxm_status rxm_to_global(rxm_mesh* m, int kind, int ncomp, const float* l, float* g, void* stream)
+{
+    return permute(m, kind, ncomp, l, g, false, stream);
+}
+
+}  // extern "C"

@@ -1,0 +1,690 @@
+// Host connectivity build: see host_build.h.
+#include "host_build.h"
+
+#include <algorithm>
+#include <atomic>
+#include <cstdlib>
+#include <cstring>
+#include <fstream>
+#include <numeric>
+#include <thread>
+
+namespace rxm {
+
+unsigned num_threads()
+{
+    static unsigned n = [] {
+        unsigned t = std::thread::hardware_concurrency();
+        if (const char* s = std::getenv("RXM_NUM_THREADS")) t = static_cast<unsigned>(std::atoi(s));
+        if (t < 1) t = 1;
+        if (t > 64) t = 64;
+        return t;
+    }();
+    return n;
+}
+
+namespace {
+
+// Dynamic-chunk parallel loop over [0, n). f(i) must be thread-safe.
+template <class F>
+void parallel_for(size_t n, F&& f, size_t chunk = 256)
+{
+    unsigned T = num_threads();
+    if (T == 1 || n <= chunk) {
+        for (size_t i = 0; i < n; ++i) f(i);
+        return;
+    }
+    std::atomic<size_t>      next{0};
+    std::vector<std::thread> pool;
+    for (unsigned t = 0; t < T; ++t)
+        pool.emplace_back([&] {
+            size_t s;
+            while ((s = next.fetch_add(chunk)) < n) {
+                size_t e = std::min(n, s + chunk);
+                for (size_t i = s; i < e; ++i) f(i);
+            }
+        });
+    for (auto& th : pool) th.join();
+}
+
+inline uint32_t round_up(uint32_t x, uint32_t a) { return (x + a - 1) / a * a; }
+
+// exclusive prefix sum of counts (n) into off (n + 1)
+void exclusive_scan(const std::vector<uint32_t>& cnt, std::vector<uint32_t>& off)
+{
+    off.resize(cnt.size() + 1);
+    off[0] = 0;
+    for (size_t i = 0; i < cnt.size(); ++i) off[i + 1] = off[i] + cnt[i];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// a1. reference: include/rxmesh/rxmesh.cpp:518-681. Edge ids are assigned in
+// order of first appearance over the half-edges (f, i) -> (fv[i], fv[i+1]);
+// instead of a hash map, half-edges are bucketed by their smaller endpoint
+// and each bucket's groups are resolved to their first half-edge.
+int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint)
+{
+    if (fv == nullptr || nF == 0) return kErrInvalidArg;
+    m.nF = nF;
+    m.fv.assign(fv, fv + 3ull * nF);
+    uint32_t vmax = 0;
+    for (uint64_t i = 0; i < 3ull * nF; ++i) vmax = std::max(vmax, m.fv[i]);
+    if (vmax == kInvalid32) return kErrInvalidArg;
+    m.nV = std::max(nV_hint, vmax + 1);
+    for (uint32_t f = 0; f < nF; ++f) {
+        const uint32_t* t = &m.fv[3ull * f];
+        if (t[0] == t[1] || t[1] == t[2] || t[0] == t[2]) return kErrInvalidArg;
+    }
+    const uint64_t nH = 3ull * nF;
+    if (nH >= kInvalid32) return kErrInvalidArg;
+
+    // vertex -> faces, face order
+    std::vector<uint32_t> cnt(m.nV, 0);
+    for (uint64_t h = 0; h < nH; ++h) cnt[m.fv[h]]++;
+    exclusive_scan(cnt, m.vf_off);
+    m.vf_val.resize(nH);
+    {
+        std::vector<uint32_t> cur(m.vf_off.begin(), m.vf_off.end() - 1);
+        for (uint64_t h = 0; h < nH; ++h) m.vf_val[cur[m.fv[h]]++] = static_cast<uint32_t>(h / 3);
+    }
+
+    // half-edges bucketed by min endpoint, in half-edge order
+    auto he_a = [&](uint64_t h) { return m.fv[h]; };
+    auto he_b = [&](uint64_t h) { return m.fv[(h % 3 == 2) ? h - 2 : h + 1]; };
+    std::fill(cnt.begin(), cnt.end(), 0);
+    for (uint64_t h = 0; h < nH; ++h) cnt[std::min(he_a(h), he_b(h))]++;
+    std::vector<uint32_t> boff;
+    exclusive_scan(cnt, boff);
+    std::vector<uint32_t> bval(nH);
+    {
+        std::vector<uint32_t> cur(boff.begin(), boff.end() - 1);
+        for (uint64_t h = 0; h < nH; ++h) bval[cur[std::min(he_a(h), he_b(h))]++] = static_cast<uint32_t>(h);
+    }
+    std::vector<uint32_t> leader(nH);
+    parallel_for(m.nV, [&](size_t v) {
+        for (uint32_t i = boff[v]; i < boff[v + 1]; ++i) {
+            uint32_t hi = bval[i];
+            uint32_t ki = std::max(he_a(hi), he_b(hi));
+            uint32_t l  = hi;
+            for (uint32_t j = boff[v]; j < i; ++j) {
+                uint32_t hj = bval[j];
+                if (std::max(he_a(hj), he_b(hj)) == ki) {
+                    l = hj;
+                    break;
+                }
+            }
+            leader[hi] = l;
+        }
+    }, 4096);
+    // edge id = rank of the leader half-edge
+    std::vector<uint32_t> eid(nH, kInvalid32);
+    uint32_t              nE = 0;
+    for (uint64_t h = 0; h < nH; ++h)
+        if (leader[h] == h) eid[h] = nE++;
+    m.nE = nE;
+    m.fe_g.resize(nH);
+    m.ev.resize(2ull * nE);
+    for (uint64_t h = 0; h < nH; ++h) {
+        uint32_t e = eid[leader[h]];
+        m.fe_g[h]  = e;
+        if (leader[h] == h) {
+            m.ev[2ull * e]     = he_a(h);
+            m.ev[2ull * e + 1] = he_b(h);
+        }
+    }
+    // edge -> faces, face order
+    std::vector<uint32_t> ecnt(nE, 0);
+    for (uint64_t h = 0; h < nH; ++h) ecnt[m.fe_g[h]]++;
+    exclusive_scan(ecnt, m.ef_off);
+    m.ef_val.resize(nH);
+    {
+        std::vector<uint32_t> cur(m.ef_off.begin(), m.ef_off.end() - 1);
+        for (uint64_t h = 0; h < nH; ++h) m.ef_val[cur[m.fe_g[h]]++] = static_cast<uint32_t>(h / 3);
+    }
+    // statistics (rxmesh.cpp:589-660)
+    m.closed = true;
+    m.manifold = true;
+    m.max_edge_faces = 0;
+    std::vector<uint32_t> val(m.nV, 0);
+    for (uint32_t e = 0; e < nE; ++e) {
+        uint32_t c = ecnt[e];
+        if (c == 1) m.closed = false;
+        if (c > 2) m.manifold = false;
+        m.max_edge_faces = std::max(m.max_edge_faces, c);
+        val[m.ev[2ull * e]]++;
+        val[m.ev[2ull * e + 1]]++;
+    }
+    m.max_valence = *std::max_element(val.begin(), val.end());
+    m.max_face_adj = 0;
+    for (uint32_t f = 0; f < nF; ++f) {
+        uint32_t s = 0;
+        for (int i = 0; i < 3; ++i) s += ecnt[m.fe_g[3ull * f + i]] - 1;
+        m.max_face_adj = std::max(m.max_face_adj, s);
+    }
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// a2. Deterministic patcher. The reference grows patches with a GPU Lloyd
+// iteration (patcher.cu:828-987) seeded from std::random_device, so its
+// layout cannot be reproduced. This one keeps the same idea -- k-means on the
+// face graph: multi-source BFS assignment, re-centre each seed on its
+// patch's most interior face, repeat -- with seeds taken at even strides of
+// a BFS order, followed by recursive bisection of any patch above
+// patch_size. Everything is sequential FIFO BFS, so it is deterministic.
+namespace {
+
+struct FaceGraph
+{
+    std::vector<uint32_t> off, val;
+};
+
+FaceGraph face_graph(const HostMesh& m)
+{
+    FaceGraph g;
+    g.off.assign(m.nF + 1, 0);
+    for (uint32_t f = 0; f < m.nF; ++f) {
+        uint32_t c = 0;
+        for (int i = 0; i < 3; ++i) {
+            uint32_t e = m.fe_g[3ull * f + i];
+            c += m.ef_off[e + 1] - m.ef_off[e] - 1;
+        }
+        g.off[f + 1] = g.off[f] + c;
+    }
+    g.val.resize(g.off[m.nF]);
+    parallel_for(m.nF, [&](size_t f) {
+        uint32_t k = g.off[f];
+        for (int i = 0; i < 3; ++i) {
+            uint32_t e = m.fe_g[3ull * f + i];
+            for (uint32_t j = m.ef_off[e]; j < m.ef_off[e + 1]; ++j)
+                if (m.ef_val[j] != f) g.val[k++] = m.ef_val[j];
+        }
+    }, 8192);
+    return g;
+}
+
+// Multi-source BFS from seeds; faces not reached get fresh seeds (smallest
+// unassigned face id). Returns the number of patches.
+uint32_t bfs_assign(const FaceGraph& g, uint32_t nF, std::vector<uint32_t>& seeds,
+                    std::vector<uint32_t>& fp)
+{
+    fp.assign(nF, kInvalid32);
+    std::vector<uint32_t> q;
+    q.reserve(nF);
+    for (uint32_t k = 0; k < seeds.size(); ++k) {
+        fp[seeds[k]] = k;
+        q.push_back(seeds[k]);
+    }
+    size_t   head = 0;
+    uint32_t scan = 0;
+    while (true) {
+        while (head < q.size()) {
+            uint32_t f = q[head++];
+            for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+                uint32_t n = g.val[j];
+                if (fp[n] == kInvalid32) {
+                    fp[n] = fp[f];
+                    q.push_back(n);
+                }
+            }
+        }
+        while (scan < nF && fp[scan] != kInvalid32) ++scan;
+        if (scan == nF) break;
+        fp[scan] = static_cast<uint32_t>(seeds.size());
+        seeds.push_back(scan);
+        q.push_back(scan);
+    }
+    return static_cast<uint32_t>(seeds.size());
+}
+
+// New seed of each patch = its face farthest from the patch border.
+void recenter(const FaceGraph& g, uint32_t nF, const std::vector<uint32_t>& fp,
+              std::vector<uint32_t>& seeds)
+{
+    std::vector<uint32_t> dist(nF, kInvalid32), q;
+    q.reserve(nF);
+    for (uint32_t f = 0; f < nF; ++f) {
+        bool border = (g.off[f + 1] - g.off[f]) < 3;
+        for (uint32_t j = g.off[f]; j < g.off[f + 1] && !border; ++j)
+            border = fp[g.val[j]] != fp[f];
+        if (border) {
+            dist[f] = 0;
+            q.push_back(f);
+        }
+    }
+    size_t head = 0;
+    while (head < q.size()) {
+        uint32_t f = q[head++];
+        for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+            uint32_t n = g.val[j];
+            if (fp[n] == fp[f] && dist[n] == kInvalid32) {
+                dist[n] = dist[f] + 1;
+                q.push_back(n);
+            }
+        }
+    }
+    std::vector<uint32_t> best(seeds.size(), 0);
+    for (uint32_t f = 0; f < nF; ++f) {
+        uint32_t k = fp[f];
+        uint32_t d = (dist[f] == kInvalid32) ? 0 : dist[f];
+        if (d > best[k] || (d == best[k] && dist[seeds[k]] != kInvalid32 && d > 0 && f < seeds[k])) {
+            best[k]  = d;
+            seeds[k] = f;
+        }
+    }
+}
+
+// BFS distances restricted to a face subset (mask == tag)
+void subset_bfs(const FaceGraph& g, const std::vector<uint32_t>& faces, uint32_t src,
+                const std::vector<uint32_t>& tag_of, uint32_t tag,
+                std::vector<uint32_t>& dist_out)
+{
+    // dist_out is indexed by position in `faces`; use a local map
+    std::vector<std::pair<uint32_t, uint32_t>> pos;
+    pos.reserve(faces.size());
+    for (uint32_t i = 0; i < faces.size(); ++i) pos.push_back({faces[i], i});
+    std::sort(pos.begin(), pos.end());
+    auto index_of = [&](uint32_t f) {
+        auto it = std::lower_bound(pos.begin(), pos.end(), std::make_pair(f, 0u));
+        return it->second;
+    };
+    dist_out.assign(faces.size(), kInvalid32);
+    std::vector<uint32_t> q{src};
+    dist_out[index_of(src)] = 0;
+    size_t head = 0;
+    while (head < q.size()) {
+        uint32_t f  = q[head++];
+        uint32_t df = dist_out[index_of(f)];
+        for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+            uint32_t n = g.val[j];
+            if (tag_of[n] != tag) continue;
+            uint32_t in = index_of(n);
+            if (dist_out[in] == kInvalid32) {
+                dist_out[in] = df + 1;
+                q.push_back(n);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+int run_patcher(HostMesh& m, uint32_t patch_size)
+{
+    if (patch_size < 16) return kErrInvalidArg;
+    m.patch_size = patch_size;
+    const uint32_t nF = m.nF;
+    std::vector<uint32_t> fp;
+    if (nF <= patch_size) {
+        fp.assign(nF, 0);
+        return set_patches(m, fp.data(), 1, nullptr, nullptr);
+    }
+    FaceGraph g = face_graph(m);
+    // BFS order over all components from face 0
+    std::vector<uint32_t> order, rank(nF, kInvalid32);
+    order.reserve(nF);
+    for (uint32_t s = 0, head = 0; s < nF; ++s) {
+        if (rank[s] != kInvalid32) continue;
+        rank[s] = static_cast<uint32_t>(order.size());
+        order.push_back(s);
+        while (head < order.size()) {
+            uint32_t f = order[head++];
+            for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+                uint32_t n = g.val[j];
+                if (rank[n] == kInvalid32) {
+                    rank[n] = static_cast<uint32_t>(order.size());
+                    order.push_back(n);
+                }
+            }
+        }
+    }
+    const uint32_t P0 = (nF + patch_size - 1) / patch_size;
+    std::vector<uint32_t> seeds(P0);
+    for (uint32_t k = 0; k < P0; ++k) seeds[k] = order[(uint64_t(k) * nF) / P0];
+    const int iters = 4;
+    for (int it = 0; it < iters; ++it) {
+        bfs_assign(g, nF, seeds, fp);
+        if (it + 1 < iters) recenter(g, nF, fp, seeds);
+    }
+    // recursive bisection of oversize patches
+    std::vector<std::vector<uint32_t>> members(seeds.size());
+    for (uint32_t f = 0; f < nF; ++f) members[fp[f]].push_back(f);
+    for (size_t k = 0; k < members.size(); ++k) {
+        while (members[k].size() > patch_size) {
+            auto& faces = members[k];
+            std::vector<uint32_t> d;
+            subset_bfs(g, faces, faces[0], fp, static_cast<uint32_t>(k), d);
+            // a = farthest from faces[0], b = farthest from a (ties: first)
+            auto far = [&](const std::vector<uint32_t>& dd) {
+                uint32_t bi = 0;
+                for (uint32_t i = 0; i < dd.size(); ++i)
+                    if (dd[i] != kInvalid32 && (dd[bi] == kInvalid32 || dd[i] > dd[bi])) bi = i;
+                return bi;
+            };
+            uint32_t a = faces[far(d)];
+            std::vector<uint32_t> da, db;
+            subset_bfs(g, faces, a, fp, static_cast<uint32_t>(k), da);
+            uint32_t b = faces[far(da)];
+            subset_bfs(g, faces, b, fp, static_cast<uint32_t>(k), db);
+            std::vector<std::pair<int64_t, uint32_t>> key(faces.size());
+            for (uint32_t i = 0; i < faces.size(); ++i) {
+                int64_t x = (da[i] == kInvalid32) ? (1 << 30) : da[i];
+                int64_t y = (db[i] == kInvalid32) ? (1 << 30) : db[i];
+                key[i] = {x - y, faces[i]};
+            }
+            std::sort(key.begin(), key.end());
+            size_t half = faces.size() / 2;
+            std::vector<uint32_t> A, B;
+            for (size_t i = 0; i < key.size(); ++i) (i < half ? A : B).push_back(key[i].second);
+            std::sort(A.begin(), A.end());
+            std::sort(B.begin(), B.end());
+            uint32_t nk = static_cast<uint32_t>(members.size());
+            for (uint32_t f : B) fp[f] = nk;
+            members[k] = A;
+            members.push_back(B);  // invalidates `faces`; loop re-reads members[k]
+        }
+    }
+    // renumber patches by the BFS rank of their earliest face (locality of
+    // the patch-linear attribute order)
+    const uint32_t P = static_cast<uint32_t>(members.size());
+    std::vector<std::pair<uint32_t, uint32_t>> first(P, {kInvalid32, 0});
+    for (uint32_t k = 0; k < P; ++k) {
+        first[k].second = k;
+        for (uint32_t f : members[k]) first[k].first = std::min(first[k].first, rank[f]);
+    }
+    std::sort(first.begin(), first.end());
+    std::vector<uint32_t> remap(P);
+    for (uint32_t i = 0; i < P; ++i) remap[first[i].second] = i;
+    for (uint32_t f = 0; f < nF; ++f) fp[f] = remap[fp[f]];
+    return set_patches(m, fp.data(), P, nullptr, nullptr);
+}
+
+int set_patches(HostMesh& m, const uint32_t* face_patch, uint32_t P, const uint32_t* patches_val,
+                const uint32_t* patches_offset)
+{
+    if (!face_patch || P == 0) return kErrInvalidArg;
+    m.P = P;
+    m.face_patch.assign(face_patch, face_patch + m.nF);
+    std::vector<uint32_t> cnt(P, 0);
+    for (uint32_t f = 0; f < m.nF; ++f) {
+        if (m.face_patch[f] >= P) return kErrInvalidArg;
+        cnt[m.face_patch[f]]++;
+    }
+    for (uint32_t p = 0; p < P; ++p)
+        if (cnt[p] == 0) return kErrInvalidArg;  // empty patch
+    if (patches_val && patches_offset) {
+        m.patches_val.assign(patches_val, patches_val + m.nF);
+        m.patches_offset.assign(patches_offset, patches_offset + P);
+        uint32_t s = 0;
+        for (uint32_t p = 0; p < P; ++p) {
+            if (m.patches_offset[p] < s || m.patches_offset[p] > m.nF) return kErrInvalidArg;
+            for (uint32_t k = s; k < m.patches_offset[p]; ++k)
+                if (m.patches_val[k] >= m.nF || m.face_patch[m.patches_val[k]] != p) return kErrInvalidArg;
+            s = m.patches_offset[p];
+        }
+        if (s != m.nF) return kErrInvalidArg;
+    } else {
+        // faces grouped by patch, ascending face order (patcher.cu:631-637)
+        m.patches_offset.resize(P);
+        std::vector<uint32_t> off;
+        exclusive_scan(cnt, off);
+        for (uint32_t p = 0; p < P; ++p) m.patches_offset[p] = off[p + 1];
+        m.patches_val.resize(m.nF);
+        std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+        for (uint32_t f = 0; f < m.nF; ++f) m.patches_val[cur[m.face_patch[f]]++] = f;
+    }
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// a3..a6. reference: patcher.cu:640-757 (ribbons, ownership) and
+// rxmesh.cpp:754-996 (ltog + topology). Ownership uses the reference rule
+// "first patch in patch order with a face touching the element", which is
+// the minimum patch id over incident faces.
+int build_patches(HostMesh& m)
+{
+    const uint32_t P = m.P;
+    m.vertex_patch.assign(m.nV, kInvalid32);
+    m.edge_patch.assign(m.nE, kInvalid32);
+    for (uint32_t v = 0; v < m.nV; ++v)
+        for (uint32_t j = m.vf_off[v]; j < m.vf_off[v + 1]; ++j)
+            m.vertex_patch[v] = std::min(m.vertex_patch[v], m.face_patch[m.vf_val[j]]);
+    for (uint32_t e = 0; e < m.nE; ++e)
+        for (uint32_t j = m.ef_off[e]; j < m.ef_off[e + 1]; ++j)
+            m.edge_patch[e] = std::min(m.edge_patch[e], m.face_patch[m.ef_val[j]]);
+
+    struct PatchLists
+    {
+        std::vector<uint32_t> ribbon;          // discovery order
+        std::vector<uint32_t> lf, le, lv;      // ltog
+        uint32_t              nfo, neo, nvo;
+    };
+    std::vector<PatchLists> pl(P);
+    std::atomic<int>        err{kOk};
+    parallel_for(P, [&](size_t pi) {
+        const uint32_t p  = static_cast<uint32_t>(pi);
+        const uint32_t ps = (p == 0) ? 0 : m.patches_offset[p - 1];
+        const uint32_t pe = m.patches_offset[p];
+        PatchLists&    L  = pl[p];
+        // ribbon: faces around the vertices of owned faces, first discovery
+        std::vector<uint32_t> cand;
+        for (uint32_t k = ps; k < pe; ++k) {
+            uint32_t f = m.patches_val[k];
+            for (int i = 0; i < 3; ++i) {
+                uint32_t v = m.fv[3ull * f + i];
+                for (uint32_t j = m.vf_off[v]; j < m.vf_off[v + 1]; ++j)
+                    if (m.face_patch[m.vf_val[j]] != p) cand.push_back(m.vf_val[j]);
+            }
+        }
+        {
+            std::vector<std::pair<uint32_t, uint32_t>> s(cand.size());
+            for (uint32_t i = 0; i < cand.size(); ++i) s[i] = {cand[i], i};
+            std::sort(s.begin(), s.end());
+            std::vector<uint32_t> firsts;
+            for (size_t i = 0; i < s.size(); ++i)
+                if (i == 0 || s[i].first != s[i - 1].first) firsts.push_back(s[i].second);
+            std::sort(firsts.begin(), firsts.end());
+            L.ribbon.clear();
+            for (uint32_t i : firsts) L.ribbon.push_back(cand[i]);
+        }
+        // faces: owned sorted, then ribbon sorted
+        L.lf.assign(m.patches_val.begin() + ps, m.patches_val.begin() + pe);
+        std::sort(L.lf.begin(), L.lf.end());
+        L.nfo = static_cast<uint32_t>(L.lf.size());
+        std::vector<uint32_t> rs = L.ribbon;
+        std::sort(rs.begin(), rs.end());
+        L.lf.insert(L.lf.end(), rs.begin(), rs.end());
+        // vertices and edges of all local faces; owned part first, each sorted
+        std::vector<uint32_t> vs, es;
+        vs.reserve(3 * L.lf.size());
+        es.reserve(3 * L.lf.size());
+        for (uint32_t f : L.lf)
+            for (int i = 0; i < 3; ++i) {
+                vs.push_back(m.fv[3ull * f + i]);
+                es.push_back(m.fe_g[3ull * f + i]);
+            }
+        auto uniq_part = [&](std::vector<uint32_t>& x, const std::vector<uint32_t>& owner,
+                             uint32_t& n_owned) {
+            std::sort(x.begin(), x.end());
+            x.erase(std::unique(x.begin(), x.end()), x.end());
+            auto it = std::stable_partition(x.begin(), x.end(),
+                                            [&](uint32_t g) { return owner[g] == p; });
+            n_owned = static_cast<uint32_t>(it - x.begin());
+        };
+        uniq_part(vs, m.vertex_patch, L.nvo);
+        uniq_part(es, m.edge_patch, L.neo);
+        L.lv = std::move(vs);
+        L.le = std::move(es);
+        if (L.lv.size() >= kInvalid16 || L.le.size() >= kInvalid16 || L.lf.size() >= kInvalid16)
+            err = kErrPatchTooBig;
+    }, 1);
+    if (err != kOk) return err;
+
+    // offsets and prefixes
+    m.nv.resize(P); m.ne.resize(P); m.nf.resize(P);
+    m.nvo.resize(P); m.neo.resize(P); m.nfo.resize(P);
+    m.v_off.assign(P + 1, 0); m.e_off.assign(P + 1, 0); m.f_off.assign(P + 1, 0);
+    m.vp.assign(P + 1, 0); m.ep.assign(P + 1, 0); m.fp.assign(P + 1, 0);
+    m.ribbon_offset.resize(P);
+    m.ribbon_val.clear();
+    m.max_nv = m.max_ne = m.max_nf = m.max_nvo = m.max_neo = m.max_nfo = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        const PatchLists& L = pl[p];
+        m.nv[p] = static_cast<uint32_t>(L.lv.size());
+        m.ne[p] = static_cast<uint32_t>(L.le.size());
+        m.nf[p] = static_cast<uint32_t>(L.lf.size());
+        m.nvo[p] = L.nvo; m.neo[p] = L.neo; m.nfo[p] = L.nfo;
+        m.v_off[p + 1] = m.v_off[p] + round_up(m.nv[p], kSlabAlign);
+        m.e_off[p + 1] = m.e_off[p] + round_up(m.ne[p], kSlabAlign);
+        m.f_off[p + 1] = m.f_off[p] + round_up(m.nf[p], kSlabAlign);
+        m.vp[p + 1] = m.vp[p] + L.nvo;
+        m.ep[p + 1] = m.ep[p] + L.neo;
+        m.fp[p + 1] = m.fp[p] + L.nfo;
+        m.max_nv = std::max(m.max_nv, m.nv[p]);
+        m.max_ne = std::max(m.max_ne, m.ne[p]);
+        m.max_nf = std::max(m.max_nf, m.nf[p]);
+        m.max_nvo = std::max(m.max_nvo, L.nvo);
+        m.max_neo = std::max(m.max_neo, L.neo);
+        m.max_nfo = std::max(m.max_nfo, L.nfo);
+        m.ribbon_val.insert(m.ribbon_val.end(), L.ribbon.begin(), L.ribbon.end());
+        m.ribbon_offset[p] = static_cast<uint32_t>(m.ribbon_val.size());
+    }
+    if (m.vp[P] != m.nV || m.ep[P] != m.nE || m.fp[P] != m.nF) return kErrInvalidArg;  // isolated vertices
+
+    m.ltog_v.assign(m.v_off[P], kInvalid32);
+    m.ltog_e.assign(m.e_off[P], kInvalid32);
+    m.ltog_f.assign(m.f_off[P], kInvalid32);
+    m.v_l2g.resize(m.nV); m.e_l2g.resize(m.nE); m.f_l2g.resize(m.nF);
+    m.v_g2l.resize(m.nV); m.e_g2l.resize(m.nE); m.f_g2l.resize(m.nF);
+    parallel_for(P, [&](size_t pi) {
+        const uint32_t p = static_cast<uint32_t>(pi);
+        const PatchLists& L = pl[p];
+        std::copy(L.lv.begin(), L.lv.end(), m.ltog_v.begin() + m.v_off[p]);
+        std::copy(L.le.begin(), L.le.end(), m.ltog_e.begin() + m.e_off[p]);
+        std::copy(L.lf.begin(), L.lf.end(), m.ltog_f.begin() + m.f_off[p]);
+        for (uint32_t l = 0; l < L.nvo; ++l) {
+            m.v_l2g[m.vp[p] + l] = L.lv[l];
+            m.v_g2l[L.lv[l]]     = m.vp[p] + l;
+        }
+        for (uint32_t l = 0; l < L.neo; ++l) {
+            m.e_l2g[m.ep[p] + l] = L.le[l];
+            m.e_g2l[L.le[l]]     = m.ep[p] + l;
+        }
+        for (uint32_t l = 0; l < L.nfo; ++l) {
+            m.f_l2g[m.fp[p] + l] = L.lf[l];
+            m.f_g2l[L.lf[l]]     = m.fp[p] + l;
+        }
+    }, 1);
+
+    // topology and owner tables
+    m.ev_local.assign(m.e_off[P], 0xFFFFFFFFu);
+    m.fe_local.assign(3ull * m.f_off[P], kInvalid16);
+    m.v_lin.assign(m.v_off[P], kInvalid32); m.v_own.assign(m.v_off[P], kInvalid32);
+    m.e_lin.assign(m.e_off[P], kInvalid32); m.e_own.assign(m.e_off[P], kInvalid32);
+    m.f_lin.assign(m.f_off[P], kInvalid32); m.f_own.assign(m.f_off[P], kInvalid32);
+    parallel_for(P, [&](size_t pi) {
+        const uint32_t p = static_cast<uint32_t>(pi);
+        const PatchLists& L = pl[p];
+        auto local_of = [&](const std::vector<uint32_t>& l, uint32_t nown, bool owned,
+                            uint32_t g) -> uint32_t {
+            auto b = owned ? l.begin() : l.begin() + nown;
+            auto e = owned ? l.begin() + nown : l.end();
+            auto it = std::lower_bound(b, e, g);
+            return static_cast<uint32_t>(it - l.begin());
+        };
+        for (uint32_t le = 0; le < L.le.size(); ++le) {
+            uint32_t g = L.le[le];
+            uint32_t a = m.ev[2ull * g], b = m.ev[2ull * g + 1];
+            uint32_t hi = std::max(a, b), lo = std::min(a, b);
+            uint32_t l0 = local_of(L.lv, L.nvo, m.vertex_patch[hi] == p, hi);
+            uint32_t l1 = local_of(L.lv, L.nvo, m.vertex_patch[lo] == p, lo);
+            m.ev_local[m.e_off[p] + le] = l0 | (l1 << 16);
+        }
+        for (uint32_t lf = 0; lf < L.lf.size(); ++lf) {
+            uint32_t g = L.lf[lf];
+            for (int i = 0; i < 3; ++i) {
+                uint32_t a  = m.fv[3ull * g + i], b = m.fv[3ull * g + (i + 1) % 3];
+                uint32_t ge = m.fe_g[3ull * g + i];
+                uint32_t le = local_of(L.le, L.neo, m.edge_patch[ge] == p, ge);
+                uint32_t dir = (a == std::max(a, b)) ? 0u : 1u;
+                m.fe_local[3ull * m.f_off[p] + 3ull * lf + i] = static_cast<uint16_t>((le << 1) | dir);
+            }
+        }
+        for (uint32_t l = 0; l < L.lv.size(); ++l) {
+            m.v_lin[m.v_off[p] + l] = m.v_g2l[L.lv[l]];
+            m.v_own[m.v_off[p] + l] = m.vertex_patch[L.lv[l]];
+        }
+        for (uint32_t l = 0; l < L.le.size(); ++l) {
+            m.e_lin[m.e_off[p] + l] = m.e_g2l[L.le[l]];
+            m.e_own[m.e_off[p] + l] = m.edge_patch[L.le[l]];
+        }
+        for (uint32_t l = 0; l < L.lf.size(); ++l) {
+            m.f_lin[m.f_off[p] + l] = m.f_g2l[L.lf[l]];
+            m.f_own[m.f_off[p] + l] = m.face_patch[L.lf[l]];
+        }
+    }, 1);
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
+// Reference patch archive: cereal PortableBinary of Patcher::serialize
+// (include/rxmesh/patcher/patcher.h:162-182): u8 little-endian flag, nine u32
+// scalars, seven (u64 size + u32[size]) vectors, one f32.
+int read_patch_file(const std::string& path, std::vector<uint32_t>& face_patch,
+                    std::vector<uint32_t>& patches_val, std::vector<uint32_t>& patches_offset,
+                    uint32_t& num_patches, uint32_t& nV, uint32_t& nE, uint32_t& nF)
+{
+    std::ifstream in(path, std::ios::binary);
+    if (!in) return kErrIO;
+    uint8_t le = 0;
+    in.read(reinterpret_cast<char*>(&le), 1);
+    if (!in || le != 1) return kErrUnsupported;
+    uint32_t hdr[9];
+    in.read(reinterpret_cast<char*>(hdr), sizeof(hdr));
+    if (!in) return kErrIO;
+    num_patches = hdr[1];
+    nV = hdr[2];
+    nE = hdr[3];
+    nF = hdr[4];
+    std::vector<uint32_t> vecs[7];
+    for (auto& v : vecs) {
+        uint64_t n = 0;
+        in.read(reinterpret_cast<char*>(&n), 8);
+        if (!in || n > (1ull << 32)) return kErrIO;
+        v.resize(n);
+        in.read(reinterpret_cast<char*>(v.data()), static_cast<std::streamsize>(4 * n));
+        if (!in) return kErrIO;
+    }
+    face_patch     = std::move(vecs[0]);
+    patches_val    = std::move(vecs[3]);
+    patches_offset = std::move(vecs[4]);
+    if (face_patch.size() != nF || patches_val.size() != nF || patches_offset.size() < num_patches)
+        return kErrIO;
+    patches_offset.resize(num_patches);
+    return kOk;
+}
+
+int write_patch_file(const std::string& path, const HostMesh& m)
+{
+    std::ofstream out(path, std::ios::binary);
+    if (!out) return kErrIO;
+    uint8_t le = 1;
+    out.write(reinterpret_cast<const char*>(&le), 1);
+    uint32_t hdr[9] = {m.patch_size, m.P, m.nV, m.nE, m.nF, m.P, m.P, 1, 0};
+    out.write(reinterpret_cast<const char*>(hdr), sizeof(hdr));
+    const std::vector<uint32_t>* vecs[7] = {&m.face_patch, &m.vertex_patch, &m.edge_patch,
+                                            &m.patches_val, &m.patches_offset, &m.ribbon_val,
+                                            &m.ribbon_offset};
+    for (auto* v : vecs) {
+        uint64_t n = v->size();
+        out.write(reinterpret_cast<const char*>(&n), 8);
+        out.write(reinterpret_cast<const char*>(v->data()), static_cast<std::streamsize>(4 * n));
+    }
+    float t = 0.f;
+    out.write(reinterpret_cast<const char*>(&t), 4);
+    return out ? kOk : kErrIO;
+}
+
+}  // namespace rxm

@@ -1,0 +1,95 @@
+// Host-side connectivity build of the static patch mesh.
+//
+// Produces the same patch-local layout the reference builds in
+// include/rxmesh/rxmesh.cpp:290-448 (edge ids in first-appearance order,
+// ribbons, first-patch ownership, owned-first sorted local numbering, EV/FE
+// encodings) but with O(F) bucket passes and a thread pool instead of hash
+// maps and per-patch O(V+E) scratch vectors, plus a deterministic patcher.
+#pragma once
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace rxm {
+
+constexpr uint32_t kInvalid32 = 0xFFFFFFFFu;
+constexpr uint16_t kInvalid16 = 0xFFFFu;
+// per-patch slabs are padded to this many elements so that every patch's
+// u16 / u32 rows start on an 8-byte / 16-byte boundary
+constexpr uint32_t kSlabAlign = 4;
+
+struct HostMesh
+{
+    uint32_t nV = 0, nE = 0, nF = 0, P = 0;
+    uint32_t patch_size = 512;
+
+    std::vector<uint32_t> fv;    // 3F
+    std::vector<uint32_t> ev;    // 2E, first-seen orientation
+    std::vector<uint32_t> fe_g;  // 3F, global edge of (fv[i], fv[i+1])
+    std::vector<uint32_t> ef_off, ef_val;  // edge -> incident faces, face order
+    std::vector<uint32_t> vf_off, vf_val;  // vertex -> incident faces, face order
+
+    bool     closed = true, manifold = true;
+    uint32_t max_valence = 0, max_edge_faces = 0, max_face_adj = 0;
+
+    // patch assignment in the reference's layout (patcher.h:162-182)
+    std::vector<uint32_t> face_patch, patches_val, patches_offset;
+    std::vector<uint32_t> ribbon_val, ribbon_offset;
+    std::vector<uint32_t> vertex_patch, edge_patch;
+
+    // per patch
+    std::vector<uint32_t> nv, ne, nf, nvo, neo, nfo;  // P each
+    std::vector<uint32_t> v_off, e_off, f_off;        // P+1, slab offsets
+    std::vector<uint32_t> vp, ep, fp;                 // P+1, linear prefix
+    uint32_t              max_nv = 0, max_ne = 0, max_nf = 0;
+    uint32_t              max_nvo = 0, max_neo = 0, max_nfo = 0;
+
+    // slabs indexed by off[p] + local
+    std::vector<uint32_t> ltog_v, ltog_e, ltog_f;
+    std::vector<uint32_t> ev_local;  // lo16 = local(max gid), hi16 = local(min gid)
+    std::vector<uint16_t> fe_local;  // 3 per face at 3*f_off[p]: (local edge << 1) | dir
+    std::vector<uint32_t> v_lin, e_lin, f_lin;  // linear id of the owner's copy
+    std::vector<uint32_t> v_own, e_own, f_own;  // owner patch
+
+    // linear <-> global
+    std::vector<uint32_t> v_l2g, e_l2g, f_l2g;
+    std::vector<uint32_t> v_g2l, e_g2l, f_g2l;
+};
+
+// status codes shared with the C-ABI
+enum Status : int
+{
+    kOk              = 0,
+    kErrInvalidArg   = 1,
+    kErrNonManifold  = 2,
+    kErrPatchTooBig  = 3,
+    kErrIO           = 4,
+    kErrDevice       = 5,
+    kErrUnsupported  = 6,
+    kErrOutOfMemory  = 7,
+};
+
+// Step a1: edges, incidence, statistics. nV_hint = 0 infers nV = max id + 1.
+int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint);
+
+// Step a2: deterministic patcher (fills face_patch / patches_val / patches_offset).
+int run_patcher(HostMesh& m, uint32_t patch_size);
+
+// Alternative to run_patcher: take a given assignment (e.g. a reference patch
+// file). patches_val / patches_offset may be null (then grouped by face order).
+int set_patches(HostMesh& m, const uint32_t* face_patch, uint32_t P,
+                const uint32_t* patches_val, const uint32_t* patches_offset);
+
+// Steps a3..a6: ribbons, ownership, local numbering, EV/FE, owner tables.
+int build_patches(HostMesh& m);
+
+// Reference patch archive (cereal PortableBinary) reader / writer.
+int read_patch_file(const std::string& path, std::vector<uint32_t>& face_patch,
+                    std::vector<uint32_t>& patches_val,
+                    std::vector<uint32_t>& patches_offset, uint32_t& num_patches,
+                    uint32_t& nV, uint32_t& nE, uint32_t& nF);
+int write_patch_file(const std::string& path, const HostMesh& m);
+
+unsigned num_threads();
+
+}  // namespace rxm

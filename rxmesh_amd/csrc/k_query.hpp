@@ -1,0 +1,340 @@
+// Query kernels: materialise any of the 8 relations as a CSR over linear ids.
+//
+// One workgroup per patch. Reference behaviour: query<op>
+// (include/rxmesh/kernels/rxmesh_queries.cuh:861-1053) and the iterator's
+// owner resolution (include/rxmesh/iterator.cuh:117-143). Rows are the owned
+// source elements of the patch (linear ids prefix[p] + local); entries are the
+// target elements' linear ids, i.e. always the owner's copy.
+#pragma once
+#include "lds_core.hpp"
+
+namespace rxm {
+
+constexpr uint32_t kQueryThreads = 256;
+
+__host__ __device__ inline uint32_t r4(uint32_t n) { return (n + 3u) & ~3u; }
+
+// ---- LDS plans (same carve on host for sizing and on device) -------------
+struct PlanVX
+{
+    uint32_t *ev, *off, *cur, *wave, *ef, *gid;
+    uint16_t *val, *rowof, *fe;
+};
+__host__ __device__ inline PlanVX plan_vx(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf, bool oriented)
+{
+    PlanVX p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.off   = b.alloc<uint32_t>(nv + 1);
+    p.cur   = b.alloc<uint32_t>(nv);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.val   = b.alloc<uint16_t>(2 * ne);
+    p.rowof = b.alloc<uint16_t>(2 * ne);
+    if (oriented) {
+        p.fe  = b.alloc<uint16_t>(r4(3 * nf));
+        p.ef  = b.alloc<uint32_t>(2 * ne);
+        p.gid = b.alloc<uint32_t>(nv);
+    }
+    return p;
+}
+
+struct PlanVF
+{
+    uint32_t *ev, *off, *cur, *wave;
+    uint16_t *fe, *fvl, *val, *rowof;
+};
+__host__ __device__ inline PlanVF plan_vf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    PlanVF p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.off   = b.alloc<uint32_t>(nv + 1);
+    p.cur   = b.alloc<uint32_t>(nv);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.fvl   = b.alloc<uint16_t>(3 * nf);
+    p.val   = b.alloc<uint16_t>(3 * nf);
+    p.rowof = b.alloc<uint16_t>(3 * nf);
+    return p;
+}
+
+struct PlanEF
+{
+    uint32_t *off, *cur, *wave, *cnt;
+    uint16_t *fe, *val, *rowof;
+};
+__host__ __device__ inline PlanEF plan_ef(Bump& b, uint32_t ne, uint32_t nf, uint32_t nfo)
+{
+    PlanEF p{};
+    p.off   = b.alloc<uint32_t>(ne + 1);
+    p.cur   = b.alloc<uint32_t>(ne);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.cnt   = b.alloc<uint32_t>(nfo + 1);
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.val   = b.alloc<uint16_t>(3 * nf);
+    p.rowof = b.alloc<uint16_t>(3 * nf);
+    return p;
+}
+
+struct PlanFX
+{
+    uint32_t* ev;
+    uint16_t* fe;
+};
+__host__ __device__ inline PlanFX plan_fx(Bump& b, uint32_t ne, uint32_t nf)
+{
+    PlanFX p{};
+    p.ev = b.alloc<uint32_t>(r4(ne));
+    p.fe = b.alloc<uint16_t>(r4(3 * nf));
+    return p;
+}
+
+// ---- canonical CCW one-ring of an owned vertex --------------------------
+// The reference orders a vertex's edges by walking "next = third vertex of
+// the face holding half-edge v->r" (rxmesh_queries.cuh:376-499) from a start
+// that depends on atomic order. The start is fixed here: among the unvisited
+// edges prefer a boundary edge from which the walk can proceed, otherwise
+// take the neighbour with the smallest global id; after a dead end the same
+// rule restarts over the remaining edges. Reorders val[k0..k1) in place.
+__device__ inline void orient_row(uint32_t v, uint32_t k0, uint32_t k1, uint16_t* val,
+                                  const uint32_t* s_ev, const uint16_t* s_fe, const uint32_t* s_ef,
+                                  const uint32_t* s_gid)
+{
+    auto other = [&](uint32_t e) {
+        const uint32_t x = s_ev[e];
+        return (lo16(x) == v) ? hi16(x) : lo16(x);
+    };
+    auto out_face = [&](uint32_t e) -> uint32_t {
+        const uint32_t r = other(e);
+        for (uint32_t s = 0; s < 2; ++s) {
+            const uint32_t f = s_ef[2 * e + s];
+            if (f == 0xFFFFFFFFu) continue;
+            for (uint32_t i = 0; i < 3; ++i)
+                if (face_vertex(s_fe, s_ev, f, i) == v &&
+                    face_vertex(s_fe, s_ev, f, (i + 1) % 3) == r)
+                    return f;
+        }
+        return 0xFFFFFFFFu;
+    };
+    auto startable = [&](uint32_t e) {
+        return s_ef[2 * e + 1] == 0xFFFFFFFFu && out_face(e) != 0xFFFFFFFFu;
+    };
+    uint32_t pos = k0;
+    while (pos < k1) {
+        uint32_t best = pos;
+        bool     bb   = startable(val[pos]);
+        uint32_t bg   = s_gid[other(val[pos])];
+        for (uint32_t j = pos + 1; j < k1; ++j) {
+            const bool     b = startable(val[j]);
+            const uint32_t g = s_gid[other(val[j])];
+            if ((b && !bb) || (b == bb && g < bg)) {
+                best = j;
+                bb   = b;
+                bg   = g;
+            }
+        }
+        uint16_t t = val[pos];
+        val[pos]   = val[best];
+        val[best]  = t;
+        while (true) {
+            const uint32_t e = val[pos];
+            const uint32_t f = out_face(e);
+            if (f == 0xFFFFFFFFu) break;
+            const uint32_t r = other(e);
+            uint32_t       q = 0xFFFFFFFFu;
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t w = face_vertex(s_fe, s_ev, f, i);
+                if (w != v && w != r) q = w;
+            }
+            uint32_t j = pos + 1;
+            while (j < k1 && other(val[j]) != q) ++j;
+            if (j >= k1) break;
+            t            = val[pos + 1];
+            val[pos + 1] = val[j];
+            val[j]       = t;
+            ++pos;
+        }
+        ++pos;
+    }
+}
+
+// Shared prologue of VV / VE: stage EV, transpose to VE, optionally orient.
+__device__ inline void vx_prologue(const DevMesh& m, const PatchDesc& d, const PlanVX& L, bool oriented)
+{
+    const uint32_t T = blockDim.x;
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    if (oriented) {
+        stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+        for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
+        for (uint32_t i = threadIdx.x; i < 2 * d.ne; i += T) L.ef[i] = 0xFFFFFFFFu;
+    }
+    __syncthreads();
+    lds_transpose<2>(2 * d.ne, d.nv,
+                     [&](uint32_t i) {
+                         const uint32_t x = L.ev[i >> 1];
+                         return (i & 1u) ? hi16(x) : lo16(x);
+                     },
+                     L.off, L.cur, L.val, L.wave, d.nvo);
+    if (oriented) {
+        for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T) {
+            const uint32_t e = L.fe[i] >> 1, f = i / 3;
+            if (atomicCAS(&L.ef[2 * e], 0xFFFFFFFFu, f) != 0xFFFFFFFFu) L.ef[2 * e + 1] = f;
+        }
+        __syncthreads();
+        for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
+            orient_row(v, L.off[v], L.off[v + 1], L.val, L.ev, L.fe, L.ef, L.gid);
+        __syncthreads();
+    }
+}
+
+// VV (IS_VV) or VE
+template <bool IS_VV>
+__global__ __launch_bounds__(kQueryThreads) void k_query_vx(DevMesh m, const uint32_t* __restrict__ base,
+                                                            int32_t* __restrict__ row_ptr,
+                                                            int32_t* __restrict__ col, int oriented)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanVX    L = plan_vx(b, d.nv, d.ne, d.nf, oriented != 0);
+    vx_prologue(m, d, L, oriented != 0);
+    const uint32_t T = blockDim.x;
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
+        for (uint32_t k = L.off[v]; k < L.off[v + 1]; ++k) L.rowof[k] = static_cast<uint16_t>(v);
+    __syncthreads();
+    const uint32_t bse  = base[p];
+    const uint32_t nent = L.off[d.nvo];
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) row_ptr[d.vp + v] = static_cast<int32_t>(bse + L.off[v]);
+    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + nent);
+    for (uint32_t k = threadIdx.x; k < nent; k += T) {
+        const uint32_t e = L.val[k];
+        uint32_t       out;
+        if constexpr (IS_VV) {
+            const uint32_t x = L.ev[e], v = L.rowof[k];
+            const uint32_t w = (lo16(x) == v) ? hi16(x) : lo16(x);
+            out              = lin_of(w, d.nvo, d.vp, m.v_lin + d.v_off);
+        } else {
+            out = lin_of(e, d.neo, d.ep, m.e_lin + d.e_off);
+        }
+        col[bse + k] = static_cast<int32_t>(out);
+    }
+}
+
+// VF: FV (from FE + EV) transposed
+__global__ __launch_bounds__(kQueryThreads) void k_query_vf(DevMesh m, const uint32_t* __restrict__ base,
+                                                            int32_t* __restrict__ row_ptr,
+                                                            int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanVF    L = plan_vf(b, d.nv, d.ne, d.nf);
+    const uint32_t  T = blockDim.x;
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T)
+        L.fvl[i] = static_cast<uint16_t>(face_vertex(L.fe, L.ev, i / 3, i % 3));
+    __syncthreads();
+    lds_transpose<3>(3 * d.nf, d.nv, [&](uint32_t i) { return static_cast<uint32_t>(L.fvl[i]); }, L.off,
+                     L.cur, L.val, L.wave, d.nvo);
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
+        for (uint32_t k = L.off[v]; k < L.off[v + 1]; ++k) L.rowof[k] = static_cast<uint16_t>(v);
+    __syncthreads();
+    const uint32_t bse  = base[p];
+    const uint32_t nent = L.off[d.nvo];
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) row_ptr[d.vp + v] = static_cast<int32_t>(bse + L.off[v]);
+    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + nent);
+    for (uint32_t k = threadIdx.x; k < nent; k += T)
+        col[bse + k] = static_cast<int32_t>(lin_of(L.val[k], d.nfo, d.fp, m.f_lin + d.f_off));
+}
+
+// EF (IS_FF == false) or FF = EF o FE minus self (reference f_f,
+// rxmesh_queries.cuh:771-858: per face, edges in FE order, faces of each edge
+// in EF order)
+template <bool IS_FF>
+__global__ __launch_bounds__(kQueryThreads) void k_query_ef(DevMesh m, const uint32_t* __restrict__ base,
+                                                            int32_t* __restrict__ row_ptr,
+                                                            int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanEF    L = plan_ef(b, d.ne, d.nf, d.nfo);
+    const uint32_t  T = blockDim.x;
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    lds_transpose<3>(3 * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
+                     L.off, L.cur, L.val, L.wave, IS_FF ? d.ne : d.neo);
+    const uint32_t bse = base[p];
+    if constexpr (!IS_FF) {
+        for (uint32_t e = threadIdx.x; e < d.neo; e += T)
+            for (uint32_t k = L.off[e]; k < L.off[e + 1]; ++k) L.rowof[k] = static_cast<uint16_t>(e);
+        __syncthreads();
+        const uint32_t nent = L.off[d.neo];
+        for (uint32_t e = threadIdx.x; e < d.neo; e += T) row_ptr[d.ep + e] = static_cast<int32_t>(bse + L.off[e]);
+        if (threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(bse + nent);
+        for (uint32_t k = threadIdx.x; k < nent; k += T)
+            col[bse + k] = static_cast<int32_t>(lin_of(L.val[k], d.nfo, d.fp, m.f_lin + d.f_off));
+    } else {
+        for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
+            uint32_t c = 0;
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t e = L.fe[3 * f + i] >> 1;
+                c += L.off[e + 1] - L.off[e] - 1;
+            }
+            L.cnt[f] = c;
+        }
+        const uint32_t total = block_exclusive_scan(L.cnt, d.nfo, L.wave);
+        for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
+            uint32_t k = bse + L.cnt[f];
+            row_ptr[d.fp + f] = static_cast<int32_t>(k);
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t e = L.fe[3 * f + i] >> 1;
+                for (uint32_t j = L.off[e]; j < L.off[e + 1]; ++j) {
+                    const uint32_t g = L.val[j];
+                    if (g != f) col[k++] = static_cast<int32_t>(lin_of(g, d.nfo, d.fp, m.f_lin + d.f_off));
+                }
+            }
+        }
+        if (threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(bse + total);
+    }
+}
+
+// Fixed-width relations EV (2), FV (3), FE (3); OP as in the reference enum.
+template <int OP>
+__global__ __launch_bounds__(kQueryThreads) void k_query_fixed(DevMesh m, int32_t* __restrict__ row_ptr,
+                                                               int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanFX    L = plan_fx(b, d.ne, d.nf);
+    const uint32_t  T = blockDim.x;
+    if constexpr (OP == 9) {  // EV
+        for (uint32_t e = threadIdx.x; e < d.neo; e += T) {
+            const uint32_t x = m.ev[d.e_off + e], r = d.ep + e;
+            row_ptr[r]       = static_cast<int32_t>(2 * r);
+            col[2 * r]       = static_cast<int32_t>(lin_of(lo16(x), d.nvo, d.vp, m.v_lin + d.v_off));
+            col[2 * r + 1]   = static_cast<int32_t>(lin_of(hi16(x), d.nvo, d.vp, m.v_lin + d.v_off));
+        }
+        if (p + 1 == gridDim.x && threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(2 * (d.ep + d.neo));
+    } else {
+        if constexpr (OP == 6) stage_u32(L.ev, m.ev + d.e_off, d.ne);
+        stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+        __syncthreads();
+        for (uint32_t i = threadIdx.x; i < 3 * d.nfo; i += T) {
+            const uint32_t r = d.fp + i / 3;
+            uint32_t       out;
+            if constexpr (OP == 6) out = lin_of(face_vertex(L.fe, L.ev, i / 3, i % 3), d.nvo, d.vp, m.v_lin + d.v_off);
+            else out = lin_of(L.fe[i] >> 1, d.neo, d.ep, m.e_lin + d.e_off);
+            col[3 * d.fp + i] = static_cast<int32_t>(out);
+            if (i % 3 == 0) row_ptr[r] = static_cast<int32_t>(3 * r);
+        }
+        if (p + 1 == gridDim.x && threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(3 * (d.fp + d.nfo));
+    }
+}
+
+}  // namespace rxm

@@ -1,0 +1,365 @@
+"""Pins of the CPU oracle (oracle/oracle.cpp) against the reference's own
+data files, its tests' known answers, and closed forms. CPU only."""
+import math
+
+import numpy as np
+import pytest
+
+from conftest import fixture_path, obj
+from oracle.oracle import OracleMesh, read_patch_archive
+from rxmesh_amd import meshgen as mg
+
+ARCHIVES = ["sphere3", "torus"]
+
+
+def oracle_from_archive(name):
+    V, F = obj(name + ".obj")
+    a = read_patch_archive(fixture_path(name + "_patches"))
+    o = OracleMesh(F).set_patches(a["face_patch"], a["patches_val"], a["patches_offset"])
+    return V, F, a, o
+
+
+# ---- a1: edges / FF / statistics -------------------------------------------
+@pytest.mark.parametrize("name", ARCHIVES)
+def test_edge_count_matches_reference_archive(name):
+    # the archives store the reference's num_edges (patcher.h:162-182)
+    V, F, a, o = oracle_from_archive(name)
+    c = o.counts()
+    assert (c["nV"], c["nE"], c["nF"]) == (a["num_vertices"], a["num_edges"], a["num_faces"])
+
+
+@pytest.mark.parametrize("name,chi", [("sphere3.obj", 2), ("torus.obj", 0), ("plane.obj", 1),
+                                      ("cube.obj", 2), ("sphere1.obj", 2), ("plane_5.obj", 1)])
+def test_euler_characteristic(name, chi):
+    V, F = obj(name)
+    c = OracleMesh(F).counts()
+    assert c["nV"] - c["nE"] + c["nF"] == chi
+
+
+@pytest.mark.parametrize("name", ["plane.obj", "cube.obj", "sphere1.obj", "bunnyhead.obj"])
+def test_edge_ids_first_appearance_bruteforce(name):
+    V, F = obj(name)
+    o = OracleMesh(F)
+    seen = {}
+    ev = []
+    for f in F:
+        for i in range(3):
+            a, b = int(f[i]), int(f[(i + 1) % 3])
+            k = (max(a, b), min(a, b))
+            if k not in seen:
+                seen[k] = len(ev)
+                ev.append((a, b))
+    assert np.array_equal(o.ev(), np.array(ev, np.uint32))
+    # FF: faces sharing an edge
+    off, val = o.ff()
+    ef = {}
+    for fi, f in enumerate(F):
+        for i in range(3):
+            a, b = int(f[i]), int(f[(i + 1) % 3])
+            ef.setdefault((max(a, b), min(a, b)), []).append(fi)
+    for fi, f in enumerate(F):
+        expect = []
+        for i in range(3):
+            a, b = int(f[i]), int(f[(i + 1) % 3])
+            expect += [g for g in ef[(max(a, b), min(a, b))] if g != fi]
+        assert sorted(val[off[fi]:off[fi + 1]].tolist()) == sorted(expect)
+
+
+def test_stats_known_answers():
+    V, F = obj("plane.obj")
+    s = OracleMesh(F).stats()
+    assert not s["closed"] and s["manifold"] and s["max_valence"] == 8
+    V, F = obj("cube.obj")
+    s = OracleMesh(F).stats()
+    assert s["closed"] and s["manifold"]
+    V, F = obj("bunnyhead.obj")
+    s = OracleMesh(F).stats()
+    assert not s["closed"]
+
+
+def test_boundary_vertex_count_bunnyhead():
+    # reference tests/RXMesh_test/test_boundary.cu:27 expects 98
+    V, F = obj("bunnyhead.obj")
+    o = OracleMesh(F).set_patches(np.zeros(len(F), np.uint32))
+    rp, col = o.global_relation("EF")
+    ev = o.ev()
+    bnd = set()
+    for e in range(len(ev)):
+        if rp[e + 1] - rp[e] == 1:
+            bnd.update(ev[e].tolist())
+    assert len(bnd) == 98
+
+
+# ---- a3: ribbons and ownership: golden vectors from the reference archives --
+@pytest.mark.parametrize("name", ARCHIVES)
+def test_ownership_matches_reference_archive(name):
+    V, F, a, o = oracle_from_archive(name)
+    vp, ep = o.ownership()
+    assert np.array_equal(vp, a["vertex_patch"])
+    assert np.array_equal(ep, a["edge_patch"])
+
+
+@pytest.mark.parametrize("name", ARCHIVES)
+def test_ribbons_match_reference_archive(name):
+    # Offsets match exactly; each patch's ribbon matches as a set. (The
+    # archives list ribbon faces in an older vertex-major discovery order;
+    # local numbering sorts ribbon faces, so nothing downstream depends on it.)
+    V, F, a, o = oracle_from_archive(name)
+    P = a["num_patches"]
+    rv, ro = o.ribbon()
+    assert np.array_equal(ro, a["ribbon_ext_offset"][:P])
+    s = 0
+    for p in range(P):
+        e = ro[p]
+        assert sorted(rv[s:e].tolist()) == sorted(a["ribbon_ext_val"][s:e].tolist())
+        s = e
+
+
+# ---- a4/a5: ltog / topology invariants (run_ltog_mapping_test) ------------
+@pytest.mark.parametrize("name", ARCHIVES)
+def test_ltog_mapping_invariants(name):
+    # restates tests/RXMesh_test/rxmesh_test.h:443-625: local EV / FE decoded
+    # through ltog equal the global edge and face definitions
+    V, F, a, o = oracle_from_archive(name)
+    ev = o.ev()
+    vp, ep = o.ownership()
+    for p in range(a["num_patches"]):
+        nv, ne, nf, nvo, neo, nfo = o.patch_counts(p)
+        lv, le, lf = o.patch_ltog(p, 0), o.patch_ltog(p, 1), o.patch_ltog(p, 2)
+        for lt, n_own, owner in ((lv, nvo, vp), (le, neo, ep), (lf, nfo, a["face_patch"])):
+            assert np.all(owner[lt[:n_own]] == p) and np.all(owner[lt[n_own:]] != p)
+            assert np.all(np.diff(lt[:n_own].astype(np.int64)) > 0)
+            assert np.all(np.diff(lt[n_own:].astype(np.int64)) > 0)
+        pev = o.patch_ev(p).reshape(-1, 2)
+        for l in range(ne):
+            g0, g1 = ev[le[l]]
+            assert (lv[pev[l, 0]], lv[pev[l, 1]]) == (max(g0, g1), min(g0, g1))
+        pfe = o.patch_fe(p).reshape(-1, 3)
+        for l in range(nf):
+            f = F[lf[l]]
+            for i in range(3):
+                x = int(pfe[l, i])
+                e, d = x >> 1, x & 1
+                a_, b_ = lv[pev[e, d]], lv[pev[e, 1 - d]]
+                assert (a_, b_) == (f[i], f[(i + 1) % 3])
+
+
+# ---- a8/a9: queries vs brute force (reference verifier semantics) ---------
+def _to_global_rows(o, op, rp, col):
+    src = o.linear_to_global({"V": 0, "E": 1, "F": 2}[op[0]])
+    dst = o.linear_to_global({"V": 0, "E": 1, "F": 2}[op[1]])
+    rows = {}
+    for l in range(len(rp) - 1):
+        rows[int(src[l])] = [int(dst[c]) for c in col[rp[l]:rp[l + 1]]]
+    return rows
+
+
+@pytest.mark.parametrize("name", ["sphere3", "torus"])
+@pytest.mark.parametrize("op", ["VV", "VE", "VF", "FV", "FE", "FF", "EV", "EF"])
+def test_queries_vs_bruteforce(name, op):
+    V, F, a, o = oracle_from_archive(name)
+    rp, col = o.query(op)
+    rows = _to_global_rows(o, op, rp, col)
+    grp, gcol = o.global_relation(op)
+    assert len(rows) == len(grp) - 1
+    for s, got in rows.items():
+        exp = gcol[grp[s]:grp[s + 1]].tolist()
+        if op in ("FV", "FE", "EV"):
+            assert got == exp  # deterministic in the reference too
+        else:
+            assert sorted(got) == exp
+
+
+def test_fv_fe_ev_exact_sequences():
+    V, F, a, o = oracle_from_archive("sphere3")
+    ev = o.ev()
+    for op in ("FV", "FE", "EV"):
+        rows = _to_global_rows(o, op, *o.query(op))
+        for s, got in rows.items():
+            if op == "FV":
+                assert got == F[s].tolist()
+            elif op == "EV":
+                assert got == [max(ev[s]), min(ev[s])]
+
+
+# ---- a10: oriented VV --------------------------------------------------------
+def _angle(p, a, b):
+    u, w = p - a, p - b
+    return math.degrees(math.acos(np.dot(u, w) / (np.linalg.norm(u) * np.linalg.norm(w))))
+
+
+def test_oriented_vv_plane_known_answer():
+    # tests/RXMesh_test/test_queries_oriented.cu:14-129
+    V, F = obj("plane.obj")
+    o = OracleMesh(F).set_patches(np.zeros(len(F), np.uint32))
+    rows = _to_global_rows(o, "VV", *o.query("VV", oriented=True))
+    for v, ring in rows.items():
+        if v == 4:
+            continue
+        s = sum(_angle(V[v], V[ring[i]], V[ring[i + 1]]) for i in range(2) if i + 1 < len(ring))
+        assert abs(s - 90) < 1e-4 or abs(s - 180) < 1e-4
+
+
+def test_oriented_vv_cube_known_answer():
+    # tests/RXMesh_test/test_queries_oriented.cu:131-225
+    V, F = obj("cube.obj")
+    o = OracleMesh(F).set_patches(np.zeros(len(F), np.uint32))
+    rows = _to_global_rows(o, "VV", *o.query("VV", oriented=True))
+    maxval = o.stats()["max_valence"]
+    for v, ring in rows.items():
+        for i in range(len(ring)):
+            j = (i + 1) % maxval
+            if j < len(ring):
+                t = _angle(V[v], V[ring[i]], V[ring[j]])
+                assert abs(t - 90) < 1e-4 or abs(t - 45) < 1e-4
+
+
+@pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "bunnyhead"])
+def test_oriented_ring_is_ccw_fan(name):
+    # every consecutive pair (r_i, r_{i+1}) must be a CCW face (v, r_i, r_{i+1});
+    # closed rings wrap; interior rings start at the smallest global id;
+    # boundary rings start at the boundary edge with an outgoing face.
+    V, F = obj(name + ".obj")
+    o = OracleMesh(F).set_patches(np.zeros(len(F), np.uint32)) if name in ("plane", "bunnyhead") \
+        else oracle_from_archive(name)[3]
+    half = set()
+    for f in F:
+        for i in range(3):
+            half.add((int(f[i]), int(f[(i + 1) % 3]), int(f[(i + 2) % 3])))
+    rows = _to_global_rows(o, "VV", *o.query("VV", oriented=True))
+    grp, gcol = o.global_relation("VV")
+    for v, ring in rows.items():
+        k = len(ring)
+        assert sorted(ring) == gcol[grp[v]:grp[v + 1]].tolist()
+        fan = [(v, ring[i], ring[(i + 1) % k]) in half for i in range(k)]
+        if all(fan):
+            assert ring[0] == min(ring)  # interior: canonical start
+        else:
+            # a boundary vertex of a manifold disk-like neighbourhood: a
+            # single open fan whose only gap is the wrap-around
+            assert all(fan[:-1]) and not fan[-1]
+
+
+# ---- a14: normals ----------------------------------------------------------
+def test_normals_single_triangle_closed_form():
+    F = np.array([[0, 1, 2]], np.uint32)
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+    o = OracleMesh(F)
+    n = o.vertex_normals(V, "max")
+    # |e01|^2 = 1, |e12|^2 = 2, |e20|^2 = 1; corner weights 1/2, 1/3, 1/3
+    assert np.allclose(n, [[0, 0, 1 / 2], [0, 0, 1 / 3], [0, 0, 1 / 3]], atol=0, rtol=1e-15)
+    assert np.allclose(o.vertex_normals(V, "area"), [[0, 0, 0.5]] * 3, atol=0, rtol=1e-15)
+
+
+def test_normals_flat_grid_point_up():
+    pos, fv = mg.grid(12, seed=3, wave=False)
+    n = OracleMesh(fv).vertex_normals(pos, "max")
+    assert np.allclose(n[:, :2], 0, atol=1e-15) and np.all(n[:, 2] > 0)
+
+
+# ---- a15: smoothing ----------------------------------------------------------
+def test_smoothing_invariants():
+    pos, fv = mg.grid(16, jitter=0.0, wave=False)
+    o = OracleMesh(fv)
+    const = np.tile([[0.3, -1.0, 2.0]], (len(pos), 1))
+    assert np.allclose(o.smooth_uniform(const, 0.5, 3), const, atol=1e-15)
+    lin = pos @ np.array([[1.0, 2.0, 0.5], [-3.0, 0.25, 1.0], [0, 0, 0]])
+    out = o.smooth_uniform(lin, 0.5, 1)
+    n = 16
+    interior = [(j * n + i) for j in range(1, n - 1) for i in range(1, n - 1)]
+    assert np.allclose(out[interior], lin[interior], atol=1e-12)
+    # one hand step on the corner vertex 0 (neighbours of vertex 0 on this grid)
+    rp, col = o.global_relation("VV")
+    nb = col[rp[0]:rp[1]]
+    exp = lin[0] + 0.5 * (lin[nb].mean(0) - lin[0])
+    assert np.allclose(out[0], exp, atol=1e-14)
+
+
+# ---- a16-a18: cotangent Laplacian + explicit MCF ---------------------------
+def test_cot_weights_flat_grid_closed_form():
+    # The reference's "cot" is dot / area = 2 cot (geometry_util.cuh:185-196),
+    # and its partial Voronoi area carries the same factor 2. On unjittered
+    # right-isosceles triangles (cot 45 = 1, cot 90 = 0) an interior axis edge
+    # of an 8-neighbour vertex weighs 2 (1 + 1) = 4, a diagonal edge 0, and
+    # the mass 2 * sum(partial) = 2 * (2 h^2) = 4 h^2.
+    n = 6
+    pos, fv = mg.grid(n, jitter=0.0, wave=False)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    rows, cols, vals, mass = o.cot_assemble(pos, 1.0)
+    A = {}
+    for r, c, v in zip(rows, cols, vals):
+        A[(int(r), int(c))] = v
+    v = 2 * n + 2  # (i, j) = (2, 2): (i + j) even -> 8 neighbours
+    assert abs(A[(v, v + 1)] + 4.0) < 1e-12 and abs(A[(v, v + n + 1)]) < 1e-12
+    h2 = (1.0 / (n - 1)) ** 2
+    assert abs(mass[v] - 4 * h2) < 1e-12
+    assert abs(A[(v, v)] - (mass[v] + 4 * 4.0)) < 1e-12
+
+
+@pytest.mark.parametrize("level", [2, 3])
+def test_cot_system_invariants_sphere(level):
+    pos, fv = mg.icosphere(level, seed=1, sigma=0.01)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    dt = 1e-3
+    rows, cols, vals, mass = o.cot_assemble(pos, dt)
+    nV = len(pos)
+    A = np.zeros((nV, nV))
+    np.add.at(A, (rows, cols), vals)
+    assert np.allclose(A, A.T, atol=1e-14)           # symmetric pattern and weights
+    assert np.allclose(A.sum(1), mass, atol=1e-12)   # rows of L sum to zero
+    off = A - np.diag(np.diag(A))
+    assert np.all(off <= 0)
+    assert abs(mass.sum() / 4 - 4 * math.pi) < 0.05 * 4 * math.pi  # 4 x area ~ sphere
+    x1 = o.mcf_explicit(rows, cols, vals, mass, pos, 1)
+    L = (A - np.diag(mass)) / dt
+    assert np.allclose(x1, pos - dt * (L @ pos) / mass[:, None], atol=1e-12)
+
+
+# ---- a19: symmetric Dirichlet -------------------------------------------------
+def _disk_problem(n=24, seed=0, sigma=0.02):
+    pos, fv = mg.disk(n, seed=seed)
+    uv = mg.uv_init(pos, fv, seed=seed, sigma=sigma)
+    return pos, fv, uv
+
+
+def test_symdir_identity_known_answer():
+    pos, fv, _ = _disk_problem()
+    o = OracleMesh(fv)
+    flat = pos.copy()
+    flat[:, 2] = 0.0
+    Mr = o.rest_shapes(flat)
+    e = o.symdir_energy(Mr, flat[:, :2])
+    area = 0.5 * (Mr[:, 0] * Mr[:, 3] - Mr[:, 2] * Mr[:, 1])
+    assert np.allclose(e, 4.0 * area, rtol=1e-12)      # J = I: |J|^2 + |J^-1|^2 = 4
+    assert np.allclose(o.symdir_grad(Mr, flat[:, :2]), 0.0, atol=1e-10)
+    # J = diag(2, 1): 5 + 1.25
+    uv2 = flat[:, :2] * np.array([2.0, 1.0])
+    assert np.allclose(o.symdir_energy(Mr, uv2), 6.25 * area, rtol=1e-12)
+
+
+def test_symdir_gradient_closed_form_vs_dual_vs_fd():
+    pos, fv, uv = _disk_problem(n=14, sigma=0.01)
+    o = OracleMesh(fv)
+    Mr = o.rest_shapes(pos)
+    g_closed = o.symdir_grad(Mr, uv, "closed")
+    g_dual = o.symdir_grad(Mr, uv, "dual")
+    assert np.allclose(g_closed, g_dual, rtol=1e-10, atol=1e-10)
+    h = 1e-6
+    rng = np.random.default_rng(0)
+    for v in rng.choice(len(uv), 12, replace=False):
+        for c in range(2):
+            up, dn = uv.copy(), uv.copy()
+            up[v, c] += h
+            dn[v, c] -= h
+            fd = (o.symdir_energy(Mr, up).sum() - o.symdir_energy(Mr, dn).sum()) / (2 * h)
+            assert abs(fd - g_closed[v, c]) <= 1e-5 * max(1.0, abs(g_closed[v, c]))
+
+
+def test_symdir_flipped_face_is_flt_max():
+    F = np.array([[0, 1, 2]], np.uint32)
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], np.float64)
+    o = OracleMesh(F)
+    Mr = o.rest_shapes(V)
+    uv = V[:, :2][[0, 2, 1]]
+    assert o.symdir_energy(Mr, uv)[0] == np.float32(np.finfo(np.float32).max)
+    assert np.all(o.symdir_grad(Mr, uv) == 0)
