@@ -1,0 +1,177 @@
+"""Benchmark of the patch query engine on one MI355X.
+
+Default workload (one "step"): on the icosphere subdivided 9 times
+(2,621,442 V / 5,242,880 F, radial noise N(0, 0.005), vertex and face ids
+randomly permuted, patch size 512):
+  VV query materialised as CSR + VF query materialised as CSR +
+  10 Jacobi iterations of uniform Laplacian smoothing (lambda 0.5, fp32).
+value = VV/VF entries delivered per second (Gentries/s), where a smoothing
+iteration consumes the 2E VV entries of its one-rings.
+
+Prints ONE JSON line. --gpus N > 1 is rejected (single-GPU build).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--workload", default="c2s_perm")
+    ap.add_argument("--patch-size", type=int, default=512)
+    ap.add_argument("--smooth-iters", type=int, default=10)
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--profile", default=None,
+                    help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
+                         "smooth_uniform, vertex_normals, copy")
+    ap.add_argument("--reps", type=int, default=20)
+    return ap.parse_args()
+
+
+def event_time_ms(fn, reps):
+    """Average duration of fn() over reps launches, HIP events on the
+    current stream (the stream every rxmesh_amd launch uses here)."""
+    import torch
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def cpu_baseline(seconds):
+    """The oracle (CPU restatement of the reference path, one core) on a
+    bounded sample: VV + VF query + 10 smoothing iterations on the icosphere
+    subdivided 6 times, repeated for about `seconds`."""
+    import numpy as np
+    from oracle.oracle import OracleMesh
+    from rxmesh_amd import meshgen as mg
+    import rxmesh_amd as rx
+    pos, fv = mg.config_mesh("c1")
+    m = rx.Mesh(fv, host_only=True)
+    o = OracleMesh(fv).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    E, F = m.nE, m.nF
+    entries = 2 * E + 3 * F + 10 * 2 * E
+    n, t0 = 0, time.perf_counter()
+    while True:
+        o.query("VV")
+        o.query("VF")
+        o.smooth_uniform(pos, 0.5, 10)
+        n += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": round(entries * n / el / 1e9, 6), "unit": "Gentries/s", "cores": 1, "kind": "port",
+            "sample": "oracle VV+VF query + 10 smoothing iters on icosphere L6 (40,962 V / 81,920 F), "
+                      "%d reps in %.1f s" % (n, el)}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus %d is not supported" % args.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import rxmesh_amd as rx
+    from rxmesh_amd import meshgen as mg
+
+    torch.cuda.set_device(0)
+    pos, fv = mg.config_mesh(args.workload)
+    m = rx.Mesh(fv, patch_size=args.patch_size)
+    V, E, F, P = m.nV, m.nE, m.nF, m.P
+    l2g = m.array("v_l2g")
+    x = torch.from_numpy(np.ascontiguousarray(pos[l2g].T.astype(np.float32))).cuda()
+    out = torch.empty_like(x)
+    tmp = torch.empty_like(x)
+    nnz_vv, nnz_vf = m.query_nnz("VV"), m.query_nnz("VF")
+    vv_rp = torch.empty(V + 1, dtype=torch.int32, device="cuda")
+    vv_col = torch.empty(nnz_vv, dtype=torch.int32, device="cuda")
+    vf_rp = torch.empty(V + 1, dtype=torch.int32, device="cuda")
+    vf_col = torch.empty(nnz_vf, dtype=torch.int32, device="cuda")
+    it = args.smooth_iters
+
+    if args.profile:
+        fns = {"query_vv": lambda: m.query("VV", row_ptr=vv_rp, col=vv_col),
+               "query_vf": lambda: m.query("VF", row_ptr=vf_rp, col=vf_col),
+               "smooth_uniform": lambda: m.smooth(x, 0.5, 1, out=out),
+               "vertex_normals": lambda: m.vertex_normals(x, "max", out=out),
+               "copy": lambda: out.copy_(x)}  # plain copy of 12 V bytes (counter calibration)
+        fn = fns[args.profile]
+        for _ in range(args.reps):
+            fn()
+        torch.cuda.synchronize()
+        print(json.dumps({"profile": args.profile, "reps": args.reps, "V": V, "E": E, "F": F,
+                          "copy_bytes": 12 * V}), flush=True)
+        return
+
+    def step():
+        m.query("VV", row_ptr=vv_rp, col=vv_col)
+        m.query("VF", row_ptr=vf_rp, col=vf_col)
+        m.smooth(x, 0.5, it, out=out, tmp=tmp)
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    entries = nnz_vv + nnz_vf + it * nnz_vv
+    value = entries / (ms * 1e-3) / 1e9
+
+    # per-kernel live timing and algorithmic bytes per launch
+    reps = args.kernel_reps
+    kern = {}
+    t_vv = event_time_ms(lambda: m.query("VV", row_ptr=vv_rp, col=vv_col), reps)
+    kern["query_vv"] = (t_vv, 4 * E + 4 * nnz_vv + 4 * (V + 1))
+    t_vf = event_time_ms(lambda: m.query("VF", row_ptr=vf_rp, col=vf_col), reps)
+    kern["query_vf"] = (t_vf, 4 * E + 6 * F + 4 * nnz_vf + 4 * (V + 1))
+    t_sm = event_time_ms(lambda: m.smooth(x, 0.5, 1, out=out), reps)
+    kern["smooth_uniform"] = (t_sm, 24 * V + 4 * E)
+    t_vn = event_time_ms(lambda: m.vertex_normals(x, "max", out=out), reps)
+    kern["vertex_normals"] = (t_vn, 24 * V + 4 * E + 6 * F)
+    kernels = {k: {"us": round(t * 1e3, 2), "bytes": b, "GB_s": round(b / (t * 1e-3) / 1e9, 1),
+                   "frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k, (t, b) in kern.items()}
+    # dominant kernel of the step: smoothing (it launches per step)
+    share = {"query_vv": t_vv, "query_vf": t_vf, "smooth_uniform": it * t_sm}
+    dom = max(share, key=share.get)
+    dt, db = kern[dom]
+    ach = db / (dt * 1e-3) / 1e9
+    line = {
+        "metric": "VV-query Gentries/s + %HBM peak; batched grad meshes/s vs plain launches",
+        "value": round(value, 4), "unit": "Gentries/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32+u16/i32", "data": "synthetic",
+        "config": {"workload": "VV+VF query CSR + %d uniform smoothing iters, icosphere L9 permuted "
+                               "(%d V / %d F), patch size %d" % (it, V, F, args.patch_size),
+                   "mesh": args.workload, "num_patches": P, "entries_per_step": entries},
+        "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+        "kernels": kernels,
+    }
+    if not args.no_cpu:
+        line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -108,6 +108,11 @@ rxm_status rxm_mesh_info(const rxm_mesh* mesh, rxm_mesh_info_t* info);
 rxm_status rxm_mesh_array_size(const rxm_mesh* mesh, int which, uint64_t* count);
 /* Copy an exportable array into host memory of `bytes` bytes. */
 rxm_status rxm_mesh_get_array(const rxm_mesh* mesh, int which, void* dst, uint64_t bytes);
+/* Copy the device view of the mesh (struct rxm::DevMesh of
+ * rxmesh_amd/csrc/device_mesh.hpp: raw device pointers + counts, the role of
+ * the reference's Context, include/rxmesh/context.h:15-441) into `out`; used
+ * by the header-only C++ API to build kernel arguments. */
+rxm_status rxm_mesh_device_view(const rxm_mesh* mesh, void* out, uint64_t bytes);
 /* Write the patch layout as a reference-compatible patch archive
  * (Patcher::save, include/rxmesh/patcher/patcher.h:154-159). */
 rxm_status rxm_mesh_save_patches(const rxm_mesh* mesh, const char* path);
@@ -136,6 +141,44 @@ rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float*
  * and the uniform branch of apps/MCF/mcf_kernels.cuh:114-213. */
 rxm_status rxm_laplacian_smooth(rxm_mesh* mesh, const float* pos, float* out, float* tmp,
                                 float lambda, int iters, void* stream);
+
+/* Entries of the cotangent system (V + 2E on closed manifolds). */
+rxm_status rxm_cot_nnz(rxm_mesh* mesh, int64_t* nnz);
+/* Assemble A = M + dt L over the oriented VV ring as CSR over linear vertex
+ * ids (row = [diagonal, CCW ring neighbours]) and the lumped mass M.
+ * Replaces SparseMatrix<T>(rx, Op::VV) (include/rxmesh/matrix/sparse_matrix.h:
+ * 133-242) filled by mcf_A_setup (apps/MCF/mcf_kernels.cuh:354-426); weights
+ * restate edge_cotan_weight / partial_voronoi_area
+ * (include/rxmesh/geometry_util.cuh:107-206). Needs an edge-manifold mesh. */
+rxm_status rxm_cot_csr_assemble(rxm_mesh* mesh, const float* pos, float dt, int32_t* row_ptr,
+                                int32_t* col, float* val, float* mass, void* stream);
+/* y = A x for 3 SoA columns over the V vertex rows. Replaces
+ * SparseMatrix::multiply (sparse_matrix.h:1317-1351, cuSPARSE SpMV). */
+rxm_status rxm_spmv3(rxm_mesh* mesh, const int32_t* row_ptr, const int32_t* col, const float* val,
+                     const float* x, float* y, void* stream);
+/* `iters` explicit mean-curvature-flow steps x <- x + (M x - A x) / M with the
+ * SpMV fused into the update; `tmp` is a 3V workspace when iters > 1. */
+rxm_status rxm_mcf_explicit(rxm_mesh* mesh, const int32_t* row_ptr, const int32_t* col,
+                            const float* val, const float* mass, const float* x, float* out,
+                            float* tmp, int iters, void* stream);
+
+/* Symmetric Dirichlet parameterisation energy (apps/Param/param.cu:113-184).
+ * rxm_symdir_rest: per-face 2x2 rest shapes Mr (SoA over linear faces, 4F
+ * floats: m00, m10, m01, m11) from 3D rest positions (param.cu:113-141). */
+rxm_status rxm_symdir_rest(rxm_mesh* mesh, const float* rest_pos, float* Mr, void* stream);
+/* Per-face energies (FLT_MAX for flipped faces) into energy[F] and their sum
+ * into total[1] (fixed-order, deterministic). Replaces the passive
+ * evaluation diff_scalar_kernel_passive (diff/diff_query_kernel.cuh:156-196)
+ * plus ReduceHandle (reduce_handle.h:23-210). */
+rxm_status rxm_symdir_energy(rxm_mesh* mesh, const float* Mr, const float* uv, float* energy,
+                             float* total, void* stream);
+/* grad (2 x V SoA) = scale[0] * d(sum_f E_f)/d uv (scale may be NULL = 1):
+ * hand-written adjoint recomputing each face's Jacobian, gathered per vertex
+ * over VF without atomics. Replaces the forward-mode AD gradient of
+ * DiffScalarProblem::eval_terms_grad_only (diff/diff_scalar_problem.h:327-335,
+ * diff/diff_query_kernel.cuh:199-330). */
+rxm_status rxm_symdir_grad(rxm_mesh* mesh, const float* Mr, const float* uv, const float* scale,
+                           float* grad, void* stream);
 
 /* Permute a vertex/edge/face attribute (kind 0/1/2) of ncomp SoA components
  * between global and linear order on device (reference: attribute copy by

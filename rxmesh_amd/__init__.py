@@ -56,6 +56,13 @@ _SIGS = {
     "rxm_vertex_normals": [_vp, _i, _vp, _vp, _vp],
     "rxm_laplacian_smooth": [_vp, _vp, _vp, _vp, _f, _i, _vp],
     "rxm_to_linear": [_vp, _i, _i, _vp, _vp, _vp],
+    "rxm_cot_nnz": [_vp, ctypes.POINTER(ctypes.c_int64)],
+    "rxm_cot_csr_assemble": [_vp, _vp, _f, _vp, _vp, _vp, _vp, _vp],
+    "rxm_spmv3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
+    "rxm_mcf_explicit": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
+    "rxm_symdir_rest": [_vp, _vp, _vp, _vp],
+    "rxm_symdir_energy": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "rxm_symdir_grad": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_to_global": [_vp, _i, _i, _vp, _vp, _vp],
 }
 
@@ -190,6 +197,70 @@ class Mesh:
                                           float(lam), int(iters), _stream(stream)), "rxm_laplacian_smooth")
         return out
 
+    # -- cotangent system / explicit MCF ------------------------------
+    def cot_nnz(self):
+        n = ctypes.c_int64()
+        _check(lib().rxm_cot_nnz(self._h, ctypes.byref(n)), "rxm_cot_nnz")
+        return n.value
+
+    def cot_assemble(self, pos, dt, stream=None):
+        """(row_ptr, col, val, mass) of A = M + dt L on device."""
+        import torch
+        nnz = self.cot_nnz()
+        rp = torch.empty(self.nV + 1, dtype=torch.int32, device="cuda")
+        col = torch.empty(nnz, dtype=torch.int32, device="cuda")
+        val = torch.empty(nnz, dtype=torch.float32, device="cuda")
+        mass = torch.empty(self.nV, dtype=torch.float32, device="cuda")
+        _check(lib().rxm_cot_csr_assemble(self._h, _ptr(pos), float(dt), _ptr(rp), _ptr(col), _ptr(val),
+                                          _ptr(mass), _stream(stream)), "rxm_cot_csr_assemble")
+        return rp, col, val, mass
+
+    def spmv3(self, A, x, out=None, stream=None):
+        import torch
+        rp, col, val = A[:3]
+        if out is None:
+            out = torch.empty_like(x)
+        _check(lib().rxm_spmv3(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(x), _ptr(out), _stream(stream)),
+               "rxm_spmv3")
+        return out
+
+    def mcf_explicit(self, A, x, iters, out=None, tmp=None, stream=None):
+        import torch
+        rp, col, val, mass = A
+        if out is None:
+            out = torch.empty_like(x)
+        if tmp is None and iters > 1:
+            tmp = torch.empty_like(x)
+        _check(lib().rxm_mcf_explicit(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(mass), _ptr(x), _ptr(out),
+                                      _ptr(tmp) if tmp is not None else None, int(iters), _stream(stream)),
+               "rxm_mcf_explicit")
+        return out
+
+    # -- symmetric Dirichlet --------------------------------------------
+    def symdir_rest(self, rest_pos, stream=None):
+        import torch
+        Mr = torch.empty(4, self.nF, dtype=torch.float32, device="cuda")
+        _check(lib().rxm_symdir_rest(self._h, _ptr(rest_pos), _ptr(Mr), _stream(stream)), "rxm_symdir_rest")
+        return Mr
+
+    def symdir_energy(self, Mr, uv, energy=None, total=None, stream=None):
+        import torch
+        if energy is None:
+            energy = torch.empty(self.nF, dtype=torch.float32, device="cuda")
+        if total is None:
+            total = torch.empty((), dtype=torch.float32, device="cuda")
+        _check(lib().rxm_symdir_energy(self._h, _ptr(Mr), _ptr(uv), _ptr(energy), _ptr(total), _stream(stream)),
+               "rxm_symdir_energy")
+        return total, energy
+
+    def symdir_grad(self, Mr, uv, scale=None, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(uv)
+        _check(lib().rxm_symdir_grad(self._h, _ptr(Mr), _ptr(uv), _ptr(scale) if scale is not None else None,
+                                     _ptr(out), _stream(stream)), "rxm_symdir_grad")
+        return out
+
     def to_linear(self, x, kind=0, out=None, stream=None):
         """x: (ncomp, n) float32 device tensor in global order -> linear order."""
         import torch
@@ -206,3 +277,41 @@ class Mesh:
         ncomp = x.shape[0] if x.dim() == 2 else 1
         _check(lib().rxm_to_global(self._h, kind, ncomp, _ptr(x), _ptr(out), _stream(stream)), "rxm_to_global")
         return out
+
+
+class SymmetricDirichlet:
+    """torch.autograd.Function wrapper: energy = sum_f E_f(uv) over a mesh.
+
+    forward runs rxm_symdir_energy; backward runs the hand-written adjoint
+    rxm_symdir_grad (per-face Jacobians are recomputed, not stored).
+    uv: (2, V) float32 SoA over linear vertex ids."""
+
+    _fn = None
+
+    @classmethod
+    def apply(cls, mesh, Mr, uv):
+        if cls._fn is None:
+            import torch
+
+            class _SymDir(torch.autograd.Function):
+                @staticmethod
+                def forward(ctx, uv_, mesh_, Mr_):
+                    total, _ = mesh_.symdir_energy(Mr_, uv_.detach())
+                    ctx.mesh = mesh_
+                    ctx.Mr = Mr_
+                    ctx.save_for_backward(uv_)
+                    return total
+
+                @staticmethod
+                def backward(ctx, g):
+                    (uv_,) = ctx.saved_tensors
+                    g = g.detach().to(torch.float32).contiguous()
+                    return ctx.mesh.symdir_grad(ctx.Mr, uv_.detach(), scale=g), None, None
+
+            cls._fn = _SymDir
+        return cls._fn.apply(uv, mesh, Mr)
+
+
+def symmetric_dirichlet(mesh, Mr, uv):
+    """Differentiable sum of symmetric Dirichlet energies (see SymmetricDirichlet)."""
+    return SymmetricDirichlet.apply(mesh, Mr, uv)

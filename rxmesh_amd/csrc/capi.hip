@@ -11,6 +11,7 @@
 #include "../../include/rxmesh_amd.h"
 #include "host_build.h"
 #include "k_geometry.hpp"
+#include "k_numerics.hpp"
 #include "k_query.hpp"
 
 using namespace rxm;
@@ -25,9 +26,12 @@ struct rxm_mesh
     uint64_t              device_bytes = 0;
     std::map<int, uint32_t*>            d_base;  // per-op patch entry bases (P+1)
     std::map<int, std::vector<uint32_t>> h_base;
+    float*                              d_partial = nullptr;  // P per-patch partial sums
 };
 
 namespace {
+
+constexpr int kOpCot = 100;  // rows of the cotangent system: VV + diagonal
 
 #define RXM_HIP(call)                                 \
     do {                                              \
@@ -103,7 +107,7 @@ const std::vector<uint32_t>& op_base(rxm_mesh* m, int op)
     const HostMesh& h = m->h;
     std::vector<uint32_t> b(h.P + 1, 0);
     std::vector<uint32_t> vval(h.nV, 0);
-    if (op == RXM_OP_VV || op == RXM_OP_VE)
+    if (op == RXM_OP_VV || op == RXM_OP_VE || op == kOpCot)
         for (uint32_t e = 0; e < h.nE; ++e) {
             vval[h.ev[2ull * e]]++;
             vval[h.ev[2ull * e + 1]]++;
@@ -137,6 +141,9 @@ const std::vector<uint32_t>& op_base(rxm_mesh* m, int op)
                 }
                 break;
             case RXM_OP_EV: c = 2ull * h.neo[p]; break;
+            case kOpCot:
+                for (uint32_t l = 0; l < h.nvo[p]; ++l) c += vval[h.ltog_v[h.v_off[p] + l]] + 1;
+                break;
             default: c = 3ull * h.nfo[p]; break;
         }
         b[p + 1] = b[p] + static_cast<uint32_t>(c);
@@ -337,6 +344,14 @@ rxm_status rxm_mesh_get_array(const rxm_mesh* m, int which, void* dst, uint64_t 
     return RXM_OK;
 }
 
+rxm_status rxm_mesh_device_view(const rxm_mesh* m, void* out, uint64_t bytes)
+{
+    if (!m || !out || bytes < sizeof(DevMesh)) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    std::memcpy(out, &m->dm, sizeof(DevMesh));
+    return RXM_OK;
+}
+
 rxm_status rxm_mesh_save_patches(const rxm_mesh* m, const char* path)
 {
     if (!m || !path) return RXM_ERR_INVALID_ARG;
@@ -446,6 +461,104 @@ rxm_status rxm_to_linear(rxm_mesh* m, int kind, int ncomp, const float* g, float
 rxm_status rxm_to_global(rxm_mesh* m, int kind, int ncomp, const float* l, float* g, void* stream)
 {
     return permute(m, kind, ncomp, l, g, false, stream);
+}
+
+rxm_status rxm_cot_nnz(rxm_mesh* m, int64_t* nnz)
+{
+    if (!m || !nnz) return RXM_ERR_INVALID_ARG;
+    *nnz = op_base(m, kOpCot).back();
+    return RXM_OK;
+}
+
+rxm_status rxm_cot_csr_assemble(rxm_mesh* m, const float* pos, float dt, int32_t* row_ptr, int32_t* col,
+                                float* val, float* mass, void* stream)
+{
+    if (!m || !pos || !row_ptr || !col || !val || !mass) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    if (!m->h.manifold) return RXM_ERR_NON_MANIFOLD;
+    const uint32_t* base = nullptr;
+    int rc = device_base(m, kOpCot, &base);
+    if (rc) return rc;
+    Bump b(nullptr);
+    plan_cot(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
+    return launch_patches(k_cot_assemble, kQueryThreads, b.bytes(), stream, m, m->dm, base, pos, dt, row_ptr,
+                          col, val, mass);
+}
+
+static constexpr uint32_t kSpmvLanes = 8;
+
+rxm_status rxm_spmv3(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* val, const float* x,
+                     float* y, void* stream)
+{
+    if (!m || !row_ptr || !col || !val || !x || !y || x == y) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const uint32_t n = m->h.nV;
+    const uint64_t threads = uint64_t(n) * kSpmvLanes;
+    hipLaunchKernelGGL((k_spmv3<kSpmvLanes, false>), dim3(static_cast<uint32_t>((threads + 255) / 256)), dim3(256),
+                       0, static_cast<hipStream_t>(stream), n, row_ptr, col, val, static_cast<const float*>(nullptr),
+                       x, y);
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+rxm_status rxm_mcf_explicit(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* val,
+                            const float* mass, const float* x, float* out, float* tmp, int iters, void* stream)
+{
+    if (!m || !row_ptr || !col || !val || !mass || !x || !out || iters < 1 || x == out) return RXM_ERR_INVALID_ARG;
+    if (iters > 1 && (!tmp || tmp == out || tmp == x)) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const uint32_t n = m->h.nV;
+    const uint64_t threads = uint64_t(n) * kSpmvLanes;
+    const float* src = x;
+    for (int it = 0; it < iters; ++it) {
+        float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
+        hipLaunchKernelGGL((k_spmv3<kSpmvLanes, true>), dim3(static_cast<uint32_t>((threads + 255) / 256)),
+                           dim3(256), 0, static_cast<hipStream_t>(stream), n, row_ptr, col, val, mass, src, dst);
+        RXM_HIP(hipGetLastError());
+        src = dst;
+    }
+    return RXM_OK;
+}
+
+rxm_status rxm_symdir_rest(rxm_mesh* m, const float* rest_pos, float* Mr, void* stream)
+{
+    if (!m || !rest_pos || !Mr) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    Bump b(nullptr);
+    plan_fv(b, m->h.max_ne, m->h.max_nf);
+    return launch_patches(k_symdir_rest, kGeomThreads, b.bytes(), stream, m, m->dm, rest_pos, Mr);
+}
+
+rxm_status rxm_symdir_energy(rxm_mesh* m, const float* Mr, const float* uv, float* energy, float* total,
+                             void* stream)
+{
+    if (!m || !Mr || !uv || !energy || !total) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    if (!m->d_partial) {
+        if (hipMalloc(&m->d_partial, sizeof(float) * m->h.P) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(m->d_partial);
+        m->device_bytes += sizeof(float) * m->h.P;
+    }
+    Bump b(nullptr);
+    plan_fv(b, m->h.max_ne, m->h.max_nf);
+    b.alloc<float>(kGeomThreads);
+    int rc = launch_patches(k_symdir_energy, kGeomThreads, b.bytes(), stream, m, m->dm, Mr, uv, energy,
+                            m->d_partial);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), m->h.P,
+                       static_cast<const float*>(m->d_partial), total);
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+rxm_status rxm_symdir_grad(rxm_mesh* m, const float* Mr, const float* uv, const float* scale, float* grad,
+                           void* stream)
+{
+    if (!m || !Mr || !uv || !grad || uv == grad) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    Bump b(nullptr);
+    plan_sg(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
+    return launch_patches(k_symdir_grad, kGeomThreads, b.bytes(), stream, m, m->dm, Mr, uv, scale, grad);
 }
 
 }  // extern "C"

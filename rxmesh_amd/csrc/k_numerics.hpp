@@ -1,0 +1,403 @@
+// Numerics on top of the query engine: cotangent CSR assembly over the
+// oriented VV ring, SpMV-3 with the fused explicit mean-curvature-flow update,
+// and the symmetric Dirichlet energy with its hand-written adjoint.
+#pragma once
+#include <cfloat>
+
+#include "k_geometry.hpp"
+
+namespace rxm {
+
+struct f3
+{
+    float x, y, z;
+};
+__device__ __forceinline__ f3 sub(f3 a, f3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__device__ __forceinline__ float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ f3 cross(f3 a, f3 b)
+{
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+// reference: include/rxmesh/geometry_util.cuh:50-58 (tri_area)
+__device__ __forceinline__ float tri_area(f3 p0, f3 p1, f3 p2)
+{
+    const f3 c = cross(sub(p1, p0), sub(p2, p0));
+    return 0.5f * sqrtf(dot(c, c));
+}
+// reference: geometry_util.cuh:107-114 (clamp_cot, +-19.1)
+__device__ __forceinline__ float clamp_cot(float v) { return (v < -19.1f) ? -19.1f : ((v > 19.1f) ? 19.1f : v); }
+
+// reference: geometry_util.cuh:120-172 (partial_voronoi_area of p in CCW p->q->r)
+__device__ __forceinline__ float partial_voronoi_area(f3 p, f3 q, f3 r)
+{
+    const f3    pq = sub(q, p), qr = sub(r, q), pr = sub(r, p);
+    const float area = tri_area(p, q, r);
+    if (area <= FLT_MIN) return -1.f;
+    const float dotp = dot(pq, pr), dotq = -dot(qr, pq), dotr = dot(qr, pr);
+    if (dotp < 0.f) return 0.25f * area;
+    if (dotq < 0.f || dotr < 0.f) return 0.125f * area;
+    const float cotq = clamp_cot(dotq / area), cotr = clamp_cot(dotr / area);
+    return 0.125f * (dot(pr, pr) * cotq + dot(pq, pq) * cotr);
+}
+
+// reference: geometry_util.cuh:178-206 (edge_cotan_weight of edge p-r with
+// opposite vertices q and s; "cot" = dot / area)
+__device__ __forceinline__ float edge_cotan_weight(f3 p, f3 r, f3 q, f3 s)
+{
+    auto part = [&](f3 v) {
+        const f3    d0 = sub(p, v), d1 = sub(r, v);
+        const float area = tri_area(p, r, v);
+        return (area > FLT_MIN) ? clamp_cot(dot(d0, d1) / area) : 0.f;
+    };
+    return part(q) + part(s);
+}
+
+// ---- cotangent CSR assembly ------------------------------------------------
+struct PlanCOT
+{
+    PlanVX vx;
+    float *x, *y, *z;
+};
+__host__ __device__ inline PlanCOT plan_cot(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    PlanCOT p{};
+    p.vx = plan_vx(b, nv, ne, nf, true);
+    p.x  = b.alloc<float>(nv);
+    p.y  = b.alloc<float>(nv);
+    p.z  = b.alloc<float>(nv);
+    return p;
+}
+
+// Rows of A = M + dt L over linear ids, restating the lambda of
+// apps/MCF/mcf_kernels.cuh:354-426 on the canonical CCW ring: the diagonal
+// first, then the ring neighbours; mass[v] = 1 / v_weight. `base` holds each
+// patch's first entry (VV base + vertex prefix).
+__global__ __launch_bounds__(kQueryThreads) void k_cot_assemble(DevMesh m, const uint32_t* __restrict__ base,
+                                                                const float* __restrict__ pos, float dt,
+                                                                int32_t* __restrict__ row_ptr,
+                                                                int32_t* __restrict__ col, float* __restrict__ val,
+                                                                float* __restrict__ mass)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanCOT   L = plan_cot(b, d.nv, d.ne, d.nf);
+    stage_positions(L.x, L.y, L.z, pos, m.nV, d, m.v_lin);
+    vx_prologue(m, d, L.vx, true);
+    const uint32_t bse = base[p];
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += blockDim.x) {
+        const uint32_t k0 = L.vx.off[v], k1 = L.vx.off[v + 1], K = k1 - k0;
+        const uint32_t rs = bse + k0 + v;
+        auto nb = [&](uint32_t i) {
+            const uint32_t x = L.vx.ev[L.vx.val[k0 + i]];
+            return (lo16(x) == v) ? hi16(x) : lo16(x);
+        };
+        auto P3 = [&](uint32_t l) { return f3{L.x[l], L.y[l], L.z[l]}; };
+        const f3 pp = P3(v);
+        float    sum_w = 0.f, vw = 0.f;
+        uint32_t q = nb(K - 1);
+        for (uint32_t i = 0; i < K; ++i) {
+            const uint32_t r = nb(i), s = (i == K - 1) ? nb(0) : nb(i + 1);
+            float          w = edge_cotan_weight(pp, P3(r), P3(q), P3(s));
+            w = (w >= 0.f) ? w : 0.f;
+            w *= dt;
+            sum_w += w;
+            col[rs + 1 + i] = static_cast<int32_t>(lin_of(r, d.nvo, d.vp, m.v_lin + d.v_off));
+            val[rs + 1 + i] = -w;
+            const float a = partial_voronoi_area(pp, P3(q), P3(r));
+            vw += (a > 0.f) ? a : 0.f;
+            q = r;
+        }
+        vw = 0.5f / vw;
+        const float ms = 1.0f / vw;
+        row_ptr[d.vp + v] = static_cast<int32_t>(rs);
+        col[rs] = static_cast<int32_t>(d.vp + v);
+        val[rs] = ms + sum_w;
+        mass[d.vp + v] = ms;
+    }
+    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.vx.off[d.nvo] + d.nvo);
+}
+
+// ---- SpMV over 3 SoA columns, optionally fused with the explicit MCF step --
+// LANES consecutive lanes share a row (rows hold ~7 entries), so the col /
+// val loads of a wave are contiguous; partial sums meet through a fixed
+// xor-shuffle tree (deterministic).
+//   FUSED == false: y = A x
+//   FUSED == true : y = x + (M x - A x) / M
+template <uint32_t LANES, bool FUSED>
+__global__ __launch_bounds__(256) void k_spmv3(uint32_t n, const int32_t* __restrict__ row_ptr,
+                                               const int32_t* __restrict__ col, const float* __restrict__ val,
+                                               const float* __restrict__ mass, const float* __restrict__ x,
+                                               float* __restrict__ y)
+{
+    const uint32_t gid  = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t row  = gid / LANES;
+    const uint32_t lane = gid % LANES;
+    float          ax = 0.f, ay = 0.f, az = 0.f;
+    const bool     ok = row < n;
+    if (ok) {
+        const int32_t s = row_ptr[row], e = row_ptr[row + 1];
+        for (int32_t k = s + static_cast<int32_t>(lane); k < e; k += LANES) {
+            const uint32_t c = static_cast<uint32_t>(col[k]);
+            const float    w = val[k];
+            ax += w * x[c];
+            ay += w * x[n + c];
+            az += w * x[2 * n + c];
+        }
+    }
+#pragma unroll
+    for (uint32_t o = LANES / 2; o > 0; o >>= 1) {
+        ax += __shfl_xor(ax, o, 64);
+        ay += __shfl_xor(ay, o, 64);
+        az += __shfl_xor(az, o, 64);
+    }
+    if (ok && lane == 0) {
+        if constexpr (FUSED) {
+            const float M = mass[row];
+            const float px = x[row], py = x[n + row], pz = x[2 * n + row];
+            y[row]         = px + (M * px - ax) / M;
+            y[n + row]     = py + (M * py - ay) / M;
+            y[2 * n + row] = pz + (M * pz - az) / M;
+        } else {
+            y[row]         = ax;
+            y[n + row]     = ay;
+            y[2 * n + row] = az;
+        }
+    }
+}
+
+// ---- symmetric Dirichlet ----------------------------------------------------
+// Rest shape of a face: apps/Param/param.cu:113-141. Mr is stored SoA over
+// linear face ids: Mr[c * F + f], c = (m00, m10, m01, m11), columns b-a, c-a
+// in the face's local 2D frame.
+struct PlanFV
+{
+    uint32_t* ev;
+    uint16_t* fe;
+};
+__host__ __device__ inline PlanFV plan_fv(Bump& b, uint32_t ne, uint32_t nf)
+{
+    PlanFV p{};
+    p.ev = b.alloc<uint32_t>(r4(ne));
+    p.fe = b.alloc<uint16_t>(r4(3 * nf));
+    return p;
+}
+
+__global__ __launch_bounds__(kGeomThreads) void k_symdir_rest(DevMesh m, const float* __restrict__ pos,
+                                                              float* __restrict__ Mr)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanFV    L = plan_fv(b, d.ne, d.nf);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    const uint32_t N = m.nV, F = m.nF;
+    for (uint32_t f = threadIdx.x; f < d.nfo; f += blockDim.x) {
+        f3 P[3];
+        for (uint32_t i = 0; i < 3; ++i) {
+            const uint32_t g = lin_of(face_vertex(L.fe, L.ev, f, i), d.nvo, d.vp, m.v_lin + d.v_off);
+            P[i] = f3{pos[g], pos[N + g], pos[2 * N + g]};
+        }
+        const f3    e1 = sub(P[1], P[0]), e2 = sub(P[2], P[0]);
+        const f3    n  = cross(e1, e2);
+        const float l1 = sqrtf(dot(e1, e1));
+        const f3    b1 = {e1.x / l1, e1.y / l1, e1.z / l1};
+        const f3    t  = cross(n, b1);
+        const float lt = sqrtf(dot(t, t));
+        const f3    b2 = {t.x / lt, t.y / lt, t.z / lt};
+        const uint32_t lf = d.fp + f;
+        Mr[lf]         = dot(e1, b1);
+        Mr[F + lf]     = 0.f;
+        Mr[2 * F + lf] = dot(e2, b1);
+        Mr[3 * F + lf] = dot(e2, b2);
+    }
+}
+
+// Energy and (optionally) corner gradients of one face. uv corners a, b, c;
+// R = Mr (m00, m10, m01, m11). Returns FLT_MAX (and zero gradient) for a
+// face whose UV triangle is not positively oriented (param.cu:165-168).
+// E = A (|J|^2 + |J^-1|^2), J = M R^-1, A = det(R) / 2; for 2x2 J,
+// |J^-1|^2 = |J|^2 / det(J)^2. Gradient (closed form):
+// dE/dJ = A (2 (1 + 1/d^2) J - 2 |J|^2 / d^3 cof(J)), dE/dM = dE/dJ R^-T.
+// a*d - b*c with one rounding error at most a few ulps (Kahan's FMA
+// algorithm), so near-degenerate faces keep their relative accuracy
+__device__ __forceinline__ float det2(float a, float d, float b, float c)
+{
+    const float w = b * c;
+    const float e = fmaf(-b, c, w);
+    const float f = fmaf(a, d, -w);
+    return f + e;
+}
+
+template <bool GRAD>
+__device__ __forceinline__ float symdir_face(const float* a, const float* b, const float* c, const float* R,
+                                             float* g /* 6: da, db, dc */)
+{
+    const float M00 = b[0] - a[0], M10 = b[1] - a[1], M01 = c[0] - a[0], M11 = c[1] - a[1];
+    const float dm  = det2(M00, M11, M01, M10);
+    if (dm <= 0.f) {
+        if (GRAD)
+            for (int i = 0; i < 6; ++i) g[i] = 0.f;
+        return FLT_MAX;
+    }
+    const float dr = det2(R[0], R[3], R[2], R[1]);
+    const float A  = 0.5f * dr;
+    const float i00 = R[3] / dr, i01 = -R[2] / dr, i10 = -R[1] / dr, i11 = R[0] / dr;
+    const float J00 = M00 * i00 + M01 * i10, J01 = M00 * i01 + M01 * i11;
+    const float J10 = M10 * i00 + M11 * i10, J11 = M10 * i01 + M11 * i11;
+    const float n2 = J00 * J00 + J01 * J01 + J10 * J10 + J11 * J11;
+    const float dj = dm / dr;  // det J = det M / det R
+    const float id2 = 1.0f / (dj * dj);
+    if (GRAD) {
+        const float s1 = 2.0f * (1.0f + id2), s2 = -2.0f * n2 * id2 / dj;
+        const float G00 = A * (s1 * J00 + s2 * J11), G01 = A * (s1 * J01 - s2 * J10);
+        const float G10 = A * (s1 * J10 - s2 * J01), G11 = A * (s1 * J11 + s2 * J00);
+        const float D00 = G00 * i00 + G01 * i01, D01 = G00 * i10 + G01 * i11;
+        const float D10 = G10 * i00 + G11 * i01, D11 = G10 * i10 + G11 * i11;
+        g[2] = D00;
+        g[3] = D10;
+        g[4] = D01;
+        g[5] = D11;
+        g[0] = -(D00 + D01);
+        g[1] = -(D10 + D11);
+    }
+    return A * (n2 + n2 * id2);
+}
+
+// Per owned face energy into energy[linear face]; each workgroup also writes
+// its patch's sum (fixed-order tree) into partial[p].
+__global__ __launch_bounds__(kGeomThreads) void k_symdir_energy(DevMesh m, const float* __restrict__ Mr,
+                                                                const float* __restrict__ uv,
+                                                                float* __restrict__ energy,
+                                                                float* __restrict__ partial)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanFV    L  = plan_fv(b, d.ne, d.nf);
+    float*          sr = b.alloc<float>(kGeomThreads);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    const uint32_t N = m.nV, F = m.nF;
+    float          acc = 0.f;
+    for (uint32_t f = threadIdx.x; f < d.nfo; f += blockDim.x) {
+        float U[3][2];
+        for (uint32_t i = 0; i < 3; ++i) {
+            const uint32_t g = lin_of(face_vertex(L.fe, L.ev, f, i), d.nvo, d.vp, m.v_lin + d.v_off);
+            U[i][0] = uv[g];
+            U[i][1] = uv[N + g];
+        }
+        const uint32_t lf   = d.fp + f;
+        const float    R[4] = {Mr[lf], Mr[F + lf], Mr[2 * F + lf], Mr[3 * F + lf]};
+        const float    e    = symdir_face<false>(U[0], U[1], U[2], R, nullptr);
+        energy[lf]          = e;
+        acc += e;
+    }
+    sr[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t s = kGeomThreads / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) sr[threadIdx.x] += sr[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) partial[p] = sr[0];
+}
+
+// Fixed-order sum of n floats by one workgroup of 1024 threads.
+__global__ __launch_bounds__(1024) void k_reduce_sum(uint32_t n, const float* __restrict__ in, float* __restrict__ out)
+{
+    __shared__ float s[1024];
+    float            acc = 0.f;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) acc += in[i];
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = 512; k > 0; k >>= 1) {
+        if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = s[0];
+}
+
+struct PlanSG
+{
+    uint32_t *ev, *off, *cur, *wave;
+    uint16_t *fe, *fvl, *val;
+    float *   u, *v, *gc;
+};
+__host__ __device__ inline PlanSG plan_sg(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    PlanSG p{};
+    p.ev   = b.alloc<uint32_t>(r4(ne));
+    p.off  = b.alloc<uint32_t>(nv + 1);
+    p.cur  = b.alloc<uint32_t>(nv);
+    p.wave = b.alloc<uint32_t>(20);
+    p.fe   = b.alloc<uint16_t>(r4(3 * nf));
+    p.fvl  = b.alloc<uint16_t>(3 * nf);
+    p.val  = b.alloc<uint16_t>(3 * nf);
+    p.u    = b.alloc<float>(nv);
+    p.v    = b.alloc<float>(nv);
+    p.gc   = b.alloc<float>(6 * nf);
+    return p;
+}
+
+// Adjoint of sum_f E_f w.r.t. UV: every local face (owned and ribbon) of the
+// patch recomputes its Jacobian and corner gradients into LDS; each owned
+// vertex then sums the corners of its VF row in ascending local-face order.
+// No global atomics (the reference scatters with atomicAdd,
+// include/rxmesh/diff/diff_query_kernel.cuh:275-282). grad = scale * dE/duv,
+// scale read from device memory (1 if null).
+__global__ __launch_bounds__(kGeomThreads) void k_symdir_grad(DevMesh m, const float* __restrict__ Mr,
+                                                              const float* __restrict__ uv,
+                                                              const float* __restrict__ scale,
+                                                              float* __restrict__ grad)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanSG    L = plan_sg(b, d.nv, d.ne, d.nf);
+    const uint32_t  T = blockDim.x, N = m.nV, F = m.nF;
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    for (uint32_t l = threadIdx.x; l < d.nv; l += T) {
+        const uint32_t g = lin_of(l, d.nvo, d.vp, m.v_lin + d.v_off);
+        L.u[l]           = uv[g];
+        L.v[l]           = uv[N + g];
+    }
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
+        float U[3][2];
+        for (uint32_t i = 0; i < 3; ++i) {
+            const uint32_t l = face_vertex(L.fe, L.ev, f, i);
+            L.fvl[3 * f + i] = static_cast<uint16_t>(l);
+            U[i][0]          = L.u[l];
+            U[i][1]          = L.v[l];
+        }
+        const uint32_t lf   = lin_of(f, d.nfo, d.fp, m.f_lin + d.f_off);
+        const float    R[4] = {Mr[lf], Mr[F + lf], Mr[2 * F + lf], Mr[3 * F + lf]};
+        symdir_face<true>(U[0], U[1], U[2], R, L.gc + 6 * f);
+    }
+    __syncthreads();
+    lds_transpose<3>(3 * d.nf, d.nv, [&](uint32_t i) { return static_cast<uint32_t>(L.fvl[i]); }, L.off, L.cur,
+                     L.val, L.wave, d.nvo);
+    const float sc = scale ? scale[0] : 1.0f;
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        float gu = 0.f, gv = 0.f;
+        for (uint32_t k = L.off[v]; k < L.off[v + 1]; ++k) {
+            const uint32_t f = L.val[k];
+            const uint32_t c = (L.fvl[3 * f] == v) ? 0u : ((L.fvl[3 * f + 1] == v) ? 1u : 2u);
+            gu += L.gc[6 * f + 2 * c];
+            gv += L.gc[6 * f + 2 * c + 1];
+        }
+        grad[d.vp + v]     = sc * gu;
+        grad[N + d.vp + v] = sc * gv;
+    }
+}
+
+}  // namespace rxm

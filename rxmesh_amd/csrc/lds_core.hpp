@@ -133,6 +133,53 @@ __device__ __forceinline__ void lds_transpose(uint32_t n, uint32_t ncols, KeyF k
     __syncthreads();
 }
 
+// Deterministic LDS transpose without fill atomics or row sorts.
+//   n entries, entry i belongs to source row i / W and targets column key(i).
+//   Wave w owns the contiguous entry range [w*span, (w+1)*span); per-wave
+//   column counts (LDS atomics, order-free) are scanned column-major, so wave
+//   w's slots of column c follow those of waves < w. Each wave then walks its
+//   range in 64-entry chunks in order; equal keys inside a chunk are ranked by
+//   lane through a ballot match over the key bits. Result: every column lists
+//   its source rows in ascending order, the order of a serial fill.
+//   cnt: nw * ncols u32 scratch (nw = blockDim / 64); key_bits >= bits(ncols-1).
+template <uint32_t W, typename KeyF>
+__device__ __forceinline__ void lds_transpose_ordered(uint32_t n, uint32_t ncols, uint32_t key_bits, KeyF key,
+                                                      uint32_t* off, uint32_t* cnt, uint16_t* val,
+                                                      uint32_t* s_wave)
+{
+    const uint32_t T = blockDim.x, nw = T >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
+    const uint32_t span = ((n + nw * 64u - 1u) / (nw * 64u)) * 64u;
+    const uint32_t b0 = min(n, w * span), b1 = min(n, b0 + span);
+    for (uint32_t c = threadIdx.x; c < ncols * nw; c += T) cnt[c] = 0;
+    __syncthreads();
+    for (uint32_t i = b0 + lane; i < b1; i += 64u) atomicAdd(&cnt[key(i) * nw + w], 1u);
+    const uint32_t total = block_exclusive_scan(cnt, ncols * nw, s_wave);
+    for (uint32_t c = threadIdx.x; c < ncols; c += T) off[c] = cnt[c * nw];
+    if (threadIdx.x == 0) off[ncols] = total;
+    __syncthreads();  // off[] must be read before the fill advances cnt[]
+    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
+    for (uint32_t base = b0; base < b1; base += 64u) {
+        const uint32_t i  = base + lane;
+        const bool     ok = i < b1;
+        const uint32_t k  = ok ? key(i) : 0u;
+        uint64_t       mk = __ballot(ok);
+        for (uint32_t bit = 0; bit < key_bits; ++bit) {
+            const bool     on = (k >> bit) & 1u;
+            const uint64_t bb = __ballot(on);
+            mk &= on ? bb : ~bb;
+        }
+        const uint32_t rank = __popcll(mk & lt), cntg = __popcll(mk);
+        if (ok) {
+            const uint32_t sb = cnt[k * nw + w];
+            val[sb + rank]    = static_cast<uint16_t>(i / W);
+            if (rank + 1u == cntg) cnt[k * nw + w] = sb + cntg;
+        }
+    }
+    __syncthreads();
+}
+
+__device__ __forceinline__ uint32_t bits_for(uint32_t n) { return (n <= 1u) ? 1u : 32u - __clz(n - 1u); }
+
 // Local vertex i (0..2) of local face f from LDS FE / EV (reference f_v,
 // rxmesh_queries.cuh:650-688: the direction bit selects the edge endpoint).
 __device__ __forceinline__ uint32_t face_vertex(const uint16_t* s_fe, const uint32_t* s_ev, uint32_t f,

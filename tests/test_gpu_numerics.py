@@ -1,0 +1,195 @@
+"""GPU parity of the numerics on top of the query engine: cotangent CSR
+assembly, SpMV-3 / explicit MCF, symmetric Dirichlet energy and its
+hand-written adjoint (also through torch.autograd)."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleMesh
+from rxmesh_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _mesh(name):
+    import rxmesh_amd as rx
+    if name == "ico4":
+        V, F = mg.icosphere(4, seed=1, sigma=0.01)
+        ps = 256
+    elif name == "ico5_perm":
+        V, F = mg.permute_mesh(*mg.icosphere(5, seed=2, sigma=0.005), seed=2)
+        ps = 512
+    elif name == "torus":
+        V, F = mg.torus(96, 48)
+        ps = 512
+    elif name == "disk":
+        V, F = mg.disk(64, seed=0)
+        ps = 256
+    elif name == "c3":
+        V, F = mg.config_mesh("c3")
+        ps = 512
+    m = rx.Mesh(F, patch_size=ps)
+    o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    return V, F, m, o
+
+
+def _lin(m, X, kind="v_l2g"):
+    import torch
+    l2g = m.array(kind)
+    return torch.from_numpy(np.ascontiguousarray(X[l2g].T.astype(np.float32))).cuda()
+
+
+def _glob(m, Y, kind="v_l2g"):
+    l2g = m.array(kind)
+    out = np.empty_like(Y)
+    out[..., l2g] = Y
+    return out
+
+
+@pytest.mark.parametrize("name", ["ico4", "ico5_perm", "torus", "disk"])
+def test_cot_assembly_vs_fp64(name):
+    import torch
+    V, F, m, o = _mesh(name)
+    V32 = V.astype(np.float32)
+    dt = 1e-3
+    rp, col, val, mass = m.cot_assemble(_lin(m, V32), dt)
+    torch.cuda.synchronize()
+    rp, col, val, mass = (t.cpu().numpy() for t in (rp, col, val, mass))
+    orows, ocols, ovals, omass = o.cot_assemble(V32.astype(np.float64), dt)
+    l2g = m.array("v_l2g")
+    # pattern: row = [diag, CCW ring] in the same order as the oracle
+    assert rp[-1] == len(ocols) == m.cot_nnz()
+    g2l = m.array("v_g2l")
+    # compare entry by entry in row order (oracle rows are in linear order too)
+    order = np.argsort(g2l[orows], kind="stable")
+    assert np.array_equal(col, g2l[ocols[order]])
+    ref = ovals[order]
+    scale = np.abs(ref).max()
+    assert np.abs(val - ref).max() <= 1e-5 * scale
+    gm = _glob(m, mass)
+    assert np.abs(gm - omass).max() <= 1e-5 * np.abs(omass).max()
+    # row sums of L vanish: A 1 = M (up to fp32 rounding)
+    rs = np.add.reduceat(val.astype(np.float64), rp[:-1])
+    assert np.abs(rs - mass).max() <= 1e-5 * np.abs(val).max()
+
+
+@pytest.mark.parametrize("name", ["ico4", "torus"])
+def test_spmv_and_mcf_vs_fp64(name):
+    import torch
+    V, F, m, o = _mesh(name)
+    V32 = V.astype(np.float32)
+    dt = 1e-5
+    x = _lin(m, V32)
+    A = m.cot_assemble(x, dt)
+    # all-ones SpMV equals the mass (SparseMatrixEdgeLen-style closed form)
+    ones = torch.ones_like(x)
+    y = m.spmv3(A, ones)
+    torch.cuda.synchronize()
+    assert torch.allclose(y[0], A[3], rtol=1e-5, atol=1e-6 * float(A[3].abs().max()))
+    # 50 explicit MCF steps vs FP64 on the oracle's system
+    out = m.mcf_explicit(A, x, 50)
+    torch.cuda.synchronize()
+    got = _glob(m, out.cpu().numpy()).T
+    orows, ocols, ovals, omass = o.cot_assemble(V32.astype(np.float64), dt)
+    ref = o.mcf_explicit(orows, ocols, ovals, omass, V32.astype(np.float64), 50)
+    err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert err.max() <= 1e-5, err.max()
+    # determinism
+    out2 = m.mcf_explicit(A, x, 50)
+    assert torch.equal(out, out2)
+
+
+@pytest.mark.parametrize("name", ["disk", "c3"])
+def test_symdir_energy_and_grad_vs_fp64(name):
+    import torch
+    V, F, m, o = _mesh(name)
+    V32 = V.astype(np.float32)
+    uv = mg.uv_init(V, F, seed=0).astype(np.float32)
+    Mr = m.symdir_rest(_lin(m, V32))
+    uvl = _lin(m, uv)
+    total, energy = m.symdir_energy(Mr, uvl)
+    grad = m.symdir_grad(Mr, uvl)
+    torch.cuda.synchronize()
+    # rest shapes vs FP64
+    oMr = o.rest_shapes(V32.astype(np.float64))
+    gMr = _glob(m, Mr.cpu().numpy(), "f_l2g").T
+    assert np.abs(gMr - oMr).max() <= 1e-5 * np.abs(oMr).max()
+    # energies use the oracle's FP64 evaluation on the GPU's fp32 rest shapes
+    # (isolates the energy kernel from the rest-shape rounding)
+    oE = o.symdir_energy(gMr.astype(np.float64), uv.astype(np.float64))
+    gE = _glob(m, energy.cpu().numpy(), "f_l2g")
+    assert np.all(np.isfinite(oE))
+    rel = np.abs(gE - oE) / oE
+    assert rel.max() <= 1e-5, rel.max()
+    assert abs(float(total) - oE.sum()) <= 1e-5 * oE.sum()
+    # gradient vs FP64 closed form and the forward-mode dual restatement
+    g64 = o.symdir_grad(gMr.astype(np.float64), uv.astype(np.float64), "closed")
+    d64 = o.symdir_grad(gMr.astype(np.float64), uv.astype(np.float64), "dual")
+    gg = _glob(m, grad.cpu().numpy()).T
+    gmax = np.abs(g64).max()
+    # per-result tolerance: max |g - g64| <= 1e-5 * max |g64|
+    assert np.abs(gg - g64).max() <= 1e-5 * gmax
+    assert np.abs(gg - d64).max() <= 1e-5 * gmax
+
+
+def test_symdir_grad_vs_central_differences():
+    import torch
+    V, F, m, o = _mesh("disk")
+    V32 = V.astype(np.float32)
+    uv = mg.uv_init(V, F, seed=0).astype(np.float32)
+    Mr = m.symdir_rest(_lin(m, V32))
+    grad = m.symdir_grad(Mr, _lin(m, uv))
+    torch.cuda.synchronize()
+    gMr = _glob(m, Mr.cpu().numpy(), "f_l2g").T.astype(np.float64)
+    gg = _glob(m, grad.cpu().numpy()).T
+    uv64 = uv.astype(np.float64)
+    h = 1e-6
+    gmax = np.abs(gg).max()
+    for v in np.random.default_rng(1).choice(len(uv), 20, replace=False):
+        for c in range(2):
+            up, dn = uv64.copy(), uv64.copy()
+            up[v, c] += h
+            dn[v, c] -= h
+            fd = (o.symdir_energy(gMr, up).sum() - o.symdir_energy(gMr, dn).sum()) / (2 * h)
+            assert abs(fd - gg[v, c]) <= 1e-5 * gmax + 1e-6 * abs(fd)
+
+
+def test_symdir_autograd_function():
+    import torch
+    import rxmesh_amd as rx
+    V, F, m, o = _mesh("disk")
+    V32 = V.astype(np.float32)
+    uv = mg.uv_init(V, F, seed=0).astype(np.float32)
+    Mr = m.symdir_rest(_lin(m, V32))
+    uvl = _lin(m, uv).requires_grad_(True)
+    e = rx.symmetric_dirichlet(m, Mr, uvl)
+    (3.0 * e).backward()
+    direct = m.symdir_grad(Mr, uvl.detach())
+    torch.cuda.synchronize()
+    assert torch.allclose(uvl.grad, 3.0 * direct, rtol=1e-6, atol=0)
+    # a backtracking gradient step decreases the energy (the energy is a
+    # barrier: too long a step flips a face and returns FLT_MAX)
+    x = uvl.detach().clone()
+    e0 = float(m.symdir_energy(Mr, x)[0])
+    g = m.symdir_grad(Mr, x)
+    step, e1 = 1e-3, float("inf")
+    while step > 1e-12:
+        e1 = float(m.symdir_energy(Mr, x - step * g)[0])
+        if e1 < e0:
+            break
+        step *= 0.5
+    assert e1 < e0
+
+
+def test_symdir_flipped_face():
+    import torch
+    import rxmesh_amd as rx
+    F = np.array([[0, 1, 2], [1, 3, 2]], np.uint32)
+    V = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], np.float64)
+    m = rx.Mesh(F, patch_size=64)
+    Mr = m.symdir_rest(_lin(m, V.astype(np.float32)))
+    uv = V[:, :2].copy()
+    uv[3] = [-1.0, -1.0]  # flips face 1
+    total, energy = m.symdir_energy(Mr, _lin(m, uv))
+    torch.cuda.synchronize()
+    gE = _glob(m, energy.cpu().numpy(), "f_l2g")
+    assert gE[1] == np.finfo(np.float32).max and np.isfinite(gE[0])
