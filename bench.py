@@ -4,7 +4,8 @@ Default workload (one "step"): on the icosphere subdivided 9 times
 (2,621,442 V / 5,242,880 F, radial noise N(0, 0.005), vertex and face ids
 randomly permuted, patch size 512):
   VV query materialised as CSR + VF query materialised as CSR +
-  10 Jacobi iterations of uniform Laplacian smoothing (lambda 0.5, fp32).
+  10 Jacobi iterations of uniform Laplacian smoothing (lambda 0.5, fp32)
+  consuming the VV CSR of this step.
 value = VV/VF entries delivered per second (Gentries/s), where a smoothing
 iteration consumes the 2E VV entries of its one-rings.
 
@@ -107,11 +108,14 @@ def main():
     vf_rp = torch.empty(V + 1, dtype=torch.int32, device="cuda")
     vf_col = torch.empty(nnz_vf, dtype=torch.int32, device="cuda")
     it = args.smooth_iters
+    m.build_vv_plan()  # static patch-local VV plan, built once per mesh
+    torch.cuda.synchronize()
 
     if args.profile:
         fns = {"query_vv": lambda: m.query("VV", row_ptr=vv_rp, col=vv_col),
                "query_vf": lambda: m.query("VF", row_ptr=vf_rp, col=vf_col),
                "smooth_uniform": lambda: m.smooth(x, 0.5, 1, out=out),
+               "smooth_csr": lambda: m.smooth_csr((vv_rp, vv_col), x, 0.5, 1, out=out),
                "vertex_normals": lambda: m.vertex_normals(x, "max", out=out),
                "copy": lambda: out.copy_(x)}  # plain copy of 12 V bytes (counter calibration)
         fn = fns[args.profile]
@@ -147,11 +151,17 @@ def main():
     kern["query_vf"] = (t_vf, 4 * E + 6 * F + 4 * nnz_vf + 4 * (V + 1))
     t_sm = event_time_ms(lambda: m.smooth(x, 0.5, 1, out=out), reps)
     kern["smooth_uniform"] = (t_sm, 24 * V + 4 * E)
+    t_sc = event_time_ms(lambda: m.smooth_csr((vv_rp, vv_col), x, 0.5, 1, out=out), reps)
+    kern["smooth_csr"] = (t_sc, 24 * V + 4 * E)
+    for th in (64, 256, 128):
+        m.set_launch_threads(0, th)
+        t = event_time_ms(lambda: m.smooth(x, 0.5, 1, out=out), reps)
+        kern["smooth_uniform_t%d" % th] = (t, 24 * V + 4 * E)
     t_vn = event_time_ms(lambda: m.vertex_normals(x, "max", out=out), reps)
     kern["vertex_normals"] = (t_vn, 24 * V + 4 * E + 6 * F)
     kernels = {k: {"us": round(t * 1e3, 2), "bytes": b, "GB_s": round(b / (t * 1e-3) / 1e9, 1),
                    "frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k, (t, b) in kern.items()}
-    # dominant kernel of the step: smoothing (it launches per step)
+    # dominant kernel of the step (by time per step)
     share = {"query_vv": t_vv, "query_vf": t_vf, "smooth_uniform": it * t_sm}
     dom = max(share, key=share.get)
     dt, db = kern[dom]

@@ -134,13 +134,30 @@ rxm_status rxm_query_csr(rxm_mesh* mesh, int op, int oriented, int32_t* row_ptr,
 rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float* normals,
                               void* stream);
 
+/* Build (once; later calls return at once) the mesh's patch-local VV plan:
+ * the VV query evaluated by the query engine and kept in its native
+ * patch-local u16 encoding for iterative consumers. rxm_laplacian_smooth
+ * builds it on first use; call this to keep the build out of a timed loop. */
+rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
+/* Workgroup size of a launch family (kernel 0 = smoothing): 64, 128, 256.
+ * Used by the autotuner. */
+rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
+
 /* `iters` Jacobi steps of uniform Laplacian smoothing
  * x <- x + lambda (mean of VV neighbours - x); `out` receives the result,
  * `tmp` is a caller workspace of 3V floats (may be NULL when iters == 1).
+ * Neighbours come from the patch-local VV plan (rxm_vv_plan_build).
  * No single reference function; nearest are apps/Smoothing/manual.h:85-103
  * and the uniform branch of apps/MCF/mcf_kernels.cuh:114-213. */
 rxm_status rxm_laplacian_smooth(rxm_mesh* mesh, const float* pos, float* out, float* tmp,
                                 float lambda, int iters, void* stream);
+
+/* Same smoothing, consuming a materialised VV query (rxm_query_csr with
+ * RXM_OP_VV, unoriented) instead of re-deriving the one-rings per launch:
+ * for iterative use on a static mesh, query once and iterate on the CSR. */
+rxm_status rxm_laplacian_smooth_csr(rxm_mesh* mesh, const int32_t* vv_row_ptr, const int32_t* vv_col,
+                                    const float* pos, float* out, float* tmp, float lambda,
+                                    int iters, void* stream);
 
 /* Entries of the cotangent system (V + 2E on closed manifolds). */
 rxm_status rxm_cot_nnz(rxm_mesh* mesh, int64_t* nnz);

@@ -56,6 +56,10 @@ _SIGS = {
     "rxm_vertex_normals": [_vp, _i, _vp, _vp, _vp],
     "rxm_laplacian_smooth": [_vp, _vp, _vp, _vp, _f, _i, _vp],
     "rxm_to_linear": [_vp, _i, _i, _vp, _vp, _vp],
+    "rxm_laplacian_smooth_csr": [_vp, _vp, _vp, _vp, _vp, _vp, _f, _i, _vp],
+    "rxm_mesh_device_view": [_vp, _vp, ctypes.c_uint64],
+    "rxm_vv_plan_build": [_vp, _vp],
+    "rxm_set_launch_threads": [_vp, _i, _i],
     "rxm_cot_nnz": [_vp, ctypes.POINTER(ctypes.c_int64)],
     "rxm_cot_csr_assemble": [_vp, _vp, _f, _vp, _vp, _vp, _vp, _vp],
     "rxm_spmv3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -195,6 +199,24 @@ class Mesh:
             tmp = torch.empty_like(pos)
         _check(lib().rxm_laplacian_smooth(self._h, _ptr(pos), _ptr(out), _ptr(tmp) if tmp is not None else None,
                                           float(lam), int(iters), _stream(stream)), "rxm_laplacian_smooth")
+        return out
+
+    def build_vv_plan(self, stream=None):
+        _check(lib().rxm_vv_plan_build(self._h, _stream(stream)), "rxm_vv_plan_build")
+
+    def set_launch_threads(self, kernel, threads):
+        _check(lib().rxm_set_launch_threads(self._h, kernel, threads), "rxm_set_launch_threads")
+
+    def smooth_csr(self, vv, pos, lam=0.5, iters=1, out=None, tmp=None, stream=None):
+        """Smoothing on a materialised VV query vv = (row_ptr, col)."""
+        import torch
+        if out is None:
+            out = torch.empty_like(pos)
+        if tmp is None and iters > 1:
+            tmp = torch.empty_like(pos)
+        _check(lib().rxm_laplacian_smooth_csr(self._h, _ptr(vv[0]), _ptr(vv[1]), _ptr(pos), _ptr(out),
+                                              _ptr(tmp) if tmp is not None else None, float(lam), int(iters),
+                                              _stream(stream)), "rxm_laplacian_smooth_csr")
         return out
 
     # -- cotangent system / explicit MCF ------------------------------

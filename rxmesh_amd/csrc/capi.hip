@@ -13,6 +13,7 @@
 #include "k_geometry.hpp"
 #include "k_numerics.hpp"
 #include "k_query.hpp"
+#include "k_query2.hpp"
 
 using namespace rxm;
 
@@ -27,6 +28,13 @@ struct rxm_mesh
     std::map<int, uint32_t*>            d_base;  // per-op patch entry bases (P+1)
     std::map<int, std::vector<uint32_t>> h_base;
     float*                              d_partial = nullptr;  // P per-patch partial sums
+    // patch-local VV plan (u16 row offsets + u16 neighbour local ids)
+    const uint32_t* d_vv_rbase = nullptr;
+    const uint32_t* d_vv_nbase = nullptr;
+    uint16_t*       d_vv_roff  = nullptr;
+    uint16_t*       d_vv_nbr   = nullptr;
+    uint32_t        vv_max_nent = 0;
+    uint32_t        smooth_threads = 128;
 };
 
 namespace {
@@ -167,14 +175,64 @@ int device_base(rxm_mesh* m, int op, const uint32_t** out)
 
 // Launch a one-workgroup-per-patch kernel with `lds` bytes of dynamic LDS.
 template <typename K, typename... Args>
-int launch_patches(K kernel, uint32_t threads, uint32_t lds, void* stream, const rxm_mesh* m, Args... args)
+int launch_grid(K kernel, uint32_t grid, uint32_t threads, uint32_t lds, void* stream, Args... args)
 {
     if (lds > 160u * 1024u) return RXM_ERR_PATCH_TOO_BIG;
     if (lds > 64u * 1024u)
         RXM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
-    hipLaunchKernelGGL(kernel, dim3(m->h.P), dim3(threads), lds, static_cast<hipStream_t>(stream), args...);
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(threads), lds, static_cast<hipStream_t>(stream), args...);
     RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+// one workgroup per patch, block index = patch
+template <typename K, typename... Args>
+int launch_patches(K kernel, uint32_t threads, uint32_t lds, void* stream, const rxm_mesh* m, Args... args)
+{
+    return launch_grid(kernel, m->h.P, threads, lds, stream, args...);
+}
+
+// one workgroup per patch, XCD-contiguous patch ranges (patch_of_block)
+template <typename K, typename... Args>
+int launch_patches_xcd(K kernel, uint32_t threads, uint32_t lds, void* stream, const rxm_mesh* m, Args... args)
+{
+    return launch_grid(kernel, patch_grid(m->h.P), threads, lds, stream, args...);
+}
+
+int ensure_vv_plan(rxm_mesh* m, void* stream)
+{
+    if (m->d_vv_nbr) return RXM_OK;
+    const HostMesh& h = m->h;
+    const auto&     vb = op_base(m, RXM_OP_VV);
+    std::vector<uint32_t> rb(h.P + 1, 0), nb(h.P + 1, 0);
+    uint32_t maxn = 0;
+    for (uint32_t p = 0; p < h.P; ++p) {
+        const uint32_t nent = vb[p + 1] - vb[p];
+        if (nent >= 0xFFFFu) return RXM_ERR_PATCH_TOO_BIG;
+        maxn = std::max(maxn, nent);
+        rb[p + 1] = rb[p] + ((h.nvo[p] + 1u + 7u) & ~7u);
+        nb[p + 1] = nb[p] + ((nent + 7u) & ~7u);
+    }
+    m->vv_max_nent = maxn;
+    int rc = upload(m, rb, m->d_vv_rbase);
+    if (!rc) rc = upload(m, nb, m->d_vv_nbase);
+    if (rc) return rc;
+    if (hipMalloc(&m->d_vv_roff, sizeof(uint16_t) * std::max<uint32_t>(rb[h.P], 8)) != hipSuccess)
+        return RXM_ERR_OUT_OF_MEMORY;
+    m->allocs.push_back(m->d_vv_roff);
+    if (hipMalloc(&m->d_vv_nbr, sizeof(uint16_t) * std::max<uint32_t>(nb[h.P], 8)) != hipSuccess)
+        return RXM_ERR_OUT_OF_MEMORY;
+    m->allocs.push_back(m->d_vv_nbr);
+    m->device_bytes += 2ull * (rb[h.P] + nb[h.P]);
+    Bump b(nullptr);
+    plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
+    rc = launch_patches(k_query2_vv_local<kQueryThreads>, kQueryThreads, b.bytes(), stream, m, m->dm,
+                        m->d_vv_rbase, m->d_vv_nbase, m->d_vv_roff, m->d_vv_nbr);
+    if (rc) {
+        m->d_vv_nbr = nullptr;  // freed with the mesh; rebuild on the next call
+        return rc;
+    }
     return RXM_OK;
 }
 
@@ -376,35 +434,38 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
     int rc = device_base(m, op, &base);
     if (rc) return rc;
     Bump b(nullptr);
+    constexpr uint32_t T = kQueryThreads;
+    if (oriented) {  // oriented VV / VE: general transpose + canonical CCW walk
+        plan_vx(b, h.max_nv, h.max_ne, h.max_nf, true);
+        if (op == RXM_OP_VV)
+            return launch_patches(k_query_vx<true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col, oriented);
+        return launch_patches(k_query_vx<false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col, oriented);
+    }
     switch (op) {
         case RXM_OP_VV:
-            plan_vx(b, h.max_nv, h.max_ne, h.max_nf, oriented != 0);
-            return launch_patches(k_query_vx<true>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
-                                  col, oriented);
+            plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
+            return launch_patches(k_query2_vx<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_VE:
-            plan_vx(b, h.max_nv, h.max_ne, h.max_nf, oriented != 0);
-            return launch_patches(k_query_vx<false>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
-                                  col, oriented);
+            plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
+            return launch_patches(k_query2_vx<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_VF:
-            plan_vf(b, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches(k_query_vf, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf);
+            return launch_patches(k_query2_vf<T>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_EF:
-            plan_ef(b, h.max_ne, h.max_nf, h.max_nfo);
-            return launch_patches(k_query_ef<false>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
-                                  col);
+            plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
+            return launch_patches(k_query2_ef<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_FF:
-            plan_ef(b, h.max_ne, h.max_nf, h.max_nfo);
-            return launch_patches(k_query_ef<true>, kQueryThreads, b.bytes(), stream, m, m->dm, base, row_ptr,
-                                  col);
+            plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
+            return launch_patches(k_query2_ef<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_EV:
             plan_fx(b, h.max_ne, h.max_nf);
-            return launch_patches(k_query_fixed<9>, kQueryThreads, b.bytes(), stream, m, m->dm, row_ptr, col);
+            return launch_patches(k_query_fixed<9>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
         case RXM_OP_FV:
             plan_fx(b, h.max_ne, h.max_nf);
-            return launch_patches(k_query_fixed<6>, kQueryThreads, b.bytes(), stream, m, m->dm, row_ptr, col);
+            return launch_patches(k_query_fixed<6>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
         case RXM_OP_FE:
             plan_fx(b, h.max_ne, h.max_nf);
-            return launch_patches(k_query_fixed<7>, kQueryThreads, b.bytes(), stream, m, m->dm, row_ptr, col);
+            return launch_patches(k_query_fixed<7>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
         default: return RXM_ERR_INVALID_ARG;
     }
 }
@@ -418,20 +479,70 @@ rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* no
     return launch_patches(k_vertex_normals, kGeomThreads, b.bytes(), stream, m, m->dm, mode, pos, normals);
 }
 
+rxm_status rxm_vv_plan_build(rxm_mesh* m, void* stream)
+{
+    if (!m) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    return ensure_vv_plan(m, stream);
+}
+
 rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float* tmp, float lambda, int iters,
                                 void* stream)
 {
     if (!m || !pos || !out || iters < 1 || pos == out) return RXM_ERR_INVALID_ARG;
     if (iters > 1 && (!tmp || tmp == out || tmp == pos)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    int rc = ensure_vv_plan(m, stream);
+    if (rc) return rc;
     Bump b(nullptr);
-    plan_sm(b, m->h.max_nv, m->h.max_ne);
-    // ping-pong so that the last step lands in `out`
+    plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent);
     const float* src = pos;
     for (int it = 0; it < iters; ++it) {
         float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
-        int rc = launch_patches(k_smooth_uniform, kGeomThreads, b.bytes(), stream, m, m->dm, lambda, src, dst);
+        switch (m->smooth_threads) {
+            case 64:
+                rc = launch_patches_xcd(k_smooth_local<64>, 64, b.bytes(), stream, m, m->dm, m->d_vv_rbase, m->d_vv_nbase,
+                                    static_cast<const uint16_t*>(m->d_vv_roff), static_cast<const uint16_t*>(m->d_vv_nbr),
+                                    lambda, src, dst);
+                break;
+            case 256:
+                rc = launch_patches_xcd(k_smooth_local<256>, 256, b.bytes(), stream, m, m->dm, m->d_vv_rbase, m->d_vv_nbase,
+                                    static_cast<const uint16_t*>(m->d_vv_roff), static_cast<const uint16_t*>(m->d_vv_nbr),
+                                    lambda, src, dst);
+                break;
+            default:
+                rc = launch_patches_xcd(k_smooth_local<128>, 128, b.bytes(), stream, m, m->dm, m->d_vv_rbase, m->d_vv_nbase,
+                                    static_cast<const uint16_t*>(m->d_vv_roff), static_cast<const uint16_t*>(m->d_vv_nbr),
+                                    lambda, src, dst);
+        }
         if (rc) return rc;
+        src = dst;
+    }
+    return RXM_OK;
+}
+
+rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
+{
+    if (!m || kernel != 0 || (threads != 64 && threads != 128 && threads != 256)) return RXM_ERR_INVALID_ARG;
+    m->smooth_threads = static_cast<uint32_t>(threads);
+    return RXM_OK;
+}
+
+rxm_status rxm_laplacian_smooth_csr(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* pos,
+                                    float* out, float* tmp, float lambda, int iters, void* stream)
+{
+    if (!m || !row_ptr || !col || !pos || !out || iters < 1 || pos == out) return RXM_ERR_INVALID_ARG;
+    if (iters > 1 && (!tmp || tmp == out || tmp == pos)) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    constexpr uint32_t LANES = 8;
+    const uint32_t     n     = m->h.nV;
+    const uint64_t     thr   = uint64_t(n) * LANES;
+    const float*       src   = pos;
+    for (int it = 0; it < iters; ++it) {
+        float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
+        hipLaunchKernelGGL((k_smooth_csr<LANES>), dim3(static_cast<uint32_t>((thr + 255) / 256)), dim3(256), 0,
+                           static_cast<hipStream_t>(stream), n, row_ptr, col, lambda, src, dst);
+        RXM_HIP(hipGetLastError());
         src = dst;
     }
     return RXM_OK;

@@ -9,6 +9,7 @@
 // output is a plain gather in LDS: no global atomics, deterministic order.
 #pragma once
 #include "k_query.hpp"
+#include "k_query2.hpp"
 
 namespace rxm {
 
@@ -169,151 +170,76 @@ __global__ __launch_bounds__(kGeomThreads) void k_smooth_uniform(DevMesh m, floa
     }
 }
 
-// ---- persistent, prefetching pipeline ---------------------------------------
-// A grid of (CUs x k) workgroups walks the patches; workgroup b sits on XCD
-// b % 8 and takes patches from that XCD's contiguous eighth of the patch
-// range, so neighbouring patches (which share ribbon vertices) share an L2.
-// While one patch is transposed and reduced in LDS, the next patch's EV words,
-// owned positions and ribbon owner ids are already in flight into registers.
-struct XcdRange
+// ---- uniform smoothing over the patch-local VV plan --------------------------
+// One workgroup per patch with a small LDS footprint (positions + the
+// patch's plan rows), so many patches are in flight per CU. Rows are read
+// with 16-byte loads; neighbours are summed in the plan's row order.
+struct PlanSL
 {
-    uint32_t first, end, step;
-};
-__device__ __forceinline__ XcdRange xcd_range(uint32_t P)
-{
-    const uint32_t G = gridDim.x, x = blockIdx.x & 7u, j = blockIdx.x >> 3;
-    const uint32_t per = (G + 7u - x) >> 3;  // workgroups on XCD x
-    const uint32_t b = static_cast<uint32_t>((uint64_t(P) * x) >> 3);
-    const uint32_t e = static_cast<uint32_t>((uint64_t(P) * (x + 1u)) >> 3);
-    return {b + j, e, per};
-}
-
-struct PlanSM2
-{
-    uint32_t *ev, *off, *cnt, *wave;
-    uint16_t* val;
+    uint16_t *ro, *nb;
     float *   x, *y, *z;
 };
-__host__ __device__ inline PlanSM2 plan_sm2(Bump& b, uint32_t nv, uint32_t ne, uint32_t nw)
+__host__ __device__ inline PlanSL plan_sl(Bump& b, uint32_t nv, uint32_t nvo, uint32_t nent)
 {
-    PlanSM2 p{};
-    p.ev   = b.alloc<uint32_t>(r4(ne));
-    p.off  = b.alloc<uint32_t>(nv + 1);
-    p.cnt  = b.alloc<uint32_t>(nw * nv);
-    p.wave = b.alloc<uint32_t>(20);
-    p.val  = b.alloc<uint16_t>(2 * ne);
-    p.x    = b.alloc<float>(nv);
-    p.y    = b.alloc<float>(nv);
-    p.z    = b.alloc<float>(nv);
+    PlanSL p{};
+    p.ro = b.alloc<uint16_t>(((nvo + 1u) + 7u) & ~7u);
+    p.nb = b.alloc<uint16_t>((nent + 7u) & ~7u);
+    p.x  = b.alloc<float>(nv);
+    p.y  = b.alloc<float>(nv);
+    p.z  = b.alloc<float>(nv);
     return p;
 }
 
-// Register-staged copy of one patch: RE uint4 of EV and RV vertex slots per
-// thread (host guarantees ne <= 4 * RE * T and nv <= RV * T).
-template <uint32_t T, uint32_t RE, uint32_t RV>
-struct SmoothStage
+// copy n u16 from a 16-byte aligned, 8-element padded run into LDS
+template <uint32_t T>
+__device__ __forceinline__ void stage_u16x8(uint16_t* dst, const uint16_t* __restrict__ src, uint32_t n)
 {
-    PatchDesc d;
-    uint4     ev[RE];
-    float     x[RV], y[RV], z[RV];
-    uint32_t  lin[RV];
+    const uint32_t n8 = (n + 7u) >> 3;
+    const uint4*   s  = reinterpret_cast<const uint4*>(src);
+    uint4*         d  = reinterpret_cast<uint4*>(dst);
+    for (uint32_t i = threadIdx.x; i < n8; i += T) d[i] = s[i];
+}
 
-    __device__ __forceinline__ void load(const DevMesh& m, const float* __restrict__ pos, uint32_t p)
-    {
-        d                 = load_desc(m.patch, p);
-        const uint4*  s   = reinterpret_cast<const uint4*>(m.ev + d.e_off);
-        const uint32_t n4 = (d.ne + 3u) >> 2;
-#pragma unroll
-        for (uint32_t r = 0; r < RE; ++r) {
-            const uint32_t i = threadIdx.x + r * T;
-            if (i < n4) ev[r] = s[i];
-        }
-        const uint32_t N = m.nV;
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = threadIdx.x + r * T;
-            if (l < d.nvo) {
-                const uint32_t g = d.vp + l;
-                x[r] = pos[g];
-                y[r] = pos[N + g];
-                z[r] = pos[2 * N + g];
-            } else if (l < d.nv) {
-                lin[r] = m.v_lin[d.v_off + l];
-            }
-        }
-    }
-};
-
-template <uint32_t T, uint32_t RE, uint32_t RV>
-__global__ __launch_bounds__(T) void k_smooth_persistent(DevMesh m, float lambda, const float* __restrict__ pos,
-                                                         float* __restrict__ out)
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* __restrict__ rbase,
+                                                    const uint32_t* __restrict__ nbase,
+                                                    const uint16_t* __restrict__ roff,
+                                                    const uint16_t* __restrict__ nbr, float lambda,
+                                                    const float* __restrict__ pos, float* __restrict__ out)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t          N  = m.nV;
-    const XcdRange          xr = xcd_range(m.P);
-    SmoothStage<T, RE, RV>  st;
-    if (xr.first < xr.end) st.load(m, pos, xr.first);
-    for (uint32_t p = xr.first; p < xr.end; p += xr.step) {
-        const PatchDesc d = st.d;
-        Bump            b(reinterpret_cast<char*>(smem));
-        const PlanSM2   L = plan_sm2(b, d.nv, d.ne, T / 64u);
-        const uint32_t  n4 = (d.ne + 3u) >> 2;
-#pragma unroll
-        for (uint32_t r = 0; r < RE; ++r) {
-            const uint32_t i = threadIdx.x + r * T;
-            if (i < n4) reinterpret_cast<uint4*>(L.ev)[i] = st.ev[r];
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d = load_desc(m.patch, p);
+    const uint32_t  rb = rbase[p], nbb = nbase[p], nent = nbase[p + 1] - nbb;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanSL    L = plan_sl(b, d.nv, d.nvo, nent);
+    const uint32_t  N = m.nV;
+    stage_u16x8<T>(L.ro, roff + rb, d.nvo + 1);
+    stage_u16x8<T>(L.nb, nbr + nbb, nent);
+    stage_positions(L.x, L.y, L.z, pos, N, d, m.v_lin);
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+        float          sx = 0.f, sy = 0.f, sz = 0.f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t w = L.nb[k];
+            sx += L.x[w];
+            sy += L.y[w];
+            sz += L.z[w];
         }
-        float gx[RV], gy[RV], gz[RV];
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = threadIdx.x + r * T;
-            if (l < d.nvo) {
-                L.x[l] = st.x[r];
-                L.y[l] = st.y[r];
-                L.z[l] = st.z[r];
-            } else if (l < d.nv) {  // ribbon: gather from the owner's slot
-                const uint32_t g = st.lin[r];
-                gx[r] = pos[g];
-                gy[r] = pos[N + g];
-                gz[r] = pos[2 * N + g];
-            }
+        const uint32_t g  = d.vp + v;
+        const float    px = L.x[v], py = L.y[v], pz = L.z[v];
+        if (k1 > k0) {
+            const float inv = 1.0f / static_cast<float>(k1 - k0);
+            out[g]          = px + lambda * (sx * inv - px);
+            out[N + g]      = py + lambda * (sy * inv - py);
+            out[2 * N + g]  = pz + lambda * (sz * inv - pz);
+        } else {
+            out[g]         = px;
+            out[N + g]     = py;
+            out[2 * N + g] = pz;
         }
-        if (p + xr.step < xr.end) st.load(m, pos, p + xr.step);
-        __syncthreads();
-        lds_transpose_ordered<2>(2 * d.ne, d.nv, bits_for(d.nv),
-                                 [&](uint32_t i) {
-                                     const uint32_t x = L.ev[i >> 1];
-                                     return (i & 1u) ? hi16(x) : lo16(x);
-                                 },
-                                 L.off, L.cnt, L.val, L.wave);
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = threadIdx.x + r * T;
-            if (l >= d.nvo && l < d.nv) {
-                L.x[l] = gx[r];
-                L.y[l] = gy[r];
-                L.z[l] = gz[r];
-            }
-        }
-        __syncthreads();
-        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
-            float          sx = 0.f, sy = 0.f, sz = 0.f;
-            const uint32_t k0 = L.off[v], k1 = L.off[v + 1];
-            for (uint32_t k = k0; k < k1; ++k) {
-                const uint32_t x = L.ev[L.val[k]];
-                const uint32_t w = (lo16(x) == v) ? hi16(x) : lo16(x);
-                sx += L.x[w];
-                sy += L.y[w];
-                sz += L.z[w];
-            }
-            const float    inv = (k1 > k0) ? 1.0f / static_cast<float>(k1 - k0) : 0.f;
-            const uint32_t g   = d.vp + v;
-            const float    px = L.x[v], py = L.y[v], pz = L.z[v];
-            out[g]         = (k1 > k0) ? px + lambda * (sx * inv - px) : px;
-            out[N + g]     = (k1 > k0) ? py + lambda * (sy * inv - py) : py;
-            out[2 * N + g] = (k1 > k0) ? pz + lambda * (sz * inv - pz) : pz;
-        }
-        __syncthreads();
     }
 }
 

@@ -168,6 +168,51 @@ __global__ __launch_bounds__(256) void k_spmv3(uint32_t n, const int32_t* __rest
     }
 }
 
+// ---- uniform smoothing over a materialised VV CSR ---------------------------
+// One Jacobi step x' = x + lambda (mean of row neighbours - x) over the CSR of
+// a VV query (linear ids). LANES lanes share a row; fixed xor-tree reduction.
+template <uint32_t LANES>
+__global__ __launch_bounds__(256) void k_smooth_csr(uint32_t n, const int32_t* __restrict__ row_ptr,
+                                                    const int32_t* __restrict__ col, float lambda,
+                                                    const float* __restrict__ x, float* __restrict__ y)
+{
+    const uint32_t gid  = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t row  = gid / LANES;
+    const uint32_t lane = gid % LANES;
+    float          sx = 0.f, sy = 0.f, sz = 0.f;
+    const bool     ok = row < n;
+    int32_t        s = 0, e = 0;
+    if (ok) {
+        s = row_ptr[row];
+        e = row_ptr[row + 1];
+        for (int32_t k = s + static_cast<int32_t>(lane); k < e; k += LANES) {
+            const uint32_t c = static_cast<uint32_t>(col[k]);
+            sx += x[c];
+            sy += x[n + c];
+            sz += x[2 * n + c];
+        }
+    }
+#pragma unroll
+    for (uint32_t o = LANES / 2; o > 0; o >>= 1) {
+        sx += __shfl_xor(sx, o, 64);
+        sy += __shfl_xor(sy, o, 64);
+        sz += __shfl_xor(sz, o, 64);
+    }
+    if (ok && lane == 0) {
+        const float px = x[row], py = x[n + row], pz = x[2 * n + row];
+        if (e > s) {
+            const float inv = 1.0f / static_cast<float>(e - s);
+            y[row]          = px + lambda * (sx * inv - px);
+            y[n + row]      = py + lambda * (sy * inv - py);
+            y[2 * n + row]  = pz + lambda * (sz * inv - pz);
+        } else {
+            y[row]         = px;
+            y[n + row]     = py;
+            y[2 * n + row] = pz;
+        }
+    }
+}
+
 // ---- symmetric Dirichlet ----------------------------------------------------
 // Rest shape of a face: apps/Param/param.cu:113-141. Mr is stored SoA over
 // linear face ids: Mr[c * F + f], c = (m00, m10, m01, m11), columns b-a, c-a

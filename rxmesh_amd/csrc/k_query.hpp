@@ -12,8 +12,6 @@ namespace rxm {
 
 constexpr uint32_t kQueryThreads = 256;
 
-__host__ __device__ inline uint32_t r4(uint32_t n) { return (n + 3u) & ~3u; }
-
 // ---- LDS plans (same carve on host for sizing and on device) -------------
 struct PlanVX
 {
@@ -320,7 +318,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query_fixed(DevMesh m, int32_
             col[2 * r]       = static_cast<int32_t>(lin_of(lo16(x), d.nvo, d.vp, m.v_lin + d.v_off));
             col[2 * r + 1]   = static_cast<int32_t>(lin_of(hi16(x), d.nvo, d.vp, m.v_lin + d.v_off));
         }
-        if (p + 1 == gridDim.x && threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(2 * (d.ep + d.neo));
+        if (p + 1 == m.P && threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(2 * (d.ep + d.neo));
     } else {
         if constexpr (OP == 6) stage_u32(L.ev, m.ev + d.e_off, d.ne);
         stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
@@ -333,7 +331,7 @@ __global__ __launch_bounds__(kQueryThreads) void k_query_fixed(DevMesh m, int32_
             col[3 * d.fp + i] = static_cast<int32_t>(out);
             if (i % 3 == 0) row_ptr[r] = static_cast<int32_t>(3 * r);
         }
-        if (p + 1 == gridDim.x && threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(3 * (d.fp + d.nfo));
+        if (p + 1 == m.P && threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(3 * (d.fp + d.nfo));
     }
 }
 

@@ -1,0 +1,299 @@
+// Lean query kernels (compile-time workgroup size, register row sorts).
+//
+// Same results as the general transpose in lds_core.hpp -- every row lists
+// its entries in ascending source-local order -- with far fewer vector
+// instructions per entry:
+//   1. degree histogram with LDS atomics (order-free);
+//   2. exclusive scan (compile-time workgroup size, no runtime division);
+//   3. slot fill with LDS atomics, each slot packing (sort key << 16 | payload);
+//   4. one thread per owned row loads its slots into registers, sorts them
+//      with a fixed compare-exchange network (rows up to 8 entries; longer
+//      rows take an LDS insertion sort) and writes the row straight to HBM.
+#pragma once
+#include "lds_core.hpp"
+
+namespace rxm {
+
+template <uint32_t T>
+__device__ __forceinline__ uint32_t scan_excl(uint32_t* s, uint32_t n, uint32_t* s_wave)
+{
+    constexpr uint32_t NW = T / 64u;
+    const uint32_t     tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    const uint32_t     per = (n + T - 1u) / T;  // T is a power of two: a shift
+    const uint32_t     b = min(n, tid * per), e = min(n, b + per);
+    __syncthreads();
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += s[i];
+    uint32_t x = sum;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63u) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        const uint32_t t = s_wave[w];
+        before += (w < wid) ? t : 0u;
+        total += t;
+    }
+    uint32_t run = before + x - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t t = s[i];
+        s[i]             = run;
+        run += t;
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
+{
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+// Sort the k packed slots of one row ascending. Rows of up to 8 entries are
+// sorted in registers with Batcher's odd-even merge network (19
+// comparators); longer rows fall back to an insertion sort in LDS. The
+// callback out(j, slot) receives the sorted entries.
+template <typename Out>
+__device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
+{
+    if (k <= 8u) {
+        uint32_t a[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; ++j) a[j] = (j < k) ? slots[j] : 0xFFFFFFFFu;
+        cswap(a[0], a[1]); cswap(a[2], a[3]); cswap(a[4], a[5]); cswap(a[6], a[7]);
+        cswap(a[0], a[2]); cswap(a[1], a[3]); cswap(a[4], a[6]); cswap(a[5], a[7]);
+        cswap(a[1], a[2]); cswap(a[5], a[6]);
+        cswap(a[0], a[4]); cswap(a[1], a[5]); cswap(a[2], a[6]); cswap(a[3], a[7]);
+        cswap(a[2], a[4]); cswap(a[3], a[5]);
+        cswap(a[1], a[2]); cswap(a[3], a[4]); cswap(a[5], a[6]);
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; ++j)
+            if (j < k) out(j, a[j]);
+    } else {
+        for (uint32_t i = 1; i < k; ++i) {
+            const uint32_t x = slots[i];
+            uint32_t       j = i;
+            while (j > 0 && slots[j - 1] > x) {
+                slots[j] = slots[j - 1];
+                --j;
+            }
+            slots[j] = x;
+        }
+        for (uint32_t j = 0; j < k; ++j) out(j, slots[j]);
+    }
+}
+
+// Histogram + scan + packed slot fill of n entries into ncols columns.
+// key(i) -> column, pack(i) -> 32-bit slot value (sort key in the high bits).
+template <uint32_t T, typename KeyF, typename PackF>
+__device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF key, PackF pack, uint32_t* off,
+                                             uint32_t* cur, uint32_t* slots, uint32_t* s_wave)
+{
+    for (uint32_t c = threadIdx.x; c < ncols; c += T) off[c] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += T) atomicAdd(&off[key(i)], 1u);
+    const uint32_t total = scan_excl<T>(off, ncols, s_wave);
+    if (threadIdx.x == 0) off[ncols] = total;
+    for (uint32_t c = threadIdx.x; c < ncols; c += T) cur[c] = off[c];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += T) slots[atomicAdd(&cur[key(i)], 1u)] = pack(i);
+    __syncthreads();
+}
+
+// ---- plans -----------------------------------------------------------------
+struct PlanQ2
+{
+    uint32_t *ev, *off, *cur, *slots, *wave, *lin;
+    uint16_t *fe;
+};
+// kind: 0 = VV/VE (EV only), 1 = VF (EV + FE), 2 = EF/FF (FE only)
+__host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    PlanQ2 p{};
+    const uint32_t ncols = (kind == 2) ? ne : nv;
+    const uint32_t n     = (kind == 0) ? 2 * ne : 3 * nf;
+    if (kind != 2) p.ev = b.alloc<uint32_t>(r4(ne));
+    if (kind != 0) p.fe = b.alloc<uint16_t>(r4(3 * nf));
+    p.off   = b.alloc<uint32_t>(ncols + 1);
+    p.cur   = b.alloc<uint32_t>((ncols > nf ? ncols : nf) + 1);  // FF reuses it per owned face
+    p.slots = b.alloc<uint32_t>(n);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.lin   = b.alloc<uint32_t>((kind == 2) ? nf : ((kind == 0) ? (nv > ne ? nv : ne) : nf));
+    return p;
+}
+
+// Target linear ids of all local elements (owned: prefix + l; ribbon: owner
+// table), staged once so row writes need no global gathers.
+__device__ __forceinline__ void stage_lin(uint32_t* dst, uint32_t n, uint32_t n_owned, uint32_t prefix,
+                                          const uint32_t* __restrict__ slab, uint32_t T)
+{
+    for (uint32_t l = threadIdx.x; l < n; l += T) dst[l] = (l < n_owned) ? prefix + l : slab[l];
+}
+
+// VV (IS_VV) or VE over linear ids.
+template <uint32_t T, bool IS_VV>
+__global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __restrict__ base,
+                                                 int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanQ2    L = plan_q2(b, 0, d.nv, d.ne, d.nf);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    if (IS_VV) stage_lin(L.lin, d.nv, d.nvo, d.vp, m.v_lin + d.v_off, T);
+    else stage_lin(L.lin, d.ne, d.neo, d.ep, m.e_lin + d.e_off, T);
+    __syncthreads();
+    // entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
+    fill_columns<T>(
+        2 * d.ne, d.nv,
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return (i & 1u) ? hi16(x) : lo16(x);
+        },
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
+        },
+        L.off, L.cur, L.slots, L.wave);
+    const uint32_t bse = base[p];
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+        row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
+        int32_t* dst = col + bse + o;
+        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) {
+            dst[j] = static_cast<int32_t>(IS_VV ? L.lin[lo16(s)] : L.lin[hi16(s)]);
+        });
+    }
+    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+}
+
+// VV in patch-local form: per patch, u16 row offsets of the owned vertices
+// (nvo + 1, relative to the patch's entry base) at roff + rbase[p], and u16
+// local neighbour ids at nbr + nbase[p], rows in ascending local-edge order.
+// This is the query engine's native (RXMesh-style) encoding of the relation;
+// iterative kernels consume it with half the index bytes of an int32 CSR.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query2_vv_local(DevMesh m, const uint32_t* __restrict__ rbase,
+                                                       const uint32_t* __restrict__ nbase,
+                                                       uint16_t* __restrict__ roff, uint16_t* __restrict__ nbr)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanQ2    L = plan_q2(b, 0, d.nv, d.ne, d.nf);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    __syncthreads();
+    fill_columns<T>(
+        2 * d.ne, d.nv,
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return (i & 1u) ? hi16(x) : lo16(x);
+        },
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
+        },
+        L.off, L.cur, L.slots, L.wave);
+    uint16_t* ro = roff + rbase[p];
+    uint16_t* nb = nbr + nbase[p];
+    for (uint32_t v = threadIdx.x; v <= d.nvo; v += T) ro[v] = static_cast<uint16_t>(L.off[v]);
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { nb[o + j] = static_cast<uint16_t>(lo16(s)); });
+    }
+}
+
+// VF: FV from FE + EV, transposed; slot = face (the sort key itself).
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __restrict__ base,
+                                                 int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    stage_lin(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, T);
+    __syncthreads();
+    fill_columns<T>(
+        3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+        [&](uint32_t i) { return i / 3u; }, L.off, L.cur, L.slots, L.wave);
+    const uint32_t bse = base[p];
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+        row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
+        int32_t* dst = col + bse + o;
+        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { dst[j] = static_cast<int32_t>(L.lin[s]); });
+    }
+    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+}
+
+// EF (IS_FF == false) or FF.
+template <uint32_t T, bool IS_FF>
+__global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __restrict__ base,
+                                                 int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanQ2    L = plan_q2(b, 2, d.nv, d.ne, d.nf);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    stage_lin(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, T);
+    __syncthreads();
+    fill_columns<T>(
+        3 * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
+        [&](uint32_t i) { return i / 3u; }, L.off, L.cur, L.slots, L.wave);
+    const uint32_t bse = base[p];
+    if constexpr (!IS_FF) {
+        for (uint32_t e = threadIdx.x; e < d.neo; e += T) {
+            const uint32_t o = L.off[e], k = L.off[e + 1] - o;
+            row_ptr[d.ep + e] = static_cast<int32_t>(bse + o);
+            int32_t* dst = col + bse + o;
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { dst[j] = static_cast<int32_t>(L.lin[s]); });
+        }
+        if (threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(bse + L.off[d.neo]);
+    } else {
+        // sort the EF rows of every edge touched by an owned face
+        for (uint32_t e = threadIdx.x; e < d.ne; e += T) {
+            const uint32_t o = L.off[e], k = L.off[e + 1] - o;
+            if (k > 1) sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = s; });
+        }
+        __syncthreads();
+        // per owned face: (faces of its edges) - 1 each, scanned into L.cur
+        for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
+            uint32_t c = 0;
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t e = L.fe[3 * f + i] >> 1;
+                c += L.off[e + 1] - L.off[e] - 1;
+            }
+            L.cur[f] = c;
+        }
+        const uint32_t total = scan_excl<T>(L.cur, d.nfo, L.wave);
+        for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
+            uint32_t k = bse + L.cur[f];
+            row_ptr[d.fp + f] = static_cast<int32_t>(k);
+            for (uint32_t i = 0; i < 3; ++i) {
+                const uint32_t e = L.fe[3 * f + i] >> 1;
+                for (uint32_t j = L.off[e]; j < L.off[e + 1]; ++j) {
+                    const uint32_t g = L.slots[j];
+                    if (g != f) col[k++] = static_cast<int32_t>(L.lin[g]);
+                }
+            }
+        }
+        if (threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(bse + total);
+    }
+}
+
+}  // namespace rxm

@@ -17,8 +17,23 @@ namespace rxm {
 
 constexpr uint16_t kInv16 = 0xFFFFu;
 
+// element count rounded up to the 4-element slab padding
+__host__ __device__ inline uint32_t r4(uint32_t n) { return (n + 3u) & ~3u; }
+
 __device__ __forceinline__ uint32_t lo16(uint32_t x) { return x & 0xFFFFu; }
 __device__ __forceinline__ uint32_t hi16(uint32_t x) { return x >> 16; }
+
+// Patch handled by this workgroup. Workgroups are dispatched round-robin over
+// the 8 XCDs (each with its own L2); block b runs patch (b % 8) * ceil(P/8) +
+// b / 8 so every XCD sweeps one contiguous range of patches, and the ribbon
+// gathers into neighbouring patches (adjacent ids) stay in that XCD's L2.
+// The grid is 8 * ceil(P/8) blocks; blocks mapped past P return at once.
+__device__ __forceinline__ uint32_t patch_of_block(uint32_t P)
+{
+    const uint32_t chunk = (P + 7u) >> 3;
+    return (blockIdx.x & 7u) * chunk + (blockIdx.x >> 3);
+}
+__host__ __device__ inline uint32_t patch_grid(uint32_t P) { return ((P + 7u) >> 3) << 3; }
 
 __device__ __forceinline__ PatchDesc load_desc(const PatchDesc* __restrict__ pd, uint32_t p)
 {

@@ -151,6 +151,53 @@ def test_smoothing_vs_fp64(name):
     assert torch.equal(out, out2)
 
 
+def test_smoothing_workgroup_sizes_agree():
+    import torch
+    V, F, m, o = case("c1")
+    x = lin_pos(m, V.astype(np.float32))
+    outs = []
+    for t in (64, 128, 256):
+        m.set_launch_threads(0, t)
+        outs.append(m.smooth(x, 0.5, 3))
+    m.set_launch_threads(0, 128)
+    torch.cuda.synchronize()
+    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[1], outs[2])
+
+
+def test_vv_plan_matches_vv_query():
+    # the patch-local plan decodes (through the owner table) to the VV CSR
+    import ctypes
+    import rxmesh_amd as rx
+    V, F, m, o = case("c1_p128")
+    m.build_vv_plan()
+    orp, ocol = o.query("VV")
+    # smoothing one step from the plan equals a host step over the oracle CSR
+    x = lin_pos(m, V.astype(np.float32))
+    out = m.smooth(x, 0.5, 1).cpu().numpy().astype(np.float64)
+    xl = x.cpu().numpy().astype(np.float64)
+    ref = np.empty_like(xl)
+    for r in range(len(orp) - 1):
+        nb = ocol[orp[r]:orp[r + 1]]
+        ref[:, r] = xl[:, r] + 0.5 * (xl[:, nb].mean(1) - xl[:, r])
+    assert np.abs(out - ref).max() <= 1e-6
+
+
+@pytest.mark.parametrize("name", ["c1", "c0", "bunnyhead"])
+def test_smoothing_on_vv_csr_vs_fp64(name):
+    import torch
+    V, F, m, o = case(name)
+    V32 = V.astype(np.float32)
+    x = lin_pos(m, V32)
+    vv = m.query("VV")
+    out = m.smooth_csr(vv, x, 0.5, 10)
+    torch.cuda.synchronize()
+    got = to_global_np(m, out.cpu().numpy()).T
+    ref = o.smooth_uniform(V32.astype(np.float64), 0.5, 10)
+    err = np.linalg.norm(got - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
+    assert err.max() <= 1e-5, err.max()
+    assert torch.equal(out, m.smooth_csr(vv, x, 0.5, 10))
+
+
 def test_permutation_round_trip():
     import torch
     V, F, m, o = case("c1")
