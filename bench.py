@@ -38,6 +38,9 @@ def parse():
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
                          "smooth_uniform, vertex_normals, copy")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--batch-count", type=int, default=256, help="meshes in the batched workload (0: skip)")
+    ap.add_argument("--batch-steps", type=int, default=20)
+    ap.add_argument("--batch-reps", type=int, default=3)
     return ap.parse_args()
 
 
@@ -81,6 +84,66 @@ def cpu_baseline(seconds):
     return {"value": round(entries * n / el / 1e9, 6), "unit": "Gentries/s", "cores": 1, "kind": "port",
             "sample": "oracle VV+VF query + 10 smoothing iters on icosphere L6 (40,962 V / 81,920 F), "
                       "%d reps in %.1f s" % (n, el)}
+
+
+def batch_bench(count, steps, reps, patch_size):
+    """Batched workload: `count` meshes (face counts log-uniform in
+    [2e3, 1.3e6]; icospheres L3-L8 and jittered grids 32..800 alternating),
+    `steps` iterations of {uniform smoothing step (lambda 0.5) + symmetric
+    Dirichlet gradient step (eta 1e-3)} each. Batched = the union of all
+    meshes, 2 launches per step, captured in one hipGraph; plain = the same
+    loop as separate per-mesh launches."""
+    import numpy as np
+    import torch
+    import rxmesh_amd as rx
+    from rxmesh_amd import autotune
+    from rxmesh_amd import meshgen as mg
+    t0 = time.perf_counter()
+    specs = mg.batch_specs(count=count)
+    geo = [mg.make_batch_mesh(s, seed=k) for k, s in enumerate(specs)]
+    parts = [rx.Mesh(f, patch_size=patch_size) for _, f in geo]
+    union = rx.Mesh(None, parts=parts)
+    xs = [torch.from_numpy(np.ascontiguousarray(p[m.array("v_l2g")].T.astype(np.float32))).cuda()
+          for m, (p, _) in zip(parts, geo)]
+    x_u = torch.cat(xs, dim=1).contiguous()
+    Mr_u = union.symdir_rest(x_u)
+    Mrs = [m.symdir_rest(x) for m, x in zip(parts, xs)]
+    for m in parts:
+        m.build_vv_plan()
+    union.build_vv_plan()
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    lam, eta = 0.5, 1e-3
+    # autotuned launch configuration of the union
+    x_t = x_u.clone()
+    t_t = torch.empty_like(x_t)
+    cfg, _ = autotune.tune_smooth_symdir(union, Mr_u, x_t, t_t, lam, eta, use_cache=False)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        a = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - a) / reps
+
+    xb = x_u.clone()
+    tb = torch.empty_like(xb)
+    graph = union.capture_smooth_symdir(Mr_u, xb, tb, lam, eta, steps)
+    t_graph = timed(graph.launch)
+    t_eager = timed(lambda: union.smooth_symdir_steps(Mr_u, xb, tb, lam, eta, steps))
+    xp = [x.clone() for x in xs]
+    tp = [torch.empty_like(x) for x in xs]
+    t_plain = timed(lambda: rx.smooth_symdir_steps_each(parts, Mrs, xp, tp, lam, eta, steps))
+    graph.close()
+    F = union.nF
+    return {"meshes": count, "faces": F, "vertices": union.nV, "patches": union.P, "steps": steps,
+            "setup_s": round(setup_s, 1), "autotune": cfg,
+            "ms_batched_graph": round(t_graph * 1e3, 3), "ms_batched_eager": round(t_eager * 1e3, 3),
+            "ms_plain_per_mesh": round(t_plain * 1e3, 3),
+            "meshes_per_s_batched": round(count / t_graph, 2), "meshes_per_s_plain": round(count / t_plain, 2),
+            "speedup_vs_plain": round(t_plain / t_graph, 3)}
 
 
 def main():
@@ -178,6 +241,12 @@ def main():
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
         "kernels": kernels,
     }
+    # free the single-mesh workload before the batch
+    del x, out, tmp, vv_rp, vv_col, vf_rp, vf_col
+    m.close()
+    torch.cuda.empty_cache()
+    if args.batch_count > 0:
+        line["batch"] = batch_bench(args.batch_count, args.batch_steps, args.batch_reps, args.patch_size)
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(args.cpu_seconds)
     print(json.dumps(line), flush=True)

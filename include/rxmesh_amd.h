@@ -54,6 +54,7 @@ enum { RXM_HOST_ONLY = 1 };
 enum { RXM_NORMALS_MAX = 0, RXM_NORMALS_AREA = 1 };
 
 typedef struct rxm_mesh rxm_mesh;
+typedef struct rxm_graph rxm_graph;
 
 typedef struct rxm_mesh_info_t {
     uint32_t num_vertices, num_edges, num_faces, num_patches;
@@ -101,6 +102,24 @@ rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_
 rxm_status rxm_mesh_create_with_patches(const uint32_t* fv, uint32_t num_faces,
                                         uint32_t num_vertices, const uint32_t* face_patch,
                                         uint32_t num_patches, uint32_t flags, rxm_mesh** out);
+/* Build `count` meshes and upload them as ONE patch mesh: their disjoint
+ * union, whose patches are the concatenation of every mesh's patches, so any
+ * entry point below runs on all meshes in a single launch. Mesh k owns the
+ * contiguous id ranges reported by rxm_batch_offsets. No reference
+ * counterpart (the reference builds one RXMeshStatic per mesh); meant for
+ * batched workloads of many uneven meshes. num_vertices may be NULL. */
+rxm_status rxm_mesh_create_batch(const uint32_t* const* fvs, const uint32_t* num_faces,
+                                 const uint32_t* num_vertices, uint32_t count, uint32_t patch_size,
+                                 uint32_t flags, rxm_mesh** out);
+/* Same union built from meshes already created (their host data is
+ * reused; the parts stay valid and independent). */
+rxm_status rxm_mesh_batch_from(const rxm_mesh* const* meshes, uint32_t count, uint32_t flags,
+                               rxm_mesh** out);
+/* Number of meshes in a batch (1 for a mesh made by rxm_mesh_create). */
+rxm_status rxm_batch_count(const rxm_mesh* mesh, uint32_t* count);
+/* count + 1 offsets of mesh k's ids: kind 0 vertices, 1 edges, 2 faces
+ * (global = linear ranges), 3 patches. */
+rxm_status rxm_batch_offsets(const rxm_mesh* mesh, int kind, uint32_t* out, uint32_t count);
 /* Frees host and device state (reference ~RXMesh, rxmesh.cpp:227-288). */
 rxm_status rxm_mesh_destroy(rxm_mesh* mesh);
 rxm_status rxm_mesh_info(const rxm_mesh* mesh, rxm_mesh_info_t* info);
@@ -139,8 +158,11 @@ rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float*
  * patch-local u16 encoding for iterative consumers. rxm_laplacian_smooth
  * builds it on first use; call this to keep the build out of a timed loop. */
 rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
-/* Workgroup size of a launch family (kernel 0 = smoothing): 64, 128, 256.
- * Used by the autotuner. */
+/* Launch configuration knobs (used by the autotuner):
+ *   kernel 0: smoothing workgroup size 64 / 128 / 256;
+ *   kernel 1: symmetric-Dirichlet step workgroup size 128 / 256;
+ *   kernel 2: symmetric-Dirichlet step launch, 0 = one workgroup per patch,
+ *             1 = persistent grid over a global patch work queue. */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 
 /* `iters` Jacobi steps of uniform Laplacian smoothing
@@ -196,6 +218,36 @@ rxm_status rxm_symdir_energy(rxm_mesh* mesh, const float* Mr, const float* uv, f
  * diff/diff_query_kernel.cuh:199-330). */
 rxm_status rxm_symdir_grad(rxm_mesh* mesh, const float* Mr, const float* uv, const float* scale,
                            float* grad, void* stream);
+
+/* Symmetric Dirichlet with 3-D positions as the variable (closed surfaces
+ * have no UV domain): J = [b-a, c-a] Mr^{-1} is 3x2 and |J^{-1}|^2 becomes
+ * |J|^2 / det(J^T J); equals the 2-D energy for planar input. Mr from
+ * rxm_symdir_rest on the rest positions. Per-face energies into energy[F]. */
+rxm_status rxm_symdir3_energy(rxm_mesh* mesh, const float* Mr, const float* x, float* energy,
+                              void* stream);
+/* x_out = x - eta * grad (sum_f E_f)(x), adjoint gathered per vertex over VF
+ * without atomics (x, x_out: 3 x V SoA, distinct). */
+rxm_status rxm_symdir3_grad_step(rxm_mesh* mesh, const float* Mr, const float* x, float eta,
+                                 float* x_out, void* stream);
+/* `steps` iterations of { x1 = one uniform smoothing step of x (lambda);
+ * x = x1 - eta grad E(x1) } in place on x, `tmp` a 3V workspace. Two launches
+ * per step over all patches (all meshes of a batch). */
+rxm_status rxm_smooth_symdir_steps(rxm_mesh* mesh, const float* Mr, float* x, float* tmp,
+                                   float lambda, float eta, int steps, void* stream);
+/* The same loop as plain per-mesh launches over `count` separate meshes (the
+ * baseline the batched form is measured against). */
+rxm_status rxm_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* const* Mr,
+                                        float* const* x, float* const* tmp, uint32_t count,
+                                        float lambda, float eta, int steps, void* stream);
+/* Capture rxm_smooth_symdir_steps into a hipGraph (no memset nodes; the VV
+ * plan and the work-queue counter are prepared before capture) and replay
+ * it with rxm_graph_launch. Pointers are baked into the graph. The
+ * reference's only graph use is a CUDA graph with a conditional node in its
+ * Geodesic app (apps/Geodesic/geodesic_ptp_rxmesh_graph.h:100-175). */
+rxm_status rxm_graph_capture_smooth_symdir(rxm_mesh* mesh, const float* Mr, float* x, float* tmp,
+                                           float lambda, float eta, int steps, rxm_graph** out);
+rxm_status rxm_graph_launch(rxm_graph* graph, void* stream);
+rxm_status rxm_graph_destroy(rxm_graph* graph);
 
 /* Permute a vertex/edge/face attribute (kind 0/1/2) of ncomp SoA components
  * between global and linear order on device (reference: attribute copy by

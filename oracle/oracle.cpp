@@ -1015,4 +1015,89 @@ void ora_symdir_grad_closed(const ora_mesh* h, const double* Mr, const double* u
     }
 }
 
+// ---------------------------------------------------------------------------
+// Symmetric Dirichlet with 3-D positions as the variable (closed surfaces have
+// no UV domain): J = [b-a, c-a] Mr^{-1} is 3x2, |J^{-1}|^2 becomes
+// |J|^2 / det(J^T J) with det(J^T J) = |J0 x J1|^2, E = A |J|^2 (1 + 1/G).
+// FLT_MAX when G <= 0. This extends apps/Param/param.cu:145-184 (2-D UV) to a
+// surface in 3-D; it reduces to the 2-D energy when the surface is planar.
+static double symdir3_face(const double* a, const double* b, const double* c, const double* R, double* g)
+{
+    double M[3][2];
+    for (int k = 0; k < 3; ++k) {
+        M[k][0] = b[k] - a[k];
+        M[k][1] = c[k] - a[k];
+    }
+    const double dr = R[0] * R[3] - R[2] * R[1];
+    const double A  = 0.5 * dr;
+    const double i00 = R[3] / dr, i01 = -R[2] / dr, i10 = -R[1] / dr, i11 = R[0] / dr;
+    double J0[3], J1[3];
+    for (int k = 0; k < 3; ++k) {
+        J0[k] = M[k][0] * i00 + M[k][1] * i10;
+        J1[k] = M[k][0] * i01 + M[k][1] * i11;
+    }
+    const double a00 = J0[0] * J0[0] + J0[1] * J0[1] + J0[2] * J0[2];
+    const double a11 = J1[0] * J1[0] + J1[1] * J1[1] + J1[2] * J1[2];
+    const double a01 = J0[0] * J1[0] + J0[1] * J1[1] + J0[2] * J1[2];
+    const double n2 = a00 + a11, G = a00 * a11 - a01 * a01;
+    if (G <= 0.0) {
+        if (g)
+            for (int i = 0; i < 9; ++i) g[i] = 0.0;
+        return static_cast<double>(FLT_MAX);
+    }
+    if (g) {
+        // dE/dJ = A [ 2 (1 + 1/G) J - n2/G^2 dG/dJ ]
+        const double s1 = 2.0 * (1.0 + 1.0 / G), s2 = -n2 / (G * G);
+        double D0[3], D1[3];
+        for (int k = 0; k < 3; ++k) {
+            const double dG0 = 2.0 * a11 * J0[k] - 2.0 * a01 * J1[k];
+            const double dG1 = 2.0 * a00 * J1[k] - 2.0 * a01 * J0[k];
+            const double E0 = A * (s1 * J0[k] + s2 * dG0), E1 = A * (s1 * J1[k] + s2 * dG1);
+            // dE/dM = dE/dJ R^T with R = Mr^{-1}
+            D0[k] = E0 * i00 + E1 * i01;
+            D1[k] = E0 * i10 + E1 * i11;
+        }
+        for (int k = 0; k < 3; ++k) {
+            g[3 + k] = D0[k];
+            g[6 + k] = D1[k];
+            g[k]     = -(D0[k] + D1[k]);
+        }
+    }
+    return A * n2 * (1.0 + 1.0 / G);
+}
+
+void ora_symdir3_energy(const ora_mesh* h, const double* Mr, const double* x, double* energy)
+{
+    const Mesh& m = *reinterpret_cast<const Mesh*>(h);
+    for (uint32_t f = 0; f < m.nF; ++f)
+        energy[f] = symdir3_face(x + 3 * m.fv[f][0], x + 3 * m.fv[f][1], x + 3 * m.fv[f][2], Mr + 4 * f, nullptr);
+}
+
+void ora_symdir3_grad(const ora_mesh* h, const double* Mr, const double* x, double* grad)
+{
+    const Mesh& m = *reinterpret_cast<const Mesh*>(h);
+    std::fill(grad, grad + 3 * m.nV, 0.0);
+    for (uint32_t f = 0; f < m.nF; ++f) {
+        double g[9];
+        symdir3_face(x + 3 * m.fv[f][0], x + 3 * m.fv[f][1], x + 3 * m.fv[f][2], Mr + 4 * f, g);
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) grad[3 * m.fv[f][i] + k] += g[3 * i + k];
+    }
+}
+
+// One step of the batched workload on one mesh: x1 = uniform smoothing of
+// x (one Jacobi step, lambda), x2 = x1 - eta * grad E_symdir3(x1).
+void ora_smooth_symdir_step(const ora_mesh* h, const double* Mr, const double* x, double lambda, double eta,
+                            int steps, double* out)
+{
+    const Mesh& m = *reinterpret_cast<const Mesh*>(h);
+    std::vector<double> cur(x, x + 3 * m.nV), sm(3 * m.nV), g(3 * m.nV);
+    for (int it = 0; it < steps; ++it) {
+        ora_smooth_uniform(h, cur.data(), lambda, 1, sm.data());
+        ora_symdir3_grad(h, Mr, sm.data(), g.data());
+        for (size_t i = 0; i < cur.size(); ++i) cur[i] = sm[i] - eta * g[i];
+    }
+    std::copy(cur.begin(), cur.end(), out);
+}
+
 }  // extern "C"

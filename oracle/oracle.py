@@ -65,6 +65,10 @@ def lib():
         L.ora_symdir_energy.argtypes = [vp, _f64p, _f64p, _f64p]
         L.ora_symdir_grad_dual.argtypes = [vp, _f64p, _f64p, _f64p]
         L.ora_symdir_grad_closed.argtypes = [vp, _f64p, _f64p, _f64p]
+        L.ora_symdir3_energy.argtypes = [vp, _f64p, _f64p, _f64p]
+        L.ora_symdir3_grad.argtypes = [vp, _f64p, _f64p, _f64p]
+        L.ora_smooth_symdir_step.argtypes = [vp, _f64p, _f64p, ctypes.c_double, ctypes.c_double,
+                                             ctypes.c_int, _f64p]
         _lib = L
     return _lib
 
@@ -258,6 +262,30 @@ class OracleMesh:
         fn = lib().ora_symdir_grad_closed if method == "closed" else lib().ora_symdir_grad_dual
         fn(self._h, _c(Mr, np.float64).ravel(), _c(uv, np.float64).ravel(), out.reshape(-1))
         return out
+
+
+def _symdir3(self, Mr, x):
+    out = np.zeros(self.nF, np.float64)
+    lib().ora_symdir3_energy(self._h, _c(Mr, np.float64).ravel(), _c(x, np.float64).ravel(), out)
+    return out
+
+
+def _symdir3_grad(self, Mr, x):
+    out = np.zeros((self.nV, 3), np.float64)
+    lib().ora_symdir3_grad(self._h, _c(Mr, np.float64).ravel(), _c(x, np.float64).ravel(), out.reshape(-1))
+    return out
+
+
+def _smooth_symdir_steps(self, Mr, x, lam, eta, steps):
+    out = np.zeros((self.nV, 3), np.float64)
+    lib().ora_smooth_symdir_step(self._h, _c(Mr, np.float64).ravel(), _c(x, np.float64).ravel(), lam, eta,
+                                 steps, out.reshape(-1))
+    return out
+
+
+OracleMesh.symdir3_energy = _symdir3
+OracleMesh.symdir3_grad = _symdir3_grad
+OracleMesh.smooth_symdir_steps = _smooth_symdir_steps
 
 
 def read_patch_archive(path):

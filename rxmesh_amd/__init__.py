@@ -60,6 +60,17 @@ _SIGS = {
     "rxm_mesh_device_view": [_vp, _vp, ctypes.c_uint64],
     "rxm_vv_plan_build": [_vp, _vp],
     "rxm_set_launch_threads": [_vp, _i, _i],
+    "rxm_mesh_create_batch": [_vp, _vp, _vp, _u32, _u32, _u32, ctypes.POINTER(_vp)],
+    "rxm_batch_count": [_vp, ctypes.POINTER(_u32)],
+    "rxm_mesh_batch_from": [_vp, _u32, _u32, ctypes.POINTER(_vp)],
+    "rxm_batch_offsets": [_vp, _i, _vp, _u32],
+    "rxm_symdir3_energy": [_vp, _vp, _vp, _vp, _vp],
+    "rxm_symdir3_grad_step": [_vp, _vp, _vp, _f, _vp, _vp],
+    "rxm_smooth_symdir_steps": [_vp, _vp, _vp, _vp, _f, _f, _i, _vp],
+    "rxm_smooth_symdir_steps_each": [_vp, _vp, _vp, _vp, _u32, _f, _f, _i, _vp],
+    "rxm_graph_capture_smooth_symdir": [_vp, _vp, _vp, _vp, _f, _f, _i, ctypes.POINTER(_vp)],
+    "rxm_graph_launch": [_vp, _vp],
+    "rxm_graph_destroy": [_vp],
     "rxm_cot_nnz": [_vp, ctypes.POINTER(ctypes.c_int64)],
     "rxm_cot_csr_assemble": [_vp, _vp, _f, _vp, _vp, _vp, _vp, _vp],
     "rxm_spmv3": [_vp, _vp, _vp, _vp, _vp, _vp, _vp],
@@ -112,20 +123,36 @@ class Mesh:
     (include/rxmesh/rxmesh_static.h:61-100)."""
 
     def __init__(self, fv, num_vertices=0, patch_size=512, patch_file=None, face_patch=None,
-                 host_only=False):
-        fv = np.ascontiguousarray(fv, dtype=np.uint32).reshape(-1, 3)
-        self._fv = fv
+                 host_only=False, batch=None, parts=None):
+        """fv: F x 3 triangles. Alternatives: batch = list of fv arrays (their
+        disjoint union, built here), parts = list of built Mesh objects
+        (their union, reusing the host build)."""
         h = ctypes.c_void_p()
         flags = RXM_HOST_ONLY if host_only else 0
-        if face_patch is not None:
-            fp = np.ascontiguousarray(face_patch, dtype=np.uint32)
-            P = int(fp.max()) + 1
-            rc = lib().rxm_mesh_create_with_patches(fv.ctypes.data, fv.shape[0], num_vertices,
-                                                    fp.ctypes.data, P, flags, ctypes.byref(h))
+        self._fv = None
+        if parts is not None:
+            self._parts = list(parts)
+            arr = (ctypes.c_void_p * len(parts))(*[q._h.value for q in parts])
+            rc = lib().rxm_mesh_batch_from(arr, len(parts), flags, ctypes.byref(h))
+        elif batch is not None:
+            fvs = [np.ascontiguousarray(f, dtype=np.uint32).reshape(-1, 3) for f in batch]
+            self._fv = fvs
+            ptrs = (ctypes.c_void_p * len(fvs))(*[f.ctypes.data for f in fvs])
+            nfs = np.array([f.shape[0] for f in fvs], np.uint32)
+            nvs = np.array([int(f.max()) + 1 for f in fvs], np.uint32)
+            rc = lib().rxm_mesh_create_batch(ptrs, nfs.ctypes.data, nvs.ctypes.data, len(fvs), patch_size,
+                                             flags, ctypes.byref(h))
         else:
-            pf = patch_file.encode() if patch_file else None
-            rc = lib().rxm_mesh_create(fv.ctypes.data, fv.shape[0], num_vertices, patch_size, pf, flags,
-                                       ctypes.byref(h))
+            fv = np.ascontiguousarray(fv, dtype=np.uint32).reshape(-1, 3)
+            self._fv = fv
+            if face_patch is not None:
+                fp = np.ascontiguousarray(face_patch, dtype=np.uint32)
+                rc = lib().rxm_mesh_create_with_patches(fv.ctypes.data, fv.shape[0], num_vertices, fp.ctypes.data,
+                                                        int(fp.max()) + 1, flags, ctypes.byref(h))
+            else:
+                pf = patch_file.encode() if patch_file else None
+                rc = lib().rxm_mesh_create(fv.ctypes.data, fv.shape[0], num_vertices, patch_size, pf, flags,
+                                           ctypes.byref(h))
         _check(rc, "rxm_mesh_create")
         self._h = h
         self.info = self._info()
@@ -219,6 +246,43 @@ class Mesh:
                                               _stream(stream)), "rxm_laplacian_smooth_csr")
         return out
 
+    # -- batches ----------------------------------------------------------
+    def batch_count(self):
+        n = ctypes.c_uint32()
+        _check(lib().rxm_batch_count(self._h, ctypes.byref(n)), "rxm_batch_count")
+        return n.value
+
+    def batch_offsets(self, kind=0):
+        k = self.batch_count()
+        out = np.zeros(k + 1, np.uint32)
+        _check(lib().rxm_batch_offsets(self._h, {"V": 0, "E": 1, "F": 2, "P": 3}.get(kind, kind),
+                                       out.ctypes.data, k + 1), "rxm_batch_offsets")
+        return out
+
+    def symdir3_energy(self, Mr, x, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty(self.nF, dtype=torch.float32, device="cuda")
+        _check(lib().rxm_symdir3_energy(self._h, _ptr(Mr), _ptr(x), _ptr(out), _stream(stream)),
+               "rxm_symdir3_energy")
+        return out
+
+    def symdir3_grad_step(self, Mr, x, eta, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        _check(lib().rxm_symdir3_grad_step(self._h, _ptr(Mr), _ptr(x), float(eta), _ptr(out), _stream(stream)),
+               "rxm_symdir3_grad_step")
+        return out
+
+    def smooth_symdir_steps(self, Mr, x, tmp, lam, eta, steps, stream=None):
+        _check(lib().rxm_smooth_symdir_steps(self._h, _ptr(Mr), _ptr(x), _ptr(tmp), float(lam), float(eta),
+                                             int(steps), _stream(stream)), "rxm_smooth_symdir_steps")
+        return x
+
+    def capture_smooth_symdir(self, Mr, x, tmp, lam, eta, steps):
+        return Graph(self, Mr, x, tmp, lam, eta, steps)
+
     # -- cotangent system / explicit MCF ------------------------------
     def cot_nnz(self):
         n = ctypes.c_int64()
@@ -299,6 +363,42 @@ class Mesh:
         ncomp = x.shape[0] if x.dim() == 2 else 1
         _check(lib().rxm_to_global(self._h, kind, ncomp, _ptr(x), _ptr(out), _stream(stream)), "rxm_to_global")
         return out
+
+
+class Graph:
+    """A hipGraph of the smoothing + symmetric-Dirichlet step loop
+    (rxm_graph_capture_smooth_symdir); the tensors are baked in."""
+
+    def __init__(self, mesh, Mr, x, tmp, lam, eta, steps):
+        self._keep = (mesh, Mr, x, tmp)
+        g = ctypes.c_void_p()
+        _check(lib().rxm_graph_capture_smooth_symdir(mesh._h, _ptr(Mr), _ptr(x), _ptr(tmp), float(lam), float(eta),
+                                                     int(steps), ctypes.byref(g)), "rxm_graph_capture_smooth_symdir")
+        self._g = g
+
+    def launch(self, stream=None):
+        _check(lib().rxm_graph_launch(self._g, _stream(stream)), "rxm_graph_launch")
+
+    def close(self):
+        if getattr(self, "_g", None) is not None and self._g.value:
+            _check(lib().rxm_graph_destroy(self._g), "rxm_graph_destroy")
+        self._g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def smooth_symdir_steps_each(meshes, Mrs, xs, tmps, lam, eta, steps, stream=None):
+    """Plain per-mesh launches of the step loop over separate meshes."""
+    n = len(meshes)
+    arr = lambda items: (ctypes.c_void_p * n)(*items)  # noqa: E731
+    _check(lib().rxm_smooth_symdir_steps_each(arr([m._h.value for m in meshes]), arr([t.data_ptr() for t in Mrs]),
+                                              arr([t.data_ptr() for t in xs]), arr([t.data_ptr() for t in tmps]),
+                                              n, float(lam), float(eta), int(steps), _stream(stream)),
+           "rxm_smooth_symdir_steps_each")
 
 
 class SymmetricDirichlet:

@@ -1,0 +1,104 @@
+"""Launch-configuration autotuner.
+
+Times the candidate launch configurations of a mesh (workgroup sizes of the
+smoothing and symmetric-Dirichlet kernels, grid vs persistent work-queue
+launch of the gradient step) on the live GPU with HIP events and keeps the
+fastest. Results are cached in a JSON file keyed by the mesh's shape (patch
+count and largest patch), so a later run on a mesh of the same shape reuses
+them without timing. Patch size is a build parameter: tune_patch_size()
+rebuilds the mesh for each candidate and times the same workload.
+"""
+import itertools
+import json
+import os
+
+CACHE_PATH = os.environ.get("RXM_AUTOTUNE_CACHE",
+                            os.path.join(os.path.expanduser("~"), ".cache", "rxmesh_amd_autotune.json"))
+
+SMOOTH_THREADS = (64, 128, 256)
+GRAD_THREADS = (128, 256)
+GRAD_QUEUE = (0, 1)
+
+
+def _key(mesh, what):
+    i = mesh.info
+    return "%s:P%d:V%d:nv%d:ne%d:nf%d" % (what, i["num_patches"], i["num_vertices"], i["max_vertices_per_patch"],
+                                          i["max_edges_per_patch"], i["max_faces_per_patch"])
+
+
+def _load():
+    try:
+        with open(CACHE_PATH) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _store(cache):
+    try:
+        os.makedirs(os.path.dirname(CACHE_PATH), exist_ok=True)
+        with open(CACHE_PATH, "w") as fh:
+            json.dump(cache, fh, indent=1, sort_keys=True)
+    except OSError:
+        pass
+
+
+def _time_ms(fn, reps):
+    import torch
+    s = torch.cuda.current_stream()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fn()
+    a.record(s)
+    for _ in range(reps):
+        fn()
+    b.record(s)
+    b.synchronize()
+    return a.elapsed_time(b) / reps
+
+
+def apply(mesh, cfg):
+    mesh.set_launch_threads(0, cfg["smooth_threads"])
+    mesh.set_launch_threads(1, cfg["grad_threads"])
+    mesh.set_launch_threads(2, cfg["grad_queue"])
+
+
+def tune_smooth_symdir(mesh, Mr, x, tmp, lam, eta, steps=2, reps=3, use_cache=True):
+    """Pick the launch configuration of the smoothing + symmetric-Dirichlet
+    step loop for `mesh`; returns (config, timings)."""
+    key = _key(mesh, "smooth_symdir")
+    cache = _load() if use_cache else {}
+    if key in cache:
+        apply(mesh, cache[key])
+        return cache[key], {}
+    x0 = x.clone()
+    timings = {}
+    best, best_t = None, float("inf")
+    for st, gt, q in itertools.product(SMOOTH_THREADS, GRAD_THREADS, GRAD_QUEUE):
+        cfg = {"smooth_threads": st, "grad_threads": gt, "grad_queue": q}
+        apply(mesh, cfg)
+        t = _time_ms(lambda: mesh.smooth_symdir_steps(Mr, x, tmp, lam, eta, steps), reps)
+        timings["%d/%d/%d" % (st, gt, q)] = t
+        if t < best_t:
+            best, best_t = cfg, t
+    x.copy_(x0)
+    apply(mesh, best)
+    cache[key] = best
+    if use_cache:
+        _store(cache)
+    return best, timings
+
+
+def tune_patch_size(fv, make_inputs, run, sizes=(256, 512, 1024)):
+    """Rebuild the mesh for each candidate patch size and time run(mesh,
+    *make_inputs(mesh)); returns (best size, {size: ms})."""
+    import rxmesh_amd as rx
+    out = {}
+    for ps in sizes:
+        try:
+            m = rx.Mesh(fv, patch_size=ps)
+        except rx.RXMError:
+            continue
+        args = make_inputs(m)
+        out[ps] = _time_ms(lambda: run(m, *args), 3)
+    best = min(out, key=out.get)
+    return best, out

@@ -1,6 +1,9 @@
 // C-ABI implementation: mesh lifetime, device upload, kernel launches.
 #include <hip/hip_runtime.h>
 
+#include <chrono>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -35,6 +38,18 @@ struct rxm_mesh
     uint16_t*       d_vv_nbr   = nullptr;
     uint32_t        vv_max_nent = 0;
     uint32_t        smooth_threads = 128;
+    uint32_t        grad_threads   = 256;
+    uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
+    uint32_t*       d_queue        = nullptr;  // work-queue ticket counter
+    int             num_cus        = 0;
+    // batch (disjoint union of meshes): per-mesh id / patch offsets
+    std::vector<uint32_t> b_v, b_e, b_f, b_p;
+};
+
+struct rxm_graph
+{
+    hipGraph_t     graph = nullptr;
+    hipGraphExec_t exec  = nullptr;
 };
 
 namespace {
@@ -236,6 +251,25 @@ int ensure_vv_plan(rxm_mesh* m, void* stream)
     return RXM_OK;
 }
 
+int smooth_once(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
+{
+    Bump b(nullptr);
+    plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent);
+    const uint16_t* ro = m->d_vv_roff;
+    const uint16_t* nb = m->d_vv_nbr;
+    switch (m->smooth_threads) {
+        case 64:
+            return launch_patches_xcd(k_smooth_local<64>, 64, b.bytes(), stream, m, m->dm, m->d_vv_rbase,
+                                      m->d_vv_nbase, ro, nb, lambda, src, dst);
+        case 256:
+            return launch_patches_xcd(k_smooth_local<256>, 256, b.bytes(), stream, m, m->dm, m->d_vv_rbase,
+                                      m->d_vv_nbase, ro, nb, lambda, src, dst);
+        default:
+            return launch_patches_xcd(k_smooth_local<128>, 128, b.bytes(), stream, m, m->dm, m->d_vv_rbase,
+                                      m->d_vv_nbase, ro, nb, lambda, src, dst);
+    }
+}
+
 bool valid_op(int op)
 {
     return op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF || op == RXM_OP_FV ||
@@ -289,6 +323,88 @@ rxm_status rxm_mesh_create_with_patches(const uint32_t* fv, uint32_t num_faces, 
     }
     m->h.patch_size = 0;
     return finish_create(m, flags, out);
+}
+
+rxm_status rxm_mesh_create_batch(const uint32_t* const* fvs, const uint32_t* num_faces,
+                                 const uint32_t* num_vertices, uint32_t count, uint32_t patch_size, uint32_t flags,
+                                 rxm_mesh** out)
+{
+    if (!out || !fvs || !num_faces || count == 0) return RXM_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::vector<std::unique_ptr<HostMesh>> parts(count);
+    int rc = RXM_OK;
+    auto t0 = std::chrono::steady_clock::now();
+    for (uint32_t k = 0; k < count && rc == RXM_OK; ++k) {
+        parts[k].reset(new (std::nothrow) HostMesh());
+        if (!parts[k]) return RXM_ERR_OUT_OF_MEMORY;
+        rc = build_supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0);
+        if (rc == RXM_OK) rc = run_patcher(*parts[k], patch_size);
+        if (rc == RXM_OK) rc = build_patches(*parts[k]);
+    }
+    if (rc != RXM_OK) return rc;
+    rxm_mesh* m = new (std::nothrow) rxm_mesh();
+    if (!m) return RXM_ERR_OUT_OF_MEMORY;
+    std::vector<const HostMesh*> ptrs;
+    for (auto& q : parts) ptrs.push_back(q.get());
+    auto t1 = std::chrono::steady_clock::now();
+    rc = concat_meshes(ptrs, m->h, m->b_v, m->b_e, m->b_f, m->b_p);
+    auto t2 = std::chrono::steady_clock::now();
+    parts.clear();
+    auto t3 = std::chrono::steady_clock::now();
+    if (std::getenv("RXM_TIMING"))
+        fprintf(stderr, "batch: build %.3f s, concat %.3f s, free %.3f s\n",
+                std::chrono::duration<double>(t1 - t0).count(), std::chrono::duration<double>(t2 - t1).count(),
+                std::chrono::duration<double>(t3 - t2).count());
+    if (rc == RXM_OK && !(flags & RXM_HOST_ONLY)) rc = upload_mesh(m);
+    if (rc != RXM_OK) {
+        rxm_mesh_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return RXM_OK;
+}
+
+rxm_status rxm_mesh_batch_from(const rxm_mesh* const* meshes, uint32_t count, uint32_t flags, rxm_mesh** out)
+{
+    if (!out || !meshes || count == 0) return RXM_ERR_INVALID_ARG;
+    *out = nullptr;
+    std::vector<const HostMesh*> ptrs;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (!meshes[k] || !meshes[k]->b_v.empty()) return RXM_ERR_INVALID_ARG;  // no nested batches
+        ptrs.push_back(&meshes[k]->h);
+    }
+    rxm_mesh* m = new (std::nothrow) rxm_mesh();
+    if (!m) return RXM_ERR_OUT_OF_MEMORY;
+    int rc = concat_meshes(ptrs, m->h, m->b_v, m->b_e, m->b_f, m->b_p);
+    if (rc == RXM_OK && !(flags & RXM_HOST_ONLY)) rc = upload_mesh(m);
+    if (rc != RXM_OK) {
+        rxm_mesh_destroy(m);
+        return rc;
+    }
+    *out = m;
+    return RXM_OK;
+}
+
+rxm_status rxm_batch_offsets(const rxm_mesh* m, int kind, uint32_t* out, uint32_t count)
+{
+    if (!m || !out || kind < 0 || kind > 3) return RXM_ERR_INVALID_ARG;
+    const std::vector<uint32_t>& v = (kind == 0) ? m->b_v : (kind == 1) ? m->b_e : (kind == 2) ? m->b_f : m->b_p;
+    if (v.empty()) {  // a single mesh is a batch of one
+        if (count < 2) return RXM_ERR_INVALID_ARG;
+        out[0] = 0;
+        out[1] = (kind == 0) ? m->h.nV : (kind == 1) ? m->h.nE : (kind == 2) ? m->h.nF : m->h.P;
+        return RXM_OK;
+    }
+    if (count < v.size()) return RXM_ERR_INVALID_ARG;
+    std::memcpy(out, v.data(), v.size() * sizeof(uint32_t));
+    return RXM_OK;
+}
+
+rxm_status rxm_batch_count(const rxm_mesh* m, uint32_t* count)
+{
+    if (!m || !count) return RXM_ERR_INVALID_ARG;
+    *count = m->b_v.empty() ? 1u : static_cast<uint32_t>(m->b_v.size() - 1);
+    return RXM_OK;
 }
 
 rxm_status rxm_mesh_destroy(rxm_mesh* m)
@@ -494,27 +610,10 @@ rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
     int rc = ensure_vv_plan(m, stream);
     if (rc) return rc;
-    Bump b(nullptr);
-    plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent);
     const float* src = pos;
     for (int it = 0; it < iters; ++it) {
         float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
-        switch (m->smooth_threads) {
-            case 64:
-                rc = launch_patches_xcd(k_smooth_local<64>, 64, b.bytes(), stream, m, m->dm, m->d_vv_rbase, m->d_vv_nbase,
-                                    static_cast<const uint16_t*>(m->d_vv_roff), static_cast<const uint16_t*>(m->d_vv_nbr),
-                                    lambda, src, dst);
-                break;
-            case 256:
-                rc = launch_patches_xcd(k_smooth_local<256>, 256, b.bytes(), stream, m, m->dm, m->d_vv_rbase, m->d_vv_nbase,
-                                    static_cast<const uint16_t*>(m->d_vv_roff), static_cast<const uint16_t*>(m->d_vv_nbr),
-                                    lambda, src, dst);
-                break;
-            default:
-                rc = launch_patches_xcd(k_smooth_local<128>, 128, b.bytes(), stream, m, m->dm, m->d_vv_rbase, m->d_vv_nbase,
-                                    static_cast<const uint16_t*>(m->d_vv_roff), static_cast<const uint16_t*>(m->d_vv_nbr),
-                                    lambda, src, dst);
-        }
+        rc = smooth_once(m, lambda, src, dst, stream);
         if (rc) return rc;
         src = dst;
     }
@@ -523,9 +622,20 @@ rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float
 
 rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
 {
-    if (!m || kernel != 0 || (threads != 64 && threads != 128 && threads != 256)) return RXM_ERR_INVALID_ARG;
-    m->smooth_threads = static_cast<uint32_t>(threads);
-    return RXM_OK;
+    if (!m) return RXM_ERR_INVALID_ARG;
+    if (kernel == 0 && (threads == 64 || threads == 128 || threads == 256)) {
+        m->smooth_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 1 && (threads == 128 || threads == 256)) {
+        m->grad_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 2 && (threads == 0 || threads == 1)) {  // gradient step launch: 0 grid, 1 work queue
+        m->grad_queue = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    return RXM_ERR_INVALID_ARG;
 }
 
 rxm_status rxm_laplacian_smooth_csr(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* pos,
@@ -670,6 +780,147 @@ rxm_status rxm_symdir_grad(rxm_mesh* m, const float* Mr, const float* uv, const 
     Bump b(nullptr);
     plan_sg(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
     return launch_patches(k_symdir_grad, kGeomThreads, b.bytes(), stream, m, m->dm, Mr, uv, scale, grad);
+}
+
+rxm_status rxm_symdir3_energy(rxm_mesh* m, const float* Mr, const float* x, float* energy, void* stream)
+{
+    if (!m || !Mr || !x || !energy) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    Bump b(nullptr);
+    plan_fv(b, m->h.max_ne, m->h.max_nf);
+    return launch_patches(k_symdir3_energy, kGeomThreads, b.bytes(), stream, m, m->dm, Mr, x, energy);
+}
+
+static int launch_symdir3_step(rxm_mesh* m, const float* Mr, float eta, const float* x, float* xo, void* stream)
+{
+    Bump b(nullptr);
+    plan_s3(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
+    const uint32_t lds = b.bytes();
+    if (m->grad_queue) {
+        if (!m->d_queue) {
+            if (hipMalloc(&m->d_queue, 256) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+            m->allocs.push_back(m->d_queue);
+        }
+        if (m->num_cus == 0) RXM_HIP(hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device));
+        hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), m->d_queue);
+        RXM_HIP(hipGetLastError());
+        // enough workgroups to fill the machine; LDS decides residency
+        const uint32_t per_cu = std::max(1u, std::min(8u, (160u * 1024u) / std::max(lds, 1u)));
+        const uint32_t grid   = std::min<uint32_t>(m->h.P, static_cast<uint32_t>(m->num_cus) * per_cu);
+        if (m->grad_threads == 128)
+            return launch_grid(k_symdir3_step_queue<128>, grid, 128, lds, stream, m->dm, m->d_queue, Mr, eta, x, xo);
+        return launch_grid(k_symdir3_step_queue<256>, grid, 256, lds, stream, m->dm, m->d_queue, Mr, eta, x, xo);
+    }
+    if (m->grad_threads == 128)
+        return launch_patches(k_symdir3_step<128>, 128, lds, stream, m, m->dm, Mr, eta, x, xo);
+    return launch_patches(k_symdir3_step<256>, 256, lds, stream, m, m->dm, Mr, eta, x, xo);
+}
+
+rxm_status rxm_symdir3_grad_step(rxm_mesh* m, const float* Mr, const float* x, float eta, float* x_out,
+                                 void* stream)
+{
+    if (!m || !Mr || !x || !x_out || x == x_out) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    return launch_symdir3_step(m, Mr, eta, x, x_out, stream);
+}
+
+// The batched workload step: x1 = smooth(x), x = x1 - eta grad E_symdir3(x1).
+static int smooth_symdir_launches(rxm_mesh* m, const float* Mr, float* x, float* tmp, float lambda, float eta,
+                                  int steps, void* stream)
+{
+    for (int s = 0; s < steps; ++s) {
+        int rc = smooth_once(m, lambda, x, tmp, stream);
+        if (!rc) rc = launch_symdir3_step(m, Mr, eta, tmp, x, stream);
+        if (rc) return rc;
+    }
+    return RXM_OK;
+}
+
+rxm_status rxm_smooth_symdir_steps(rxm_mesh* m, const float* Mr, float* x, float* tmp, float lambda, float eta,
+                                   int steps, void* stream)
+{
+    if (!m || !Mr || !x || !tmp || x == tmp || steps < 1) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    int rc = ensure_vv_plan(m, stream);
+    if (rc) return rc;
+    return smooth_symdir_launches(m, Mr, x, tmp, lambda, eta, steps, stream);
+}
+
+rxm_status rxm_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* const* Mr, float* const* x,
+                                        float* const* tmp, uint32_t count, float lambda, float eta, int steps,
+                                        void* stream)
+{
+    if (!meshes || !Mr || !x || !tmp || count == 0 || steps < 1) return RXM_ERR_INVALID_ARG;
+    for (uint32_t k = 0; k < count; ++k) {
+        if (!meshes[k] || !meshes[k]->on_device) return RXM_ERR_INVALID_ARG;
+        int rc = ensure_vv_plan(meshes[k], stream);
+        if (rc) return rc;
+    }
+    // plain launches: every mesh gets its own two launches per step
+    for (int s = 0; s < steps; ++s)
+        for (uint32_t k = 0; k < count; ++k) {
+            int rc = smooth_symdir_launches(meshes[k], Mr[k], x[k], tmp[k], lambda, eta, 1, stream);
+            if (rc) return rc;
+        }
+    return RXM_OK;
+}
+
+rxm_status rxm_graph_capture_smooth_symdir(rxm_mesh* m, const float* Mr, float* x, float* tmp, float lambda,
+                                           float eta, int steps, rxm_graph** out)
+{
+    if (!m || !Mr || !x || !tmp || !out || x == tmp || steps < 1) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    *out = nullptr;
+    hipStream_t s;
+    RXM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    int rc = ensure_vv_plan(m, s);  // allocation and the plan build stay outside the capture
+    if (!rc && m->grad_queue && !m->d_queue) {
+        if (hipMalloc(&m->d_queue, 256) != hipSuccess) rc = RXM_ERR_OUT_OF_MEMORY;
+        else m->allocs.push_back(m->d_queue);
+    }
+    if (!rc && m->num_cus == 0 &&
+        hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess)
+        rc = RXM_ERR_DEVICE;
+    if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = RXM_ERR_DEVICE;
+    rxm_graph* g = nullptr;
+    if (!rc) {
+        g = new (std::nothrow) rxm_graph();
+        if (!g) rc = RXM_ERR_OUT_OF_MEMORY;
+    }
+    if (!rc) {
+        if (hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal) != hipSuccess) rc = RXM_ERR_DEVICE;
+        else {
+            int lrc = smooth_symdir_launches(m, Mr, x, tmp, lambda, eta, steps, s);
+            hipError_t e = hipStreamEndCapture(s, &g->graph);
+            if (lrc) rc = lrc;
+            else if (e != hipSuccess) rc = RXM_ERR_DEVICE;
+            else if (hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0) != hipSuccess) rc = RXM_ERR_DEVICE;
+        }
+    }
+    if (hipStreamDestroy(s) != hipSuccess && !rc) rc = RXM_ERR_DEVICE;
+    if (rc) {
+        if (g) rxm_graph_destroy(g);
+        return rc;
+    }
+    *out = g;
+    return RXM_OK;
+}
+
+rxm_status rxm_graph_launch(rxm_graph* g, void* stream)
+{
+    if (!g || !g->exec) return RXM_ERR_INVALID_ARG;
+    RXM_HIP(hipGraphLaunch(g->exec, static_cast<hipStream_t>(stream)));
+    return RXM_OK;
+}
+
+rxm_status rxm_graph_destroy(rxm_graph* g)
+{
+    if (!g) return RXM_ERR_INVALID_ARG;
+    int rc = RXM_OK;
+    if (g->exec && hipGraphExecDestroy(g->exec) != hipSuccess) rc = RXM_ERR_DEVICE;
+    if (g->graph && hipGraphDestroy(g->graph) != hipSuccess) rc = RXM_ERR_DEVICE;
+    delete g;
+    return rc;
 }
 
 }  // extern "C"

@@ -629,6 +629,121 @@ int build_patches(HostMesh& m)
 }
 
 // ---------------------------------------------------------------------------
+int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& o, std::vector<uint32_t>& vb,
+                  std::vector<uint32_t>& eb, std::vector<uint32_t>& fb, std::vector<uint32_t>& pb)
+{
+    const size_t K = parts.size();
+    if (K == 0) return kErrInvalidArg;
+    vb.assign(K + 1, 0); eb.assign(K + 1, 0); fb.assign(K + 1, 0); pb.assign(K + 1, 0);
+    std::vector<uint32_t> vs(K + 1, 0), es(K + 1, 0), fs(K + 1, 0);  // slab bases
+    std::vector<uint64_t> hb(K + 1, 0), rb(K + 1, 0);                  // half-edge, ribbon bases
+    for (size_t k = 0; k < K; ++k) {
+        const HostMesh& m = *parts[k];
+        uint64_t nv = uint64_t(vb[k]) + m.nV, ne = uint64_t(eb[k]) + m.nE, nf = uint64_t(fb[k]) + m.nF;
+        uint64_t np = uint64_t(pb[k]) + m.P;
+        uint64_t sv = uint64_t(vs[k]) + m.v_off[m.P], se = uint64_t(es[k]) + m.e_off[m.P];
+        uint64_t sf = uint64_t(fs[k]) + m.f_off[m.P];
+        if (nv >= kInvalid32 || ne >= kInvalid32 || 3 * nf >= kInvalid32 || np >= kInvalid32 ||
+            sv >= kInvalid32 || se >= kInvalid32 || 3 * sf >= kInvalid32)
+            return kErrInvalidArg;
+        vb[k + 1] = uint32_t(nv); eb[k + 1] = uint32_t(ne); fb[k + 1] = uint32_t(nf); pb[k + 1] = uint32_t(np);
+        vs[k + 1] = uint32_t(sv); es[k + 1] = uint32_t(se); fs[k + 1] = uint32_t(sf);
+        hb[k + 1] = hb[k] + 3ull * m.nF;
+        rb[k + 1] = rb[k] + m.ribbon_val.size();
+    }
+    o = HostMesh();
+    o.nV = vb[K]; o.nE = eb[K]; o.nF = fb[K]; o.P = pb[K];
+    o.patch_size = parts[0]->patch_size;
+    o.closed = true; o.manifold = true;
+    o.fv.resize(3ull * o.nF); o.fe_g.resize(3ull * o.nF); o.ev.resize(2ull * o.nE);
+    o.vf_off.assign(o.nV + 1, 0); o.vf_val.resize(3ull * o.nF);
+    o.ef_off.assign(o.nE + 1, 0); o.ef_val.resize(3ull * o.nF);
+    o.face_patch.resize(o.nF); o.patches_val.resize(o.nF); o.patches_offset.resize(o.P);
+    o.ribbon_val.resize(rb[K]); o.ribbon_offset.resize(o.P);
+    o.vertex_patch.resize(o.nV); o.edge_patch.resize(o.nE);
+    for (auto* v : {&o.nv, &o.ne, &o.nf, &o.nvo, &o.neo, &o.nfo}) v->resize(o.P);
+    for (auto* v : {&o.v_off, &o.e_off, &o.f_off, &o.vp, &o.ep, &o.fp}) v->assign(o.P + 1, 0);
+    o.ltog_v.resize(vs[K]); o.v_lin.resize(vs[K]); o.v_own.resize(vs[K]);
+    o.ltog_e.resize(es[K]); o.e_lin.resize(es[K]); o.e_own.resize(es[K]); o.ev_local.resize(es[K]);
+    o.ltog_f.resize(fs[K]); o.f_lin.resize(fs[K]); o.f_own.resize(fs[K]); o.fe_local.resize(3ull * fs[K]);
+    o.v_l2g.resize(o.nV); o.v_g2l.resize(o.nV); o.e_l2g.resize(o.nE); o.e_g2l.resize(o.nE);
+    o.f_l2g.resize(o.nF); o.f_g2l.resize(o.nF);
+    for (size_t k = 0; k < K; ++k) {
+        const HostMesh& m = *parts[k];
+        o.closed = o.closed && m.closed;
+        o.manifold = o.manifold && m.manifold;
+        o.max_valence = std::max(o.max_valence, m.max_valence);
+        o.max_edge_faces = std::max(o.max_edge_faces, m.max_edge_faces);
+        o.max_face_adj = std::max(o.max_face_adj, m.max_face_adj);
+        o.max_nv = std::max(o.max_nv, m.max_nv); o.max_ne = std::max(o.max_ne, m.max_ne);
+        o.max_nf = std::max(o.max_nf, m.max_nf); o.max_nvo = std::max(o.max_nvo, m.max_nvo);
+        o.max_neo = std::max(o.max_neo, m.max_neo); o.max_nfo = std::max(o.max_nfo, m.max_nfo);
+    }
+    parallel_for(K, [&](size_t k) {
+        const HostMesh& m = *parts[k];
+        const uint32_t V0 = vb[k], E0 = eb[k], F0 = fb[k], P0 = pb[k];
+        for (uint64_t i = 0; i < 3ull * m.nF; ++i) {
+            o.fv[3ull * F0 + i]     = m.fv[i] + V0;
+            o.fe_g[3ull * F0 + i]   = m.fe_g[i] + E0;
+            o.vf_val[3ull * F0 + i] = m.vf_val[i] + F0;
+            o.ef_val[3ull * F0 + i] = m.ef_val[i] + F0;
+        }
+        for (uint64_t i = 0; i < 2ull * m.nE; ++i) o.ev[2ull * E0 + i] = m.ev[i] + V0;
+        for (uint32_t v = 0; v < m.nV; ++v) {
+            o.vf_off[V0 + v + 1] = m.vf_off[v + 1] + uint32_t(hb[k]);
+            o.vertex_patch[V0 + v] = m.vertex_patch[v] + P0;
+            o.v_l2g[V0 + v] = m.v_l2g[v] + V0;
+            o.v_g2l[V0 + v] = m.v_g2l[v] + V0;
+        }
+        for (uint32_t e = 0; e < m.nE; ++e) {
+            o.ef_off[E0 + e + 1] = m.ef_off[e + 1] + uint32_t(hb[k]);
+            o.edge_patch[E0 + e] = m.edge_patch[e] + P0;
+            o.e_l2g[E0 + e] = m.e_l2g[e] + E0;
+            o.e_g2l[E0 + e] = m.e_g2l[e] + E0;
+        }
+        for (uint32_t f = 0; f < m.nF; ++f) {
+            o.face_patch[F0 + f] = m.face_patch[f] + P0;
+            o.patches_val[F0 + f] = m.patches_val[f] + F0;
+            o.f_l2g[F0 + f] = m.f_l2g[f] + F0;
+            o.f_g2l[F0 + f] = m.f_g2l[f] + F0;
+        }
+        for (size_t i = 0; i < m.ribbon_val.size(); ++i) o.ribbon_val[rb[k] + i] = m.ribbon_val[i] + F0;
+        for (uint32_t p = 0; p < m.P; ++p) {
+            const uint32_t q = P0 + p;
+            o.patches_offset[q] = m.patches_offset[p] + F0;
+            o.ribbon_offset[q] = m.ribbon_offset[p] + uint32_t(rb[k]);
+            o.nv[q] = m.nv[p]; o.ne[q] = m.ne[p]; o.nf[q] = m.nf[p];
+            o.nvo[q] = m.nvo[p]; o.neo[q] = m.neo[p]; o.nfo[q] = m.nfo[p];
+            o.v_off[q] = m.v_off[p] + vs[k]; o.e_off[q] = m.e_off[p] + es[k]; o.f_off[q] = m.f_off[p] + fs[k];
+            o.vp[q] = m.vp[p] + V0; o.ep[q] = m.ep[p] + E0; o.fp[q] = m.fp[p] + F0;
+        }
+        for (uint32_t i = 0; i < m.v_off[m.P]; ++i) {
+            const bool pad = m.ltog_v[i] == kInvalid32;
+            o.ltog_v[vs[k] + i] = pad ? kInvalid32 : m.ltog_v[i] + V0;
+            o.v_lin[vs[k] + i] = pad ? kInvalid32 : m.v_lin[i] + V0;
+            o.v_own[vs[k] + i] = pad ? kInvalid32 : m.v_own[i] + P0;
+        }
+        for (uint32_t i = 0; i < m.e_off[m.P]; ++i) {
+            const bool pad = m.ltog_e[i] == kInvalid32;
+            o.ltog_e[es[k] + i] = pad ? kInvalid32 : m.ltog_e[i] + E0;
+            o.e_lin[es[k] + i] = pad ? kInvalid32 : m.e_lin[i] + E0;
+            o.e_own[es[k] + i] = pad ? kInvalid32 : m.e_own[i] + P0;
+            o.ev_local[es[k] + i] = m.ev_local[i];
+        }
+        for (uint32_t i = 0; i < m.f_off[m.P]; ++i) {
+            const bool pad = m.ltog_f[i] == kInvalid32;
+            o.ltog_f[fs[k] + i] = pad ? kInvalid32 : m.ltog_f[i] + F0;
+            o.f_lin[fs[k] + i] = pad ? kInvalid32 : m.f_lin[i] + F0;
+            o.f_own[fs[k] + i] = pad ? kInvalid32 : m.f_own[i] + P0;
+        }
+        for (uint64_t i = 0; i < 3ull * m.f_off[m.P]; ++i) o.fe_local[3ull * fs[k] + i] = m.fe_local[i];
+    }, 1);
+    o.v_off[o.P] = vs[K]; o.e_off[o.P] = es[K]; o.f_off[o.P] = fs[K];
+    o.vp[o.P] = o.nV; o.ep[o.P] = o.nE; o.fp[o.P] = o.nF;
+    return kOk;
+}
+
+// ---------------------------------------------------------------------------
 // Reference patch archive: cereal PortableBinary of Patcher::serialize
 // (include/rxmesh/patcher/patcher.h:162-182): u8 little-endian flag, nine u32
 // scalars, seven (u64 size + u32[size]) vectors, one f32.

@@ -90,6 +90,14 @@ int read_patch_file(const std::string& path, std::vector<uint32_t>& face_patch,
                     uint32_t& nV, uint32_t& nE, uint32_t& nF);
 int write_patch_file(const std::string& path, const HostMesh& m);
 
+// Disjoint union of already built meshes (a batch): element ids, patch ids,
+// linear ids and slab offsets of mesh k are shifted by the totals of meshes
+// < k, so the union is itself a valid patch mesh whose patches are the
+// concatenation of all meshes' patches. v/e/f_base receive count + 1 linear
+// (= global) id offsets per mesh, p_base the patch offsets.
+int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& out, std::vector<uint32_t>& v_base,
+                  std::vector<uint32_t>& e_base, std::vector<uint32_t>& f_base, std::vector<uint32_t>& p_base);
+
 unsigned num_threads();
 
 }  // namespace rxm

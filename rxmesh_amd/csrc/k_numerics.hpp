@@ -445,4 +445,168 @@ __global__ __launch_bounds__(kGeomThreads) void k_symdir_grad(DevMesh m, const f
     }
 }
 
+// ---- symmetric Dirichlet with 3-D positions as the variable ------------------
+// For a closed surface (no UV domain) the current triangle is compared with
+// its rest shape directly in 3-D: J = [b-a, c-a] Mr^{-1} is 3x2,
+// |J^{-1}|^2 -> |J|^2 / G with G = det(J^T J) = |J0 x J1|^2, so
+// E = A |J|^2 (1 + 1/G). Reduces to symdir_face for planar input.
+// g receives the 9 corner gradients (da, db, dc); FLT_MAX when G <= 0.
+__device__ __forceinline__ float symdir3_face(const f3 a, const f3 b, const f3 c, const float* R, float* g)
+{
+    const f3    e1 = sub(b, a), e2 = sub(c, a);
+    const float dr = det2(R[0], R[3], R[2], R[1]);
+    const float A  = 0.5f * dr;
+    const float i00 = R[3] / dr, i01 = -R[2] / dr, i10 = -R[1] / dr, i11 = R[0] / dr;
+    const f3    J0 = {e1.x * i00 + e2.x * i10, e1.y * i00 + e2.y * i10, e1.z * i00 + e2.z * i10};
+    const f3    J1 = {e1.x * i01 + e2.x * i11, e1.y * i01 + e2.y * i11, e1.z * i01 + e2.z * i11};
+    const float a00 = dot(J0, J0), a11 = dot(J1, J1), a01 = dot(J0, J1);
+    const float n2 = a00 + a11, G = det2(a00, a11, a01, a01);
+    if (G <= 0.f) {
+        for (int i = 0; i < 9; ++i) g[i] = 0.f;
+        return FLT_MAX;
+    }
+    const float iG = 1.0f / G;
+    const float s1 = 2.0f * (1.0f + iG), s2 = -n2 * iG * iG;
+    const float q0[3] = {J0.x, J0.y, J0.z}, q1[3] = {J1.x, J1.y, J1.z};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float dG0 = 2.0f * (a11 * q0[k] - a01 * q1[k]);
+        const float dG1 = 2.0f * (a00 * q1[k] - a01 * q0[k]);
+        const float E0 = A * (s1 * q0[k] + s2 * dG0), E1 = A * (s1 * q1[k] + s2 * dG1);
+        const float D0 = E0 * i00 + E1 * i01, D1 = E0 * i10 + E1 * i11;
+        g[3 + k] = D0;
+        g[6 + k] = D1;
+        g[k]     = -(D0 + D1);
+    }
+    return A * n2 * (1.0f + iG);
+}
+
+struct PlanS3
+{
+    uint32_t *ev, *off, *cur, *slots, *wave;
+    uint16_t *fe, *fvl;
+    float *   x, *y, *z, *gc;
+};
+__host__ __device__ inline PlanS3 plan_s3(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    PlanS3 p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.off   = b.alloc<uint32_t>(nv + 1);
+    p.cur   = b.alloc<uint32_t>(nv + 1);
+    p.slots = b.alloc<uint32_t>(3 * nf);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.fvl   = b.alloc<uint16_t>(3 * nf);
+    p.x     = b.alloc<float>(nv);
+    p.y     = b.alloc<float>(nv);
+    p.z     = b.alloc<float>(nv);
+    p.gc    = b.alloc<float>(9 * nf);
+    return p;
+}
+
+// One gradient-descent step x_out = x - eta * dE/dx on one patch: every
+// local face recomputes its corner gradients into LDS; each owned vertex sums
+// its VF corners in ascending (face, corner) order. No global atomics.
+template <uint32_t T>
+__device__ __forceinline__ void symdir3_step_patch(const DevMesh& m, uint32_t p, const float* __restrict__ Mr,
+                                                   float eta, const float* __restrict__ x, float* __restrict__ xo,
+                                                   char* smem)
+{
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(smem);
+    const PlanS3    L = plan_s3(b, d.nv, d.ne, d.nf);
+    const uint32_t  N = m.nV, F = m.nF;
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    stage_positions(L.x, L.y, L.z, x, N, d, m.v_lin);
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
+        uint32_t l[3];
+        for (uint32_t i = 0; i < 3; ++i) {
+            l[i]             = face_vertex(L.fe, L.ev, f, i);
+            L.fvl[3 * f + i] = static_cast<uint16_t>(l[i]);
+        }
+        const uint32_t lf   = lin_of(f, d.nfo, d.fp, m.f_lin + d.f_off);
+        const float    R[4] = {Mr[lf], Mr[F + lf], Mr[2 * F + lf], Mr[3 * F + lf]};
+        symdir3_face(f3{L.x[l[0]], L.y[l[0]], L.z[l[0]]}, f3{L.x[l[1]], L.y[l[1]], L.z[l[1]]},
+                     f3{L.x[l[2]], L.y[l[2]], L.z[l[2]]}, R, L.gc + 9 * f);
+    }
+    __syncthreads();
+    fill_columns<T>(
+        3 * d.nf, d.nv, [&](uint32_t i) { return static_cast<uint32_t>(L.fvl[i]); }, [&](uint32_t i) { return i; },
+        L.off, L.cur, L.slots, L.wave);
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+        float          gx = 0.f, gy = 0.f, gz = 0.f;
+        sort_row(L.slots + o, k, [&](uint32_t, uint32_t s) {
+            const float* c = L.gc + 9 * (s / 3u) + 3 * (s % 3u);
+            gx += c[0];
+            gy += c[1];
+            gz += c[2];
+        });
+        const uint32_t g = d.vp + v;
+        xo[g]            = L.x[v] - eta * gx;
+        xo[N + g]        = L.y[v] - eta * gy;
+        xo[2 * N + g]    = L.z[v] - eta * gz;
+    }
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_symdir3_step(DevMesh m, const float* __restrict__ Mr, float eta,
+                                                    const float* __restrict__ x, float* __restrict__ xo)
+{
+    extern __shared__ uint4 smem[];
+    symdir3_step_patch<T>(m, blockIdx.x, Mr, eta, x, xo, reinterpret_cast<char*>(smem));
+}
+
+// Persistent form: a grid sized to the machine pulls patch ids from a global
+// work queue (one atomic ticket per patch); every workgroup exits once the
+// queue is drained. The counter is cleared by k_reset_counter before launch.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_symdir3_step_queue(DevMesh m, uint32_t* __restrict__ queue,
+                                                          const float* __restrict__ Mr, float eta,
+                                                          const float* __restrict__ x, float* __restrict__ xo)
+{
+    extern __shared__ uint4 smem[];
+    __shared__ uint32_t s_p;
+    while (true) {
+        if (threadIdx.x == 0) s_p = atomicAdd(queue, 1u);
+        __syncthreads();
+        const uint32_t p = s_p;
+        __syncthreads();
+        if (p >= m.P) break;
+        symdir3_step_patch<T>(m, p, Mr, eta, x, xo, reinterpret_cast<char*>(smem));
+        __syncthreads();
+    }
+}
+
+__global__ void k_reset_counter(uint32_t* c) { c[0] = 0u; }
+
+// Per-face 3-D symmetric Dirichlet energy of owned faces (FLT_MAX if degenerate).
+__global__ __launch_bounds__(kGeomThreads) void k_symdir3_energy(DevMesh m, const float* __restrict__ Mr,
+                                                                 const float* __restrict__ x,
+                                                                 float* __restrict__ energy)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanFV    L = plan_fv(b, d.ne, d.nf);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    const uint32_t N = m.nV, F = m.nF;
+    for (uint32_t f = threadIdx.x; f < d.nfo; f += blockDim.x) {
+        f3 P[3];
+        for (uint32_t i = 0; i < 3; ++i) {
+            const uint32_t g = lin_of(face_vertex(L.fe, L.ev, f, i), d.nvo, d.vp, m.v_lin + d.v_off);
+            P[i] = f3{x[g], x[N + g], x[2 * N + g]};
+        }
+        const uint32_t lf   = d.fp + f;
+        const float    R[4] = {Mr[lf], Mr[F + lf], Mr[2 * F + lf], Mr[3 * F + lf]};
+        float          gdummy[9];
+        energy[lf] = symdir3_face(P[0], P[1], P[2], R, gdummy);
+    }
+}
+
 }  // namespace rxm

@@ -1,0 +1,178 @@
+"""GPU parity of batched execution: a batch is the disjoint union of its
+meshes, run in one launch per kernel (or a persistent work-queue launch), its
+step loop captured in a hipGraph; results must equal per-mesh runs bit for
+bit and the FP64 oracle within tolerance."""
+import numpy as np
+import pytest
+
+from oracle.oracle import OracleMesh
+from rxmesh_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+LAM, ETA = 0.5, 1e-3
+
+
+def _meshes():
+    return [mg.icosphere(3, seed=1, sigma=0.01), mg.grid(20, seed=2), mg.icosphere(4, seed=3, sigma=0.005),
+            mg.disk(24, seed=4), mg.grid(33, seed=5)]
+
+
+_C = {}
+
+
+def setup():
+    if "b" in _C:
+        return _C["b"]
+    import torch
+    import rxmesh_amd as rx
+    ms = _meshes()
+    b = rx.Mesh(None, batch=[f for _, f in ms], patch_size=128)
+    sep = [rx.Mesh(f, patch_size=128) for _, f in ms]
+    ov = b.batch_offsets("V")
+    X = np.concatenate([p for p, _ in ms]).astype(np.float32)
+    l2g = b.array("v_l2g")
+    x = torch.from_numpy(np.ascontiguousarray(X[l2g].T)).cuda()
+    _C["b"] = (ms, b, sep, ov, X, x)
+    return _C["b"]
+
+
+def lin(m, X):
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(X[m.array("v_l2g")].T.astype(np.float32))).cuda()
+
+
+def glob(m, Y):
+    out = np.empty_like(Y)
+    out[:, m.array("v_l2g")] = Y
+    return out
+
+
+def test_union_queries_equal_per_mesh_queries():
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    ofs = {"V": ov, "E": b.batch_offsets("E"), "F": b.batch_offsets("F")}
+    for op in ("VV", "VF", "FF", "EV"):
+        rp, col = (t.cpu().numpy() for t in b.query(op))
+        src, dst = ofs[op[0]], ofs[op[1]]
+        for k, m in enumerate(sep):
+            mrp, mcol = (t.cpu().numpy() for t in m.query(op))
+            r = rp[src[k]:src[k + 1] + 1]
+            assert np.array_equal(r - r[0], mrp)
+            assert np.array_equal(col[r[0]:r[-1]] - dst[k], mcol[: len(mcol)])
+
+
+def test_symdir3_step_vs_fp64():
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    # deform, then one gradient step
+    Xd = (X.astype(np.float64) * 1.05 + 0.01 * np.sin(5 * X)).astype(np.float32)
+    xd = lin(b, Xd)
+    out = b.symdir3_grad_step(Mr, xd, ETA)
+    e = b.symdir3_energy(Mr, xd)
+    torch.cuda.synchronize()
+    fv_all = np.concatenate([f + ov[k] for k, (_, f) in enumerate(ms)])
+    o = OracleMesh(fv_all).set_patches(b.array("face_patch"), b.array("patches_val"), b.array("patches_offset"))
+    gMr = np.empty((4, b.nF), np.float32)
+    gMr[:, b.array("f_l2g")] = Mr.cpu().numpy()
+    g64 = o.symdir3_grad(gMr.T.astype(np.float64), Xd.astype(np.float64))
+    got = glob(b, out.cpu().numpy()).T
+    step = (Xd.astype(np.float64) - got) / ETA
+    assert np.abs(step - g64).max() <= 1e-3 * np.abs(g64).max() + 2e-7 / ETA
+    E64 = o.symdir3_energy(gMr.T.astype(np.float64), Xd.astype(np.float64))
+    gE = np.empty(b.nF, np.float32)
+    gE[b.array("f_l2g")] = e.cpu().numpy()
+    assert np.abs(gE - E64).max() <= 1e-5 * np.abs(E64).max()
+
+
+def test_batched_steps_vs_fp64_oracle():
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    xs = x.clone()
+    tmp = torch.empty_like(xs)
+    b.smooth_symdir_steps(Mr, xs, tmp, LAM, ETA, 5)
+    torch.cuda.synchronize()
+    got = glob(b, xs.cpu().numpy()).T
+    fv_all = np.concatenate([f + ov[k] for k, (_, f) in enumerate(ms)])
+    o = OracleMesh(fv_all).set_patches(b.array("face_patch"), b.array("patches_val"), b.array("patches_offset"))
+    gMr = np.empty((4, b.nF), np.float32)
+    gMr[:, b.array("f_l2g")] = Mr.cpu().numpy()
+    ref = o.smooth_symdir_steps(gMr.T.astype(np.float64), X.astype(np.float64), LAM, ETA, 5)
+    err = np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)
+    assert err.max() <= 1e-5, err.max()
+
+
+def test_batched_equals_plain_per_mesh_bitwise():
+    import torch
+    import rxmesh_amd as rx
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    xb = x.clone()
+    tb = torch.empty_like(xb)
+    b.smooth_symdir_steps(Mr, xb, tb, LAM, ETA, 4)
+    xs, ts, Mrs = [], [], []
+    for k, m in enumerate(sep):
+        xk = lin(m, X[ov[k]:ov[k + 1]])
+        Mrs.append(m.symdir_rest(xk))
+        xs.append(xk.clone())
+        ts.append(torch.empty_like(xk))
+    rx.smooth_symdir_steps_each(sep, Mrs, xs, ts, LAM, ETA, 4)
+    torch.cuda.synchronize()
+    gb = glob(b, xb.cpu().numpy())
+    for k, m in enumerate(sep):
+        assert np.array_equal(gb[:, ov[k]:ov[k + 1]], glob(m, xs[k].cpu().numpy()))
+
+
+def test_graph_replays_match_eager():
+    # every replay checked against the eager loop, with eager work and
+    # synchronisations between replays
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    xe = x.clone()
+    te = torch.empty_like(xe)
+    xg = x.clone()
+    tg = torch.empty_like(xg)
+    g = b.capture_smooth_symdir(Mr, xg, tg, LAM, ETA, 3)
+    for rep in range(4):
+        b.smooth_symdir_steps(Mr, xe, te, LAM, ETA, 3)
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(xg, xe), rep
+        _ = b.query("VV")  # unrelated eager work between replays
+        torch.cuda.synchronize()
+    g.close()
+
+
+def test_work_queue_mode_equals_grid_mode():
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    outs = []
+    for mode, th in ((0, 256), (1, 256), (1, 128), (0, 128)):
+        b.set_launch_threads(2, mode)
+        b.set_launch_threads(1, th)
+        xs = x.clone()
+        tmp = torch.empty_like(xs)
+        b.smooth_symdir_steps(Mr, xs, tmp, LAM, ETA, 3)
+        outs.append(xs)
+    b.set_launch_threads(2, 0)
+    b.set_launch_threads(1, 256)
+    torch.cuda.synchronize()
+    for o in outs[1:]:
+        assert torch.equal(o, outs[0])
+    # graph capture of the work-queue launch (counter reset by a kernel node)
+    b.set_launch_threads(2, 1)
+    xg = x.clone()
+    tg = torch.empty_like(xg)
+    g = b.capture_smooth_symdir(Mr, xg, tg, LAM, ETA, 3)
+    for rep in range(3):
+        xg.copy_(x)
+        torch.cuda.synchronize()
+        g.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(xg, outs[0]), rep
+    g.close()
+    b.set_launch_threads(2, 0)

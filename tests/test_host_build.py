@@ -151,3 +151,30 @@ def test_capi_exports_every_declared_symbol():
     L = ctypes.CDLL(rx.LIB_PATH)
     missing = [n for n in sorted(names) if not hasattr(L, n)]
     assert not missing, missing
+
+
+# ---- batches: the disjoint union of meshes ---------------------------------
+def _batch_meshes():
+    return [mg.icosphere(3, seed=1, sigma=0.01), mg.grid(20, seed=2), mg.icosphere(2, seed=3),
+            mg.disk(24, seed=4)]
+
+
+def test_batch_union_matches_separate_meshes():
+    ms = _batch_meshes()
+    b = rx.Mesh(None, batch=[f for _, f in ms], patch_size=128, host_only=True)
+    sep = [rx.Mesh(f, patch_size=128, host_only=True) for _, f in ms]
+    assert b.batch_count() == len(ms)
+    ov, oe, of, op = (b.batch_offsets(k) for k in "VEFP")
+    assert ov[-1] == b.nV == sum(m.nV for m in sep) and of[-1] == b.nF and op[-1] == b.P
+    pc = b.array("patch_counts").reshape(-1, 6)
+    for k, m in enumerate(sep):
+        assert np.array_equal(b.array("v_l2g")[ov[k]:ov[k + 1]] - ov[k], m.array("v_l2g"))
+        assert np.array_equal(b.array("f_l2g")[of[k]:of[k + 1]] - of[k], m.array("f_l2g"))
+        assert np.array_equal(b.array("face_patch")[of[k]:of[k + 1]] - op[k], m.array("face_patch"))
+        assert np.array_equal(pc[op[k]:op[k + 1]], m.array("patch_counts").reshape(-1, 6))
+    for q in ("VV", "VE", "VF", "FV", "FE", "FF", "EV", "EF"):
+        assert b.query_nnz(q) == sum(m.query_nnz(q) for m in sep)
+    # the union is itself a mesh the oracle accepts, with identical layout
+    fv_all = np.concatenate([f + ov[k] for k, (_, f) in enumerate(ms)])
+    o = OracleMesh(fv_all).set_patches(b.array("face_patch"), b.array("patches_val"), b.array("patches_offset"))
+    compare_with_oracle(b, o, fv_all)
