@@ -99,10 +99,17 @@ def batch_bench(count, steps, reps, patch_size):
     from rxmesh_amd import autotune
     from rxmesh_amd import meshgen as mg
     t0 = time.perf_counter()
+    log = lambda msg: print("[batch %.1fs] %s" % (time.perf_counter() - t0, msg), file=sys.stderr, flush=True)  # noqa: E731
     specs = mg.batch_specs(count=count)
     geo = [mg.make_batch_mesh(s, seed=k) for k, s in enumerate(specs)]
-    parts = [rx.Mesh(f, patch_size=patch_size) for _, f in geo]
+    log("generated %d meshes, %d faces" % (count, sum(len(f) for _, f in geo)))
+    parts = []
+    for k, (_, f) in enumerate(geo):
+        parts.append(rx.Mesh(f, patch_size=patch_size))
+        if (k + 1) % 32 == 0:
+            log("built %d meshes" % (k + 1))
     union = rx.Mesh(None, parts=parts)
+    log("union built: %d patches" % union.P)
     xs = [torch.from_numpy(np.ascontiguousarray(p[m.array("v_l2g")].T.astype(np.float32))).cuda()
           for m, (p, _) in zip(parts, geo)]
     x_u = torch.cat(xs, dim=1).contiguous()
@@ -118,6 +125,7 @@ def batch_bench(count, steps, reps, patch_size):
     x_t = x_u.clone()
     t_t = torch.empty_like(x_t)
     cfg, _ = autotune.tune_smooth_symdir(union, Mr_u, x_t, t_t, lam, eta, use_cache=False)
+    log("autotuned: %s" % cfg)
 
     def timed(fn):
         fn()
@@ -135,7 +143,9 @@ def batch_bench(count, steps, reps, patch_size):
     t_eager = timed(lambda: union.smooth_symdir_steps(Mr_u, xb, tb, lam, eta, steps))
     xp = [x.clone() for x in xs]
     tp = [torch.empty_like(x) for x in xs]
+    log("batched graph %.2f ms, eager %.2f ms" % (t_graph * 1e3, t_eager * 1e3))
     t_plain = timed(lambda: rx.smooth_symdir_steps_each(parts, Mrs, xp, tp, lam, eta, steps))
+    log("plain per-mesh %.2f ms" % (t_plain * 1e3))
     graph.close()
     F = union.nF
     return {"meshes": count, "faces": F, "vertices": union.nV, "patches": union.P, "steps": steps,

@@ -37,6 +37,14 @@ struct rxm_mesh
     uint16_t*       d_vv_roff  = nullptr;
     uint16_t*       d_vv_nbr   = nullptr;
     uint32_t        vv_max_nent = 0;
+    // patch-local face plan (FV table + VF rows of (face << 2 | corner))
+    const uint32_t* d_fp_fbase = nullptr;
+    const uint32_t* d_fp_rbase = nullptr;
+    const uint32_t* d_fp_nbase = nullptr;
+    uint16_t*       d_fp_fvl   = nullptr;
+    uint16_t*       d_fp_roff  = nullptr;
+    uint16_t*       d_fp_vf    = nullptr;
+    uint32_t        fp_max_nent = 0;
     uint32_t        smooth_threads = 128;
     uint32_t        grad_threads   = 256;
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
@@ -248,6 +256,45 @@ int ensure_vv_plan(rxm_mesh* m, void* stream)
         m->d_vv_nbr = nullptr;  // freed with the mesh; rebuild on the next call
         return rc;
     }
+    return RXM_OK;
+}
+
+int ensure_face_plan(rxm_mesh* m, void* stream)
+{
+    if (m->d_fp_vf) return RXM_OK;
+    const HostMesh& h = m->h;
+    const auto&     vb = op_base(m, RXM_OP_VF);
+    std::vector<uint32_t> fb(h.P + 1, 0), rb(h.P + 1, 0), nb(h.P + 1, 0);
+    uint32_t maxn = 0;
+    for (uint32_t p = 0; p < h.P; ++p) {
+        const uint32_t nent = vb[p + 1] - vb[p];
+        if (nent >= 0xFFFFu || h.nf[p] >= 16384u) return RXM_ERR_PATCH_TOO_BIG;
+        maxn = std::max(maxn, nent);
+        fb[p + 1] = fb[p] + ((3u * h.nf[p] + 7u) & ~7u);
+        rb[p + 1] = rb[p] + ((h.nvo[p] + 1u + 7u) & ~7u);
+        nb[p + 1] = nb[p] + ((nent + 7u) & ~7u);
+    }
+    int rc = upload(m, fb, m->d_fp_fbase);
+    if (!rc) rc = upload(m, rb, m->d_fp_rbase);
+    if (!rc) rc = upload(m, nb, m->d_fp_nbase);
+    if (rc) return rc;
+    uint16_t* bufs[3] = {nullptr, nullptr, nullptr};
+    const uint32_t sizes[3] = {fb[h.P], rb[h.P], nb[h.P]};
+    for (int i = 0; i < 3; ++i) {
+        if (hipMalloc(&bufs[i], sizeof(uint16_t) * std::max<uint32_t>(sizes[i], 8)) != hipSuccess)
+            return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(bufs[i]);
+        m->device_bytes += 2ull * sizes[i];
+    }
+    Bump b(nullptr);
+    plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf);
+    rc = launch_patches(k_query2_vf_local<kQueryThreads>, kQueryThreads, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
+                        m->d_fp_rbase, m->d_fp_nbase, bufs[0], bufs[1], bufs[2]);
+    if (rc) return rc;
+    m->d_fp_fvl = bufs[0];
+    m->d_fp_roff = bufs[1];
+    m->d_fp_vf = bufs[2];
+    m->fp_max_nent = maxn;
     return RXM_OK;
 }
 
@@ -794,26 +841,43 @@ rxm_status rxm_symdir3_energy(rxm_mesh* m, const float* Mr, const float* x, floa
 static int launch_symdir3_step(rxm_mesh* m, const float* Mr, float eta, const float* x, float* xo, void* stream)
 {
     Bump b(nullptr);
-    plan_s3(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
-    const uint32_t lds = b.bytes();
+    plan_s3p(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
+    const uint32_t  lds = b.bytes();
+    const uint16_t* fvl = m->d_fp_fvl;
+    const uint16_t* ro  = m->d_fp_roff;
+    const uint16_t* vf  = m->d_fp_vf;
     if (m->grad_queue) {
-        if (!m->d_queue) {
-            if (hipMalloc(&m->d_queue, 256) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
-            m->allocs.push_back(m->d_queue);
-        }
-        if (m->num_cus == 0) RXM_HIP(hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device));
         hipLaunchKernelGGL(k_reset_counter, dim3(1), dim3(1), 0, static_cast<hipStream_t>(stream), m->d_queue);
         RXM_HIP(hipGetLastError());
         // enough workgroups to fill the machine; LDS decides residency
         const uint32_t per_cu = std::max(1u, std::min(8u, (160u * 1024u) / std::max(lds, 1u)));
         const uint32_t grid   = std::min<uint32_t>(m->h.P, static_cast<uint32_t>(m->num_cus) * per_cu);
         if (m->grad_threads == 128)
-            return launch_grid(k_symdir3_step_queue<128>, grid, 128, lds, stream, m->dm, m->d_queue, Mr, eta, x, xo);
-        return launch_grid(k_symdir3_step_queue<256>, grid, 256, lds, stream, m->dm, m->d_queue, Mr, eta, x, xo);
+            return launch_grid(k_symdir3_step_plan_queue<128>, grid, 128, lds, stream, m->dm, m->d_queue,
+                               m->d_fp_fbase, m->d_fp_rbase, m->d_fp_nbase, fvl, ro, vf, Mr, eta, x, xo);
+        return launch_grid(k_symdir3_step_plan_queue<256>, grid, 256, lds, stream, m->dm, m->d_queue, m->d_fp_fbase,
+                           m->d_fp_rbase, m->d_fp_nbase, fvl, ro, vf, Mr, eta, x, xo);
     }
     if (m->grad_threads == 128)
-        return launch_patches(k_symdir3_step<128>, 128, lds, stream, m, m->dm, Mr, eta, x, xo);
-    return launch_patches(k_symdir3_step<256>, 256, lds, stream, m, m->dm, Mr, eta, x, xo);
+        return launch_patches(k_symdir3_step_plan<128>, 128, lds, stream, m, m->dm, m->d_fp_fbase, m->d_fp_rbase,
+                              m->d_fp_nbase, fvl, ro, vf, Mr, eta, x, xo);
+    return launch_patches(k_symdir3_step_plan<256>, 256, lds, stream, m, m->dm, m->d_fp_fbase, m->d_fp_rbase,
+                          m->d_fp_nbase, fvl, ro, vf, Mr, eta, x, xo);
+}
+
+// Everything the step loop needs on the device before launching (or
+// capturing): the VV and face plans, the work-queue counter, the CU count.
+static int prepare_steps(rxm_mesh* m, void* stream)
+{
+    int rc = ensure_vv_plan(m, stream);
+    if (!rc) rc = ensure_face_plan(m, stream);
+    if (!rc && !m->d_queue) {
+        if (hipMalloc(&m->d_queue, 256) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(m->d_queue);
+    }
+    if (!rc && m->num_cus == 0)
+        RXM_HIP(hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device));
+    return rc;
 }
 
 rxm_status rxm_symdir3_grad_step(rxm_mesh* m, const float* Mr, const float* x, float eta, float* x_out,
@@ -821,6 +885,8 @@ rxm_status rxm_symdir3_grad_step(rxm_mesh* m, const float* Mr, const float* x, f
 {
     if (!m || !Mr || !x || !x_out || x == x_out) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    int rc = prepare_steps(m, stream);
+    if (rc) return rc;
     return launch_symdir3_step(m, Mr, eta, x, x_out, stream);
 }
 
@@ -841,7 +907,7 @@ rxm_status rxm_smooth_symdir_steps(rxm_mesh* m, const float* Mr, float* x, float
 {
     if (!m || !Mr || !x || !tmp || x == tmp || steps < 1) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    int rc = ensure_vv_plan(m, stream);
+    int rc = prepare_steps(m, stream);
     if (rc) return rc;
     return smooth_symdir_launches(m, Mr, x, tmp, lambda, eta, steps, stream);
 }
@@ -853,7 +919,7 @@ rxm_status rxm_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* co
     if (!meshes || !Mr || !x || !tmp || count == 0 || steps < 1) return RXM_ERR_INVALID_ARG;
     for (uint32_t k = 0; k < count; ++k) {
         if (!meshes[k] || !meshes[k]->on_device) return RXM_ERR_INVALID_ARG;
-        int rc = ensure_vv_plan(meshes[k], stream);
+        int rc = prepare_steps(meshes[k], stream);
         if (rc) return rc;
     }
     // plain launches: every mesh gets its own two launches per step
@@ -873,14 +939,7 @@ rxm_status rxm_graph_capture_smooth_symdir(rxm_mesh* m, const float* Mr, float* 
     *out = nullptr;
     hipStream_t s;
     RXM_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    int rc = ensure_vv_plan(m, s);  // allocation and the plan build stay outside the capture
-    if (!rc && m->grad_queue && !m->d_queue) {
-        if (hipMalloc(&m->d_queue, 256) != hipSuccess) rc = RXM_ERR_OUT_OF_MEMORY;
-        else m->allocs.push_back(m->d_queue);
-    }
-    if (!rc && m->num_cus == 0 &&
-        hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device) != hipSuccess)
-        rc = RXM_ERR_DEVICE;
+    int rc = prepare_steps(m, s);  // allocations and plan builds stay outside the capture
     if (!rc && hipStreamSynchronize(s) != hipSuccess) rc = RXM_ERR_DEVICE;
     rxm_graph* g = nullptr;
     if (!rc) {

@@ -481,69 +481,99 @@ __device__ __forceinline__ float symdir3_face(const f3 a, const f3 b, const f3 c
     return A * n2 * (1.0f + iG);
 }
 
-struct PlanS3
+// Work-queue counter reset (a kernel, so graphs hold no memset nodes).
+__global__ void k_reset_counter(uint32_t* c) { c[0] = 0u; }
+
+// Gradient of corner c of one face only (3 values), recomputed per (vertex,
+// face) pair from the face plan instead of stored per face; the corner is
+// selected with conditional moves (no runtime-indexed local array).
+__device__ __forceinline__ f3 symdir3_corner(const f3 a, const f3 b, const f3 c, const float* R, uint32_t corner)
 {
-    uint32_t *ev, *off, *cur, *slots, *wave;
-    uint16_t *fe, *fvl;
-    float *   x, *y, *z, *gc;
+    const f3    e1 = sub(b, a), e2 = sub(c, a);
+    const float dr = det2(R[0], R[3], R[2], R[1]);
+    const float A  = 0.5f * dr;
+    const float i00 = R[3] / dr, i01 = -R[2] / dr, i10 = -R[1] / dr, i11 = R[0] / dr;
+    const f3    J0 = {e1.x * i00 + e2.x * i10, e1.y * i00 + e2.y * i10, e1.z * i00 + e2.z * i10};
+    const f3    J1 = {e1.x * i01 + e2.x * i11, e1.y * i01 + e2.y * i11, e1.z * i01 + e2.z * i11};
+    const float a00 = dot(J0, J0), a11 = dot(J1, J1), a01 = dot(J0, J1);
+    const float n2 = a00 + a11, G = det2(a00, a11, a01, a01);
+    if (G <= 0.f) return f3{0.f, 0.f, 0.f};
+    const float iG = 1.0f / G;
+    const float s1 = 2.0f * (1.0f + iG), s2 = -n2 * iG * iG;
+    auto d = [&](float j0, float j1, float& D0, float& D1) {
+        const float E0 = A * (s1 * j0 + s2 * 2.0f * (a11 * j0 - a01 * j1));
+        const float E1 = A * (s1 * j1 + s2 * 2.0f * (a00 * j1 - a01 * j0));
+        D0 = E0 * i00 + E1 * i01;
+        D1 = E0 * i10 + E1 * i11;
+    };
+    float bx, by, bz, cx, cy, cz;
+    d(J0.x, J1.x, bx, cx);
+    d(J0.y, J1.y, by, cy);
+    d(J0.z, J1.z, bz, cz);
+    if (corner == 1u) return f3{bx, by, bz};
+    if (corner == 2u) return f3{cx, cy, cz};
+    return f3{-(bx + cx), -(by + cy), -(bz + cz)};
+}
+
+struct PlanS3P
+{
+    uint16_t *fvl, *ro, *vf;
+    float *   x, *y, *z, *mr;
 };
-__host__ __device__ inline PlanS3 plan_s3(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+__host__ __device__ inline PlanS3P plan_s3p(Bump& b, uint32_t nv, uint32_t nvo, uint32_t nf, uint32_t nent)
 {
-    PlanS3 p{};
-    p.ev    = b.alloc<uint32_t>(r4(ne));
-    p.off   = b.alloc<uint32_t>(nv + 1);
-    p.cur   = b.alloc<uint32_t>(nv + 1);
-    p.slots = b.alloc<uint32_t>(3 * nf);
-    p.wave  = b.alloc<uint32_t>(20);
-    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
-    p.fvl   = b.alloc<uint16_t>(3 * nf);
-    p.x     = b.alloc<float>(nv);
-    p.y     = b.alloc<float>(nv);
-    p.z     = b.alloc<float>(nv);
-    p.gc    = b.alloc<float>(9 * nf);
+    PlanS3P p{};
+    p.fvl = b.alloc<uint16_t>((3 * nf + 7u) & ~7u);
+    p.ro  = b.alloc<uint16_t>((nvo + 1u + 7u) & ~7u);
+    p.vf  = b.alloc<uint16_t>((nent + 7u) & ~7u);
+    p.x   = b.alloc<float>(nv);
+    p.y   = b.alloc<float>(nv);
+    p.z   = b.alloc<float>(nv);
+    p.mr  = b.alloc<float>(4 * nf);
     return p;
 }
 
-// One gradient-descent step x_out = x - eta * dE/dx on one patch: every
-// local face recomputes its corner gradients into LDS; each owned vertex sums
-// its VF corners in ascending (face, corner) order. No global atomics.
+// x_out = x - eta * dE/dx over the patch-local face plan: positions, rest
+// shapes and plan rows are staged into LDS; each owned vertex recomputes the
+// corner gradient of every incident face in the plan's ascending order.
 template <uint32_t T>
-__device__ __forceinline__ void symdir3_step_patch(const DevMesh& m, uint32_t p, const float* __restrict__ Mr,
+__device__ __forceinline__ void symdir3_plan_patch(const DevMesh& m, uint32_t p, const uint32_t* __restrict__ fbase,
+                                                   const uint32_t* __restrict__ rbase,
+                                                   const uint32_t* __restrict__ nbase,
+                                                   const uint16_t* __restrict__ fvl,
+                                                   const uint16_t* __restrict__ roff,
+                                                   const uint16_t* __restrict__ vf, const float* __restrict__ Mr,
                                                    float eta, const float* __restrict__ x, float* __restrict__ xo,
                                                    char* smem)
 {
     const PatchDesc d = load_desc(m.patch, p);
+    const uint32_t  nb0 = nbase[p], nent = nbase[p + 1] - nb0;
     Bump            b(smem);
-    const PlanS3    L = plan_s3(b, d.nv, d.ne, d.nf);
+    const PlanS3P   L = plan_s3p(b, d.nv, d.nvo, d.nf, nent);
     const uint32_t  N = m.nV, F = m.nF;
-    stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    stage_u16x8<T>(L.fvl, fvl + fbase[p], 3 * d.nf);
+    stage_u16x8<T>(L.ro, roff + rbase[p], d.nvo + 1);
+    stage_u16x8<T>(L.vf, vf + nb0, nent);
     stage_positions(L.x, L.y, L.z, x, N, d, m.v_lin);
-    __syncthreads();
     for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
-        uint32_t l[3];
-        for (uint32_t i = 0; i < 3; ++i) {
-            l[i]             = face_vertex(L.fe, L.ev, f, i);
-            L.fvl[3 * f + i] = static_cast<uint16_t>(l[i]);
-        }
-        const uint32_t lf   = lin_of(f, d.nfo, d.fp, m.f_lin + d.f_off);
-        const float    R[4] = {Mr[lf], Mr[F + lf], Mr[2 * F + lf], Mr[3 * F + lf]};
-        symdir3_face(f3{L.x[l[0]], L.y[l[0]], L.z[l[0]]}, f3{L.x[l[1]], L.y[l[1]], L.z[l[1]]},
-                     f3{L.x[l[2]], L.y[l[2]], L.z[l[2]]}, R, L.gc + 9 * f);
+        const uint32_t lf = lin_of(f, d.nfo, d.fp, m.f_lin + d.f_off);
+        L.mr[4 * f]       = Mr[lf];
+        L.mr[4 * f + 1]   = Mr[F + lf];
+        L.mr[4 * f + 2]   = Mr[2 * F + lf];
+        L.mr[4 * f + 3]   = Mr[3 * F + lf];
     }
     __syncthreads();
-    fill_columns<T>(
-        3 * d.nf, d.nv, [&](uint32_t i) { return static_cast<uint32_t>(L.fvl[i]); }, [&](uint32_t i) { return i; },
-        L.off, L.cur, L.slots, L.wave);
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
-        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
-        float          gx = 0.f, gy = 0.f, gz = 0.f;
-        sort_row(L.slots + o, k, [&](uint32_t, uint32_t s) {
-            const float* c = L.gc + 9 * (s / 3u) + 3 * (s % 3u);
-            gx += c[0];
-            gy += c[1];
-            gz += c[2];
-        });
+        float gx = 0.f, gy = 0.f, gz = 0.f;
+        for (uint32_t k = L.ro[v]; k < L.ro[v + 1]; ++k) {
+            const uint32_t s = L.vf[k], f = s >> 2, c = s & 3u;
+            const uint32_t a = L.fvl[3 * f], bb = L.fvl[3 * f + 1], cc = L.fvl[3 * f + 2];
+            const f3 g = symdir3_corner(f3{L.x[a], L.y[a], L.z[a]}, f3{L.x[bb], L.y[bb], L.z[bb]},
+                                        f3{L.x[cc], L.y[cc], L.z[cc]}, L.mr + 4 * f, c);
+            gx += g.x;
+            gy += g.y;
+            gz += g.z;
+        }
         const uint32_t g = d.vp + v;
         xo[g]            = L.x[v] - eta * gx;
         xo[N + g]        = L.y[v] - eta * gy;
@@ -552,20 +582,28 @@ __device__ __forceinline__ void symdir3_step_patch(const DevMesh& m, uint32_t p,
 }
 
 template <uint32_t T>
-__global__ __launch_bounds__(T) void k_symdir3_step(DevMesh m, const float* __restrict__ Mr, float eta,
-                                                    const float* __restrict__ x, float* __restrict__ xo)
+__global__ __launch_bounds__(T) void k_symdir3_step_plan(DevMesh m, const uint32_t* __restrict__ fbase,
+                                                         const uint32_t* __restrict__ rbase,
+                                                         const uint32_t* __restrict__ nbase,
+                                                         const uint16_t* __restrict__ fvl,
+                                                         const uint16_t* __restrict__ roff,
+                                                         const uint16_t* __restrict__ vf, const float* __restrict__ Mr,
+                                                         float eta, const float* __restrict__ x, float* __restrict__ xo)
 {
     extern __shared__ uint4 smem[];
-    symdir3_step_patch<T>(m, blockIdx.x, Mr, eta, x, xo, reinterpret_cast<char*>(smem));
+    symdir3_plan_patch<T>(m, blockIdx.x, fbase, rbase, nbase, fvl, roff, vf, Mr, eta, x, xo,
+                          reinterpret_cast<char*>(smem));
 }
 
 // Persistent form: a grid sized to the machine pulls patch ids from a global
-// work queue (one atomic ticket per patch); every workgroup exits once the
-// queue is drained. The counter is cleared by k_reset_counter before launch.
+// work queue (one atomic ticket per patch, counter cleared by k_reset_counter
+// before the launch); every workgroup exits once the queue is drained.
 template <uint32_t T>
-__global__ __launch_bounds__(T) void k_symdir3_step_queue(DevMesh m, uint32_t* __restrict__ queue,
-                                                          const float* __restrict__ Mr, float eta,
-                                                          const float* __restrict__ x, float* __restrict__ xo)
+__global__ __launch_bounds__(T) void k_symdir3_step_plan_queue(
+    DevMesh m, uint32_t* __restrict__ queue, const uint32_t* __restrict__ fbase, const uint32_t* __restrict__ rbase,
+    const uint32_t* __restrict__ nbase, const uint16_t* __restrict__ fvl, const uint16_t* __restrict__ roff,
+    const uint16_t* __restrict__ vf, const float* __restrict__ Mr, float eta, const float* __restrict__ x,
+    float* __restrict__ xo)
 {
     extern __shared__ uint4 smem[];
     __shared__ uint32_t s_p;
@@ -575,12 +613,10 @@ __global__ __launch_bounds__(T) void k_symdir3_step_queue(DevMesh m, uint32_t* _
         const uint32_t p = s_p;
         __syncthreads();
         if (p >= m.P) break;
-        symdir3_step_patch<T>(m, p, Mr, eta, x, xo, reinterpret_cast<char*>(smem));
+        symdir3_plan_patch<T>(m, p, fbase, rbase, nbase, fvl, roff, vf, Mr, eta, x, xo, reinterpret_cast<char*>(smem));
         __syncthreads();
     }
 }
-
-__global__ void k_reset_counter(uint32_t* c) { c[0] = 0u; }
 
 // Per-face 3-D symmetric Dirichlet energy of owned faces (FLT_MAX if degenerate).
 __global__ __launch_bounds__(kGeomThreads) void k_symdir3_energy(DevMesh m, const float* __restrict__ Mr,

@@ -212,6 +212,41 @@ __global__ __launch_bounds__(T) void k_query2_vv_local(DevMesh m, const uint32_t
     }
 }
 
+// Patch-local face plan: per patch, the FV table of every local face (3 u16
+// per face at fvl + fbase[p]) and, for every owned vertex, its VF row of
+// (local face << 2 | corner) slots in ascending order (u16 row offsets at
+// roff + rbase[p], entries at vf + nbase[p]). Corner c of face f is the
+// vertex at fvl[3 f + c]. Requires nf < 16384 (checked on the host).
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query2_vf_local(DevMesh m, const uint32_t* __restrict__ fbase,
+                                                       const uint32_t* __restrict__ rbase,
+                                                       const uint32_t* __restrict__ nbase,
+                                                       uint16_t* __restrict__ fvl, uint16_t* __restrict__ roff,
+                                                       uint16_t* __restrict__ vf)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    uint16_t* fo = fvl + fbase[p];
+    for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T)
+        fo[i] = static_cast<uint16_t>(face_vertex(L.fe, L.ev, i / 3u, i % 3u));
+    fill_columns<T>(
+        3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+        [&](uint32_t i) { return ((i / 3u) << 2) | (i % 3u); }, L.off, L.cur, L.slots, L.wave);
+    uint16_t* ro = roff + rbase[p];
+    uint16_t* nb = vf + nbase[p];
+    for (uint32_t v = threadIdx.x; v <= d.nvo; v += T) ro[v] = static_cast<uint16_t>(L.off[v]);
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { nb[o + j] = static_cast<uint16_t>(s); });
+    }
+}
+
 // VF: FV from FE + EV, transposed; slot = face (the sort key itself).
 template <uint32_t T>
 __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __restrict__ base,
