@@ -1,0 +1,102 @@
+// LDS counting-sort primitives shared by the query kernels and the C++
+// query API: compile-time-size block scan, register sorting network for
+// short rows, and the histogram + scan + packed-slot fill.
+#pragma once
+#include "lds_core.hpp"
+
+namespace rxm {
+
+template <uint32_t T>
+__device__ __forceinline__ uint32_t scan_excl(uint32_t* s, uint32_t n, uint32_t* s_wave)
+{
+    constexpr uint32_t NW = T / 64u;
+    const uint32_t     tid = threadIdx.x, lane = tid & 63u, wid = tid >> 6;
+    const uint32_t     per = (n + T - 1u) / T;  // T is a power of two: a shift
+    const uint32_t     b = min(n, tid * per), e = min(n, b + per);
+    __syncthreads();
+    uint32_t sum = 0;
+    for (uint32_t i = b; i < e; ++i) sum += s[i];
+    uint32_t x = sum;
+#pragma unroll
+    for (uint32_t d = 1; d < 64u; d <<= 1) {
+        const uint32_t y = __shfl_up(x, d, 64);
+        if (lane >= d) x += y;
+    }
+    if (lane == 63u) s_wave[wid] = x;
+    __syncthreads();
+    uint32_t before = 0, total = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < NW; ++w) {
+        const uint32_t t = s_wave[w];
+        before += (w < wid) ? t : 0u;
+        total += t;
+    }
+    uint32_t run = before + x - sum;
+    for (uint32_t i = b; i < e; ++i) {
+        const uint32_t t = s[i];
+        s[i]             = run;
+        run += t;
+    }
+    __syncthreads();
+    return total;
+}
+
+__device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
+{
+    const uint32_t lo = min(a, b), hi = max(a, b);
+    a = lo;
+    b = hi;
+}
+
+// Sort the k packed slots of one row ascending. Rows of up to 8 entries are
+// sorted in registers with Batcher's odd-even merge network (19
+// comparators); longer rows fall back to an insertion sort in LDS. The
+// callback out(j, slot) receives the sorted entries.
+template <typename Out>
+__device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
+{
+    if (k <= 8u) {
+        uint32_t a[8];
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; ++j) a[j] = (j < k) ? slots[j] : 0xFFFFFFFFu;
+        cswap(a[0], a[1]); cswap(a[2], a[3]); cswap(a[4], a[5]); cswap(a[6], a[7]);
+        cswap(a[0], a[2]); cswap(a[1], a[3]); cswap(a[4], a[6]); cswap(a[5], a[7]);
+        cswap(a[1], a[2]); cswap(a[5], a[6]);
+        cswap(a[0], a[4]); cswap(a[1], a[5]); cswap(a[2], a[6]); cswap(a[3], a[7]);
+        cswap(a[2], a[4]); cswap(a[3], a[5]);
+        cswap(a[1], a[2]); cswap(a[3], a[4]); cswap(a[5], a[6]);
+#pragma unroll
+        for (uint32_t j = 0; j < 8u; ++j)
+            if (j < k) out(j, a[j]);
+    } else {
+        for (uint32_t i = 1; i < k; ++i) {
+            const uint32_t x = slots[i];
+            uint32_t       j = i;
+            while (j > 0 && slots[j - 1] > x) {
+                slots[j] = slots[j - 1];
+                --j;
+            }
+            slots[j] = x;
+        }
+        for (uint32_t j = 0; j < k; ++j) out(j, slots[j]);
+    }
+}
+
+// Histogram + scan + packed slot fill of n entries into ncols columns.
+// key(i) -> column, pack(i) -> 32-bit slot value (sort key in the high bits).
+template <uint32_t T, typename KeyF, typename PackF>
+__device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF key, PackF pack, uint32_t* off,
+                                             uint32_t* cur, uint32_t* slots, uint32_t* s_wave)
+{
+    for (uint32_t c = threadIdx.x; c < ncols; c += T) off[c] = 0;
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += T) atomicAdd(&off[key(i)], 1u);
+    const uint32_t total = scan_excl<T>(off, ncols, s_wave);
+    if (threadIdx.x == 0) off[ncols] = total;
+    for (uint32_t c = threadIdx.x; c < ncols; c += T) cur[c] = off[c];
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < n; i += T) slots[atomicAdd(&cur[key(i)], 1u)] = pack(i);
+    __syncthreads();
+}
+
+}  // namespace rxm
