@@ -17,7 +17,8 @@
 //   normals .................... hand-computed single triangle / flat grid closed form
 //   smoothing .................. constants are fixed points; linear fields preserved inside
 //   cot CSR / MCF .............. row sums of L = 0, symmetry, flat-grid closed form
-//   symmetric Dirichlet ........ J=I => E=2A, grad=0; FD; forward-mode dual restatement
+//   symmetric Dirichlet ........ J=I => E=4A, grad=0; J=diag(2,1) => 6.25A; FD;
+//                                forward-mode dual restatement
 #include <algorithm>
 #include <array>
 #include <cfloat>

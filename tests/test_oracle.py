@@ -363,3 +363,71 @@ def test_symdir_flipped_face_is_flt_max():
     uv = V[:, :2][[0, 2, 1]]
     assert o.symdir_energy(Mr, uv)[0] == np.float32(np.finfo(np.float32).max)
     assert np.all(o.symdir_grad(Mr, uv) == 0)
+
+
+# ---- 3-D symmetric Dirichlet (variable = surface positions) -------------------
+def _rot(a, b, c):
+    ca, sa, cb, sb, cc, sc = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(c), np.sin(c)
+    rz = np.array([[ca, -sa, 0], [sa, ca, 0], [0, 0, 1]])
+    ry = np.array([[cb, 0, sb], [0, 1, 0], [-sb, 0, cb]])
+    rx_ = np.array([[1, 0, 0], [0, cc, -sc], [0, sc, cc]])
+    return rz @ ry @ rx_
+
+
+def test_symdir3_identity_and_planar_reduction():
+    """x = rest: J^T J = I, so E = 4A and the gradient vanishes; on a planar
+    surface the 3-D energy equals the 2-D (UV) energy of the same map."""
+    pos, fv = mg.icosphere(2, seed=3, sigma=0.02)
+    o = OracleMesh(fv)
+    Mr = o.rest_shapes(pos)
+    area = 0.5 * (Mr[:, 0] * Mr[:, 3] - Mr[:, 2] * Mr[:, 1])
+    a, b, c = (pos[fv[:, i].astype(np.int64)] for i in range(3))
+    assert np.allclose(area, 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1), rtol=1e-12)
+    assert np.allclose(o.symdir3_energy(Mr, pos), 4.0 * area, rtol=1e-12)
+    assert np.allclose(o.symdir3_grad(Mr, pos), 0.0, atol=1e-10)
+    # planar: rest = flat disk, current = UV embedded at z = 0
+    dpos, dfv, uv = _disk_problem(n=14, sigma=0.01)
+    od = OracleMesh(dfv)
+    flat = dpos.copy()
+    flat[:, 2] = 0.0
+    Mr2 = od.rest_shapes(flat)
+    x3 = np.concatenate([uv, np.zeros((len(uv), 1))], axis=1)
+    assert np.allclose(od.symdir3_energy(Mr2, x3), od.symdir_energy(Mr2, uv), rtol=1e-10)
+    g3 = od.symdir3_grad(Mr2, x3)
+    assert np.allclose(g3[:, :2], od.symdir_grad(Mr2, uv), rtol=1e-9, atol=1e-9)
+    assert np.allclose(g3[:, 2], 0.0, atol=1e-12)
+
+
+def test_symdir3_gradient_vs_fd_and_rotation():
+    pos, fv = mg.icosphere(2, seed=4, sigma=0.02)
+    o = OracleMesh(fv)
+    Mr = o.rest_shapes(pos)
+    rng = np.random.default_rng(1)
+    x = pos + rng.normal(0, 0.01, pos.shape)
+    g = o.symdir3_grad(Mr, x)
+    h = 1e-6
+    for v in rng.choice(len(x), 10, replace=False):
+        for c in range(3):
+            up, dn = x.copy(), x.copy()
+            up[v, c] += h
+            dn[v, c] -= h
+            fd = (o.symdir3_energy(Mr, up).sum() - o.symdir3_energy(Mr, dn).sum()) / (2 * h)
+            assert abs(fd - g[v, c]) <= 1e-5 * max(1.0, abs(g[v, c]))
+    # a rigid rotation of the current shape leaves E unchanged and rotates the gradient
+    R = _rot(0.3, -0.7, 1.1)
+    assert np.allclose(o.symdir3_energy(Mr, x @ R.T), o.symdir3_energy(Mr, x), rtol=1e-10)
+    assert np.allclose(o.symdir3_grad(Mr, x @ R.T), g @ R.T, rtol=1e-8, atol=1e-10)
+
+
+def test_smooth_symdir_step_is_the_composition():
+    pos, fv = mg.icosphere(2, seed=5, sigma=0.02)
+    o = OracleMesh(fv)
+    Mr = o.rest_shapes(pos)
+    x = pos * 1.01
+    lam, eta = 0.5, 1e-3
+    s = o.smooth_uniform(x, lam, 1)
+    want = s - eta * o.symdir3_grad(Mr, s)
+    assert np.allclose(o.smooth_symdir_steps(Mr, x, lam, eta, 1), want, rtol=0, atol=1e-14)
+    w2 = o.smooth_uniform(want, lam, 1)
+    w2 = w2 - eta * o.symdir3_grad(Mr, w2)
+    assert np.allclose(o.smooth_symdir_steps(Mr, x, lam, eta, 2), w2, rtol=0, atol=1e-14)
