@@ -30,6 +30,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2s_perm")
     ap.add_argument("--patch-size", type=int, default=512)
+    ap.add_argument("--patcher", default="kmeans", choices=("kmeans", "bisect"))
     ap.add_argument("--smooth-iters", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
@@ -42,6 +43,20 @@ def parse():
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-reps", type=int, default=3)
     return ap.parse_args()
+
+
+def measured_traffic(workload, patch_size, kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 counter
+    summary of this workload (FETCH_SIZE + WRITE_SIZE passes, each scaled by
+    a plain copy's reading), or None when no summary covers it."""
+    path = os.path.join(ROOT, "profiles", "pmc_%s.md" % workload)
+    if patch_size != 512 or not os.path.exists(path):
+        return None
+    for line in open(path):
+        f = line.split()
+        if len(f) == 3 and f[0] == "traffic_bytes_per_launch" and f[1] == kernel:
+            return int(f[2])
+    return None
 
 
 def event_time_ms(fn, reps):
@@ -169,7 +184,7 @@ def main():
 
     torch.cuda.set_device(0)
     pos, fv = mg.config_mesh(args.workload)
-    m = rx.Mesh(fv, patch_size=args.patch_size)
+    m = rx.Mesh(fv, patch_size=args.patch_size, patcher=args.patcher)
     V, E, F, P = m.nV, m.nE, m.nF, m.P
     l2g = m.array("v_l2g")
     x = torch.from_numpy(np.ascontiguousarray(pos[l2g].T.astype(np.float32))).cuda()
@@ -246,9 +261,13 @@ def main():
         "vs_baseline": None, "dtype": "fp32+u16/i32", "data": "synthetic",
         "config": {"workload": "VV+VF query CSR + %d uniform smoothing iters, icosphere L9 permuted "
                                "(%d V / %d F), patch size %d" % (it, V, F, args.patch_size),
-                   "mesh": args.workload, "num_patches": P, "entries_per_step": entries},
+                   "mesh": args.workload, "num_patches": P, "patcher": args.patcher,
+                   "entries_per_step": entries},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None},
+                     "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                     "traffic": (measured_traffic(args.workload, args.patch_size, dom)
+                                 if args.patcher == "kmeans" else None),
+                     "algorithmic_bytes": db},
         "kernels": kernels,
     }
     # free the single-mesh workload before the batch

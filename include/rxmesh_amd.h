@@ -47,8 +47,11 @@ enum {
     RXM_OP_FE = 7, RXM_OP_FF = 8, RXM_OP_EV = 9, RXM_OP_EF = 11
 };
 
-/* rxm_mesh_create flags */
-enum { RXM_HOST_ONLY = 1 };
+/* rxm_mesh_create flags. RXM_PATCHER_BISECT selects the recursive
+ * proportional bisection patcher (ceil(F / patch_size) connected patches, the
+ * fewest the cap allows, and the smallest ribbons) instead of the default
+ * k-means patcher (faster to build, patches ~0.7 of the cap). */
+enum { RXM_HOST_ONLY = 1, RXM_PATCHER_BISECT = 2 };
 
 /* vertex normal weighting */
 enum { RXM_NORMALS_MAX = 0, RXM_NORMALS_AREA = 1 };

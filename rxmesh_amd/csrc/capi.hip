@@ -344,7 +344,7 @@ rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_
             if (rc == RXM_OK) rc = set_patches(m->h, fp.data(), P, pv.data(), po.data());
             m->h.patch_size = patch_size;
         } else {
-            rc = run_patcher(m->h, patch_size);
+            rc = run_patcher(m->h, patch_size, (flags & RXM_PATCHER_BISECT) ? kPatcherBisect : kPatcherKMeans);
         }
     }
     if (rc != RXM_OK) {
@@ -385,7 +385,8 @@ rxm_status rxm_mesh_create_batch(const uint32_t* const* fvs, const uint32_t* num
         parts[k].reset(new (std::nothrow) HostMesh());
         if (!parts[k]) return RXM_ERR_OUT_OF_MEMORY;
         rc = build_supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0);
-        if (rc == RXM_OK) rc = run_patcher(*parts[k], patch_size);
+        if (rc == RXM_OK)
+            rc = run_patcher(*parts[k], patch_size, (flags & RXM_PATCHER_BISECT) ? kPatcherBisect : kPatcherKMeans);
         if (rc == RXM_OK) rc = build_patches(*parts[k]);
     }
     if (rc != RXM_OK) return rc;
@@ -637,6 +638,16 @@ rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* no
 {
     if (!m || !pos || !normals || (mode != RXM_NORMALS_MAX && mode != RXM_NORMALS_AREA)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    // the face plan (built once per mesh) carries the VF rows; meshes whose
+    // patches exceed its u16 encoding use the self-contained kernel
+    const int prc = ensure_face_plan(m, stream);
+    if (prc == RXM_OK) {
+        Bump b(nullptr);
+        plan_vnp(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
+        return launch_patches(k_vertex_normals_plan<128>, 128, b.bytes(), stream, m, m->dm, mode, m->d_fp_fbase,
+                              m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, normals);
+    }
+    if (prc != RXM_ERR_PATCH_TOO_BIG) return prc;
     Bump b(nullptr);
     plan_vn(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
     return launch_patches(k_vertex_normals, kGeomThreads, b.bytes(), stream, m, m->dm, mode, pos, normals);

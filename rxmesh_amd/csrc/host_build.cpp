@@ -2,6 +2,7 @@
 #include "host_build.h"
 
 #include <algorithm>
+#include <cmath>
 #include <atomic>
 #include <cstdlib>
 #include <cstring>
@@ -167,13 +168,14 @@ int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_h
 }
 
 // ---------------------------------------------------------------------------
-// a2. Deterministic patcher. The reference grows patches with a GPU Lloyd
+// a2. Deterministic patchers. The reference grows patches with a GPU Lloyd
 // iteration (patcher.cu:828-987) seeded from std::random_device, so its
-// layout cannot be reproduced. This one keeps the same idea -- k-means on the
-// face graph: multi-source BFS assignment, re-centre each seed on its
+// layout cannot be reproduced. The default keeps the same idea -- k-means on
+// the face graph: multi-source BFS assignment, re-centre each seed on its
 // patch's most interior face, repeat -- with seeds taken at even strides of
-// a BFS order, followed by recursive bisection of any patch above
-// patch_size. Everything is sequential FIFO BFS, so it is deterministic.
+// a BFS order, followed by halving of any patch above patch_size. The
+// alternative is a recursive proportional bisection (Bisector below).
+// Everything is sequential FIFO BFS with id tie-breaks: deterministic.
 namespace {
 
 struct FaceGraph
@@ -276,6 +278,187 @@ void recenter(const FaceGraph& g, uint32_t nF, const std::vector<uint32_t>& fp,
     }
 }
 
+// Recursive proportional bisection: a face set that must become k patches is
+// cut in two at the median of (d_a - d_b), where a and b are two mutually far
+// faces of the set (BFS distances inside the set), into parts of
+// round(n * k1 / k) and the rest faces for k1 = k / 2 and k - k1 patches.
+// With k = ceil(F / cap) at the root every leaf holds about F / k <= cap
+// faces, the fewest patches the cap allows. Sets are processed depth-first
+// (patch ids follow the recursion, so neighbouring patches get nearby ids);
+// ties break on face id, so the result is deterministic.
+class Bisector
+{
+   public:
+    Bisector(const FaceGraph& g, uint32_t nF) : g_(g), pos_(nF, kInvalid32), dist_(nF, kInvalid32), side_(nF, 0) {}
+
+    void run(std::vector<uint32_t> faces, uint32_t k, uint32_t cap, std::vector<uint32_t>& fp, uint32_t& next_id)
+    {
+        if (k <= 1 && faces.size() <= cap) {
+            for (uint32_t f : faces) fp[f] = next_id;
+            ++next_id;
+            return;
+        }
+        if (k <= 1) k = 2;
+        const uint32_t n  = static_cast<uint32_t>(faces.size());
+        const uint32_t k1 = k / 2, k2 = k - k1;
+        const uint32_t n1 = static_cast<uint32_t>((uint64_t(n) * k1 + k / 2) / k);
+        for (uint32_t i = 0; i < n; ++i) pos_[faces[i]] = i;
+        // three candidate cut axes between mutually far faces a, b, c; the
+        // one whose split cuts the fewest face adjacencies wins (keeps cuts
+        // short on meshes with handles, e.g. around a torus tube)
+        const uint32_t a = farthest(faces, faces[0]);
+        std::vector<uint32_t> d[3];
+        const uint32_t        b = farthest(faces, a);
+        d[0]                    = dists(faces);
+        bfs(faces, b);
+        d[1]         = dists(faces);
+        uint32_t c   = faces[0];
+        uint32_t cbest = 0;
+        for (uint32_t i = 0; i < n; ++i) {
+            const uint32_t m = std::min(d[0][i], d[1][i]);
+            if (m != kInvalid32 && m > cbest) {
+                cbest = m;
+                c     = faces[i];
+            }
+        }
+        bfs(faces, c);
+        d[2] = dists(faces);
+        // (u, w, z): cut along d_u - d_w; faces tied on it (a level set)
+        // are ordered by the distance to the third face z, so the median
+        // splits a tie band into two connected runs, then by face id
+        const int pairs[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
+        std::vector<std::pair<int64_t, uint64_t>> key(n), best_key;
+        uint64_t best_cut = ~0ull;
+        for (const auto& pr : pairs) {
+            const std::vector<uint32_t>& du = d[pr[0]];
+            const std::vector<uint32_t>& dw = d[pr[1]];
+            const std::vector<uint32_t>& dz = d[pr[2]];
+            for (uint32_t i = 0; i < n; ++i) {
+                // faces not connected to u / w (another component) sort last
+                const int64_t x = (du[i] == kInvalid32) ? (int64_t(1) << 40) : int64_t(du[i]);
+                const int64_t y = (dw[i] == kInvalid32) ? 0 : int64_t(dw[i]);
+                key[i] = {x - y, (uint64_t(dz[i]) << 32) | faces[i]};
+            }
+            std::nth_element(key.begin(), key.begin() + n1, key.end());
+            for (uint32_t i = 0; i < n; ++i) side_[uint32_t(key[i].second)] = (i < n1) ? 1u : 2u;
+            uint64_t cut = 0;
+            for (uint32_t i = 0; i < n1; ++i) {
+                const uint32_t f = uint32_t(key[i].second);
+                for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) cut += (side_[g_.val[j]] == 2u);
+            }
+            if (cut < best_cut) {
+                best_cut = cut;
+                best_key = key;
+            }
+        }
+        for (uint32_t f : faces) {
+            pos_[f]  = kInvalid32;
+            side_[f] = 0;
+        }
+        std::vector<uint32_t> A, B;
+        A.reserve(n1);
+        B.reserve(n - n1);
+        for (uint32_t i = 0; i < n; ++i) (i < n1 ? A : B).push_back(uint32_t(best_key[i].second));
+        std::sort(A.begin(), A.end());
+        // keep both halves connected: pieces cut off from the larger
+        // component of one half border the other half, so they move there
+        std::vector<uint32_t>().swap(faces);
+        reconnect(A, B);
+        reconnect(B, A);
+        std::sort(A.begin(), A.end());
+        std::sort(B.begin(), B.end());
+        const uint32_t kA = std::max<uint32_t>(k1, static_cast<uint32_t>((A.size() + cap - 1) / cap));
+        const uint32_t kB = std::max<uint32_t>(k2, static_cast<uint32_t>((B.size() + cap - 1) / cap));
+        run(std::move(A), kA, cap, fp, next_id);
+        run(std::move(B), kB, cap, fp, next_id);
+    }
+
+    // Entry: every connected component gets ceil(size / cap) patches.
+    void run_all(uint32_t nF, uint32_t cap, std::vector<uint32_t>& fp, uint32_t& next_id)
+    {
+        std::vector<uint32_t> all(nF);
+        std::iota(all.begin(), all.end(), 0u);
+        for (auto& c : components(all)) {
+            std::sort(c.begin(), c.end());
+            const uint32_t k = static_cast<uint32_t>((c.size() + cap - 1) / cap);
+            run(std::move(c), k, cap, fp, next_id);
+        }
+    }
+
+   private:
+    // connected components of a face set, largest first (ties: smaller first face)
+    std::vector<std::vector<uint32_t>> components(const std::vector<uint32_t>& faces)
+    {
+        for (uint32_t i = 0; i < faces.size(); ++i) pos_[faces[i]] = i;
+        for (uint32_t f : faces) dist_[f] = kInvalid32;
+        std::vector<std::vector<uint32_t>> out;
+        for (uint32_t s : faces) {
+            if (dist_[s] != kInvalid32) continue;
+            std::vector<uint32_t> c{s};
+            dist_[s] = 0;
+            for (size_t h = 0; h < c.size(); ++h) {
+                const uint32_t f = c[h];
+                for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) {
+                    const uint32_t n = g_.val[j];
+                    if (pos_[n] != kInvalid32 && dist_[n] == kInvalid32) {
+                        dist_[n] = 0;
+                        c.push_back(n);
+                    }
+                }
+            }
+            out.push_back(std::move(c));
+        }
+        for (uint32_t f : faces) pos_[f] = kInvalid32;
+        std::stable_sort(out.begin(), out.end(),
+                         [](const std::vector<uint32_t>& x, const std::vector<uint32_t>& y) { return x.size() > y.size(); });
+        return out;
+    }
+
+    // move every component of `from` but the largest into `to`
+    void reconnect(std::vector<uint32_t>& from, std::vector<uint32_t>& to)
+    {
+        auto comps = components(from);
+        if (comps.size() <= 1) return;
+        from = std::move(comps[0]);
+        for (size_t i = 1; i < comps.size(); ++i) to.insert(to.end(), comps[i].begin(), comps[i].end());
+    }
+
+    // BFS inside the current set (pos_[f] != kInvalid32); fills dist_ for its faces
+    void bfs(const std::vector<uint32_t>& faces, uint32_t src)
+    {
+        for (uint32_t f : faces) dist_[f] = kInvalid32;
+        q_.clear();
+        q_.push_back(src);
+        dist_[src] = 0;
+        for (size_t h = 0; h < q_.size(); ++h) {
+            const uint32_t f = q_[h];
+            for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) {
+                const uint32_t n = g_.val[j];
+                if (pos_[n] != kInvalid32 && dist_[n] == kInvalid32) {
+                    dist_[n] = dist_[f] + 1;
+                    q_.push_back(n);
+                }
+            }
+        }
+    }
+    // last face reached by a BFS from src (largest distance; FIFO order breaks ties)
+    uint32_t farthest(const std::vector<uint32_t>& faces, uint32_t src)
+    {
+        bfs(faces, src);
+        return q_.back();
+    }
+
+    std::vector<uint32_t> dists(const std::vector<uint32_t>& faces) const
+    {
+        std::vector<uint32_t> r(faces.size());
+        for (size_t i = 0; i < faces.size(); ++i) r[i] = dist_[faces[i]];
+        return r;
+    }
+
+    const FaceGraph&      g_;
+    std::vector<uint32_t> pos_, dist_, side_, q_;
+};
+
 // BFS distances restricted to a face subset (mask == tag)
 void subset_bfs(const FaceGraph& g, const std::vector<uint32_t>& faces, uint32_t src,
                 const std::vector<uint32_t>& tag_of, uint32_t tag,
@@ -311,7 +494,7 @@ void subset_bfs(const FaceGraph& g, const std::vector<uint32_t>& faces, uint32_t
 
 }  // namespace
 
-int run_patcher(HostMesh& m, uint32_t patch_size)
+int run_patcher(HostMesh& m, uint32_t patch_size, int mode)
 {
     if (patch_size < 16) return kErrInvalidArg;
     m.patch_size = patch_size;
@@ -340,16 +523,24 @@ int run_patcher(HostMesh& m, uint32_t patch_size)
             }
         }
     }
-    const uint32_t P0 = (nF + patch_size - 1) / patch_size;
-    std::vector<uint32_t> seeds(P0);
-    for (uint32_t k = 0; k < P0; ++k) seeds[k] = order[(uint64_t(k) * nF) / P0];
-    const int iters = 4;
-    for (int it = 0; it < iters; ++it) {
-        bfs_assign(g, nF, seeds, fp);
-        if (it + 1 < iters) recenter(g, nF, fp, seeds);
+    uint32_t nP = 0;
+    if (mode == kPatcherKMeans) {
+        const uint32_t P0 = (nF + patch_size - 1) / patch_size;
+        std::vector<uint32_t> seeds(P0);
+        for (uint32_t k = 0; k < P0; ++k) seeds[k] = order[(uint64_t(k) * nF) / P0];
+        const int iters = 4;
+        for (int it = 0; it < iters; ++it) {
+            bfs_assign(g, nF, seeds, fp);
+            if (it + 1 < iters) recenter(g, nF, fp, seeds);
+        }
+        nP = static_cast<uint32_t>(seeds.size());
+    } else {
+        fp.assign(nF, kInvalid32);
+        Bisector bs(g, nF);
+        bs.run_all(nF, patch_size, fp, nP);
     }
     // recursive bisection of oversize patches
-    std::vector<std::vector<uint32_t>> members(seeds.size());
+    std::vector<std::vector<uint32_t>> members(nP);
     for (uint32_t f = 0; f < nF; ++f) members[fp[f]].push_back(f);
     for (size_t k = 0; k < members.size(); ++k) {
         while (members[k].size() > patch_size) {

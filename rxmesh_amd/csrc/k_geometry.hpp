@@ -243,6 +243,77 @@ __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* _
     }
 }
 
+// ---- vertex normals over the patch-local face plan ----------------------------
+// Same weights and summation order as k_vertex_normals (incident faces in
+// ascending local order), but the VF rows come precomputed from the face
+// plan and every owned vertex evaluates its incident faces' normal and
+// corner weight itself, so LDS holds only the plan rows and positions.
+struct PlanVNP
+{
+    uint16_t *fvl, *ro, *vf;
+    float *   x, *y, *z;
+};
+__host__ __device__ inline PlanVNP plan_vnp(Bump& b, uint32_t nv, uint32_t nvo, uint32_t nf, uint32_t nent)
+{
+    PlanVNP p{};
+    p.fvl = b.alloc<uint16_t>((3 * nf + 7u) & ~7u);
+    p.ro  = b.alloc<uint16_t>((nvo + 1u + 7u) & ~7u);
+    p.vf  = b.alloc<uint16_t>((nent + 7u) & ~7u);
+    p.x   = b.alloc<float>(nv);
+    p.y   = b.alloc<float>(nv);
+    p.z   = b.alloc<float>(nv);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_vertex_normals_plan(DevMesh m, int mode, const uint32_t* __restrict__ fbase,
+                                                           const uint32_t* __restrict__ rbase,
+                                                           const uint32_t* __restrict__ nbase,
+                                                           const uint16_t* __restrict__ fvl,
+                                                           const uint16_t* __restrict__ roff,
+                                                           const uint16_t* __restrict__ vf,
+                                                           const float* __restrict__ pos, float* __restrict__ nrm)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = blockIdx.x;
+    const PatchDesc d = load_desc(m.patch, p);
+    const uint32_t  nb0 = nbase[p], nent = nbase[p + 1] - nb0;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanVNP   L = plan_vnp(b, d.nv, d.nvo, d.nf, nent);
+    const uint32_t  N = m.nV;
+    stage_u16x8<T>(L.fvl, fvl + fbase[p], 3 * d.nf);
+    stage_u16x8<T>(L.ro, roff + rbase[p], d.nvo + 1);
+    stage_u16x8<T>(L.vf, vf + nb0, nent);
+    stage_positions(L.x, L.y, L.z, pos, N, d, m.v_lin);
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        float sx = 0.f, sy = 0.f, sz = 0.f;
+        for (uint32_t k = L.ro[v]; k < L.ro[v + 1]; ++k) {
+            const uint32_t s = L.vf[k], f = s >> 2, c = s & 3u;
+            const uint32_t a = L.fvl[3 * f], bb = L.fvl[3 * f + 1], cc = L.fvl[3 * f + 2];
+            const float    ux = L.x[bb] - L.x[a], uy = L.y[bb] - L.y[a], uz = L.z[bb] - L.z[a];
+            const float    wx = L.x[cc] - L.x[a], wy = L.y[cc] - L.y[a], wz = L.z[cc] - L.z[a];
+            const float    nx = uy * wz - uz * wy, ny = uz * wx - ux * wz, nz = ux * wy - uy * wx;
+            float          w = 0.5f;
+            if (mode == 0) {
+                const float vx = L.x[cc] - L.x[bb], vy = L.y[cc] - L.y[bb], vz = L.z[cc] - L.z[bb];
+                const float l0 = ux * ux + uy * uy + uz * uz;  // a-b
+                const float l1 = vx * vx + vy * vy + vz * vz;  // b-c
+                const float l2 = wx * wx + wy * wy + wz * wz;  // c-a
+                const float den = (c == 0u) ? (l0 + l2) : ((c == 1u) ? (l1 + l0) : (l2 + l1));
+                w = 1.0f / den;
+            }
+            sx += nx * w;
+            sy += ny * w;
+            sz += nz * w;
+        }
+        const uint32_t g = d.vp + v;
+        nrm[g]           = sx;
+        nrm[N + g]       = sy;
+        nrm[2 * N + g]   = sz;
+    }
+}
+
 // ---- permutations between global (input) order and linear order ----------
 __global__ void k_gather_rows(uint32_t n, uint32_t ncomp, const uint32_t* __restrict__ map,
                               const float* __restrict__ src, float* __restrict__ dst)

@@ -44,6 +44,14 @@ __device__ __forceinline__ void stage_lin(uint32_t* dst, uint32_t n, uint32_t n_
     for (uint32_t l = threadIdx.x; l < n; l += T) dst[l] = (l < n_owned) ? prefix + l : slab[l];
 }
 
+// Coalesced copy of n u32 from LDS to a global int32 run (thread i writes
+// entries i, i + T, ...).
+template <uint32_t T>
+__device__ __forceinline__ void copy_out(int32_t* __restrict__ dst, const uint32_t* src, uint32_t n)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += T) dst[i] = static_cast<int32_t>(src[i]);
+}
+
 // VV (IS_VV) or VE over linear ids.
 template <uint32_t T, bool IS_VV>
 __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __restrict__ base,
@@ -71,15 +79,18 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
         },
         L.off, L.cur, L.slots, L.wave);
     const uint32_t bse = base[p];
+    // sort each owned row in place, replacing slots by target linear ids;
+    // owned rows are the prefix [0, off[nvo]) of the slot array
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         const uint32_t o = L.off[v], k = L.off[v + 1] - o;
         row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
-        int32_t* dst = col + bse + o;
         sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) {
-            dst[j] = static_cast<int32_t>(IS_VV ? L.lin[lo16(s)] : L.lin[hi16(s)]);
+            L.slots[o + j] = IS_VV ? L.lin[lo16(s)] : L.lin[hi16(s)];
         });
     }
     if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+    __syncthreads();
+    copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
 }
 
 // VV in patch-local form: per patch, u16 row offsets of the owned vertices
@@ -175,10 +186,11 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         const uint32_t o = L.off[v], k = L.off[v + 1] - o;
         row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
-        int32_t* dst = col + bse + o;
-        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { dst[j] = static_cast<int32_t>(L.lin[s]); });
+        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = L.lin[s]; });
     }
     if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+    __syncthreads();
+    copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
 }
 
 // EF (IS_FF == false) or FF.
@@ -202,10 +214,11 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
         for (uint32_t e = threadIdx.x; e < d.neo; e += T) {
             const uint32_t o = L.off[e], k = L.off[e + 1] - o;
             row_ptr[d.ep + e] = static_cast<int32_t>(bse + o);
-            int32_t* dst = col + bse + o;
-            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { dst[j] = static_cast<int32_t>(L.lin[s]); });
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = L.lin[s]; });
         }
         if (threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(bse + L.off[d.neo]);
+        __syncthreads();
+        copy_out<T>(col + bse, L.slots, L.off[d.neo]);
     } else {
         // sort the EF rows of every edge touched by an owned face
         for (uint32_t e = threadIdx.x; e < d.ne; e += T) {

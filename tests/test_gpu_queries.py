@@ -45,7 +45,10 @@ def case(name):
         elif name == "disk":
             V, F = mg.disk(64, seed=1)
             ps = 256
-        m = rx.Mesh(F, patch_size=ps)
+        elif name == "c1_bisect":
+            V, F = mg.config_mesh("c1")
+            ps = 512
+        m = rx.Mesh(F, patch_size=ps, patcher="bisect" if name.endswith("_bisect") else "kmeans")
         o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"),
                                       m.array("patches_offset"))
     _CACHE[name] = (V, F, m, o)
@@ -66,7 +69,7 @@ def to_global_np(m, x_lin):
     return out
 
 
-NAMES = ["sphere3", "torus", "plane", "cube", "bunnyhead", "c0", "c1", "c1_p128", "disk"]
+NAMES = ["sphere3", "torus", "plane", "cube", "bunnyhead", "c0", "c1", "c1_p128", "disk", "c1_bisect"]
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -81,7 +84,7 @@ def test_query_bit_exact(name, op):
     assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
 
 
-@pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "disk"])
+@pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "disk", "c1_bisect"])
 @pytest.mark.parametrize("op", ["VV", "VE"])
 def test_oriented_query_bit_exact(name, op):
     import torch

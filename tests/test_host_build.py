@@ -178,3 +178,44 @@ def test_batch_union_matches_separate_meshes():
     fv_all = np.concatenate([f + ov[k] for k, (_, f) in enumerate(ms)])
     o = OracleMesh(fv_all).set_patches(b.array("face_patch"), b.array("patches_val"), b.array("patches_offset"))
     compare_with_oracle(b, o, fv_all)
+
+
+# ---- the recursive-bisection patcher ------------------------------------------
+@pytest.mark.parametrize("case", ["c0", "c1", "disk", "torus"])
+def test_bisect_patcher_invariants_and_layout(case):
+    if case == "disk":
+        V, F = mg.disk(48)
+    elif case == "torus":
+        V, F = mg.torus(48, 24)
+    else:
+        V, F = mg.config_mesh(case)
+    ps = 256
+    m = rx.Mesh(F, patch_size=ps, host_only=True, patcher="bisect")
+    fp = m.array("face_patch")
+    cnt = np.bincount(fp, minlength=m.P)
+    # none above the cap; close to the fewest the cap allows (pieces moved to
+    # keep both halves of a cut connected can add a few patches)
+    pmin = -(-len(F) // ps)
+    assert pmin <= m.P <= 1.1 * pmin and cnt.min() >= 1 and cnt.max() <= ps
+    # every patch is edge-connected
+    import scipy.sparse as sps
+    from scipy.sparse.csgraph import connected_components
+    o = OracleMesh(F)
+    off, cols = o.ff()
+    rows = np.repeat(np.arange(len(F)), np.diff(off.astype(np.int64)))
+    keep = fp[rows] == fp[cols]
+    A = sps.coo_matrix((np.ones(keep.sum()), (rows[keep], cols[keep])), shape=(len(F), len(F)))
+    ncomp, lab = connected_components(A, directed=False)
+    assert ncomp == m.P
+    assert np.array_equal(rx.Mesh(F, patch_size=ps, host_only=True, patcher="bisect").array("face_patch"), fp)
+    o.set_patches(fp, m.array("patches_val"), m.array("patches_offset"))
+    compare_with_oracle(m, o, F)
+
+
+def test_bisect_patcher_components_get_their_own_patches():
+    (V1, F1), (V2, F2) = mg.icosphere(2), mg.grid(12)
+    F = np.concatenate([F1, F2 + len(V1)]).astype(np.uint32)
+    m = rx.Mesh(F, patch_size=100, host_only=True, patcher="bisect")
+    fp = m.array("face_patch")
+    assert m.P <= 1.1 * (-(-len(F1) // 100) + -(-len(F2) // 100))
+    assert not set(fp[:len(F1)]) & set(fp[len(F1):])
