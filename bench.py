@@ -30,7 +30,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--workload", default="c2s_perm")
     ap.add_argument("--patch-size", type=int, default=512)
-    ap.add_argument("--patcher", default="kmeans", choices=("kmeans", "bisect"))
+    ap.add_argument("--patcher", default="bisect", choices=("kmeans", "bisect"))
     ap.add_argument("--smooth-iters", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
@@ -39,6 +39,12 @@ def parse():
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
                          "smooth_uniform, vertex_normals, copy")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
+    ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
+    ap.add_argument("--query-pipe", type=int, default=None,
+                    help="persistent pipelined VV/VF query workgroups per CU, 0 = one per patch (default: autotuned)")
+    ap.add_argument("--smooth-pipe", type=int, default=None,
+                    help="persistent pipelined smoothing workgroups per CU, 0 = one per patch (default: autotuned)")
     ap.add_argument("--batch-count", type=int, default=256, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -198,6 +204,25 @@ def main():
     it = args.smooth_iters
     m.build_vv_plan()  # static patch-local VV plan, built once per mesh
     torch.cuda.synchronize()
+    from rxmesh_amd import autotune
+    if args.smooth_threads or args.smooth_layout is not None or args.smooth_pipe is not None:
+        smooth_cfg = {"smooth_threads": args.smooth_threads or 128,
+                      "smooth_layout": args.smooth_layout if args.smooth_layout is not None else 0,
+                      "smooth_pipe": args.smooth_pipe if args.smooth_pipe is not None else 0}
+        autotune.apply(m, smooth_cfg)
+        smooth_sweep = {}
+    elif args.profile:
+        smooth_cfg, smooth_sweep = {}, {}  # library defaults; --smooth-threads / --smooth-layout choose
+    else:
+        smooth_cfg, smooth_sweep = autotune.tune_smooth(m, x, out, use_cache=False)
+    if args.query_pipe is not None:
+        query_cfg, query_sweep = {"query_pipe": args.query_pipe}, {}
+        autotune.apply(m, query_cfg)
+    elif args.profile:
+        query_cfg, query_sweep = {}, {}
+    else:
+        query_cfg, query_sweep = autotune.tune_query(m, (vv_rp, vv_col, vf_col), use_cache=False)
+    torch.cuda.synchronize()
 
     if args.profile:
         fns = {"query_vv": lambda: m.query("VV", row_ptr=vv_rp, col=vv_col),
@@ -241,10 +266,6 @@ def main():
     kern["smooth_uniform"] = (t_sm, 24 * V + 4 * E)
     t_sc = event_time_ms(lambda: m.smooth_csr((vv_rp, vv_col), x, 0.5, 1, out=out), reps)
     kern["smooth_csr"] = (t_sc, 24 * V + 4 * E)
-    for th in (64, 256, 128):
-        m.set_launch_threads(0, th)
-        t = event_time_ms(lambda: m.smooth(x, 0.5, 1, out=out), reps)
-        kern["smooth_uniform_t%d" % th] = (t, 24 * V + 4 * E)
     t_vn = event_time_ms(lambda: m.vertex_normals(x, "max", out=out), reps)
     kern["vertex_normals"] = (t_vn, 24 * V + 4 * E + 6 * F)
     kernels = {k: {"us": round(t * 1e3, 2), "bytes": b, "GB_s": round(b / (t * 1e-3) / 1e9, 1),
@@ -266,9 +287,11 @@ def main():
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": (measured_traffic(args.workload, args.patch_size, dom)
-                                 if args.patcher == "kmeans" else None),
+                                 if args.patcher == "bisect" else None),
                      "algorithmic_bytes": db},
         "kernels": kernels,
+        "smooth_autotune": {"chosen": smooth_cfg, "us": smooth_sweep},
+        "query_autotune": {"chosen": query_cfg, "us": query_sweep},
     }
     # free the single-mesh workload before the batch
     del x, out, tmp, vv_rp, vv_col, vf_rp, vf_col

@@ -47,11 +47,11 @@ enum {
     RXM_OP_FE = 7, RXM_OP_FF = 8, RXM_OP_EV = 9, RXM_OP_EF = 11
 };
 
-/* rxm_mesh_create flags. RXM_PATCHER_BISECT selects the recursive
- * proportional bisection patcher (ceil(F / patch_size) connected patches, the
- * fewest the cap allows, and the smallest ribbons) instead of the default
- * k-means patcher (faster to build, patches ~0.7 of the cap). */
-enum { RXM_HOST_ONLY = 1, RXM_PATCHER_BISECT = 2 };
+/* rxm_mesh_create flags. The default patcher is a recursive proportional
+ * bisection (about ceil(F / patch_size) connected patches, the fewest the cap
+ * allows, with the smallest ribbons); RXM_PATCHER_KMEANS selects the k-means
+ * patcher instead (patches ~0.7 of the cap). */
+enum { RXM_HOST_ONLY = 1, RXM_PATCHER_KMEANS = 2 };
 
 /* vertex normal weighting */
 enum { RXM_NORMALS_MAX = 0, RXM_NORMALS_AREA = 1 };
@@ -165,7 +165,14 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *   kernel 0: smoothing workgroup size 64 / 128 / 256;
  *   kernel 1: symmetric-Dirichlet step workgroup size 128 / 256;
  *   kernel 2: symmetric-Dirichlet step launch, 0 = one workgroup per patch,
- *             1 = persistent grid over a global patch work queue. */
+ *             1 = persistent grid over a global patch work queue;
+ *   kernel 3: smoothing LDS layout of positions, 0 = three float arrays,
+ *             1 = one float4 per vertex;
+ *   kernel 4: smoothing grid, 0 = one workgroup per patch, n in 1..32 = a
+ *             persistent grid of n workgroups per CU that prefetches the next
+ *             patch while computing the current one;
+ *   kernel 5: the same choice for the VV / VE / VF queries.
+ * All settings give bit-identical results. */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 
 /* `iters` Jacobi steps of uniform Laplacian smoothing

@@ -22,7 +22,7 @@ ARR = {"ev": 0, "face_patch": 1, "vertex_patch": 2, "edge_patch": 3, "patches_va
        "patch_offsets": 15, "ltog_v": 16, "ltog_e": 17, "ltog_f": 18, "ev_local": 19,
        "fe_local": 20}
 RXM_HOST_ONLY = 1
-RXM_PATCHER_BISECT = 2
+RXM_PATCHER_KMEANS = 2
 
 
 class RXMError(RuntimeError):
@@ -124,14 +124,14 @@ class Mesh:
     (include/rxmesh/rxmesh_static.h:61-100)."""
 
     def __init__(self, fv, num_vertices=0, patch_size=512, patch_file=None, face_patch=None,
-                 host_only=False, batch=None, parts=None, patcher="kmeans"):
+                 host_only=False, batch=None, parts=None, patcher="bisect"):
         """fv: F x 3 triangles. Alternatives: batch = list of fv arrays (their
         disjoint union, built here), parts = list of built Mesh objects
         (their union, reusing the host build)."""
         h = ctypes.c_void_p()
         if patcher not in ("kmeans", "bisect"):
             raise ValueError("patcher must be 'kmeans' or 'bisect'")
-        flags = (RXM_HOST_ONLY if host_only else 0) | (RXM_PATCHER_BISECT if patcher == "bisect" else 0)
+        flags = (RXM_HOST_ONLY if host_only else 0) | (RXM_PATCHER_KMEANS if patcher == "kmeans" else 0)
         self._fv = None
         if parts is not None:
             self._parts = list(parts)

@@ -7,6 +7,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <numeric>
 #include <thread>
 
@@ -170,12 +171,12 @@ int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_h
 // ---------------------------------------------------------------------------
 // a2. Deterministic patchers. The reference grows patches with a GPU Lloyd
 // iteration (patcher.cu:828-987) seeded from std::random_device, so its
-// layout cannot be reproduced. The default keeps the same idea -- k-means on
-// the face graph: multi-source BFS assignment, re-centre each seed on its
-// patch's most interior face, repeat -- with seeds taken at even strides of
-// a BFS order, followed by halving of any patch above patch_size. The
-// alternative is a recursive proportional bisection (Bisector below).
-// Everything is sequential FIFO BFS with id tie-breaks: deterministic.
+// layout cannot be reproduced. The default is a recursive proportional
+// bisection (Bisector below). The alternative keeps the reference's idea --
+// k-means on the face graph: multi-source BFS assignment, re-centre each seed
+// on its patch's most interior face, repeat -- with seeds taken at even
+// strides of a BFS order, followed by halving of any patch above
+// patch_size. Both use FIFO BFS with id tie-breaks: deterministic.
 namespace {
 
 struct FaceGraph
@@ -283,36 +284,83 @@ void recenter(const FaceGraph& g, uint32_t nF, const std::vector<uint32_t>& fp,
 // faces of the set (BFS distances inside the set), into parts of
 // round(n * k1 / k) and the rest faces for k1 = k / 2 and k - k1 patches.
 // With k = ceil(F / cap) at the root every leaf holds about F / k <= cap
-// faces, the fewest patches the cap allows. Sets are processed depth-first
-// (patch ids follow the recursion, so neighbouring patches get nearby ids);
-// ties break on face id, so the result is deterministic.
+// faces, the fewest patches the cap allows. Leaves are listed depth-first
+// (neighbouring patches get nearby ids); ties break on distances and face
+// ids, so the result is deterministic. Large independent subtrees run on
+// worker threads: each set in flight carries its own tag, and a task reads
+// or writes the per-face scratch of its own faces only.
 class Bisector
 {
    public:
-    Bisector(const FaceGraph& g, uint32_t nF) : g_(g), pos_(nF, kInvalid32), dist_(nF, kInvalid32), side_(nF, 0) {}
+    using Leaves = std::vector<std::vector<uint32_t>>;
 
-    void run(std::vector<uint32_t> faces, uint32_t k, uint32_t cap, std::vector<uint32_t>& fp, uint32_t& next_id)
+    Bisector(const FaceGraph& g, uint32_t nF)
+        : g_(g), tag_(new std::atomic<uint32_t>[nF]), dist_(nF, kInvalid32), side_(nF, 0)
     {
-        if (k <= 1 && faces.size() <= cap) {
-            for (uint32_t f : faces) fp[f] = next_id;
-            ++next_id;
-            return;
+        for (uint32_t f = 0; f < nF; ++f) tag_[f].store(kInvalid32, std::memory_order_relaxed);
+    }
+
+    // Entry: every connected component gets ceil(size / cap) patches; fp
+    // receives patch ids in leaf order, the return value is the patch count.
+    uint32_t run_all(uint32_t nF, uint32_t cap, std::vector<uint32_t>& fp)
+    {
+        std::vector<uint32_t> all(nF);
+        std::iota(all.begin(), all.end(), 0u);
+        uint32_t next = 0;
+        for (auto& c : components(all)) {
+            std::sort(c.begin(), c.end());
+            const uint32_t k = static_cast<uint32_t>((c.size() + cap - 1) / cap);
+            for (const auto& leaf : run(std::move(c), k, cap)) {
+                for (uint32_t f : leaf) fp[f] = next;
+                ++next;
+            }
         }
+        return next;
+    }
+
+   private:
+    static constexpr size_t kParallelMin = size_t(1) << 15;
+
+    Leaves run(std::vector<uint32_t> faces, uint32_t k, uint32_t cap)
+    {
+        if (k <= 1 && faces.size() <= cap) return Leaves{std::move(faces)};
         if (k <= 1) k = 2;
+        std::vector<uint32_t> A, B;
+        uint32_t              kA, kB;
+        split(std::move(faces), k, cap, A, B, kA, kB);
+        Leaves left, right;
+        if (A.size() + B.size() >= kParallelMin && workers_.fetch_add(1) < num_threads()) {
+            std::thread th([&] { left = run(std::move(A), kA, cap); });
+            right = run(std::move(B), kB, cap);
+            th.join();
+            workers_.fetch_sub(1);
+        } else {
+            if (A.size() + B.size() >= kParallelMin) workers_.fetch_sub(1);
+            left  = run(std::move(A), kA, cap);
+            right = run(std::move(B), kB, cap);
+        }
+        for (auto& l : right) left.push_back(std::move(l));
+        return left;
+    }
+
+    void split(std::vector<uint32_t> faces, uint32_t k, uint32_t cap, std::vector<uint32_t>& A,
+               std::vector<uint32_t>& B, uint32_t& kA, uint32_t& kB)
+    {
         const uint32_t n  = static_cast<uint32_t>(faces.size());
         const uint32_t k1 = k / 2, k2 = k - k1;
         const uint32_t n1 = static_cast<uint32_t>((uint64_t(n) * k1 + k / 2) / k);
-        for (uint32_t i = 0; i < n; ++i) pos_[faces[i]] = i;
+        const uint32_t tag = mark(faces);
         // three candidate cut axes between mutually far faces a, b, c; the
         // one whose split cuts the fewest face adjacencies wins (keeps cuts
         // short on meshes with handles, e.g. around a torus tube)
-        const uint32_t a = farthest(faces, faces[0]);
+        std::vector<uint32_t> q;
+        const uint32_t        a = farthest(faces, faces[0], tag, q);
         std::vector<uint32_t> d[3];
-        const uint32_t        b = farthest(faces, a);
+        const uint32_t        b = farthest(faces, a, tag, q);
         d[0]                    = dists(faces);
-        bfs(faces, b);
-        d[1]         = dists(faces);
-        uint32_t c   = faces[0];
+        bfs(faces, b, tag, q);
+        d[1]           = dists(faces);
+        uint32_t c     = faces[0];
         uint32_t cbest = 0;
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t m = std::min(d[0][i], d[1][i]);
@@ -321,7 +369,7 @@ class Bisector
                 c     = faces[i];
             }
         }
-        bfs(faces, c);
+        bfs(faces, c, tag, q);
         d[2] = dists(faces);
         // (u, w, z): cut along d_u - d_w; faces tied on it (a level set)
         // are ordered by the distance to the third face z, so the median
@@ -344,54 +392,50 @@ class Bisector
             uint64_t cut = 0;
             for (uint32_t i = 0; i < n1; ++i) {
                 const uint32_t f = uint32_t(key[i].second);
-                for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) cut += (side_[g_.val[j]] == 2u);
+                for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) {
+                    const uint32_t o = g_.val[j];
+                    cut += (in(o, tag) && side_[o] == 2u);
+                }
             }
             if (cut < best_cut) {
                 best_cut = cut;
                 best_key = key;
             }
         }
-        for (uint32_t f : faces) {
-            pos_[f]  = kInvalid32;
-            side_[f] = 0;
-        }
-        std::vector<uint32_t> A, B;
+        for (uint32_t f : faces) side_[f] = 0;
+        A.clear();
+        B.clear();
         A.reserve(n1);
         B.reserve(n - n1);
         for (uint32_t i = 0; i < n; ++i) (i < n1 ? A : B).push_back(uint32_t(best_key[i].second));
-        std::sort(A.begin(), A.end());
+        std::vector<uint32_t>().swap(faces);
         // keep both halves connected: pieces cut off from the larger
         // component of one half border the other half, so they move there
-        std::vector<uint32_t>().swap(faces);
+        std::sort(A.begin(), A.end());
         reconnect(A, B);
+        std::sort(B.begin(), B.end());
         reconnect(B, A);
         std::sort(A.begin(), A.end());
         std::sort(B.begin(), B.end());
-        const uint32_t kA = std::max<uint32_t>(k1, static_cast<uint32_t>((A.size() + cap - 1) / cap));
-        const uint32_t kB = std::max<uint32_t>(k2, static_cast<uint32_t>((B.size() + cap - 1) / cap));
-        run(std::move(A), kA, cap, fp, next_id);
-        run(std::move(B), kB, cap, fp, next_id);
+        kA = std::max<uint32_t>(k1, static_cast<uint32_t>((A.size() + cap - 1) / cap));
+        kB = std::max<uint32_t>(k2, static_cast<uint32_t>((B.size() + cap - 1) / cap));
     }
 
-    // Entry: every connected component gets ceil(size / cap) patches.
-    void run_all(uint32_t nF, uint32_t cap, std::vector<uint32_t>& fp, uint32_t& next_id)
+    bool in(uint32_t f, uint32_t tag) const { return tag_[f].load(std::memory_order_relaxed) == tag; }
+
+    uint32_t mark(const std::vector<uint32_t>& faces)
     {
-        std::vector<uint32_t> all(nF);
-        std::iota(all.begin(), all.end(), 0u);
-        for (auto& c : components(all)) {
-            std::sort(c.begin(), c.end());
-            const uint32_t k = static_cast<uint32_t>((c.size() + cap - 1) / cap);
-            run(std::move(c), k, cap, fp, next_id);
-        }
+        const uint32_t tag = next_tag_.fetch_add(1);
+        for (uint32_t f : faces) tag_[f].store(tag, std::memory_order_relaxed);
+        return tag;
     }
 
-   private:
     // connected components of a face set, largest first (ties: smaller first face)
-    std::vector<std::vector<uint32_t>> components(const std::vector<uint32_t>& faces)
+    Leaves components(const std::vector<uint32_t>& faces)
     {
-        for (uint32_t i = 0; i < faces.size(); ++i) pos_[faces[i]] = i;
+        const uint32_t tag = mark(faces);
         for (uint32_t f : faces) dist_[f] = kInvalid32;
-        std::vector<std::vector<uint32_t>> out;
+        Leaves out;
         for (uint32_t s : faces) {
             if (dist_[s] != kInvalid32) continue;
             std::vector<uint32_t> c{s};
@@ -399,16 +443,15 @@ class Bisector
             for (size_t h = 0; h < c.size(); ++h) {
                 const uint32_t f = c[h];
                 for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) {
-                    const uint32_t n = g_.val[j];
-                    if (pos_[n] != kInvalid32 && dist_[n] == kInvalid32) {
-                        dist_[n] = 0;
-                        c.push_back(n);
+                    const uint32_t o = g_.val[j];
+                    if (in(o, tag) && dist_[o] == kInvalid32) {
+                        dist_[o] = 0;
+                        c.push_back(o);
                     }
                 }
             }
             out.push_back(std::move(c));
         }
-        for (uint32_t f : faces) pos_[f] = kInvalid32;
         std::stable_sort(out.begin(), out.end(),
                          [](const std::vector<uint32_t>& x, const std::vector<uint32_t>& y) { return x.size() > y.size(); });
         return out;
@@ -423,29 +466,29 @@ class Bisector
         for (size_t i = 1; i < comps.size(); ++i) to.insert(to.end(), comps[i].begin(), comps[i].end());
     }
 
-    // BFS inside the current set (pos_[f] != kInvalid32); fills dist_ for its faces
-    void bfs(const std::vector<uint32_t>& faces, uint32_t src)
+    // BFS inside the set tagged `tag`; fills dist_ for its faces
+    void bfs(const std::vector<uint32_t>& faces, uint32_t src, uint32_t tag, std::vector<uint32_t>& q)
     {
         for (uint32_t f : faces) dist_[f] = kInvalid32;
-        q_.clear();
-        q_.push_back(src);
+        q.clear();
+        q.push_back(src);
         dist_[src] = 0;
-        for (size_t h = 0; h < q_.size(); ++h) {
-            const uint32_t f = q_[h];
+        for (size_t h = 0; h < q.size(); ++h) {
+            const uint32_t f = q[h];
             for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) {
-                const uint32_t n = g_.val[j];
-                if (pos_[n] != kInvalid32 && dist_[n] == kInvalid32) {
-                    dist_[n] = dist_[f] + 1;
-                    q_.push_back(n);
+                const uint32_t o = g_.val[j];
+                if (in(o, tag) && dist_[o] == kInvalid32) {
+                    dist_[o] = dist_[f] + 1;
+                    q.push_back(o);
                 }
             }
         }
     }
     // last face reached by a BFS from src (largest distance; FIFO order breaks ties)
-    uint32_t farthest(const std::vector<uint32_t>& faces, uint32_t src)
+    uint32_t farthest(const std::vector<uint32_t>& faces, uint32_t src, uint32_t tag, std::vector<uint32_t>& q)
     {
-        bfs(faces, src);
-        return q_.back();
+        bfs(faces, src, tag, q);
+        return q.back();
     }
 
     std::vector<uint32_t> dists(const std::vector<uint32_t>& faces) const
@@ -455,8 +498,11 @@ class Bisector
         return r;
     }
 
-    const FaceGraph&      g_;
-    std::vector<uint32_t> pos_, dist_, side_, q_;
+    const FaceGraph&                         g_;
+    std::unique_ptr<std::atomic<uint32_t>[]> tag_;
+    std::vector<uint32_t>                    dist_, side_;
+    std::atomic<uint32_t>                    next_tag_{0};
+    std::atomic<uint32_t>                    workers_{0};
 };
 
 // BFS distances restricted to a face subset (mask == tag)
@@ -537,7 +583,7 @@ int run_patcher(HostMesh& m, uint32_t patch_size, int mode)
     } else {
         fp.assign(nF, kInvalid32);
         Bisector bs(g, nF);
-        bs.run_all(nF, patch_size, fp, nP);
+        nP = bs.run_all(nF, patch_size, fp);
     }
     // recursive bisection of oversize patches
     std::vector<std::vector<uint32_t>> members(nP);

@@ -73,16 +73,17 @@ enum Status : int
 int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint);
 
 // Step a2: deterministic patcher (fills face_patch / patches_val / patches_offset).
+//   kPatcherBisect: recursive proportional bisection into about
+//     ceil(F / patch_size) connected patches per component (fewest patches,
+//     least ribbon; subtrees run on worker threads);
 //   kPatcherKMeans: k-means on the face graph with ceil(F / patch_size) seeds,
-//     oversize patches halved (bounded patches; about 0.7 patch_size faces each);
-//   kPatcherBisect: recursive proportional bisection into ceil(F / patch_size)
-//     connected patches per component (fewest patches, least ribbon; slower build).
+//     oversize patches halved (about 0.7 patch_size faces each).
 enum PatcherMode : int
 {
-    kPatcherKMeans = 0,
-    kPatcherBisect = 1,
+    kPatcherBisect = 0,
+    kPatcherKMeans = 1,
 };
-int run_patcher(HostMesh& m, uint32_t patch_size, int mode = kPatcherKMeans);
+int run_patcher(HostMesh& m, uint32_t patch_size, int mode = kPatcherBisect);
 
 // Alternative to run_patcher: take a given assignment (e.g. a reference patch
 // file). patches_val / patches_offset may be null (then grouped by face order).

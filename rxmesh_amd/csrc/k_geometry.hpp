@@ -26,6 +26,56 @@ __device__ __forceinline__ void stage_positions(float* sx, float* sy, float* sz,
     }
 }
 
+// Same staging with all of a thread's loads issued before any LDS store:
+// R rounds of T vertices are unrolled, so a thread's owner-table loads go out
+// together, then its position gathers, instead of one dependent load chain
+// per loop iteration. Vertices past R * T (larger patches) take the loop.
+template <uint32_t T, uint32_t R, bool AOS4>
+__device__ __forceinline__ void stage_positions_batched(float* sx, float* sy, float* sz, float4* s4,
+                                                        const float* __restrict__ pos, uint32_t N,
+                                                        const PatchDesc& d, const uint32_t* __restrict__ v_lin)
+{
+    uint32_t g[R];
+    float    x[R], y[R], z[R];
+    const uint32_t* lin = v_lin + d.v_off;
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t l = threadIdx.x + r * T;
+        g[r]             = (l < d.nvo) ? d.vp + l : ((l < d.nv) ? lin[l] : 0u);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        if (threadIdx.x + r * T < d.nv) {
+            x[r] = pos[g[r]];
+            y[r] = pos[N + g[r]];
+            z[r] = pos[2 * N + g[r]];
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t l = threadIdx.x + r * T;
+        if (l < d.nv) {
+            if constexpr (AOS4) {
+                s4[l] = make_float4(x[r], y[r], z[r], 0.f);
+            } else {
+                sx[l] = x[r];
+                sy[l] = y[r];
+                sz[l] = z[r];
+            }
+        }
+    }
+    for (uint32_t l = threadIdx.x + R * T; l < d.nv; l += T) {
+        const uint32_t gg = lin_of(l, d.nvo, d.vp, lin);
+        if constexpr (AOS4) {
+            s4[l] = make_float4(pos[gg], pos[N + gg], pos[2 * N + gg], 0.f);
+        } else {
+            sx[l] = pos[gg];
+            sy[l] = pos[N + gg];
+            sz[l] = pos[2 * N + gg];
+        }
+    }
+}
+
 // ---- vertex normals -------------------------------------------------------
 struct PlanVN
 {
@@ -110,66 +160,6 @@ __global__ __launch_bounds__(kGeomThreads) void k_vertex_normals(DevMesh m, int 
     }
 }
 
-// ---- uniform Laplacian smoothing (one Jacobi step) -------------------------
-struct PlanSM
-{
-    uint32_t *ev, *off, *cur, *wave;
-    uint16_t* val;
-    float *   x, *y, *z;
-};
-__host__ __device__ inline PlanSM plan_sm(Bump& b, uint32_t nv, uint32_t ne)
-{
-    PlanSM p{};
-    p.ev   = b.alloc<uint32_t>(r4(ne));
-    p.off  = b.alloc<uint32_t>(nv + 1);
-    p.cur  = b.alloc<uint32_t>(nv);
-    p.wave = b.alloc<uint32_t>(20);
-    p.val  = b.alloc<uint16_t>(2 * ne);
-    p.x    = b.alloc<float>(nv);
-    p.y    = b.alloc<float>(nv);
-    p.z    = b.alloc<float>(nv);
-    return p;
-}
-
-// x' = x + lambda * (mean of VV neighbours - x); neighbours summed in
-// ascending local-edge order.
-__global__ __launch_bounds__(kGeomThreads) void k_smooth_uniform(DevMesh m, float lambda, const float* __restrict__ pos,
-                                                                 float* __restrict__ out)
-{
-    extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
-    const PatchDesc d = load_desc(m.patch, p);
-    Bump            b(reinterpret_cast<char*>(smem));
-    const PlanSM    L = plan_sm(b, d.nv, d.ne);
-    const uint32_t  T = blockDim.x, N = m.nV;
-    stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_positions(L.x, L.y, L.z, pos, N, d, m.v_lin);
-    __syncthreads();
-    lds_transpose<2>(2 * d.ne, d.nv,
-                     [&](uint32_t i) {
-                         const uint32_t x = L.ev[i >> 1];
-                         return (i & 1u) ? hi16(x) : lo16(x);
-                     },
-                     L.off, L.cur, L.val, L.wave, d.nvo);
-    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
-        float          sx = 0.f, sy = 0.f, sz = 0.f;
-        const uint32_t k0 = L.off[v], k1 = L.off[v + 1];
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t x = L.ev[L.val[k]];
-            const uint32_t w = (lo16(x) == v) ? hi16(x) : lo16(x);
-            sx += L.x[w];
-            sy += L.y[w];
-            sz += L.z[w];
-        }
-        const float    inv = (k1 > k0) ? 1.0f / static_cast<float>(k1 - k0) : 0.f;
-        const uint32_t g   = d.vp + v;
-        const float    px = L.x[v], py = L.y[v], pz = L.z[v];
-        out[g]         = (k1 > k0) ? px + lambda * (sx * inv - px) : px;
-        out[N + g]     = (k1 > k0) ? py + lambda * (sy * inv - py) : py;
-        out[2 * N + g] = (k1 > k0) ? pz + lambda * (sz * inv - pz) : pz;
-    }
-}
-
 // ---- uniform smoothing over the patch-local VV plan --------------------------
 // One workgroup per patch with a small LDS footprint (positions + the
 // patch's plan rows), so many patches are in flight per CU. Rows are read
@@ -178,15 +168,22 @@ struct PlanSL
 {
     uint16_t *ro, *nb;
     float *   x, *y, *z;
+    float4*   p4;
 };
-__host__ __device__ inline PlanSL plan_sl(Bump& b, uint32_t nv, uint32_t nvo, uint32_t nent)
+// AOS4: positions staged as one float4 per vertex (one 16-byte LDS access
+// per neighbour) instead of three float arrays
+__host__ __device__ inline PlanSL plan_sl(Bump& b, uint32_t nv, uint32_t nvo, uint32_t nent, bool aos4 = false)
 {
     PlanSL p{};
     p.ro = b.alloc<uint16_t>(((nvo + 1u) + 7u) & ~7u);
     p.nb = b.alloc<uint16_t>((nent + 7u) & ~7u);
-    p.x  = b.alloc<float>(nv);
-    p.y  = b.alloc<float>(nv);
-    p.z  = b.alloc<float>(nv);
+    if (aos4) {
+        p.p4 = b.alloc<float4>(nv);
+    } else {
+        p.x = b.alloc<float>(nv);
+        p.y = b.alloc<float>(nv);
+        p.z = b.alloc<float>(nv);
+    }
     return p;
 }
 
@@ -200,7 +197,7 @@ __device__ __forceinline__ void stage_u16x8(uint16_t* dst, const uint16_t* __res
     for (uint32_t i = threadIdx.x; i < n8; i += T) d[i] = s[i];
 }
 
-template <uint32_t T>
+template <uint32_t T, bool AOS4>
 __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* __restrict__ rbase,
                                                     const uint32_t* __restrict__ nbase,
                                                     const uint16_t* __restrict__ roff,
@@ -213,23 +210,39 @@ __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* _
     const PatchDesc d = load_desc(m.patch, p);
     const uint32_t  rb = rbase[p], nbb = nbase[p], nent = nbase[p + 1] - nbb;
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanSL    L = plan_sl(b, d.nv, d.nvo, nent);
+    const PlanSL    L = plan_sl(b, d.nv, d.nvo, nent, AOS4);
     const uint32_t  N = m.nV;
     stage_u16x8<T>(L.ro, roff + rb, d.nvo + 1);
     stage_u16x8<T>(L.nb, nbr + nbb, nent);
-    stage_positions(L.x, L.y, L.z, pos, N, d, m.v_lin);
+    stage_positions_batched<T, 512u / T, AOS4>(L.x, L.y, L.z, L.p4, pos, N, d, m.v_lin);
     __syncthreads();
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
         float          sx = 0.f, sy = 0.f, sz = 0.f;
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t w = L.nb[k];
-            sx += L.x[w];
-            sy += L.y[w];
-            sz += L.z[w];
+        float          px, py, pz;
+        if constexpr (AOS4) {
+            for (uint32_t k = k0; k < k1; ++k) {
+                const float4 q = L.p4[L.nb[k]];
+                sx += q.x;
+                sy += q.y;
+                sz += q.z;
+            }
+            const float4 q = L.p4[v];
+            px             = q.x;
+            py             = q.y;
+            pz             = q.z;
+        } else {
+            for (uint32_t k = k0; k < k1; ++k) {
+                const uint32_t w = L.nb[k];
+                sx += L.x[w];
+                sy += L.y[w];
+                sz += L.z[w];
+            }
+            px = L.x[v];
+            py = L.y[v];
+            pz = L.z[v];
         }
-        const uint32_t g  = d.vp + v;
-        const float    px = L.x[v], py = L.y[v], pz = L.z[v];
+        const uint32_t g = d.vp + v;
         if (k1 > k0) {
             const float inv = 1.0f / static_cast<float>(k1 - k0);
             out[g]          = px + lambda * (sx * inv - px);
@@ -240,6 +253,194 @@ __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* _
             out[N + g]     = py;
             out[2 * N + g] = pz;
         }
+    }
+}
+
+// ---- persistent, software-pipelined smoothing ---------------------------------
+// A machine-sized grid of workgroups; workgroup b belongs to XCD b % 8 and
+// walks that XCD's contiguous patch range with stride gridDim.x / 8. While
+// patch i is computed from LDS, patch i+1's plan rows, owned positions and
+// ribbon owner-table entries are already in flight into registers (RP uint4
+// of plan and RV vertices per thread); per patch only the ribbon position
+// gathers, which need those entries, are waited on. Anything beyond the
+// register capacity is loaded directly. Same arithmetic and summation order
+// as k_smooth_local, so the results are identical.
+template <uint32_t T, uint32_t RP, uint32_t RV>
+struct SmoothStage
+{
+    uint4    plan[RP];
+    float    x[RV], y[RV], z[RV];
+    uint32_t lin[RV];
+};
+
+template <uint32_t T, bool AOS4, uint32_t RP, uint32_t RV>
+__global__ __launch_bounds__(T) void k_smooth_pipe(DevMesh m, const uint32_t* __restrict__ rbase,
+                                                   const uint32_t* __restrict__ nbase,
+                                                   const uint16_t* __restrict__ roff,
+                                                   const uint16_t* __restrict__ nbr, float lambda,
+                                                   const float* __restrict__ pos, float* __restrict__ out)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t P = m.P, N = m.nV, tid = threadIdx.x;
+    const uint32_t G8 = gridDim.x >> 3, xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
+    const uint32_t chunk = (P + 7u) >> 3;
+    const uint32_t p_end = min(P, (xcd + 1u) * chunk);
+    uint32_t       p     = xcd * chunk + j;
+    if (p >= p_end) return;
+
+    SmoothStage<T, RP, RV> s;
+    // issue the loads of patch q's plan rows, owned positions and ribbon entries
+    auto prefetch = [&](const PatchDesc& d, uint32_t rb, uint32_t nb0, uint32_t nent) {
+        const uint32_t n8r = (d.nvo + 1u + 7u) >> 3, n8 = n8r + ((nent + 7u) >> 3);
+        const uint4*   r4p = reinterpret_cast<const uint4*>(roff + rb);
+        const uint4*   n4p = reinterpret_cast<const uint4*>(nbr + nb0);
+#pragma unroll
+        for (uint32_t r = 0; r < RP; ++r) {
+            const uint32_t i = tid + r * T;
+            if (i < n8) s.plan[r] = (i < n8r) ? r4p[i] : n4p[i - n8r];
+        }
+        const uint32_t* lin = m.v_lin + d.v_off;
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            const uint32_t l = tid + r * T;
+            if (l < d.nvo) {
+                const uint32_t g = d.vp + l;
+                s.x[r]           = pos[g];
+                s.y[r]           = pos[N + g];
+                s.z[r]           = pos[2 * N + g];
+            } else if (l < d.nv) {
+                s.lin[r] = lin[l];
+            }
+        }
+    };
+    // move the prefetched plan rows and owned positions into LDS (+ overflow)
+    auto commit = [&](const PatchDesc& d, uint32_t rb, uint32_t nb0, uint32_t nent, const PlanSL& L) {
+        const uint32_t n8r = (d.nvo + 1u + 7u) >> 3, n8 = n8r + ((nent + 7u) >> 3);
+        uint4*         lp  = reinterpret_cast<uint4*>(L.ro);  // ro and nb are contiguous uint4 runs
+#pragma unroll
+        for (uint32_t r = 0; r < RP; ++r) {
+            const uint32_t i = tid + r * T;
+            if (i < n8) lp[i] = s.plan[r];
+        }
+        for (uint32_t i = tid + RP * T; i < n8; i += T)
+            lp[i] = (i < n8r) ? reinterpret_cast<const uint4*>(roff + rb)[i]
+                              : reinterpret_cast<const uint4*>(nbr + nb0)[i - n8r];
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            const uint32_t l = tid + r * T;
+            if (l < d.nvo) {
+                if constexpr (AOS4) L.p4[l] = make_float4(s.x[r], s.y[r], s.z[r], 0.f);
+                else {
+                    L.x[l] = s.x[r];
+                    L.y[l] = s.y[r];
+                    L.z[l] = s.z[r];
+                }
+            }
+        }
+    };
+    // gather the ribbon positions (owner-table entries already in registers)
+    auto ribbon = [&](const PatchDesc& d, const PlanSL& L) {
+        float x[RV], y[RV], z[RV];
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            const uint32_t l = tid + r * T;
+            if (l >= d.nvo && l < d.nv) {
+                const uint32_t g = s.lin[r];
+                x[r]             = pos[g];
+                y[r]             = pos[N + g];
+                z[r]             = pos[2 * N + g];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            const uint32_t l = tid + r * T;
+            if (l >= d.nvo && l < d.nv) {
+                if constexpr (AOS4) L.p4[l] = make_float4(x[r], y[r], z[r], 0.f);
+                else {
+                    L.x[l] = x[r];
+                    L.y[l] = y[r];
+                    L.z[l] = z[r];
+                }
+            }
+        }
+        // vertices past the register capacity
+        for (uint32_t l = tid + RV * T; l < d.nv; l += T) {
+            const uint32_t g = lin_of(l, d.nvo, d.vp, m.v_lin + d.v_off);
+            if constexpr (AOS4) L.p4[l] = make_float4(pos[g], pos[N + g], pos[2 * N + g], 0.f);
+            else {
+                L.x[l] = pos[g];
+                L.y[l] = pos[N + g];
+                L.z[l] = pos[2 * N + g];
+            }
+        }
+    };
+
+    PatchDesc d  = load_desc(m.patch, p);
+    uint32_t  rb = rbase[p], nb0 = nbase[p], nent = nbase[p + 1] - nb0;
+    prefetch(d, rb, nb0, nent);
+    while (true) {
+        Bump         b(reinterpret_cast<char*>(smem));
+        const PlanSL L = plan_sl(b, d.nv, d.nvo, nent, AOS4);
+        commit(d, rb, nb0, nent, L);
+        ribbon(d, L);
+        // next patch: descriptor, then its loads go out before this patch's compute
+        const uint32_t pn   = p + G8;
+        const bool     more = pn < p_end;
+        PatchDesc      dn{};
+        uint32_t       rbn = 0, nbn = 0, nentn = 0;
+        if (more) {
+            dn    = load_desc(m.patch, pn);
+            rbn   = rbase[pn];
+            nbn   = nbase[pn];
+            nentn = nbase[pn + 1] - nbn;
+        }
+        __syncthreads();
+        if (more) prefetch(dn, rbn, nbn, nentn);
+        for (uint32_t v = tid; v < d.nvo; v += T) {
+            const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+            float          sx = 0.f, sy = 0.f, sz = 0.f;
+            float          px, py, pz;
+            if constexpr (AOS4) {
+                for (uint32_t k = k0; k < k1; ++k) {
+                    const float4 q = L.p4[L.nb[k]];
+                    sx += q.x;
+                    sy += q.y;
+                    sz += q.z;
+                }
+                const float4 q = L.p4[v];
+                px = q.x;
+                py = q.y;
+                pz = q.z;
+            } else {
+                for (uint32_t k = k0; k < k1; ++k) {
+                    const uint32_t w = L.nb[k];
+                    sx += L.x[w];
+                    sy += L.y[w];
+                    sz += L.z[w];
+                }
+                px = L.x[v];
+                py = L.y[v];
+                pz = L.z[v];
+            }
+            const uint32_t g = d.vp + v;
+            if (k1 > k0) {
+                const float inv = 1.0f / static_cast<float>(k1 - k0);
+                out[g]          = px + lambda * (sx * inv - px);
+                out[N + g]      = py + lambda * (sy * inv - py);
+                out[2 * N + g]  = pz + lambda * (sz * inv - pz);
+            } else {
+                out[g]         = px;
+                out[N + g]     = py;
+                out[2 * N + g] = pz;
+            }
+        }
+        if (!more) break;
+        __syncthreads();  // LDS is reused by the next patch
+        p    = pn;
+        d    = dn;
+        rb   = rbn;
+        nb0  = nbn;
+        nent = nentn;
     }
 }
 
@@ -284,7 +485,7 @@ __global__ __launch_bounds__(T) void k_vertex_normals_plan(DevMesh m, int mode, 
     stage_u16x8<T>(L.fvl, fvl + fbase[p], 3 * d.nf);
     stage_u16x8<T>(L.ro, roff + rbase[p], d.nvo + 1);
     stage_u16x8<T>(L.vf, vf + nb0, nent);
-    stage_positions(L.x, L.y, L.z, pos, N, d, m.v_lin);
+    stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, pos, N, d, m.v_lin);
     __syncthreads();
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         float sx = 0.f, sy = 0.f, sz = 0.f;

@@ -35,43 +35,6 @@ __host__ __device__ inline PlanVX plan_vx(Bump& b, uint32_t nv, uint32_t ne, uin
     return p;
 }
 
-struct PlanVF
-{
-    uint32_t *ev, *off, *cur, *wave;
-    uint16_t *fe, *fvl, *val, *rowof;
-};
-__host__ __device__ inline PlanVF plan_vf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
-{
-    PlanVF p{};
-    p.ev    = b.alloc<uint32_t>(r4(ne));
-    p.off   = b.alloc<uint32_t>(nv + 1);
-    p.cur   = b.alloc<uint32_t>(nv);
-    p.wave  = b.alloc<uint32_t>(20);
-    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
-    p.fvl   = b.alloc<uint16_t>(3 * nf);
-    p.val   = b.alloc<uint16_t>(3 * nf);
-    p.rowof = b.alloc<uint16_t>(3 * nf);
-    return p;
-}
-
-struct PlanEF
-{
-    uint32_t *off, *cur, *wave, *cnt;
-    uint16_t *fe, *val, *rowof;
-};
-__host__ __device__ inline PlanEF plan_ef(Bump& b, uint32_t ne, uint32_t nf, uint32_t nfo)
-{
-    PlanEF p{};
-    p.off   = b.alloc<uint32_t>(ne + 1);
-    p.cur   = b.alloc<uint32_t>(ne);
-    p.wave  = b.alloc<uint32_t>(20);
-    p.cnt   = b.alloc<uint32_t>(nfo + 1);
-    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
-    p.val   = b.alloc<uint16_t>(3 * nf);
-    p.rowof = b.alloc<uint16_t>(3 * nf);
-    return p;
-}
-
 struct PlanFX
 {
     uint32_t* ev;
@@ -145,89 +108,6 @@ __global__ __launch_bounds__(kQueryThreads) void k_query_vx(DevMesh m, const uin
             out = lin_of(e, d.neo, d.ep, m.e_lin + d.e_off);
         }
         col[bse + k] = static_cast<int32_t>(out);
-    }
-}
-
-// VF: FV (from FE + EV) transposed
-__global__ __launch_bounds__(kQueryThreads) void k_query_vf(DevMesh m, const uint32_t* __restrict__ base,
-                                                            int32_t* __restrict__ row_ptr,
-                                                            int32_t* __restrict__ col)
-{
-    extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
-    const PatchDesc d = load_desc(m.patch, p);
-    Bump            b(reinterpret_cast<char*>(smem));
-    const PlanVF    L = plan_vf(b, d.nv, d.ne, d.nf);
-    const uint32_t  T = blockDim.x;
-    stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T)
-        L.fvl[i] = static_cast<uint16_t>(face_vertex(L.fe, L.ev, i / 3, i % 3));
-    __syncthreads();
-    lds_transpose<3>(3 * d.nf, d.nv, [&](uint32_t i) { return static_cast<uint32_t>(L.fvl[i]); }, L.off,
-                     L.cur, L.val, L.wave, d.nvo);
-    for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
-        for (uint32_t k = L.off[v]; k < L.off[v + 1]; ++k) L.rowof[k] = static_cast<uint16_t>(v);
-    __syncthreads();
-    const uint32_t bse  = base[p];
-    const uint32_t nent = L.off[d.nvo];
-    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) row_ptr[d.vp + v] = static_cast<int32_t>(bse + L.off[v]);
-    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + nent);
-    for (uint32_t k = threadIdx.x; k < nent; k += T)
-        col[bse + k] = static_cast<int32_t>(lin_of(L.val[k], d.nfo, d.fp, m.f_lin + d.f_off));
-}
-
-// EF (IS_FF == false) or FF = EF o FE minus self (reference f_f,
-// rxmesh_queries.cuh:771-858: per face, edges in FE order, faces of each edge
-// in EF order)
-template <bool IS_FF>
-__global__ __launch_bounds__(kQueryThreads) void k_query_ef(DevMesh m, const uint32_t* __restrict__ base,
-                                                            int32_t* __restrict__ row_ptr,
-                                                            int32_t* __restrict__ col)
-{
-    extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
-    const PatchDesc d = load_desc(m.patch, p);
-    Bump            b(reinterpret_cast<char*>(smem));
-    const PlanEF    L = plan_ef(b, d.ne, d.nf, d.nfo);
-    const uint32_t  T = blockDim.x;
-    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
-    __syncthreads();
-    lds_transpose<3>(3 * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
-                     L.off, L.cur, L.val, L.wave, IS_FF ? d.ne : d.neo);
-    const uint32_t bse = base[p];
-    if constexpr (!IS_FF) {
-        for (uint32_t e = threadIdx.x; e < d.neo; e += T)
-            for (uint32_t k = L.off[e]; k < L.off[e + 1]; ++k) L.rowof[k] = static_cast<uint16_t>(e);
-        __syncthreads();
-        const uint32_t nent = L.off[d.neo];
-        for (uint32_t e = threadIdx.x; e < d.neo; e += T) row_ptr[d.ep + e] = static_cast<int32_t>(bse + L.off[e]);
-        if (threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(bse + nent);
-        for (uint32_t k = threadIdx.x; k < nent; k += T)
-            col[bse + k] = static_cast<int32_t>(lin_of(L.val[k], d.nfo, d.fp, m.f_lin + d.f_off));
-    } else {
-        for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
-            uint32_t c = 0;
-            for (uint32_t i = 0; i < 3; ++i) {
-                const uint32_t e = L.fe[3 * f + i] >> 1;
-                c += L.off[e + 1] - L.off[e] - 1;
-            }
-            L.cnt[f] = c;
-        }
-        const uint32_t total = block_exclusive_scan(L.cnt, d.nfo, L.wave);
-        for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
-            uint32_t k = bse + L.cnt[f];
-            row_ptr[d.fp + f] = static_cast<int32_t>(k);
-            for (uint32_t i = 0; i < 3; ++i) {
-                const uint32_t e = L.fe[3 * f + i] >> 1;
-                for (uint32_t j = L.off[e]; j < L.off[e + 1]; ++j) {
-                    const uint32_t g = L.val[j];
-                    if (g != f) col[k++] = static_cast<int32_t>(lin_of(g, d.nfo, d.fp, m.f_lin + d.f_off));
-                }
-            }
-        }
-        if (threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(bse + total);
     }
 }
 

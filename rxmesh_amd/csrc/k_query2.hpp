@@ -20,28 +20,44 @@ struct PlanQ2
     uint32_t *ev, *off, *cur, *slots, *wave, *lin;
     uint16_t *fe;
 };
-// kind: 0 = VV/VE (EV only), 1 = VF (EV + FE), 2 = EF/FF (FE only)
+// kind: 0 = VV, 3 = VE (EV only), 1 = VF (EV + FE), 2 = EF/FF (FE only).
+// Every array is sized for its kind, so a workgroup holds no more LDS than
+// its relation needs (occupancy is LDS-bound).
 __host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32_t ne, uint32_t nf)
 {
     PlanQ2 p{};
+    const bool     vx    = (kind == 0 || kind == 3);
     const uint32_t ncols = (kind == 2) ? ne : nv;
-    const uint32_t n     = (kind == 0) ? 2 * ne : 3 * nf;
+    const uint32_t n     = vx ? 2 * ne : 3 * nf;
     if (kind != 2) p.ev = b.alloc<uint32_t>(r4(ne));
-    if (kind != 0) p.fe = b.alloc<uint16_t>(r4(3 * nf));
+    if (kind == 1 || kind == 2) p.fe = b.alloc<uint16_t>(r4(3 * nf));
     p.off   = b.alloc<uint32_t>(ncols + 1);
-    p.cur   = b.alloc<uint32_t>((ncols > nf ? ncols : nf) + 1);  // FF reuses it per owned face
+    p.cur   = b.alloc<uint32_t>(((kind == 2 && nf > ncols) ? nf : ncols) + 1);  // FF reuses it per owned face
     p.slots = b.alloc<uint32_t>(n);
     p.wave  = b.alloc<uint32_t>(20);
-    p.lin   = b.alloc<uint32_t>((kind == 2) ? nf : ((kind == 0) ? (nv > ne ? nv : ne) : nf));
+    p.lin   = b.alloc<uint32_t>((kind == 0) ? nv : ((kind == 3) ? ne : nf));
     return p;
 }
 
 // Target linear ids of all local elements (owned: prefix + l; ribbon: owner
 // table), staged once so row writes need no global gathers.
+// Loads of four rounds are issued before their LDS stores.
 __device__ __forceinline__ void stage_lin(uint32_t* dst, uint32_t n, uint32_t n_owned, uint32_t prefix,
                                           const uint32_t* __restrict__ slab, uint32_t T)
 {
-    for (uint32_t l = threadIdx.x; l < n; l += T) dst[l] = (l < n_owned) ? prefix + l : slab[l];
+    for (uint32_t base = 0; base < n; base += 4u * T) {
+        uint32_t v[4];
+#pragma unroll
+        for (uint32_t r = 0; r < 4u; ++r) {
+            const uint32_t l = base + threadIdx.x + r * T;
+            v[r]             = (l < n_owned) ? prefix + l : ((l < n) ? slab[l] : 0u);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < 4u; ++r) {
+            const uint32_t l = base + threadIdx.x + r * T;
+            if (l < n) dst[l] = v[r];
+        }
+    }
 }
 
 // Coalesced copy of n u32 from LDS to a global int32 run (thread i writes
@@ -61,7 +77,7 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
     const uint32_t  p = blockIdx.x;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanQ2    L = plan_q2(b, 0, d.nv, d.ne, d.nf);
+    const PlanQ2    L = plan_q2(b, IS_VV ? 0 : 3, d.nv, d.ne, d.nf);
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
     if (IS_VV) stage_lin(L.lin, d.nv, d.nvo, d.vp, m.v_lin + d.v_off, T);
     else stage_lin(L.lin, d.ne, d.neo, d.ep, m.e_lin + d.e_off, T);
@@ -91,6 +107,177 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
     if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
     __syncthreads();
     copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
+}
+
+// ---- persistent, software-pipelined VV / VE / VF ------------------------------
+// A machine-sized grid; workgroup b walks XCD (b % 8)'s contiguous patch
+// range with stride gridDim.x / 8. The next patch's topology (EV, FE) and
+// ribbon owner-table entries are loaded into registers (RE / RF / RL rounds
+// per thread) while the current patch's LDS sort runs, so the global round
+// trips of a patch overlap the previous patch's work. Rows and entries are
+// identical to the one-workgroup-per-patch kernels.
+struct PipeRange
+{
+    uint32_t p, p_end, stride;
+};
+__device__ __forceinline__ PipeRange pipe_range(uint32_t P)
+{
+    const uint32_t chunk = (P + 7u) >> 3, xcd = blockIdx.x & 7u;
+    return PipeRange{xcd * chunk + (blockIdx.x >> 3), min(P, (xcd + 1u) * chunk), gridDim.x >> 3};
+}
+
+// owned ids map by prefix, ribbon ids come from registers (first RL rounds)
+// or straight from the slab (overflow)
+template <uint32_t T, uint32_t RL>
+__device__ __forceinline__ void commit_lin(uint32_t* dst, uint32_t n, uint32_t n_owned, uint32_t prefix,
+                                           const uint32_t* __restrict__ slab, const uint32_t (&ln)[RL])
+{
+    for (uint32_t l = threadIdx.x; l < n_owned; l += T) dst[l] = prefix + l;
+#pragma unroll
+    for (uint32_t r = 0; r < RL; ++r) {
+        const uint32_t l = n_owned + threadIdx.x + r * T;
+        if (l < n) dst[l] = ln[r];
+    }
+    for (uint32_t l = n_owned + threadIdx.x + RL * T; l < n; l += T) dst[l] = slab[l];
+}
+template <uint32_t T, uint32_t RL>
+__device__ __forceinline__ void prefetch_lin(uint32_t (&ln)[RL], uint32_t n, uint32_t n_owned,
+                                             const uint32_t* __restrict__ slab)
+{
+#pragma unroll
+    for (uint32_t r = 0; r < RL; ++r) {
+        const uint32_t l = n_owned + threadIdx.x + r * T;
+        if (l < n) ln[r] = slab[l];
+    }
+}
+template <uint32_t T, uint32_t R, typename V>
+__device__ __forceinline__ void prefetch_vec(V (&reg)[R], const V* __restrict__ src, uint32_t n)
+{
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t i = threadIdx.x + r * T;
+        if (i < n) reg[r] = src[i];
+    }
+}
+template <uint32_t T, uint32_t R, typename V>
+__device__ __forceinline__ void commit_vec(V* dst, const V (&reg)[R], const V* __restrict__ src, uint32_t n)
+{
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t i = threadIdx.x + r * T;
+        if (i < n) dst[i] = reg[r];
+    }
+    for (uint32_t i = threadIdx.x + R * T; i < n; i += T) dst[i] = src[i];
+}
+
+template <uint32_t T, bool IS_VV, uint32_t RE, uint32_t RL>
+__global__ __launch_bounds__(T) void k_query2_vx_pipe(DevMesh m, const uint32_t* __restrict__ base,
+                                                      int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    PipeRange rg = pipe_range(m.P);
+    if (rg.p >= rg.p_end) return;
+    uint4     ev[RE];
+    uint32_t  ln[RL];
+    auto      slab = [&](const PatchDesc& d) { return IS_VV ? m.v_lin + d.v_off : m.e_lin + d.e_off; };
+    auto      prefetch = [&](const PatchDesc& d) {
+        prefetch_vec<T, RE>(ev, reinterpret_cast<const uint4*>(m.ev + d.e_off), (d.ne + 3u) >> 2);
+        prefetch_lin<T, RL>(ln, IS_VV ? d.nv : d.ne, IS_VV ? d.nvo : d.neo, slab(d));
+    };
+    PatchDesc d = load_desc(m.patch, rg.p);
+    prefetch(d);
+    while (true) {
+        const uint32_t p = rg.p;
+        Bump           b(reinterpret_cast<char*>(smem));
+        const PlanQ2   L = plan_q2(b, IS_VV ? 0 : 3, d.nv, d.ne, d.nf);
+        commit_vec<T, RE>(reinterpret_cast<uint4*>(L.ev), ev, reinterpret_cast<const uint4*>(m.ev + d.e_off),
+                          (d.ne + 3u) >> 2);
+        if (IS_VV) commit_lin<T, RL>(L.lin, d.nv, d.nvo, d.vp, slab(d), ln);
+        else commit_lin<T, RL>(L.lin, d.ne, d.neo, d.ep, slab(d), ln);
+        const uint32_t pn   = p + rg.stride;
+        const bool     more = pn < rg.p_end;
+        PatchDesc      dn{};
+        if (more) dn = load_desc(m.patch, pn);
+        const uint32_t bse = base[p];
+        __syncthreads();
+        if (more) prefetch(dn);
+        fill_columns<T>(
+            2 * d.ne, d.nv,
+            [&](uint32_t i) {
+                const uint32_t x = L.ev[i >> 1];
+                return (i & 1u) ? hi16(x) : lo16(x);
+            },
+            [&](uint32_t i) {
+                const uint32_t x = L.ev[i >> 1];
+                return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
+            },
+            L.off, L.cur, L.slots, L.wave);
+        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+            const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+            row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
+            sort_row(L.slots + o, k, [&](uint32_t jj, uint32_t s) {
+                L.slots[o + jj] = IS_VV ? L.lin[lo16(s)] : L.lin[hi16(s)];
+            });
+        }
+        if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+        __syncthreads();
+        copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
+        if (!more) break;
+        __syncthreads();  // LDS is reused by the next patch
+        rg.p = pn;
+        d    = dn;
+    }
+}
+
+template <uint32_t T, uint32_t RE, uint32_t RF, uint32_t RL>
+__global__ __launch_bounds__(T) void k_query2_vf_pipe(DevMesh m, const uint32_t* __restrict__ base,
+                                                      int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    PipeRange rg = pipe_range(m.P);
+    if (rg.p >= rg.p_end) return;
+    uint4     ev[RE];
+    uint2     fe[RF];
+    uint32_t  ln[RL];
+    auto      prefetch = [&](const PatchDesc& d) {
+        prefetch_vec<T, RE>(ev, reinterpret_cast<const uint4*>(m.ev + d.e_off), (d.ne + 3u) >> 2);
+        prefetch_vec<T, RF>(fe, reinterpret_cast<const uint2*>(m.fe + 3u * d.f_off), (3u * d.nf + 3u) >> 2);
+        prefetch_lin<T, RL>(ln, d.nf, d.nfo, m.f_lin + d.f_off);
+    };
+    PatchDesc d = load_desc(m.patch, rg.p);
+    prefetch(d);
+    while (true) {
+        const uint32_t p = rg.p;
+        Bump           b(reinterpret_cast<char*>(smem));
+        const PlanQ2   L = plan_q2(b, 1, d.nv, d.ne, d.nf);
+        commit_vec<T, RE>(reinterpret_cast<uint4*>(L.ev), ev, reinterpret_cast<const uint4*>(m.ev + d.e_off),
+                          (d.ne + 3u) >> 2);
+        commit_vec<T, RF>(reinterpret_cast<uint2*>(L.fe), fe, reinterpret_cast<const uint2*>(m.fe + 3u * d.f_off),
+                          (3u * d.nf + 3u) >> 2);
+        commit_lin<T, RL>(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, ln);
+        const uint32_t pn   = p + rg.stride;
+        const bool     more = pn < rg.p_end;
+        PatchDesc      dn{};
+        if (more) dn = load_desc(m.patch, pn);
+        const uint32_t bse = base[p];
+        __syncthreads();
+        if (more) prefetch(dn);
+        fill_columns<T>(
+            3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+            [&](uint32_t i) { return i / 3u; }, L.off, L.cur, L.slots, L.wave);
+        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+            const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+            row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
+            sort_row(L.slots + o, k, [&](uint32_t jj, uint32_t s) { L.slots[o + jj] = L.lin[s]; });
+        }
+        if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+        __syncthreads();
+        copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
+        if (!more) break;
+        __syncthreads();
+        rg.p = pn;
+        d    = dn;
+    }
 }
 
 // VV in patch-local form: per patch, u16 row offsets of the owned vertices

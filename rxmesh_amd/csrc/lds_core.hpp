@@ -46,24 +46,34 @@ __device__ __forceinline__ PatchDesc load_desc(const PatchDesc* __restrict__ pd,
     return d;
 }
 
+// Copy n vector elements global -> LDS, two rounds' loads issued before
+// their stores.
+template <typename V>
+__device__ __forceinline__ void stage_vec2(V* __restrict__ d, const V* __restrict__ s, uint32_t n)
+{
+    const uint32_t T = blockDim.x;
+    for (uint32_t base = 0; base < n; base += 2u * T) {
+        const uint32_t i0 = base + threadIdx.x, i1 = i0 + T;
+        V              a{}, b{};
+        if (i0 < n) a = s[i0];
+        if (i1 < n) b = s[i1];
+        if (i0 < n) d[i0] = a;
+        if (i1 < n) d[i1] = b;
+    }
+}
+
 // Copy n u32 from a 16-byte aligned, 4-element padded slab into LDS.
 __device__ __forceinline__ void stage_u32(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
                                           uint32_t n)
 {
-    const uint32_t n4 = (n + 3u) >> 2;
-    const uint4*   s  = reinterpret_cast<const uint4*>(src);
-    uint4*         d  = reinterpret_cast<uint4*>(dst);
-    for (uint32_t i = threadIdx.x; i < n4; i += blockDim.x) d[i] = s[i];
+    stage_vec2(reinterpret_cast<uint4*>(dst), reinterpret_cast<const uint4*>(src), (n + 3u) >> 2);
 }
 
 // Copy 3*nf u16 of a face row (8-byte aligned, padded to 4 faces) into LDS.
 __device__ __forceinline__ void stage_fe(uint16_t* __restrict__ dst, const uint16_t* __restrict__ src,
                                          uint32_t nf)
 {
-    const uint32_t n2 = (3u * nf + 3u) >> 2;  // uint2 = 4 x u16
-    const uint2*   s  = reinterpret_cast<const uint2*>(src);
-    uint2*         d  = reinterpret_cast<uint2*>(dst);
-    for (uint32_t i = threadIdx.x; i < n2; i += blockDim.x) d[i] = s[i];
+    stage_vec2(reinterpret_cast<uint2*>(dst), reinterpret_cast<const uint2*>(src), (3u * nf + 3u) >> 2);
 }
 
 // Exclusive scan of s[0..n) in place; returns the total. s_wave holds
@@ -148,52 +158,6 @@ __device__ __forceinline__ void lds_transpose(uint32_t n, uint32_t ncols, KeyF k
     __syncthreads();
 }
 
-// Deterministic LDS transpose without fill atomics or row sorts.
-//   n entries, entry i belongs to source row i / W and targets column key(i).
-//   Wave w owns the contiguous entry range [w*span, (w+1)*span); per-wave
-//   column counts (LDS atomics, order-free) are scanned column-major, so wave
-//   w's slots of column c follow those of waves < w. Each wave then walks its
-//   range in 64-entry chunks in order; equal keys inside a chunk are ranked by
-//   lane through a ballot match over the key bits. Result: every column lists
-//   its source rows in ascending order, the order of a serial fill.
-//   cnt: nw * ncols u32 scratch (nw = blockDim / 64); key_bits >= bits(ncols-1).
-template <uint32_t W, typename KeyF>
-__device__ __forceinline__ void lds_transpose_ordered(uint32_t n, uint32_t ncols, uint32_t key_bits, KeyF key,
-                                                      uint32_t* off, uint32_t* cnt, uint16_t* val,
-                                                      uint32_t* s_wave)
-{
-    const uint32_t T = blockDim.x, nw = T >> 6, w = threadIdx.x >> 6, lane = threadIdx.x & 63u;
-    const uint32_t span = ((n + nw * 64u - 1u) / (nw * 64u)) * 64u;
-    const uint32_t b0 = min(n, w * span), b1 = min(n, b0 + span);
-    for (uint32_t c = threadIdx.x; c < ncols * nw; c += T) cnt[c] = 0;
-    __syncthreads();
-    for (uint32_t i = b0 + lane; i < b1; i += 64u) atomicAdd(&cnt[key(i) * nw + w], 1u);
-    const uint32_t total = block_exclusive_scan(cnt, ncols * nw, s_wave);
-    for (uint32_t c = threadIdx.x; c < ncols; c += T) off[c] = cnt[c * nw];
-    if (threadIdx.x == 0) off[ncols] = total;
-    __syncthreads();  // off[] must be read before the fill advances cnt[]
-    const uint64_t lt = (lane == 0) ? 0ull : (~0ull >> (64u - lane));
-    for (uint32_t base = b0; base < b1; base += 64u) {
-        const uint32_t i  = base + lane;
-        const bool     ok = i < b1;
-        const uint32_t k  = ok ? key(i) : 0u;
-        uint64_t       mk = __ballot(ok);
-        for (uint32_t bit = 0; bit < key_bits; ++bit) {
-            const bool     on = (k >> bit) & 1u;
-            const uint64_t bb = __ballot(on);
-            mk &= on ? bb : ~bb;
-        }
-        const uint32_t rank = __popcll(mk & lt), cntg = __popcll(mk);
-        if (ok) {
-            const uint32_t sb = cnt[k * nw + w];
-            val[sb + rank]    = static_cast<uint16_t>(i / W);
-            if (rank + 1u == cntg) cnt[k * nw + w] = sb + cntg;
-        }
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ uint32_t bits_for(uint32_t n) { return (n <= 1u) ? 1u : 32u - __clz(n - 1u); }
 
 // Local vertex i (0..2) of local face f from LDS FE / EV (reference f_v,
 // rxmesh_queries.cuh:650-688: the direction bit selects the edge endpoint).

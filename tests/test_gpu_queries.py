@@ -45,10 +45,10 @@ def case(name):
         elif name == "disk":
             V, F = mg.disk(64, seed=1)
             ps = 256
-        elif name == "c1_bisect":
+        elif name == "c1_kmeans":
             V, F = mg.config_mesh("c1")
             ps = 512
-        m = rx.Mesh(F, patch_size=ps, patcher="bisect" if name.endswith("_bisect") else "kmeans")
+        m = rx.Mesh(F, patch_size=ps, patcher="kmeans" if name.endswith("_kmeans") else "bisect")
         o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"),
                                       m.array("patches_offset"))
     _CACHE[name] = (V, F, m, o)
@@ -69,7 +69,7 @@ def to_global_np(m, x_lin):
     return out
 
 
-NAMES = ["sphere3", "torus", "plane", "cube", "bunnyhead", "c0", "c1", "c1_p128", "disk", "c1_bisect"]
+NAMES = ["sphere3", "torus", "plane", "cube", "bunnyhead", "c0", "c1", "c1_p128", "disk", "c1_kmeans"]
 
 
 @pytest.mark.parametrize("name", NAMES)
@@ -84,7 +84,26 @@ def test_query_bit_exact(name, op):
     assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
 
 
-@pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "disk", "c1_bisect"])
+@pytest.mark.parametrize("name", ["c1", "c1_kmeans", "c1_p128", "disk", "bunnyhead"])
+@pytest.mark.parametrize("op", ["VV", "VE", "VF"])
+@pytest.mark.parametrize("pipe", [1, 4, 8])
+def test_pipelined_query_bit_exact(name, op, pipe):
+    """The persistent pipelined grid (workgroups walking patch ranges with
+    prefetch) gives the same CSR as the oracle."""
+    import torch
+    V, F, m, o = case(name)
+    m.set_launch_threads(5, pipe)
+    try:
+        rp, col = m.query(op)
+        torch.cuda.synchronize()
+    finally:
+        m.set_launch_threads(5, 0)
+    orp, ocol = o.query(op)
+    assert np.array_equal(rp.cpu().numpy(), orp)
+    assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
+
+
+@pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "disk", "c1_kmeans"])
 @pytest.mark.parametrize("op", ["VV", "VE"])
 def test_oriented_query_bit_exact(name, op):
     import torch
@@ -154,17 +173,22 @@ def test_smoothing_vs_fp64(name):
     assert torch.equal(out, out2)
 
 
-def test_smoothing_workgroup_sizes_agree():
+def test_smoothing_launch_configs_agree():
+    """Every launch configuration the autotuner may pick (workgroup size, LDS
+    layout, per-patch or persistent pipelined grid) gives bit-identical
+    results: same neighbours, same summation order."""
     import torch
     V, F, m, o = case("c1")
     x = lin_pos(m, V.astype(np.float32))
+    from rxmesh_amd import autotune
     outs = []
-    for t in (64, 128, 256):
-        m.set_launch_threads(0, t)
+    for cfg in autotune.smooth_configs():
+        autotune.apply(m, cfg)
         outs.append(m.smooth(x, 0.5, 3))
-    m.set_launch_threads(0, 128)
+    autotune.apply(m, {"smooth_threads": 128, "smooth_layout": 0, "smooth_pipe": 0})
     torch.cuda.synchronize()
-    assert torch.equal(outs[0], outs[1]) and torch.equal(outs[1], outs[2])
+    for y in outs[1:]:
+        assert torch.equal(outs[0], y)
 
 
 def test_vv_plan_matches_vv_query():

@@ -51,14 +51,15 @@ def test_reference_patch_archive_replay(name):
 
 @pytest.mark.parametrize("case", ["plane", "cube", "bunnyhead", "c0", "c1", "disk"])
 @pytest.mark.parametrize("patch_size", [64, 512])
-def test_patcher_then_build_matches_oracle(case, patch_size):
+@pytest.mark.parametrize("patcher", ["bisect", "kmeans"])
+def test_patcher_then_build_matches_oracle(case, patch_size, patcher):
     if case in ("plane", "cube", "bunnyhead"):
         V, F = obj(case + ".obj")
     elif case == "disk":
         V, F = mg.disk(48)
     else:
         V, F = mg.config_mesh(case)
-    m = rx.Mesh(F, patch_size=patch_size, host_only=True)
+    m = rx.Mesh(F, patch_size=patch_size, host_only=True, patcher=patcher)
     fp = m.array("face_patch")
     # patcher invariants: every face in exactly one patch, no patch above the cap
     cnt = np.bincount(fp, minlength=m.P)
@@ -67,10 +68,11 @@ def test_patcher_then_build_matches_oracle(case, patch_size):
     compare_with_oracle(m, o, F)
 
 
-def test_patcher_is_deterministic():
+@pytest.mark.parametrize("patcher", ["bisect", "kmeans"])
+def test_patcher_is_deterministic(patcher):
     V, F = mg.config_mesh("c1")
-    a = rx.Mesh(F, host_only=True).array("face_patch")
-    b = rx.Mesh(F, host_only=True).array("face_patch")
+    a = rx.Mesh(F, host_only=True, patcher=patcher).array("face_patch")
+    b = rx.Mesh(F, host_only=True, patcher=patcher).array("face_patch")
     assert np.array_equal(a, b)
 
 
