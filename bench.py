@@ -41,10 +41,6 @@ def parse():
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
-    ap.add_argument("--query-pipe", type=int, default=None,
-                    help="persistent pipelined VV/VF query workgroups per CU, 0 = one per patch (default: autotuned)")
-    ap.add_argument("--smooth-pipe", type=int, default=None,
-                    help="persistent pipelined smoothing workgroups per CU, 0 = one per patch (default: autotuned)")
     ap.add_argument("--batch-count", type=int, default=256, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -205,23 +201,15 @@ def main():
     m.build_vv_plan()  # static patch-local VV plan, built once per mesh
     torch.cuda.synchronize()
     from rxmesh_amd import autotune
-    if args.smooth_threads or args.smooth_layout is not None or args.smooth_pipe is not None:
+    if args.smooth_threads or args.smooth_layout is not None:
         smooth_cfg = {"smooth_threads": args.smooth_threads or 128,
-                      "smooth_layout": args.smooth_layout if args.smooth_layout is not None else 0,
-                      "smooth_pipe": args.smooth_pipe if args.smooth_pipe is not None else 0}
+                      "smooth_layout": args.smooth_layout if args.smooth_layout is not None else 0}
         autotune.apply(m, smooth_cfg)
         smooth_sweep = {}
     elif args.profile:
         smooth_cfg, smooth_sweep = {}, {}  # library defaults; --smooth-threads / --smooth-layout choose
     else:
         smooth_cfg, smooth_sweep = autotune.tune_smooth(m, x, out, use_cache=False)
-    if args.query_pipe is not None:
-        query_cfg, query_sweep = {"query_pipe": args.query_pipe}, {}
-        autotune.apply(m, query_cfg)
-    elif args.profile:
-        query_cfg, query_sweep = {}, {}
-    else:
-        query_cfg, query_sweep = autotune.tune_query(m, (vv_rp, vv_col, vf_col), use_cache=False)
     torch.cuda.synchronize()
 
     if args.profile:
@@ -291,7 +279,6 @@ def main():
                      "algorithmic_bytes": db},
         "kernels": kernels,
         "smooth_autotune": {"chosen": smooth_cfg, "us": smooth_sweep},
-        "query_autotune": {"chosen": query_cfg, "us": query_sweep},
     }
     # free the single-mesh workload before the batch
     del x, out, tmp, vv_rp, vv_col, vf_rp, vf_col

@@ -94,7 +94,9 @@ enum {
  * Replaces RXMeshStatic(fv, patcher_file, patch_size, ...)
  * (include/rxmesh/rxmesh_static.h:61-100) -> RXMesh::init / build
  * (include/rxmesh/rxmesh.cpp:89-225, 290-448).
- * num_vertices = 0 infers max index + 1. patch_file (may be NULL) is a
+ * num_vertices = 0 infers max index + 1; every vertex must belong to a face
+ * (RXM_ERR_INVALID_ARG otherwise, as for an empty or degenerate face list).
+ * patch_file (may be NULL) is a
  * reference patch archive (cereal PortableBinary, patcher.h:154-182) whose
  * layout is replayed instead of running the patcher. */
 rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_vertices,
@@ -167,11 +169,7 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *   kernel 2: symmetric-Dirichlet step launch, 0 = one workgroup per patch,
  *             1 = persistent grid over a global patch work queue;
  *   kernel 3: smoothing LDS layout of positions, 0 = three float arrays,
- *             1 = one float4 per vertex;
- *   kernel 4: smoothing grid, 0 = one workgroup per patch, n in 1..32 = a
- *             persistent grid of n workgroups per CU that prefetches the next
- *             patch while computing the current one;
- *   kernel 5: the same choice for the VV / VE / VF queries.
+ *             1 = one float4 per vertex.
  * All settings give bit-identical results. */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 

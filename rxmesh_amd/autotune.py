@@ -57,16 +57,12 @@ def _time_ms(fn, reps):
 
 
 SMOOTH_LAYOUTS = (0, 1)  # LDS staging of positions: 0 three float arrays, 1 float4 per vertex
-SMOOTH_PIPE = (0, 4, 8, 12, 16)  # 0: one workgroup per patch; n: persistent pipelined, n workgroups per CU
 
 
 def smooth_configs():
-    for pipe in SMOOTH_PIPE:
-        for st in SMOOTH_THREADS:
-            if pipe and st == 64:
-                continue
-            for lay in SMOOTH_LAYOUTS:
-                yield {"smooth_threads": st, "smooth_layout": lay, "smooth_pipe": pipe}
+    for st in SMOOTH_THREADS:
+        for lay in SMOOTH_LAYOUTS:
+            yield {"smooth_threads": st, "smooth_layout": lay}
 
 
 def apply(mesh, cfg):
@@ -78,39 +74,6 @@ def apply(mesh, cfg):
         mesh.set_launch_threads(2, cfg["grad_queue"])
     if "smooth_layout" in cfg:
         mesh.set_launch_threads(3, cfg["smooth_layout"])
-    if "smooth_pipe" in cfg:
-        mesh.set_launch_threads(4, cfg["smooth_pipe"])
-    if "query_pipe" in cfg:
-        mesh.set_launch_threads(5, cfg["query_pipe"])
-
-
-QUERY_PIPE = (0, 2, 4, 6, 8)  # 0: one workgroup per patch; n: persistent pipelined, n workgroups per CU
-
-
-def tune_query(mesh, buffers, reps=10, use_cache=True):
-    """Pick the grid of the VV / VE / VF query kernels (per patch, or a
-    persistent pipelined grid) from the time of one VV and one VF query.
-    buffers: (row_ptr, vv_col, vf_col) device tensors."""
-    key = _key(mesh, "query")
-    cache = _load() if use_cache else {}
-    if key in cache:
-        apply(mesh, cache[key])
-        return cache[key], {}
-    rp, cv, cf = buffers
-    timings = {}
-    best, best_t = None, float("inf")
-    for q in QUERY_PIPE:
-        cfg = {"query_pipe": q}
-        apply(mesh, cfg)
-        t = _time_ms(lambda: (mesh.query("VV", row_ptr=rp, col=cv), mesh.query("VF", row_ptr=rp, col=cf)), reps)
-        timings["p%d" % q] = round(t * 1e3, 2)
-        if t < best_t:
-            best, best_t = cfg, t
-    apply(mesh, best)
-    cache[key] = best
-    if use_cache:
-        _store(cache)
-    return best, timings
 
 
 def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True):
@@ -127,7 +90,7 @@ def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True):
     for cfg in smooth_configs():
         apply(mesh, cfg)
         t = _time_ms(lambda: mesh.smooth(x, lam, 1, out=out), reps)
-        timings["t%d/l%d/p%d" % (cfg["smooth_threads"], cfg["smooth_layout"], cfg["smooth_pipe"])] = round(t * 1e3, 2)
+        timings["t%d/l%d" % (cfg["smooth_threads"], cfg["smooth_layout"])] = round(t * 1e3, 2)
         if t < best_t:
             best, best_t = cfg, t
     apply(mesh, best)

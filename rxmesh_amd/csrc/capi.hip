@@ -47,8 +47,6 @@ struct rxm_mesh
     uint32_t        fp_max_nent = 0;
     uint32_t        smooth_threads = 128;
     uint32_t        smooth_aos4    = 0;        // 1: smoothing stages positions as float4
-    uint32_t        smooth_pipe    = 0;        // > 0: persistent pipelined smoothing, workgroups per CU
-    uint32_t        query_pipe     = 0;        // > 0: persistent pipelined VV / VE / VF queries, workgroups per CU
     uint32_t        grad_threads   = 256;
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
     uint32_t*       d_queue        = nullptr;  // work-queue ticket counter
@@ -321,30 +319,8 @@ int smooth_once_t(rxm_mesh* m, float lambda, const float* src, float* dst, void*
     }
 }
 
-// persistent pipelined form: a grid of num_cus * smooth_pipe workgroups
-template <bool AOS4>
-int smooth_pipe_t(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
-{
-    Bump b(nullptr);
-    plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent, AOS4);
-    if (m->num_cus == 0)
-        RXM_HIP(hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device));
-    // a multiple of 8 so every XCD gets the same number of workgroups
-    const uint32_t grid = std::max<uint32_t>(8u, (static_cast<uint32_t>(m->num_cus) * m->smooth_pipe) & ~7u);
-    const uint16_t* ro  = m->d_vv_roff;
-    const uint16_t* nb  = m->d_vv_nbr;
-    if (m->smooth_threads == 256)
-        return launch_grid(k_smooth_pipe<256, AOS4, 2, 2>, grid, 256, b.bytes(), stream, m->dm, m->d_vv_rbase,
-                           m->d_vv_nbase, ro, nb, lambda, src, dst);
-    return launch_grid(k_smooth_pipe<128, AOS4, 3, 4>, grid, 128, b.bytes(), stream, m->dm, m->d_vv_rbase,
-                       m->d_vv_nbase, ro, nb, lambda, src, dst);
-}
-
 int smooth_once(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
 {
-    if (m->smooth_pipe)
-        return m->smooth_aos4 ? smooth_pipe_t<true>(m, lambda, src, dst, stream)
-                              : smooth_pipe_t<false>(m, lambda, src, dst, stream);
     return m->smooth_aos4 ? smooth_once_t<true>(m, lambda, src, dst, stream)
                           : smooth_once_t<false>(m, lambda, src, dst, stream);
 }
@@ -637,25 +613,6 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
             return launch_patches(k_query_vx<true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col, oriented);
         return launch_patches(k_query_vx<false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col, oriented);
     }
-    if (m->query_pipe && (op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF)) {
-        if (m->num_cus == 0)
-            RXM_HIP(hipDeviceGetAttribute(&m->num_cus, hipDeviceAttributeMultiprocessorCount, m->device));
-        const uint32_t grid = std::max<uint32_t>(8u, (static_cast<uint32_t>(m->num_cus) * m->query_pipe) & ~7u);
-        switch (op) {
-            case RXM_OP_VV:
-                plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
-                return launch_grid(k_query2_vx_pipe<T, true, 2, 2>, grid, T, b.bytes(), stream, m->dm, base, row_ptr,
-                                   col);
-            case RXM_OP_VE:
-                plan_q2(b, 3, h.max_nv, h.max_ne, h.max_nf);
-                return launch_grid(k_query2_vx_pipe<T, false, 2, 2>, grid, T, b.bytes(), stream, m->dm, base,
-                                   row_ptr, col);
-            default:
-                plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf);
-                return launch_grid(k_query2_vf_pipe<T, 2, 3, 2>, grid, T, b.bytes(), stream, m->dm, base, row_ptr,
-                                   col);
-        }
-    }
     switch (op) {
         case RXM_OP_VV:
             plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
@@ -746,14 +703,6 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
     }
     if (kernel == 3 && (threads == 0 || threads == 1)) {  // smoothing LDS layout: 0 three float arrays, 1 float4
         m->smooth_aos4 = static_cast<uint32_t>(threads);
-        return RXM_OK;
-    }
-    if (kernel == 4 && threads >= 0 && threads <= 32) {  // smoothing: 0 one workgroup per patch, n persistent per CU
-        m->smooth_pipe = static_cast<uint32_t>(threads);
-        return RXM_OK;
-    }
-    if (kernel == 5 && threads >= 0 && threads <= 32) {  // VV / VE / VF queries: same choice
-        m->query_pipe = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
     return RXM_ERR_INVALID_ARG;

@@ -256,194 +256,6 @@ __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* _
     }
 }
 
-// ---- persistent, software-pipelined smoothing ---------------------------------
-// A machine-sized grid of workgroups; workgroup b belongs to XCD b % 8 and
-// walks that XCD's contiguous patch range with stride gridDim.x / 8. While
-// patch i is computed from LDS, patch i+1's plan rows, owned positions and
-// ribbon owner-table entries are already in flight into registers (RP uint4
-// of plan and RV vertices per thread); per patch only the ribbon position
-// gathers, which need those entries, are waited on. Anything beyond the
-// register capacity is loaded directly. Same arithmetic and summation order
-// as k_smooth_local, so the results are identical.
-template <uint32_t T, uint32_t RP, uint32_t RV>
-struct SmoothStage
-{
-    uint4    plan[RP];
-    float    x[RV], y[RV], z[RV];
-    uint32_t lin[RV];
-};
-
-template <uint32_t T, bool AOS4, uint32_t RP, uint32_t RV>
-__global__ __launch_bounds__(T) void k_smooth_pipe(DevMesh m, const uint32_t* __restrict__ rbase,
-                                                   const uint32_t* __restrict__ nbase,
-                                                   const uint16_t* __restrict__ roff,
-                                                   const uint16_t* __restrict__ nbr, float lambda,
-                                                   const float* __restrict__ pos, float* __restrict__ out)
-{
-    extern __shared__ uint4 smem[];
-    const uint32_t P = m.P, N = m.nV, tid = threadIdx.x;
-    const uint32_t G8 = gridDim.x >> 3, xcd = blockIdx.x & 7u, j = blockIdx.x >> 3;
-    const uint32_t chunk = (P + 7u) >> 3;
-    const uint32_t p_end = min(P, (xcd + 1u) * chunk);
-    uint32_t       p     = xcd * chunk + j;
-    if (p >= p_end) return;
-
-    SmoothStage<T, RP, RV> s;
-    // issue the loads of patch q's plan rows, owned positions and ribbon entries
-    auto prefetch = [&](const PatchDesc& d, uint32_t rb, uint32_t nb0, uint32_t nent) {
-        const uint32_t n8r = (d.nvo + 1u + 7u) >> 3, n8 = n8r + ((nent + 7u) >> 3);
-        const uint4*   r4p = reinterpret_cast<const uint4*>(roff + rb);
-        const uint4*   n4p = reinterpret_cast<const uint4*>(nbr + nb0);
-#pragma unroll
-        for (uint32_t r = 0; r < RP; ++r) {
-            const uint32_t i = tid + r * T;
-            if (i < n8) s.plan[r] = (i < n8r) ? r4p[i] : n4p[i - n8r];
-        }
-        const uint32_t* lin = m.v_lin + d.v_off;
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = tid + r * T;
-            if (l < d.nvo) {
-                const uint32_t g = d.vp + l;
-                s.x[r]           = pos[g];
-                s.y[r]           = pos[N + g];
-                s.z[r]           = pos[2 * N + g];
-            } else if (l < d.nv) {
-                s.lin[r] = lin[l];
-            }
-        }
-    };
-    // move the prefetched plan rows and owned positions into LDS (+ overflow)
-    auto commit = [&](const PatchDesc& d, uint32_t rb, uint32_t nb0, uint32_t nent, const PlanSL& L) {
-        const uint32_t n8r = (d.nvo + 1u + 7u) >> 3, n8 = n8r + ((nent + 7u) >> 3);
-        uint4*         lp  = reinterpret_cast<uint4*>(L.ro);  // ro and nb are contiguous uint4 runs
-#pragma unroll
-        for (uint32_t r = 0; r < RP; ++r) {
-            const uint32_t i = tid + r * T;
-            if (i < n8) lp[i] = s.plan[r];
-        }
-        for (uint32_t i = tid + RP * T; i < n8; i += T)
-            lp[i] = (i < n8r) ? reinterpret_cast<const uint4*>(roff + rb)[i]
-                              : reinterpret_cast<const uint4*>(nbr + nb0)[i - n8r];
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = tid + r * T;
-            if (l < d.nvo) {
-                if constexpr (AOS4) L.p4[l] = make_float4(s.x[r], s.y[r], s.z[r], 0.f);
-                else {
-                    L.x[l] = s.x[r];
-                    L.y[l] = s.y[r];
-                    L.z[l] = s.z[r];
-                }
-            }
-        }
-    };
-    // gather the ribbon positions (owner-table entries already in registers)
-    auto ribbon = [&](const PatchDesc& d, const PlanSL& L) {
-        float x[RV], y[RV], z[RV];
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = tid + r * T;
-            if (l >= d.nvo && l < d.nv) {
-                const uint32_t g = s.lin[r];
-                x[r]             = pos[g];
-                y[r]             = pos[N + g];
-                z[r]             = pos[2 * N + g];
-            }
-        }
-#pragma unroll
-        for (uint32_t r = 0; r < RV; ++r) {
-            const uint32_t l = tid + r * T;
-            if (l >= d.nvo && l < d.nv) {
-                if constexpr (AOS4) L.p4[l] = make_float4(x[r], y[r], z[r], 0.f);
-                else {
-                    L.x[l] = x[r];
-                    L.y[l] = y[r];
-                    L.z[l] = z[r];
-                }
-            }
-        }
-        // vertices past the register capacity
-        for (uint32_t l = tid + RV * T; l < d.nv; l += T) {
-            const uint32_t g = lin_of(l, d.nvo, d.vp, m.v_lin + d.v_off);
-            if constexpr (AOS4) L.p4[l] = make_float4(pos[g], pos[N + g], pos[2 * N + g], 0.f);
-            else {
-                L.x[l] = pos[g];
-                L.y[l] = pos[N + g];
-                L.z[l] = pos[2 * N + g];
-            }
-        }
-    };
-
-    PatchDesc d  = load_desc(m.patch, p);
-    uint32_t  rb = rbase[p], nb0 = nbase[p], nent = nbase[p + 1] - nb0;
-    prefetch(d, rb, nb0, nent);
-    while (true) {
-        Bump         b(reinterpret_cast<char*>(smem));
-        const PlanSL L = plan_sl(b, d.nv, d.nvo, nent, AOS4);
-        commit(d, rb, nb0, nent, L);
-        ribbon(d, L);
-        // next patch: descriptor, then its loads go out before this patch's compute
-        const uint32_t pn   = p + G8;
-        const bool     more = pn < p_end;
-        PatchDesc      dn{};
-        uint32_t       rbn = 0, nbn = 0, nentn = 0;
-        if (more) {
-            dn    = load_desc(m.patch, pn);
-            rbn   = rbase[pn];
-            nbn   = nbase[pn];
-            nentn = nbase[pn + 1] - nbn;
-        }
-        __syncthreads();
-        if (more) prefetch(dn, rbn, nbn, nentn);
-        for (uint32_t v = tid; v < d.nvo; v += T) {
-            const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
-            float          sx = 0.f, sy = 0.f, sz = 0.f;
-            float          px, py, pz;
-            if constexpr (AOS4) {
-                for (uint32_t k = k0; k < k1; ++k) {
-                    const float4 q = L.p4[L.nb[k]];
-                    sx += q.x;
-                    sy += q.y;
-                    sz += q.z;
-                }
-                const float4 q = L.p4[v];
-                px = q.x;
-                py = q.y;
-                pz = q.z;
-            } else {
-                for (uint32_t k = k0; k < k1; ++k) {
-                    const uint32_t w = L.nb[k];
-                    sx += L.x[w];
-                    sy += L.y[w];
-                    sz += L.z[w];
-                }
-                px = L.x[v];
-                py = L.y[v];
-                pz = L.z[v];
-            }
-            const uint32_t g = d.vp + v;
-            if (k1 > k0) {
-                const float inv = 1.0f / static_cast<float>(k1 - k0);
-                out[g]          = px + lambda * (sx * inv - px);
-                out[N + g]      = py + lambda * (sy * inv - py);
-                out[2 * N + g]  = pz + lambda * (sz * inv - pz);
-            } else {
-                out[g]         = px;
-                out[N + g]     = py;
-                out[2 * N + g] = pz;
-            }
-        }
-        if (!more) break;
-        __syncthreads();  // LDS is reused by the next patch
-        p    = pn;
-        d    = dn;
-        rb   = rbn;
-        nb0  = nbn;
-        nent = nentn;
-    }
-}
-
 // ---- vertex normals over the patch-local face plan ----------------------------
 // Same weights and summation order as k_vertex_normals (incident faces in
 // ascending local order), but the VF rows come precomputed from the face

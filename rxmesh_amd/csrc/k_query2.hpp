@@ -109,177 +109,6 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
     copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
 }
 
-// ---- persistent, software-pipelined VV / VE / VF ------------------------------
-// A machine-sized grid; workgroup b walks XCD (b % 8)'s contiguous patch
-// range with stride gridDim.x / 8. The next patch's topology (EV, FE) and
-// ribbon owner-table entries are loaded into registers (RE / RF / RL rounds
-// per thread) while the current patch's LDS sort runs, so the global round
-// trips of a patch overlap the previous patch's work. Rows and entries are
-// identical to the one-workgroup-per-patch kernels.
-struct PipeRange
-{
-    uint32_t p, p_end, stride;
-};
-__device__ __forceinline__ PipeRange pipe_range(uint32_t P)
-{
-    const uint32_t chunk = (P + 7u) >> 3, xcd = blockIdx.x & 7u;
-    return PipeRange{xcd * chunk + (blockIdx.x >> 3), min(P, (xcd + 1u) * chunk), gridDim.x >> 3};
-}
-
-// owned ids map by prefix, ribbon ids come from registers (first RL rounds)
-// or straight from the slab (overflow)
-template <uint32_t T, uint32_t RL>
-__device__ __forceinline__ void commit_lin(uint32_t* dst, uint32_t n, uint32_t n_owned, uint32_t prefix,
-                                           const uint32_t* __restrict__ slab, const uint32_t (&ln)[RL])
-{
-    for (uint32_t l = threadIdx.x; l < n_owned; l += T) dst[l] = prefix + l;
-#pragma unroll
-    for (uint32_t r = 0; r < RL; ++r) {
-        const uint32_t l = n_owned + threadIdx.x + r * T;
-        if (l < n) dst[l] = ln[r];
-    }
-    for (uint32_t l = n_owned + threadIdx.x + RL * T; l < n; l += T) dst[l] = slab[l];
-}
-template <uint32_t T, uint32_t RL>
-__device__ __forceinline__ void prefetch_lin(uint32_t (&ln)[RL], uint32_t n, uint32_t n_owned,
-                                             const uint32_t* __restrict__ slab)
-{
-#pragma unroll
-    for (uint32_t r = 0; r < RL; ++r) {
-        const uint32_t l = n_owned + threadIdx.x + r * T;
-        if (l < n) ln[r] = slab[l];
-    }
-}
-template <uint32_t T, uint32_t R, typename V>
-__device__ __forceinline__ void prefetch_vec(V (&reg)[R], const V* __restrict__ src, uint32_t n)
-{
-#pragma unroll
-    for (uint32_t r = 0; r < R; ++r) {
-        const uint32_t i = threadIdx.x + r * T;
-        if (i < n) reg[r] = src[i];
-    }
-}
-template <uint32_t T, uint32_t R, typename V>
-__device__ __forceinline__ void commit_vec(V* dst, const V (&reg)[R], const V* __restrict__ src, uint32_t n)
-{
-#pragma unroll
-    for (uint32_t r = 0; r < R; ++r) {
-        const uint32_t i = threadIdx.x + r * T;
-        if (i < n) dst[i] = reg[r];
-    }
-    for (uint32_t i = threadIdx.x + R * T; i < n; i += T) dst[i] = src[i];
-}
-
-template <uint32_t T, bool IS_VV, uint32_t RE, uint32_t RL>
-__global__ __launch_bounds__(T) void k_query2_vx_pipe(DevMesh m, const uint32_t* __restrict__ base,
-                                                      int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
-{
-    extern __shared__ uint4 smem[];
-    PipeRange rg = pipe_range(m.P);
-    if (rg.p >= rg.p_end) return;
-    uint4     ev[RE];
-    uint32_t  ln[RL];
-    auto      slab = [&](const PatchDesc& d) { return IS_VV ? m.v_lin + d.v_off : m.e_lin + d.e_off; };
-    auto      prefetch = [&](const PatchDesc& d) {
-        prefetch_vec<T, RE>(ev, reinterpret_cast<const uint4*>(m.ev + d.e_off), (d.ne + 3u) >> 2);
-        prefetch_lin<T, RL>(ln, IS_VV ? d.nv : d.ne, IS_VV ? d.nvo : d.neo, slab(d));
-    };
-    PatchDesc d = load_desc(m.patch, rg.p);
-    prefetch(d);
-    while (true) {
-        const uint32_t p = rg.p;
-        Bump           b(reinterpret_cast<char*>(smem));
-        const PlanQ2   L = plan_q2(b, IS_VV ? 0 : 3, d.nv, d.ne, d.nf);
-        commit_vec<T, RE>(reinterpret_cast<uint4*>(L.ev), ev, reinterpret_cast<const uint4*>(m.ev + d.e_off),
-                          (d.ne + 3u) >> 2);
-        if (IS_VV) commit_lin<T, RL>(L.lin, d.nv, d.nvo, d.vp, slab(d), ln);
-        else commit_lin<T, RL>(L.lin, d.ne, d.neo, d.ep, slab(d), ln);
-        const uint32_t pn   = p + rg.stride;
-        const bool     more = pn < rg.p_end;
-        PatchDesc      dn{};
-        if (more) dn = load_desc(m.patch, pn);
-        const uint32_t bse = base[p];
-        __syncthreads();
-        if (more) prefetch(dn);
-        fill_columns<T>(
-            2 * d.ne, d.nv,
-            [&](uint32_t i) {
-                const uint32_t x = L.ev[i >> 1];
-                return (i & 1u) ? hi16(x) : lo16(x);
-            },
-            [&](uint32_t i) {
-                const uint32_t x = L.ev[i >> 1];
-                return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
-            },
-            L.off, L.cur, L.slots, L.wave);
-        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
-            const uint32_t o = L.off[v], k = L.off[v + 1] - o;
-            row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
-            sort_row(L.slots + o, k, [&](uint32_t jj, uint32_t s) {
-                L.slots[o + jj] = IS_VV ? L.lin[lo16(s)] : L.lin[hi16(s)];
-            });
-        }
-        if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
-        __syncthreads();
-        copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
-        if (!more) break;
-        __syncthreads();  // LDS is reused by the next patch
-        rg.p = pn;
-        d    = dn;
-    }
-}
-
-template <uint32_t T, uint32_t RE, uint32_t RF, uint32_t RL>
-__global__ __launch_bounds__(T) void k_query2_vf_pipe(DevMesh m, const uint32_t* __restrict__ base,
-                                                      int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
-{
-    extern __shared__ uint4 smem[];
-    PipeRange rg = pipe_range(m.P);
-    if (rg.p >= rg.p_end) return;
-    uint4     ev[RE];
-    uint2     fe[RF];
-    uint32_t  ln[RL];
-    auto      prefetch = [&](const PatchDesc& d) {
-        prefetch_vec<T, RE>(ev, reinterpret_cast<const uint4*>(m.ev + d.e_off), (d.ne + 3u) >> 2);
-        prefetch_vec<T, RF>(fe, reinterpret_cast<const uint2*>(m.fe + 3u * d.f_off), (3u * d.nf + 3u) >> 2);
-        prefetch_lin<T, RL>(ln, d.nf, d.nfo, m.f_lin + d.f_off);
-    };
-    PatchDesc d = load_desc(m.patch, rg.p);
-    prefetch(d);
-    while (true) {
-        const uint32_t p = rg.p;
-        Bump           b(reinterpret_cast<char*>(smem));
-        const PlanQ2   L = plan_q2(b, 1, d.nv, d.ne, d.nf);
-        commit_vec<T, RE>(reinterpret_cast<uint4*>(L.ev), ev, reinterpret_cast<const uint4*>(m.ev + d.e_off),
-                          (d.ne + 3u) >> 2);
-        commit_vec<T, RF>(reinterpret_cast<uint2*>(L.fe), fe, reinterpret_cast<const uint2*>(m.fe + 3u * d.f_off),
-                          (3u * d.nf + 3u) >> 2);
-        commit_lin<T, RL>(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, ln);
-        const uint32_t pn   = p + rg.stride;
-        const bool     more = pn < rg.p_end;
-        PatchDesc      dn{};
-        if (more) dn = load_desc(m.patch, pn);
-        const uint32_t bse = base[p];
-        __syncthreads();
-        if (more) prefetch(dn);
-        fill_columns<T>(
-            3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
-            [&](uint32_t i) { return i / 3u; }, L.off, L.cur, L.slots, L.wave);
-        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
-            const uint32_t o = L.off[v], k = L.off[v + 1] - o;
-            row_ptr[d.vp + v] = static_cast<int32_t>(bse + o);
-            sort_row(L.slots + o, k, [&](uint32_t jj, uint32_t s) { L.slots[o + jj] = L.lin[s]; });
-        }
-        if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
-        __syncthreads();
-        copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
-        if (!more) break;
-        __syncthreads();
-        rg.p = pn;
-        d    = dn;
-    }
-}
-
 // VV in patch-local form: per patch, u16 row offsets of the owned vertices
 // (nvo + 1, relative to the patch's entry base) at roff + rbase[p], and u16
 // local neighbour ids at nbr + nbase[p], rows in ascending local-edge order.

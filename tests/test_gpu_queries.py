@@ -84,25 +84,6 @@ def test_query_bit_exact(name, op):
     assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
 
 
-@pytest.mark.parametrize("name", ["c1", "c1_kmeans", "c1_p128", "disk", "bunnyhead"])
-@pytest.mark.parametrize("op", ["VV", "VE", "VF"])
-@pytest.mark.parametrize("pipe", [1, 4, 8])
-def test_pipelined_query_bit_exact(name, op, pipe):
-    """The persistent pipelined grid (workgroups walking patch ranges with
-    prefetch) gives the same CSR as the oracle."""
-    import torch
-    V, F, m, o = case(name)
-    m.set_launch_threads(5, pipe)
-    try:
-        rp, col = m.query(op)
-        torch.cuda.synchronize()
-    finally:
-        m.set_launch_threads(5, 0)
-    orp, ocol = o.query(op)
-    assert np.array_equal(rp.cpu().numpy(), orp)
-    assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
-
-
 @pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "disk", "c1_kmeans"])
 @pytest.mark.parametrize("op", ["VV", "VE"])
 def test_oriented_query_bit_exact(name, op):
@@ -175,8 +156,7 @@ def test_smoothing_vs_fp64(name):
 
 def test_smoothing_launch_configs_agree():
     """Every launch configuration the autotuner may pick (workgroup size, LDS
-    layout, per-patch or persistent pipelined grid) gives bit-identical
-    results: same neighbours, same summation order."""
+    layout) gives bit-identical results: same neighbours, same summation order."""
     import torch
     V, F, m, o = case("c1")
     x = lin_pos(m, V.astype(np.float32))
@@ -185,7 +165,7 @@ def test_smoothing_launch_configs_agree():
     for cfg in autotune.smooth_configs():
         autotune.apply(m, cfg)
         outs.append(m.smooth(x, 0.5, 3))
-    autotune.apply(m, {"smooth_threads": 128, "smooth_layout": 0, "smooth_pipe": 0})
+    autotune.apply(m, {"smooth_threads": 128, "smooth_layout": 0})
     torch.cuda.synchronize()
     for y in outs[1:]:
         assert torch.equal(outs[0], y)

@@ -114,6 +114,13 @@ def test_errors_are_status_codes(tmp_path):
     with pytest.raises(rx.RXMError) as e:
         rx.Mesh(np.array([[0, 0, 1]], np.uint32), host_only=True)  # degenerate face
     assert e.value.code == 1
+    with pytest.raises(rx.RXMError) as e:  # no faces
+        rx.Mesh(np.zeros((0, 3), np.uint32), host_only=True)
+    assert e.value.code == 1
+    Vg, Fg = mg.grid(8)
+    with pytest.raises(rx.RXMError) as e:  # vertices referenced by no face have no owner patch
+        rx.Mesh(Fg, num_vertices=len(Vg) + 3, host_only=True)
+    assert e.value.code == 1
     bad = tmp_path / "bad_patches"
     bad.write_bytes(b"\x01" + b"\x00" * 10)
     V, F = obj("sphere3.obj")
