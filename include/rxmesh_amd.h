@@ -87,7 +87,14 @@ enum {
     RXM_ARR_PATCH_OFFSETS = 15,/* 3(P+1): v_off, e_off, f_off slab offsets */
     RXM_ARR_LTOG_V = 16, RXM_ARR_LTOG_E = 17, RXM_ARR_LTOG_F = 18, /* slabs */
     RXM_ARR_EV_LOCAL = 19,     /* e-slab u32: lo16 local(max gid), hi16 local(min gid) */
-    RXM_ARR_FE_LOCAL = 20      /* 3 * f-slab uint16: (local edge << 1) | dir */
+    RXM_ARR_FE_LOCAL = 20,     /* 3 * f-slab uint16: (local edge << 1) | dir */
+    /* two-step smoothing plan (built on demand): per patch p, a row for every
+     * local vertex in its owner's VV order, neighbours in p's extended local
+     * numbering ([0, nv) local vertices, then the second-ring extras) */
+    RXM_ARR_VV2_ROFF = 21,     /* uint16 row offsets, run of patch p at rbase[p] */
+    RXM_ARR_VV2_NBR = 22,      /* uint16 neighbours, run of patch p at nbase[p] */
+    RXM_ARR_VV2_XLIN = 23,     /* linear ids of the extras, run of patch p at xbase[p] */
+    RXM_ARR_VV2_OFFSETS = 24   /* rbase (P+1), nbase (P+1), xbase (P+1), extras per patch (P) */
 };
 
 /* Build the patch mesh from F x 3 triangle indices and upload it.
@@ -169,7 +176,9 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *   kernel 2: symmetric-Dirichlet step launch, 0 = one workgroup per patch,
  *             1 = persistent grid over a global patch work queue;
  *   kernel 3: smoothing LDS layout of positions, 0 = three float arrays,
- *             1 = one float4 per vertex.
+ *             1 = one float4 per vertex;
+ *   kernel 4: smoothing, 1 = two Jacobi steps per launch over the two-step
+ *             plan (RXM_ARR_VV2_*), 0 = one step per launch.
  * All settings give bit-identical results. */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 

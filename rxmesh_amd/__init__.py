@@ -20,7 +20,7 @@ ARR = {"ev": 0, "face_patch": 1, "vertex_patch": 2, "edge_patch": 3, "patches_va
        "patches_offset": 5, "ribbon_val": 6, "ribbon_offset": 7, "v_l2g": 8, "e_l2g": 9,
        "f_l2g": 10, "v_g2l": 11, "e_g2l": 12, "f_g2l": 13, "patch_counts": 14,
        "patch_offsets": 15, "ltog_v": 16, "ltog_e": 17, "ltog_f": 18, "ev_local": 19,
-       "fe_local": 20}
+       "fe_local": 20, "vv2_roff": 21, "vv2_nbr": 22, "vv2_xlin": 23, "vv2_offsets": 24}
 RXM_HOST_ONLY = 1
 RXM_PATCHER_KMEANS = 2
 
@@ -184,7 +184,7 @@ class Mesh:
         which = ARR[name]
         n = ctypes.c_uint64()
         _check(lib().rxm_mesh_array_size(self._h, which, ctypes.byref(n)), "rxm_mesh_array_size")
-        dt = np.uint16 if name == "fe_local" else np.uint32
+        dt = np.uint16 if name in ("fe_local", "vv2_roff", "vv2_nbr") else np.uint32
         out = np.zeros(max(n.value, 1), dt)
         _check(lib().rxm_mesh_get_array(self._h, which, out.ctypes.data, out.nbytes), "rxm_mesh_get_array")
         return out[: n.value]

@@ -59,10 +59,16 @@ def _time_ms(fn, reps):
 SMOOTH_LAYOUTS = (0, 1)  # LDS staging of positions: 0 three float arrays, 1 float4 per vertex
 
 
+SMOOTH_FUSED = (0, 1)  # 1: two Jacobi steps per launch over the two-step plan
+
+
 def smooth_configs():
-    for st in SMOOTH_THREADS:
-        for lay in SMOOTH_LAYOUTS:
-            yield {"smooth_threads": st, "smooth_layout": lay}
+    for fused in SMOOTH_FUSED:
+        for st in SMOOTH_THREADS:
+            for lay in SMOOTH_LAYOUTS:
+                if fused and lay:  # the two-step kernel stages three float arrays only
+                    continue
+                yield {"smooth_threads": st, "smooth_layout": lay, "smooth_fused": fused}
 
 
 def apply(mesh, cfg):
@@ -74,23 +80,28 @@ def apply(mesh, cfg):
         mesh.set_launch_threads(2, cfg["grad_queue"])
     if "smooth_layout" in cfg:
         mesh.set_launch_threads(3, cfg["smooth_layout"])
+    if "smooth_fused" in cfg:
+        mesh.set_launch_threads(4, cfg["smooth_fused"])
 
 
 def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True):
-    """Pick workgroup size and LDS layout of the uniform smoothing kernel
-    (one Jacobi iteration on the VV plan); returns (config, timings)."""
+    """Pick workgroup size, LDS layout and steps per launch of uniform
+    smoothing from the time of two Jacobi iterations; returns (config,
+    timings in us per iteration)."""
     key = _key(mesh, "smooth")
     cache = _load() if use_cache else {}
     if key in cache:
         apply(mesh, cache[key])
         return cache[key], {}
+    import torch
     mesh.build_vv_plan()
+    tmp = torch.empty_like(x)
     timings = {}
     best, best_t = None, float("inf")
     for cfg in smooth_configs():
         apply(mesh, cfg)
-        t = _time_ms(lambda: mesh.smooth(x, lam, 1, out=out), reps)
-        timings["t%d/l%d" % (cfg["smooth_threads"], cfg["smooth_layout"])] = round(t * 1e3, 2)
+        t = _time_ms(lambda: mesh.smooth(x, lam, 2, out=out, tmp=tmp), reps) / 2
+        timings["t%d/l%d/f%d" % (cfg["smooth_threads"], cfg["smooth_layout"], cfg["smooth_fused"])] = round(t * 1e3, 2)
         if t < best_t:
             best, best_t = cfg, t
     apply(mesh, best)

@@ -37,6 +37,17 @@ struct rxm_mesh
     uint16_t*       d_vv_roff  = nullptr;
     uint16_t*       d_vv_nbr   = nullptr;
     uint32_t        vv_max_nent = 0;
+    // two-step smoothing plan (host copy built on demand, device copy on first use)
+    mutable Vv2Plan   vv2;
+    mutable bool      vv2_built = false;
+    const uint32_t*   d_vv2_rbase = nullptr;
+    const uint32_t*   d_vv2_nbase = nullptr;
+    const uint32_t*   d_vv2_xbase = nullptr;
+    const uint32_t*   d_vv2_xcnt  = nullptr;
+    const uint16_t*   d_vv2_roff  = nullptr;
+    const uint16_t*   d_vv2_nbr   = nullptr;
+    const uint32_t*   d_vv2_xlin  = nullptr;
+    uint32_t          smooth_fused = 0;  // 1: two Jacobi steps per launch
     // patch-local face plan (FV table + VF rows of (face << 2 | corner))
     const uint32_t* d_fp_fbase = nullptr;
     const uint32_t* d_fp_rbase = nullptr;
@@ -319,6 +330,47 @@ int smooth_once_t(rxm_mesh* m, float lambda, const float* src, float* dst, void*
     }
 }
 
+int host_vv2_plan(const rxm_mesh* m)
+{
+    if (m->vv2_built) return RXM_OK;
+    int rc = build_vv2_plan(m->h, m->vv2);
+    if (rc == RXM_OK) m->vv2_built = true;
+    return rc;
+}
+
+int ensure_vv2_plan(rxm_mesh* m)
+{
+    if (m->d_vv2_nbr) return RXM_OK;
+    int rc = host_vv2_plan(m);
+    const Vv2Plan& v = m->vv2;
+    if (!rc) rc = upload(m, v.rbase, m->d_vv2_rbase);
+    if (!rc) rc = upload(m, v.nbase, m->d_vv2_nbase);
+    if (!rc) rc = upload(m, v.xbase, m->d_vv2_xbase);
+    if (!rc) rc = upload(m, v.nx, m->d_vv2_xcnt);
+    if (!rc) rc = upload(m, v.roff, m->d_vv2_roff);
+    if (!rc) rc = upload(m, v.xlin, m->d_vv2_xlin);
+    if (!rc) rc = upload(m, v.nbr, m->d_vv2_nbr);  // last: marks the plan complete
+    if (rc) m->d_vv2_nbr = nullptr;
+    return rc;
+}
+
+// two Jacobi steps in one launch
+int smooth_twice(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
+{
+    Bump b(nullptr);
+    plan_s2(b, m->h.max_nv, m->vv2.max_nx, m->vv2.max_nent);
+    auto go = [&](auto kernel, uint32_t T) {
+        return launch_patches_xcd(kernel, T, b.bytes(), stream, m, m->dm, m->d_vv2_rbase, m->d_vv2_nbase,
+                                  m->d_vv2_xbase, m->d_vv2_xcnt, m->d_vv2_roff, m->d_vv2_nbr, m->d_vv2_xlin, lambda,
+                                  src, dst);
+    };
+    switch (m->smooth_threads) {
+        case 64: return go(k_smooth2<64>, 64);
+        case 256: return go(k_smooth2<256>, 256);
+        default: return go(k_smooth2<128>, 128);
+    }
+}
+
 int smooth_once(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
 {
     return m->smooth_aos4 ? smooth_once_t<true>(m, lambda, src, dst, stream)
@@ -548,6 +600,30 @@ static int array_ref(const rxm_mesh* m, int which, const void** ptr, uint64_t* c
             *count = h.fe_local.size();
             *esize = 2;
             break;
+        case RXM_ARR_VV2_ROFF:
+        case RXM_ARR_VV2_NBR:
+        case RXM_ARR_VV2_XLIN:
+        case RXM_ARR_VV2_OFFSETS: {
+            const int rc = host_vv2_plan(m);
+            if (rc) return rc;
+            const Vv2Plan& v = m->vv2;
+            if (which == RXM_ARR_VV2_ROFF || which == RXM_ARR_VV2_NBR) {
+                const auto& a = (which == RXM_ARR_VV2_ROFF) ? v.roff : v.nbr;
+                *ptr   = a.data();
+                *count = a.size();
+                *esize = 2;
+            } else if (which == RXM_ARR_VV2_XLIN) {
+                set(v.xlin);
+            } else {
+                tmp.clear();
+                tmp.insert(tmp.end(), v.rbase.begin(), v.rbase.end());
+                tmp.insert(tmp.end(), v.nbase.begin(), v.nbase.end());
+                tmp.insert(tmp.end(), v.xbase.begin(), v.xbase.end());
+                tmp.insert(tmp.end(), v.nx.begin(), v.nx.end());
+                set(tmp);
+            }
+            break;
+        }
         default: return RXM_ERR_INVALID_ARG;
     }
     return RXM_OK;
@@ -676,10 +752,14 @@ rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
     int rc = ensure_vv_plan(m, stream);
     if (rc) return rc;
-    const float* src = pos;
-    for (int it = 0; it < iters; ++it) {
-        float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
-        rc = smooth_once(m, lambda, src, dst, stream);
+    const bool fused = m->smooth_fused && iters > 1 && ensure_vv2_plan(m) == RXM_OK;
+    // launches: pairs of steps (fused) then a single one; the last writes out
+    const int    launches = fused ? iters / 2 + iters % 2 : iters;
+    const float* src      = pos;
+    for (int i = 0; i < launches; ++i) {
+        float* dst = ((launches - 1 - i) % 2 == 0) ? out : tmp;
+        const bool pair = fused && (i < iters / 2);
+        rc = pair ? smooth_twice(m, lambda, src, dst, stream) : smooth_once(m, lambda, src, dst, stream);
         if (rc) return rc;
         src = dst;
     }
@@ -703,6 +783,10 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
     }
     if (kernel == 3 && (threads == 0 || threads == 1)) {  // smoothing LDS layout: 0 three float arrays, 1 float4
         m->smooth_aos4 = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 4 && (threads == 0 || threads == 1)) {  // smoothing: 1 = two Jacobi steps per launch
+        m->smooth_fused = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
     return RXM_ERR_INVALID_ARG;

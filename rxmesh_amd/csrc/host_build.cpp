@@ -866,6 +866,107 @@ int build_patches(HostMesh& m)
 }
 
 // ---------------------------------------------------------------------------
+// ---------------------------------------------------------------------------
+// Two-step smoothing plan (host_build.h).
+int build_vv2_plan(const HostMesh& m, Vv2Plan& out)
+{
+    const uint32_t P = m.P, V = m.nV;
+    // canonical VV rows in global ids, indexed by linear vertex id: an owned
+    // vertex's row lists the other endpoints of its incident edges in
+    // ascending owner-local edge order
+    std::vector<uint32_t> deg(V, 0);
+    for (uint64_t e = 0; e < m.nE; ++e) {
+        deg[m.ev[2 * e]]++;
+        deg[m.ev[2 * e + 1]]++;
+    }
+    std::vector<uint32_t> roff_g(V + 1, 0);
+    for (uint32_t l = 0; l < V; ++l) roff_g[l + 1] = roff_g[l] + deg[m.v_l2g[l]];
+    std::vector<uint32_t> rows(roff_g[V]);
+    parallel_for(P, [&](size_t q) {
+        const uint32_t        vo = m.v_off[q], eo = m.e_off[q], nvo = m.nvo[q], ne = m.ne[q];
+        std::vector<uint32_t> fill(nvo, 0);
+        for (uint32_t e = 0; e < ne; ++e) {
+            const uint32_t x = m.ev_local[eo + e], a = x & 0xFFFFu, b = x >> 16;
+            if (a < nvo) rows[roff_g[m.vp[q] + a] + fill[a]++] = m.ltog_v[vo + b];
+            if (b < nvo) rows[roff_g[m.vp[q] + b] + fill[b]++] = m.ltog_v[vo + a];
+        }
+    }, 1);
+    // per patch: extended numbering, rows of all local vertices, extras
+    std::vector<std::vector<uint16_t>> pr(P), pn(P);
+    std::vector<std::vector<uint32_t>> px(P);
+    std::atomic<int>                   err{kOk};
+    parallel_for(P, [&](size_t p) {
+        const uint32_t  vo = m.v_off[p], nv = m.nv[p], nvo = m.nvo[p];
+        const uint32_t* gl = &m.ltog_v[vo];  // owned part sorted, ribbon part sorted
+        auto local_of = [&](uint32_t g) -> uint32_t {
+            const uint32_t* a = std::lower_bound(gl, gl + nvo, g);
+            if (a != gl + nvo && *a == g) return static_cast<uint32_t>(a - gl);
+            const uint32_t* b = std::lower_bound(gl + nvo, gl + nv, g);
+            if (b != gl + nv && *b == g) return static_cast<uint32_t>(b - gl);
+            return kInvalid32;
+        };
+        auto row_of = [&](uint32_t l) {
+            const uint32_t lin = (l < nvo) ? m.vp[p] + l : m.v_lin[vo + l];
+            return std::make_pair(roff_g[lin], roff_g[lin + 1]);
+        };
+        std::vector<uint32_t>& X = px[p];
+        for (uint32_t l = 0; l < nv; ++l) {
+            const auto r = row_of(l);
+            for (uint32_t k = r.first; k < r.second; ++k)
+                if (local_of(rows[k]) == kInvalid32) X.push_back(rows[k]);
+        }
+        std::sort(X.begin(), X.end());
+        X.erase(std::unique(X.begin(), X.end()), X.end());
+        std::vector<uint16_t>& R = pr[p];
+        std::vector<uint16_t>& Nb = pn[p];
+        R.assign(nv + 1, 0);
+        uint64_t total = 0;
+        for (uint32_t l = 0; l < nv; ++l) {
+            const auto r = row_of(l);
+            total += r.second - r.first;
+        }
+        if (nv + X.size() >= 0xFFFFu || total >= 0xFFFFu) {
+            err = kErrPatchTooBig;
+            return;
+        }
+        Nb.reserve(total);
+        for (uint32_t l = 0; l < nv; ++l) {
+            const auto r = row_of(l);
+            for (uint32_t k = r.first; k < r.second; ++k) {
+                uint32_t t = local_of(rows[k]);
+                if (t == kInvalid32)
+                    t = nv + static_cast<uint32_t>(std::lower_bound(X.begin(), X.end(), rows[k]) - X.begin());
+                Nb.push_back(static_cast<uint16_t>(t));
+            }
+            R[l + 1] = static_cast<uint16_t>(Nb.size());
+        }
+        for (uint32_t& g : X) g = m.v_g2l[g];  // gather list: linear ids
+    });
+    if (err != kOk) return err;
+    out.rbase.assign(P + 1, 0);
+    out.nbase.assign(P + 1, 0);
+    out.xbase.assign(P + 1, 0);
+    out.nx.assign(P, 0);
+    out.max_nx = out.max_nent = 0;
+    for (uint32_t p = 0; p < P; ++p) {
+        out.rbase[p + 1] = out.rbase[p] + round_up(static_cast<uint32_t>(pr[p].size()), 8);
+        out.nbase[p + 1] = out.nbase[p] + round_up(static_cast<uint32_t>(pn[p].size()), 8);
+        out.xbase[p + 1] = out.xbase[p] + round_up(static_cast<uint32_t>(px[p].size()), 4);
+        out.nx[p]        = static_cast<uint32_t>(px[p].size());
+        out.max_nx       = std::max(out.max_nx, out.nx[p]);
+        out.max_nent     = std::max(out.max_nent, static_cast<uint32_t>(pn[p].size()));
+    }
+    out.roff.assign(std::max<uint32_t>(out.rbase[P], 8), 0);
+    out.nbr.assign(std::max<uint32_t>(out.nbase[P], 8), 0);
+    out.xlin.assign(std::max<uint32_t>(out.xbase[P], 4), 0);
+    parallel_for(P, [&](size_t p) {
+        std::copy(pr[p].begin(), pr[p].end(), out.roff.begin() + out.rbase[p]);
+        std::copy(pn[p].begin(), pn[p].end(), out.nbr.begin() + out.nbase[p]);
+        std::copy(px[p].begin(), px[p].end(), out.xlin.begin() + out.xbase[p]);
+    }, 64);
+    return kOk;
+}
+
 int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& o, std::vector<uint32_t>& vb,
                   std::vector<uint32_t>& eb, std::vector<uint32_t>& fb, std::vector<uint32_t>& pb)
 {

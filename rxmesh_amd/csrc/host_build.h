@@ -108,6 +108,24 @@ int write_patch_file(const std::string& path, const HostMesh& m);
 int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& out, std::vector<uint32_t>& v_base,
                   std::vector<uint32_t>& e_base, std::vector<uint32_t>& f_base, std::vector<uint32_t>& p_base);
 
+// Two-step smoothing plan. For every patch p, a row for EVERY local vertex
+// l in [0, nv) -- owned or ribbon -- listing its full one-ring in the
+// canonical order of its owner patch's VV row (ascending owner-local edge
+// id), with neighbours numbered in p's extended local space: [0, nv) are p's
+// local vertices, [nv, nv + nx) the extra second-ring vertices, sorted by
+// global id. One workgroup can then run two Jacobi steps of its patch: step 1
+// on all local vertices, step 2 on the owned ones, with the same neighbour
+// order (hence the same results) as two single steps.
+struct Vv2Plan
+{
+    std::vector<uint32_t> rbase, nbase, xbase;  // P + 1 each: u16 / u16 / u32 offsets, runs padded to 8 / 8 / 4
+    std::vector<uint32_t> nx;                   // P: extra vertices per patch
+    std::vector<uint16_t> roff, nbr;            // per patch: nv + 1 row offsets; entries
+    std::vector<uint32_t> xlin;                 // per patch: linear ids of the extra vertices
+    uint32_t              max_nx = 0, max_nent = 0;
+};
+int build_vv2_plan(const HostMesh& m, Vv2Plan& out);
+
 unsigned num_threads();
 
 }  // namespace rxm

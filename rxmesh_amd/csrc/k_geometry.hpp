@@ -256,6 +256,105 @@ __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* _
     }
 }
 
+// ---- two Jacobi steps per launch over the two-step plan ------------------------
+// The workgroup stages the positions of its patch's local vertices and of
+// their second ring (extra vertices, gathered through the plan's linear-id
+// list), runs step 1 for every local vertex (owned and ribbon, each with
+// its owner's neighbour order) into LDS, then step 2 for the owned vertices
+// from those values. Each value equals the one two separate steps produce.
+struct PlanS2
+{
+    uint16_t *ro, *nb;
+    float *   x, *y, *z;     // nv + nx: input positions
+    float *   x1, *y1, *z1;  // nv: after step 1
+};
+__host__ __device__ inline PlanS2 plan_s2(Bump& b, uint32_t nv, uint32_t nx, uint32_t nent)
+{
+    PlanS2 p{};
+    p.ro = b.alloc<uint16_t>(((nv + 1u) + 7u) & ~7u);
+    p.nb = b.alloc<uint16_t>((nent + 7u) & ~7u);
+    p.x  = b.alloc<float>(nv + nx);
+    p.y  = b.alloc<float>(nv + nx);
+    p.z  = b.alloc<float>(nv + nx);
+    p.x1 = b.alloc<float>(nv);
+    p.y1 = b.alloc<float>(nv);
+    p.z1 = b.alloc<float>(nv);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __restrict__ rbase,
+                                               const uint32_t* __restrict__ nbase, const uint32_t* __restrict__ xbase,
+                                               const uint32_t* __restrict__ xcnt, const uint16_t* __restrict__ roff,
+                                               const uint16_t* __restrict__ nbr, const uint32_t* __restrict__ xlin,
+                                               float lambda, const float* __restrict__ pos, float* __restrict__ out)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d  = load_desc(m.patch, p);
+    const uint32_t  rb = rbase[p], nb0 = nbase[p], nent = nbase[p + 1] - nb0, xb = xbase[p], nx = xcnt[p];
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanS2    L = plan_s2(b, d.nv, nx, nent);
+    const uint32_t  N = m.nV;
+    stage_u16x8<T>(L.ro, roff + rb, d.nv + 1);
+    stage_u16x8<T>(L.nb, nbr + nb0, nent);
+    stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, pos, N, d, m.v_lin);
+    for (uint32_t i = threadIdx.x; i < nx; i += T) {
+        const uint32_t g = xlin[xb + i];
+        L.x[d.nv + i]    = pos[g];
+        L.y[d.nv + i]    = pos[N + g];
+        L.z[d.nv + i]    = pos[2 * N + g];
+    }
+    __syncthreads();
+    // step 1: every local vertex
+    for (uint32_t v = threadIdx.x; v < d.nv; v += T) {
+        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+        float          sx = 0.f, sy = 0.f, sz = 0.f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t w = L.nb[k];
+            sx += L.x[w];
+            sy += L.y[w];
+            sz += L.z[w];
+        }
+        const float px = L.x[v], py = L.y[v], pz = L.z[v];
+        if (k1 > k0) {
+            const float inv = 1.0f / static_cast<float>(k1 - k0);
+            L.x1[v]         = px + lambda * (sx * inv - px);
+            L.y1[v]         = py + lambda * (sy * inv - py);
+            L.z1[v]         = pz + lambda * (sz * inv - pz);
+        } else {
+            L.x1[v] = px;
+            L.y1[v] = py;
+            L.z1[v] = pz;
+        }
+    }
+    __syncthreads();
+    // step 2: owned vertices (their neighbours are all local)
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+        float          sx = 0.f, sy = 0.f, sz = 0.f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t w = L.nb[k];
+            sx += L.x1[w];
+            sy += L.y1[w];
+            sz += L.z1[w];
+        }
+        const uint32_t g  = d.vp + v;
+        const float    px = L.x1[v], py = L.y1[v], pz = L.z1[v];
+        if (k1 > k0) {
+            const float inv = 1.0f / static_cast<float>(k1 - k0);
+            out[g]          = px + lambda * (sx * inv - px);
+            out[N + g]      = py + lambda * (sy * inv - py);
+            out[2 * N + g]  = pz + lambda * (sz * inv - pz);
+        } else {
+            out[g]         = px;
+            out[N + g]     = py;
+            out[2 * N + g] = pz;
+        }
+    }
+}
+
 // ---- vertex normals over the patch-local face plan ----------------------------
 // Same weights and summation order as k_vertex_normals (incident faces in
 // ascending local order), but the VF rows come precomputed from the face

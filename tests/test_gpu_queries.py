@@ -156,19 +156,21 @@ def test_smoothing_vs_fp64(name):
 
 def test_smoothing_launch_configs_agree():
     """Every launch configuration the autotuner may pick (workgroup size, LDS
-    layout) gives bit-identical results: same neighbours, same summation order."""
+    layout, one or two Jacobi steps per launch) gives bit-identical results:
+    same neighbours, same summation order."""
     import torch
     V, F, m, o = case("c1")
     x = lin_pos(m, V.astype(np.float32))
     from rxmesh_amd import autotune
-    outs = []
-    for cfg in autotune.smooth_configs():
-        autotune.apply(m, cfg)
-        outs.append(m.smooth(x, 0.5, 3))
-    autotune.apply(m, {"smooth_threads": 128, "smooth_layout": 0})
-    torch.cuda.synchronize()
-    for y in outs[1:]:
-        assert torch.equal(outs[0], y)
+    for iters in (2, 3, 4):
+        outs = []
+        for cfg in autotune.smooth_configs():
+            autotune.apply(m, cfg)
+            outs.append(m.smooth(x, 0.5, iters))
+        autotune.apply(m, {"smooth_threads": 128, "smooth_layout": 0, "smooth_fused": 0})
+        torch.cuda.synchronize()
+        for y in outs[1:]:
+            assert torch.equal(outs[0], y)
 
 
 def test_vv_plan_matches_vv_query():

@@ -228,3 +228,41 @@ def test_bisect_patcher_components_get_their_own_patches():
     fp = m.array("face_patch")
     assert m.P <= 1.1 * (-(-len(F1) // 100) + -(-len(F2) // 100))
     assert not set(fp[:len(F1)]) & set(fp[len(F1):])
+
+
+# ---- the two-step smoothing plan ------------------------------------------------
+@pytest.mark.parametrize("case", ["c1", "disk", "bunnyhead"])
+def test_vv2_plan_rows_are_owner_rows(case):
+    """Every local vertex's row in the two-step plan lists its full one-ring
+    in its owner's canonical VV order (the oracle's VV row), and every
+    neighbour index stays inside the patch's extended local space."""
+    if case == "disk":
+        V, F = mg.disk(40, seed=2)
+    elif case == "bunnyhead":
+        V, F = obj("bunnyhead.obj")
+    else:
+        V, F = mg.config_mesh(case)
+    m = rx.Mesh(F, patch_size=256, host_only=True)
+    o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    orp, ocol = o.query("VV")                    # linear-id CSR
+    l2g = m.array("v_l2g")
+    P = m.P
+    offs = m.array("vv2_offsets")
+    rbase, nbase, xbase, nx = offs[:P + 1], offs[P + 1:2 * P + 2], offs[2 * P + 2:3 * P + 3], offs[3 * P + 3:]
+    roff, nbr, xlin = m.array("vv2_roff"), m.array("vv2_nbr"), m.array("vv2_xlin")
+    pc = m.array("patch_counts").reshape(-1, 6)
+    po = m.array("patch_offsets").reshape(3, -1)
+    ltog = m.array("ltog_v")
+    g2l = m.array("v_g2l")
+    for p in range(P):
+        nv = pc[p, 0]
+        ext = np.concatenate([ltog[po[0][p]:po[0][p] + nv], l2g[xlin[xbase[p]:xbase[p] + nx[p]]]])
+        assert len(np.unique(ext)) == len(ext)
+        ro = roff[rbase[p]:rbase[p] + nv + 1]
+        assert ro[0] == 0 and ro[-1] <= nbase[p + 1] - nbase[p]
+        for l in range(nv):
+            row = nbr[nbase[p] + ro[l]:nbase[p] + ro[l + 1]]
+            assert np.all(row < nv + nx[p])
+            lin = g2l[ext[l]]
+            want = l2g[ocol[orp[lin]:orp[lin + 1]]]
+            assert np.array_equal(ext[row], want)
