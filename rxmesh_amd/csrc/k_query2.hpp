@@ -32,7 +32,7 @@ __host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32
     if (kind != 2) p.ev = b.alloc<uint32_t>(r4(ne));
     if (kind == 1 || kind == 2) p.fe = b.alloc<uint16_t>(r4(3 * nf));
     p.off   = b.alloc<uint32_t>(ncols + 1);
-    p.cur   = b.alloc<uint32_t>(((kind == 2 && nf > ncols) ? nf : ncols) + 1);  // FF reuses it per owned face
+    if (kind == 2) p.cur = b.alloc<uint32_t>(nf + 1);  // FF: per owned face entry counts
     p.slots = b.alloc<uint32_t>(n);
     p.wave  = b.alloc<uint32_t>(20);
     p.lin   = b.alloc<uint32_t>((kind == 0) ? nv : ((kind == 3) ? ne : nf));
@@ -93,7 +93,7 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
             const uint32_t x = L.ev[i >> 1];
             return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
         },
-        L.off, L.cur, L.slots, L.wave);
+        L.off, L.slots, L.wave);
     const uint32_t bse = base[p];
     // sort each owned row in place, replacing slots by target linear ids;
     // owned rows are the prefix [0, off[nvo]) of the slot array
@@ -136,7 +136,7 @@ __global__ __launch_bounds__(T) void k_query2_vv_local(DevMesh m, const uint32_t
             const uint32_t x = L.ev[i >> 1];
             return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
         },
-        L.off, L.cur, L.slots, L.wave);
+        L.off, L.slots, L.wave);
     uint16_t* ro = roff + rbase[p];
     uint16_t* nb = nbr + nbase[p];
     for (uint32_t v = threadIdx.x; v <= d.nvo; v += T) ro[v] = static_cast<uint16_t>(L.off[v]);
@@ -171,7 +171,7 @@ __global__ __launch_bounds__(T) void k_query2_vf_local(DevMesh m, const uint32_t
         fo[i] = static_cast<uint16_t>(face_vertex(L.fe, L.ev, i / 3u, i % 3u));
     fill_columns<T>(
         3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
-        [&](uint32_t i) { return ((i / 3u) << 2) | (i % 3u); }, L.off, L.cur, L.slots, L.wave);
+        [&](uint32_t i) { return ((i / 3u) << 2) | (i % 3u); }, L.off, L.slots, L.wave);
     uint16_t* ro = roff + rbase[p];
     uint16_t* nb = vf + nbase[p];
     for (uint32_t v = threadIdx.x; v <= d.nvo; v += T) ro[v] = static_cast<uint16_t>(L.off[v]);
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
     __syncthreads();
     fill_columns<T>(
         3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
-        [&](uint32_t i) { return i / 3u; }, L.off, L.cur, L.slots, L.wave);
+        [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bse = base[p];
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         const uint32_t o = L.off[v], k = L.off[v + 1] - o;
@@ -224,7 +224,7 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
     __syncthreads();
     fill_columns<T>(
         3 * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
-        [&](uint32_t i) { return i / 3u; }, L.off, L.cur, L.slots, L.wave);
+        [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bse = base[p];
     if constexpr (!IS_FF) {
         for (uint32_t e = threadIdx.x; e < d.neo; e += T) {

@@ -84,18 +84,19 @@ __device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
 
 // Histogram + scan + packed slot fill of n entries into ncols columns.
 // key(i) -> column, pack(i) -> 32-bit slot value (sort key in the high bits).
+// Counts of column c accumulate in off[c + 1]; the exclusive scan turns them
+// into column starts, and the fill advances off[c + 1] from the start of c to
+// the start of c + 1, so no cursor array is needed. On return off[c] is the
+// start of column c and off[ncols] the total.
 template <uint32_t T, typename KeyF, typename PackF>
 __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF key, PackF pack, uint32_t* off,
-                                             uint32_t* cur, uint32_t* slots, uint32_t* s_wave)
+                                             uint32_t* slots, uint32_t* s_wave)
 {
-    for (uint32_t c = threadIdx.x; c < ncols; c += T) off[c] = 0;
+    for (uint32_t c = threadIdx.x; c <= ncols; c += T) off[c] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += T) atomicAdd(&off[key(i)], 1u);
-    const uint32_t total = scan_excl<T>(off, ncols, s_wave);
-    if (threadIdx.x == 0) off[ncols] = total;
-    for (uint32_t c = threadIdx.x; c < ncols; c += T) cur[c] = off[c];
-    __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += T) slots[atomicAdd(&cur[key(i)], 1u)] = pack(i);
+    for (uint32_t i = threadIdx.x; i < n; i += T) atomicAdd(&off[key(i) + 1u], 1u);
+    scan_excl<T>(off + 1, ncols, s_wave);
+    for (uint32_t i = threadIdx.x; i < n; i += T) slots[atomicAdd(&off[key(i) + 1u], 1u)] = pack(i);
     __syncthreads();
 }
 

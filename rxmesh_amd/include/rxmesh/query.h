@@ -126,14 +126,12 @@ __host__ __device__ inline QueryLDS plan_query(rxm::Bump& b, Op op, uint32_t nv,
                 q.gid = b.alloc<uint32_t>(nv);
                 q.cur = b.alloc<uint32_t>(nv);
             } else {
-                q.cur   = b.alloc<uint32_t>(nv + 1u);
                 q.slots = b.alloc<uint32_t>(2u * ne);
             }
             break;
         case Op::VF:
             q.ev    = b.alloc<uint32_t>(rxm::r4(ne));
             q.fe    = b.alloc<uint16_t>(rxm::r4(3u * nf));
-            q.cur   = b.alloc<uint32_t>(nv + 1u);
             q.slots = b.alloc<uint32_t>(3u * nf);
             break;
         case Op::FV:
@@ -142,13 +140,11 @@ __host__ __device__ inline QueryLDS plan_query(rxm::Bump& b, Op op, uint32_t nv,
             break;
         case Op::EF:
             q.fe    = b.alloc<uint16_t>(rxm::r4(3u * nf));
-            q.cur   = b.alloc<uint32_t>(ne + 1u);
             q.slots = b.alloc<uint32_t>(3u * nf);
             break;
         case Op::FF:
             q.fe    = b.alloc<uint16_t>(rxm::r4(3u * nf));
             q.eoff  = b.alloc<uint32_t>(ne + 1u);
-            q.cur   = b.alloc<uint32_t>(ne + 1u);
             q.slots = b.alloc<uint32_t>(3u * nf);
             break;
         default: break;
@@ -321,7 +317,7 @@ struct Query
                         const uint32_t x = L.ev[i >> 1];
                         return ((i >> 1) << 16) | ((i & 1u) ? rxm::lo16(x) : rxm::hi16(x));
                     },
-                    m_off, L.cur, L.slots, L.wave);
+                    m_off, L.slots, L.wave);
                 for (uint32_t v = threadIdx.x; v < rows; v += T) {
                     const uint32_t o = m_off[v];
                     rxm::sort_row(L.slots + o, m_off[v + 1] - o, [&](uint32_t j, uint32_t s) {
@@ -335,7 +331,7 @@ struct Query
             block.sync();
             rxm::fill_columns<T>(
                 3u * d.nf, d.nv, [&](uint32_t i) { return rxm::face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
-                [&](uint32_t i) { return i / 3u; }, m_off, L.cur, L.slots, L.wave);
+                [&](uint32_t i) { return i / 3u; }, m_off, L.slots, L.wave);
             for (uint32_t v = threadIdx.x; v < rows; v += T) {
                 const uint32_t o = m_off[v];
                 rxm::sort_row(L.slots + o, m_off[v + 1] - o,
@@ -358,7 +354,7 @@ struct Query
             block.sync();
             rxm::fill_columns<T>(
                 3u * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
-                [&](uint32_t i) { return i / 3u; }, m_off, L.cur, L.slots, L.wave);
+                [&](uint32_t i) { return i / 3u; }, m_off, L.slots, L.wave);
             for (uint32_t e = threadIdx.x; e < rows; e += T) {
                 const uint32_t o = m_off[e];
                 rxm::sort_row(L.slots + o, m_off[e + 1] - o,
@@ -369,7 +365,7 @@ struct Query
             block.sync();
             rxm::fill_columns<T>(
                 3u * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
-                [&](uint32_t i) { return i / 3u; }, L.eoff, L.cur, L.slots, L.wave);
+                [&](uint32_t i) { return i / 3u; }, L.eoff, L.slots, L.wave);
             for (uint32_t e = threadIdx.x; e < d.ne; e += T) {
                 const uint32_t o = L.eoff[e], k = L.eoff[e + 1] - o;
                 if (k > 1u) rxm::sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = s; });
