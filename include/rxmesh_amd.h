@@ -171,7 +171,7 @@ rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float*
  * builds it on first use; call this to keep the build out of a timed loop. */
 rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
 /* Launch configuration knobs (used by the autotuner):
- *   kernel 0: smoothing workgroup size 64 / 128 / 256;
+ *   kernel 0: smoothing workgroup size 64 / 128 / 256 / 512;
  *   kernel 1: symmetric-Dirichlet step workgroup size 128 / 256;
  *   kernel 2: symmetric-Dirichlet step launch, 0 = one workgroup per patch,
  *             1 = persistent grid over a global patch work queue;

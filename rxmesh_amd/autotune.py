@@ -15,7 +15,7 @@ import os
 CACHE_PATH = os.environ.get("RXM_AUTOTUNE_CACHE",
                             os.path.join(os.path.expanduser("~"), ".cache", "rxmesh_amd_autotune.json"))
 
-SMOOTH_THREADS = (64, 128, 256)
+SMOOTH_THREADS = (64, 128, 256, 512)
 GRAD_THREADS = (128, 256)
 GRAD_QUEUE = (0, 1)
 

@@ -324,6 +324,9 @@ int smooth_once_t(rxm_mesh* m, float lambda, const float* src, float* dst, void*
         case 256:
             return launch_patches_xcd(k_smooth_local<256, AOS4>, 256, b.bytes(), stream, m, m->dm, m->d_vv_rbase,
                                       m->d_vv_nbase, ro, nb, lambda, src, dst);
+        case 512:
+            return launch_patches_xcd(k_smooth_local<512, AOS4>, 512, b.bytes(), stream, m, m->dm, m->d_vv_rbase,
+                                      m->d_vv_nbase, ro, nb, lambda, src, dst);
         default:
             return launch_patches_xcd(k_smooth_local<128, AOS4>, 128, b.bytes(), stream, m, m->dm, m->d_vv_rbase,
                                       m->d_vv_nbase, ro, nb, lambda, src, dst);
@@ -367,6 +370,7 @@ int smooth_twice(rxm_mesh* m, float lambda, const float* src, float* dst, void* 
     switch (m->smooth_threads) {
         case 64: return go(k_smooth2<64>, 64);
         case 256: return go(k_smooth2<256>, 256);
+        case 512: return go(k_smooth2<512>, 512);
         default: return go(k_smooth2<128>, 128);
     }
 }
@@ -769,7 +773,7 @@ rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float
 rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
 {
     if (!m) return RXM_ERR_INVALID_ARG;
-    if (kernel == 0 && (threads == 64 || threads == 128 || threads == 256)) {
+    if (kernel == 0 && (threads == 64 || threads == 128 || threads == 256 || threads == 512)) {
         m->smooth_threads = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
