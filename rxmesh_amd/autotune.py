@@ -66,8 +66,6 @@ def smooth_configs():
     for fused in SMOOTH_FUSED:
         for st in SMOOTH_THREADS:
             for lay in SMOOTH_LAYOUTS:
-                if fused and lay:  # the two-step kernel stages three float arrays only
-                    continue
                 yield {"smooth_threads": st, "smooth_layout": lay, "smooth_fused": fused}
 
 

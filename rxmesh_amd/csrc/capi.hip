@@ -358,21 +358,28 @@ int ensure_vv2_plan(rxm_mesh* m)
 }
 
 // two Jacobi steps in one launch
-int smooth_twice(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
+template <bool AOS4>
+int smooth_twice_t(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
 {
     Bump b(nullptr);
-    plan_s2(b, m->h.max_nv, m->vv2.max_nx, m->vv2.max_nent);
+    plan_s2(b, m->h.max_nv, m->vv2.max_nx, m->vv2.max_nent, AOS4);
     auto go = [&](auto kernel, uint32_t T) {
         return launch_patches_xcd(kernel, T, b.bytes(), stream, m, m->dm, m->d_vv2_rbase, m->d_vv2_nbase,
                                   m->d_vv2_xbase, m->d_vv2_xcnt, m->d_vv2_roff, m->d_vv2_nbr, m->d_vv2_xlin, lambda,
                                   src, dst);
     };
     switch (m->smooth_threads) {
-        case 64: return go(k_smooth2<64>, 64);
-        case 256: return go(k_smooth2<256>, 256);
-        case 512: return go(k_smooth2<512>, 512);
-        default: return go(k_smooth2<128>, 128);
+        case 64: return go(k_smooth2<64, AOS4>, 64);
+        case 256: return go(k_smooth2<256, AOS4>, 256);
+        case 512: return go(k_smooth2<512, AOS4>, 512);
+        default: return go(k_smooth2<128, AOS4>, 128);
     }
+}
+
+int smooth_twice(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
+{
+    return m->smooth_aos4 ? smooth_twice_t<true>(m, lambda, src, dst, stream)
+                          : smooth_twice_t<false>(m, lambda, src, dst, stream);
 }
 
 int smooth_once(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)

@@ -262,27 +262,66 @@ __global__ __launch_bounds__(T) void k_smooth_local(DevMesh m, const uint32_t* _
 // list), runs step 1 for every local vertex (owned and ribbon, each with
 // its owner's neighbour order) into LDS, then step 2 for the owned vertices
 // from those values. Each value equals the one two separate steps produce.
+// AOS4 keeps both position sets as one float4 per vertex in LDS.
 struct PlanS2
 {
     uint16_t *ro, *nb;
     float *   x, *y, *z;     // nv + nx: input positions
     float *   x1, *y1, *z1;  // nv: after step 1
+    float4 *  p4, *q4;       // AOS4 forms of the two sets
 };
-__host__ __device__ inline PlanS2 plan_s2(Bump& b, uint32_t nv, uint32_t nx, uint32_t nent)
+__host__ __device__ inline PlanS2 plan_s2(Bump& b, uint32_t nv, uint32_t nx, uint32_t nent, bool aos4 = false)
 {
     PlanS2 p{};
     p.ro = b.alloc<uint16_t>(((nv + 1u) + 7u) & ~7u);
     p.nb = b.alloc<uint16_t>((nent + 7u) & ~7u);
-    p.x  = b.alloc<float>(nv + nx);
-    p.y  = b.alloc<float>(nv + nx);
-    p.z  = b.alloc<float>(nv + nx);
-    p.x1 = b.alloc<float>(nv);
-    p.y1 = b.alloc<float>(nv);
-    p.z1 = b.alloc<float>(nv);
+    if (aos4) {
+        p.p4 = b.alloc<float4>(nv + nx);
+        p.q4 = b.alloc<float4>(nv);
+    } else {
+        p.x  = b.alloc<float>(nv + nx);
+        p.y  = b.alloc<float>(nv + nx);
+        p.z  = b.alloc<float>(nv + nx);
+        p.x1 = b.alloc<float>(nv);
+        p.y1 = b.alloc<float>(nv);
+        p.z1 = b.alloc<float>(nv);
+    }
     return p;
 }
 
-template <uint32_t T>
+template <bool AOS4>
+__device__ __forceinline__ float3 s2_get(const PlanS2& L, uint32_t w, bool second)
+{
+    if constexpr (AOS4) {
+        const float4 q = second ? L.q4[w] : L.p4[w];
+        return make_float3(q.x, q.y, q.z);
+    } else {
+        return second ? make_float3(L.x1[w], L.y1[w], L.z1[w]) : make_float3(L.x[w], L.y[w], L.z[w]);
+    }
+}
+
+// one Jacobi update of vertex v from set `second` (false: input, true: step 1)
+template <bool AOS4>
+__device__ __forceinline__ float3 s2_update(const PlanS2& L, uint32_t v, bool second, float lambda)
+{
+    const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+    float          sx = 0.f, sy = 0.f, sz = 0.f;
+    for (uint32_t k = k0; k < k1; ++k) {
+        const float3 q = s2_get<AOS4>(L, L.nb[k], second);
+        sx += q.x;
+        sy += q.y;
+        sz += q.z;
+    }
+    const float3 pv = s2_get<AOS4>(L, v, second);
+    if (k1 > k0) {
+        const float inv = 1.0f / static_cast<float>(k1 - k0);
+        return make_float3(pv.x + lambda * (sx * inv - pv.x), pv.y + lambda * (sy * inv - pv.y),
+                           pv.z + lambda * (sz * inv - pv.z));
+    }
+    return pv;
+}
+
+template <uint32_t T, bool AOS4>
 __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __restrict__ rbase,
                                                const uint32_t* __restrict__ nbase, const uint32_t* __restrict__ xbase,
                                                const uint32_t* __restrict__ xcnt, const uint16_t* __restrict__ roff,
@@ -295,63 +334,42 @@ __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __rest
     const PatchDesc d  = load_desc(m.patch, p);
     const uint32_t  rb = rbase[p], nb0 = nbase[p], nent = nbase[p + 1] - nb0, xb = xbase[p], nx = xcnt[p];
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanS2    L = plan_s2(b, d.nv, nx, nent);
+    const PlanS2    L = plan_s2(b, d.nv, nx, nent, AOS4);
     const uint32_t  N = m.nV;
     stage_u16x8<T>(L.ro, roff + rb, d.nv + 1);
     stage_u16x8<T>(L.nb, nbr + nb0, nent);
-    stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, pos, N, d, m.v_lin);
+    stage_positions_batched<T, 512u / T, AOS4>(L.x, L.y, L.z, L.p4, pos, N, d, m.v_lin);
     for (uint32_t i = threadIdx.x; i < nx; i += T) {
         const uint32_t g = xlin[xb + i];
-        L.x[d.nv + i]    = pos[g];
-        L.y[d.nv + i]    = pos[N + g];
-        L.z[d.nv + i]    = pos[2 * N + g];
+        const float    x = pos[g], y = pos[N + g], z = pos[2 * N + g];
+        if constexpr (AOS4) {
+            L.p4[d.nv + i] = make_float4(x, y, z, 0.f);
+        } else {
+            L.x[d.nv + i] = x;
+            L.y[d.nv + i] = y;
+            L.z[d.nv + i] = z;
+        }
     }
     __syncthreads();
     // step 1: every local vertex
     for (uint32_t v = threadIdx.x; v < d.nv; v += T) {
-        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
-        float          sx = 0.f, sy = 0.f, sz = 0.f;
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t w = L.nb[k];
-            sx += L.x[w];
-            sy += L.y[w];
-            sz += L.z[w];
-        }
-        const float px = L.x[v], py = L.y[v], pz = L.z[v];
-        if (k1 > k0) {
-            const float inv = 1.0f / static_cast<float>(k1 - k0);
-            L.x1[v]         = px + lambda * (sx * inv - px);
-            L.y1[v]         = py + lambda * (sy * inv - py);
-            L.z1[v]         = pz + lambda * (sz * inv - pz);
+        const float3 r = s2_update<AOS4>(L, v, false, lambda);
+        if constexpr (AOS4) {
+            L.q4[v] = make_float4(r.x, r.y, r.z, 0.f);
         } else {
-            L.x1[v] = px;
-            L.y1[v] = py;
-            L.z1[v] = pz;
+            L.x1[v] = r.x;
+            L.y1[v] = r.y;
+            L.z1[v] = r.z;
         }
     }
     __syncthreads();
     // step 2: owned vertices (their neighbours are all local)
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
-        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
-        float          sx = 0.f, sy = 0.f, sz = 0.f;
-        for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t w = L.nb[k];
-            sx += L.x1[w];
-            sy += L.y1[w];
-            sz += L.z1[w];
-        }
-        const uint32_t g  = d.vp + v;
-        const float    px = L.x1[v], py = L.y1[v], pz = L.z1[v];
-        if (k1 > k0) {
-            const float inv = 1.0f / static_cast<float>(k1 - k0);
-            out[g]          = px + lambda * (sx * inv - px);
-            out[N + g]      = py + lambda * (sy * inv - py);
-            out[2 * N + g]  = pz + lambda * (sz * inv - pz);
-        } else {
-            out[g]         = px;
-            out[N + g]     = py;
-            out[2 * N + g] = pz;
-        }
+        const float3   r = s2_update<AOS4>(L, v, true, lambda);
+        const uint32_t g = d.vp + v;
+        out[g]           = r.x;
+        out[N + g]       = r.y;
+        out[2 * N + g]   = r.z;
     }
 }
 
