@@ -3,7 +3,7 @@
 // Same results as the general transpose in lds_core.hpp -- every row lists
 // its entries in ascending source-local order -- with far fewer vector
 // instructions per entry:
-//   1. degree histogram with LDS atomics (order-free);
+//   1. degree histogram with LDS atomics (order-free), owned rows only;
 //   2. exclusive scan (compile-time workgroup size, no runtime division);
 //   3. slot fill with LDS atomics, each slot packing (sort key << 16 | payload);
 //   4. one thread per owned row loads its slots into registers, sorts them
@@ -84,7 +84,7 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
     __syncthreads();
     // entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
     fill_columns<T>(
-        2 * d.ne, d.nv,
+        2 * d.ne, d.nvo,
         [&](uint32_t i) {
             const uint32_t x = L.ev[i >> 1];
             return (i & 1u) ? hi16(x) : lo16(x);
@@ -127,7 +127,7 @@ __global__ __launch_bounds__(T) void k_query2_vv_local(DevMesh m, const uint32_t
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
     __syncthreads();
     fill_columns<T>(
-        2 * d.ne, d.nv,
+        2 * d.ne, d.nvo,
         [&](uint32_t i) {
             const uint32_t x = L.ev[i >> 1];
             return (i & 1u) ? hi16(x) : lo16(x);
@@ -170,7 +170,7 @@ __global__ __launch_bounds__(T) void k_query2_vf_local(DevMesh m, const uint32_t
     for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T)
         fo[i] = static_cast<uint16_t>(face_vertex(L.fe, L.ev, i / 3u, i % 3u));
     fill_columns<T>(
-        3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+        3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
         [&](uint32_t i) { return ((i / 3u) << 2) | (i % 3u); }, L.off, L.slots, L.wave);
     uint16_t* ro = roff + rbase[p];
     uint16_t* nb = vf + nbase[p];
@@ -196,7 +196,7 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
     stage_lin(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, T);
     __syncthreads();
     fill_columns<T>(
-        3 * d.nf, d.nv, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+        3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
         [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bse = base[p];
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
@@ -223,7 +223,7 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
     stage_lin(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, T);
     __syncthreads();
     fill_columns<T>(
-        3 * d.nf, d.ne, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
+        3 * d.nf, IS_FF ? d.ne : d.neo, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
         [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bse = base[p];
     if constexpr (!IS_FF) {

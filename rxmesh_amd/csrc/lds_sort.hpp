@@ -84,19 +84,46 @@ __device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
 
 // Histogram + scan + packed slot fill of n entries into ncols columns.
 // key(i) -> column, pack(i) -> 32-bit slot value (sort key in the high bits).
+// Entries whose column is >= ncols are dropped, so a caller that outputs
+// only owned rows (owned ids are the low local ids) sorts only those.
 // Counts of column c accumulate in off[c + 1]; the exclusive scan turns them
 // into column starts, and the fill advances off[c + 1] from the start of c to
 // the start of c + 1, so no cursor array is needed. On return off[c] is the
 // start of column c and off[ncols] the total.
-template <uint32_t T, typename KeyF, typename PackF>
+// When n <= R * T the keys and slot values of a thread's entries are
+// computed once and held in registers between the two passes.
+template <uint32_t T, uint32_t R = 12, typename KeyF, typename PackF>
 __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF key, PackF pack, uint32_t* off,
                                              uint32_t* slots, uint32_t* s_wave)
 {
     for (uint32_t c = threadIdx.x; c <= ncols; c += T) off[c] = 0;
     __syncthreads();
-    for (uint32_t i = threadIdx.x; i < n; i += T) atomicAdd(&off[key(i) + 1u], 1u);
-    scan_excl<T>(off + 1, ncols, s_wave);
-    for (uint32_t i = threadIdx.x; i < n; i += T) slots[atomicAdd(&off[key(i) + 1u], 1u)] = pack(i);
+    if (n <= R * T) {
+        uint32_t kr[R], pr[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t i = threadIdx.x + r * T;
+            kr[r]            = (i < n) ? key(i) : ncols;
+            if (kr[r] < ncols) {
+                pr[r] = pack(i);
+                atomicAdd(&off[kr[r] + 1u], 1u);
+            }
+        }
+        scan_excl<T>(off + 1, ncols, s_wave);
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r)
+            if (kr[r] < ncols) slots[atomicAdd(&off[kr[r] + 1u], 1u)] = pr[r];
+    } else {
+        for (uint32_t i = threadIdx.x; i < n; i += T) {
+            const uint32_t k = key(i);
+            if (k < ncols) atomicAdd(&off[k + 1u], 1u);
+        }
+        scan_excl<T>(off + 1, ncols, s_wave);
+        for (uint32_t i = threadIdx.x; i < n; i += T) {
+            const uint32_t k = key(i);
+            if (k < ncols) slots[atomicAdd(&off[k + 1u], 1u)] = pack(i);
+        }
+    }
     __syncthreads();
 }
 
