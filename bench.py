@@ -34,6 +34,7 @@ def parse():
     ap.add_argument("--smooth-iters", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-configs", action="store_true", help="skip the c0 / c2s / c2t / c3 workloads")
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", default=None,
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
@@ -175,6 +176,109 @@ def batch_bench(count, steps, reps, patch_size):
             "speedup_vs_plain": round(t_plain / t_graph, 3)}
 
 
+def _entry(t_ms, nbytes, unit_count=None, unit=None):
+    r = {"us": round(t_ms * 1e3, 2), "algorithmic_bytes": int(nbytes),
+         "GB_s": round(nbytes / (t_ms * 1e-3) / 1e9, 1),
+         "frac": round(nbytes / (t_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)}
+    if unit_count is not None:
+        r[unit] = round(unit_count / (t_ms * 1e-3) / 1e9, 4)
+    return r
+
+
+def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
+    """The other workloads of the suite, each measured with HIP events on
+    the launch stream against the kernel's algorithmic bytes (fp32 = 4 B,
+    int32 index = 4 B, patch-local u16 = 2 B):
+      c0: area-weighted vertex normals + VV query on the jittered 128 x 128 grid;
+      c2s / c2t: cotangent CSR assembly (A = M + dt L, oriented VV rows) and
+          explicit mean-curvature flow x <- x + (M x - A x) / M, 50 SpMV-3
+          iterations, dt = 1e-5, on the icosphere L9 and the 2048 x 1024 torus;
+      c3: symmetric-Dirichlet energy and its adjoint gradient on the
+          256 x 256 grid clipped to a disk, UV = rest x, y + N(0, 0.002)."""
+    import numpy as np
+    import torch
+    import rxmesh_amd as rx
+    from rxmesh_amd import meshgen as mg
+    out = {}
+    t0 = time.perf_counter()
+    log = lambda msg: print("[configs %.1fs] %s" % (time.perf_counter() - t0, msg), file=sys.stderr, flush=True)  # noqa: E731
+
+    def lin(m, pos):
+        return torch.from_numpy(np.ascontiguousarray(pos[m.array("v_l2g")].T.astype(np.float32))).cuda()
+
+    # c0
+    pos, fv = mg.config_mesh("c0")
+    m = rx.Mesh(fv)
+    V, E, F = m.nV, m.nE, m.nF
+    x = lin(m, pos)
+    n = torch.empty_like(x)
+    rp = torch.empty(V + 1, dtype=torch.int32, device="cuda")
+    col = torch.empty(m.query_nnz("VV"), dtype=torch.int32, device="cuda")
+    m.vertex_normals(x, "area", out=n)
+    t_n = event_time_ms(lambda: m.vertex_normals(x, "area", out=n), reps)
+    t_q = event_time_ms(lambda: m.query("VV", row_ptr=rp, col=col), reps)
+    out["c0_grid128"] = {"V": V, "F": F, "normals_area": _entry(t_n, 24 * V + 4 * E + 6 * F, F, "Gfaces_s"),
+                         "query_vv": _entry(t_q, 4 * E + 8 * E + 4 * (V + 1), 2 * E, "Gentries_s")}
+    m.close()
+    log("c0 done")
+
+    def mcf(m, x, name):
+        V, E, F = m.nV, m.nE, m.nF
+        dt = 1e-5
+        A = m.cot_assemble(x, dt)
+        nnz = int(A[1].numel())
+        t_a = event_time_ms(lambda: m.cot_assemble(x, dt), max(3, reps // 5))
+        y = torch.empty_like(x)
+        tmp = torch.empty_like(x)
+        sweep = {}
+        for lanes in (1, 2, 4, 8):  # lanes per CSR row: autotuned here
+            m.set_launch_threads(5, lanes)
+            m.mcf_explicit(A, x, 50, out=y, tmp=tmp)
+            sweep[lanes] = event_time_ms(lambda: m.mcf_explicit(A, x, 50, out=y, tmp=tmp), max(3, reps // 10))
+        best = min(sweep, key=sweep.get)
+        m.set_launch_threads(5, best)
+        t_m = sweep[best]
+        out[name] = {"V": V, "F": F, "nnz": nnz,
+                     "cot_csr_assembly": _entry(t_a, 12 * V + 4 * E + 6 * F + 8 * nnz + 4 * (V + 1) + 4 * V,
+                                                nnz, "Gnnz_s"),
+                     "mcf_spmv3_iteration": _entry(t_m / 50, 8 * nnz + 4 * (V + 1) + 12 * V + 4 * V + 12 * V,
+                                                   nnz, "Gnnz_s"),
+                     "mcf_50_iterations_ms": round(t_m, 4), "spmv_lanes": best,
+                     "spmv_lanes_sweep_ms": {str(k): round(v, 4) for k, v in sweep.items()}}
+        del A, y, tmp
+
+    if c2s_mesh is not None:
+        mcf(c2s_mesh, c2s_pos, "c2s_icosphere_L9")
+        log("c2s done")
+    pos, fv = mg.config_mesh("c2t")
+    m = rx.Mesh(fv)
+    x = lin(m, pos)
+    mcf(m, x, "c2t_torus_2048x1024")
+    del x
+    m.close()
+    log("c2t done")
+
+    # c3
+    pos, fv = mg.config_mesh("c3")
+    m = rx.Mesh(fv)
+    V, E, F = m.nV, m.nE, m.nF
+    uv = mg.uv_init(pos, fv, seed=0)
+    rest = lin(m, pos)
+    u = torch.from_numpy(np.ascontiguousarray(uv[m.array("v_l2g")].T.astype(np.float32))).cuda()
+    Mr = m.symdir_rest(rest)
+    total, energy = m.symdir_energy(Mr, u)
+    g = m.symdir_grad(Mr, u)
+    t_e = event_time_ms(lambda: m.symdir_energy(Mr, u, energy=energy, total=total), reps)
+    t_g = event_time_ms(lambda: m.symdir_grad(Mr, u, out=g), reps)
+    topo = 6 * F + 4 * E
+    out["c3_disk256"] = {"V": V, "F": F,
+                         "symdir_energy": _entry(t_e, 16 * F + 8 * V + topo + 4 * F, F, "Gfaces_s"),
+                         "symdir_grad": _entry(t_g, 16 * F + 8 * V + topo + 8 * V, F, "Gfaces_s")}
+    m.close()
+    log("c3 done")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -292,8 +396,12 @@ def main():
         "kernels": kernels,
         "smooth_autotune": {"chosen": smooth_cfg, "us": smooth_sweep},
     }
+    if not args.no_configs:
+        del out, tmp, vv_rp, vv_col, vf_rp, vf_col
+        torch.cuda.empty_cache()
+        line["configs"] = configs_bench(reps, m, x)
     # free the single-mesh workload before the batch
-    del x, out, tmp, vv_rp, vv_col, vf_rp, vf_col
+    del x
     m.close()
     torch.cuda.empty_cache()
     if args.batch_count > 0:

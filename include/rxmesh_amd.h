@@ -178,8 +178,10 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *   kernel 3: smoothing LDS layout of positions, 0 = three float arrays,
  *             1 = one float4 per vertex;
  *   kernel 4: smoothing, 1 = two Jacobi steps per launch over the two-step
- *             plan (RXM_ARR_VV2_*), 0 = one step per launch.
- * All settings give bit-identical results. */
+ *             plan (RXM_ARR_VV2_*), 0 = one step per launch;
+ *   kernel 5: SpMV-3 / explicit MCF lanes per CSR row, 1 / 2 / 4 / 8.
+ * Settings of kernels 0-4 give bit-identical results; kernel 5 changes the
+ * summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 
 /* `iters` Jacobi steps of uniform Laplacian smoothing

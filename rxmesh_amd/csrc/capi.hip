@@ -48,6 +48,7 @@ struct rxm_mesh
     const uint16_t*   d_vv2_nbr   = nullptr;
     const uint32_t*   d_vv2_xlin  = nullptr;
     uint32_t          smooth_fused = 0;  // 1: two Jacobi steps per launch
+    uint32_t          spmv_lanes   = 8;  // lanes per CSR row in the SpMV-3 kernels
     // patch-local face plan (FV table + VF rows of (face << 2 | corner))
     const uint32_t* d_fp_fbase = nullptr;
     const uint32_t* d_fp_rbase = nullptr;
@@ -800,6 +801,10 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
         m->smooth_fused = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
+    if (kernel == 5 && (threads == 1 || threads == 2 || threads == 4 || threads == 8)) {  // SpMV-3 lanes per row
+        m->spmv_lanes = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
     return RXM_ERR_INVALID_ARG;
 }
 
@@ -871,20 +876,36 @@ rxm_status rxm_cot_csr_assemble(rxm_mesh* m, const float* pos, float dt, int32_t
                           col, val, mass);
 }
 
-static constexpr uint32_t kSpmvLanes = 8;
+}  // extern "C"
+
+// lanes per CSR row of the SpMV-3 kernels (autotuned: 1, 2, 4 or 8)
+template <bool FUSED>
+int launch_spmv3(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* val, const float* mass,
+                 const float* x, float* y, void* stream)
+{
+    const uint32_t n       = m->h.nV;
+    const uint32_t lanes   = m->spmv_lanes;
+    const uint64_t threads = uint64_t(n) * lanes;
+    const dim3     grid(static_cast<uint32_t>((threads + 255) / 256)), block(256);
+    hipStream_t    s = static_cast<hipStream_t>(stream);
+    switch (lanes) {
+        case 1: hipLaunchKernelGGL((k_spmv3<1, FUSED>), grid, block, 0, s, n, row_ptr, col, val, mass, x, y); break;
+        case 2: hipLaunchKernelGGL((k_spmv3<2, FUSED>), grid, block, 0, s, n, row_ptr, col, val, mass, x, y); break;
+        case 4: hipLaunchKernelGGL((k_spmv3<4, FUSED>), grid, block, 0, s, n, row_ptr, col, val, mass, x, y); break;
+        default: hipLaunchKernelGGL((k_spmv3<8, FUSED>), grid, block, 0, s, n, row_ptr, col, val, mass, x, y); break;
+    }
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+extern "C" {
 
 rxm_status rxm_spmv3(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* val, const float* x,
                      float* y, void* stream)
 {
     if (!m || !row_ptr || !col || !val || !x || !y || x == y) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    const uint32_t n = m->h.nV;
-    const uint64_t threads = uint64_t(n) * kSpmvLanes;
-    hipLaunchKernelGGL((k_spmv3<kSpmvLanes, false>), dim3(static_cast<uint32_t>((threads + 255) / 256)), dim3(256),
-                       0, static_cast<hipStream_t>(stream), n, row_ptr, col, val, static_cast<const float*>(nullptr),
-                       x, y);
-    RXM_HIP(hipGetLastError());
-    return RXM_OK;
+    return launch_spmv3<false>(m, row_ptr, col, val, nullptr, x, y, stream);
 }
 
 rxm_status rxm_mcf_explicit(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* val,
@@ -893,14 +914,11 @@ rxm_status rxm_mcf_explicit(rxm_mesh* m, const int32_t* row_ptr, const int32_t* 
     if (!m || !row_ptr || !col || !val || !mass || !x || !out || iters < 1 || x == out) return RXM_ERR_INVALID_ARG;
     if (iters > 1 && (!tmp || tmp == out || tmp == x)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    const uint32_t n = m->h.nV;
-    const uint64_t threads = uint64_t(n) * kSpmvLanes;
     const float* src = x;
     for (int it = 0; it < iters; ++it) {
         float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
-        hipLaunchKernelGGL((k_spmv3<kSpmvLanes, true>), dim3(static_cast<uint32_t>((threads + 255) / 256)),
-                           dim3(256), 0, static_cast<hipStream_t>(stream), n, row_ptr, col, val, mass, src, dst);
-        RXM_HIP(hipGetLastError());
+        const int rc = launch_spmv3<true>(m, row_ptr, col, val, mass, src, dst, stream);
+        if (rc) return rc;
         src = dst;
     }
     return RXM_OK;

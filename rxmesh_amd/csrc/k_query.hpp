@@ -59,12 +59,13 @@ __device__ inline void vx_prologue(const DevMesh& m, const PatchDesc& d, const P
         for (uint32_t i = threadIdx.x; i < 2 * d.ne; i += T) L.ef[i] = 0xFFFFFFFFu;
     }
     __syncthreads();
+    // the canonical ring does not depend on the row order: no row sort needed
     lds_transpose<2>(2 * d.ne, d.nv,
                      [&](uint32_t i) {
                          const uint32_t x = L.ev[i >> 1];
                          return (i & 1u) ? hi16(x) : lo16(x);
                      },
-                     L.off, L.cur, L.val, L.wave, d.nvo);
+                     L.off, L.cur, L.val, L.wave, oriented ? 0u : d.nvo);
     if (oriented) {
         for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T) {
             const uint32_t e = L.fe[i] >> 1, f = i / 3;
@@ -72,7 +73,7 @@ __device__ inline void vx_prologue(const DevMesh& m, const PatchDesc& d, const P
         }
         __syncthreads();
         for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
-            orient_row(v, L.off[v], L.off[v + 1], L.val, L.ev, L.fe, L.ef, L.gid);
+            orient_row_fast<12>(v, L.off[v], L.off[v + 1], L.val, L.ev, L.fe, L.ef, L.gid);
         __syncthreads();
     }
 }

@@ -74,4 +74,79 @@ __device__ inline void orient_row(uint32_t v, uint32_t k0, uint32_t k1, uint16_t
     }
 }
 
+// Same ordering computed in registers: each edge's successor (the third
+// vertex of the face holding half-edge v->r) and its startable flag are
+// gathered once (independent LDS reads), then the start selection and the
+// walk are compares over register arrays. The ordering does not depend on
+// the order val[k0..k1) comes in. Rows longer than KMAX use orient_row.
+template <uint32_t KMAX>
+__device__ inline void orient_row_fast(uint32_t v, uint32_t k0, uint32_t k1, uint16_t* val, const uint32_t* s_ev,
+                                       const uint16_t* s_fe, const uint32_t* s_ef, const uint32_t* s_gid)
+{
+    const uint32_t K = k1 - k0;
+    if (K > KMAX) {
+        orient_row(v, k0, k1, val, s_ev, s_fe, s_ef, s_gid);
+        return;
+    }
+    uint32_t er[KMAX];  // edge | other endpoint << 16
+    uint32_t sq[KMAX];  // successor vertex (0xFFFF: none) | startable << 16
+#pragma unroll
+    for (uint32_t i = 0; i < KMAX; ++i) {
+        er[i] = 0;
+        sq[i] = 0xFFFFu;
+        if (i < K) {
+            const uint32_t e = val[k0 + i], x = s_ev[e];
+            const uint32_t r = (lo16(x) == v) ? hi16(x) : lo16(x);
+            uint32_t       q = 0xFFFFu;
+            for (uint32_t t = 0; t < 2; ++t) {
+                const uint32_t f = s_ef[2 * e + t];
+                if (f == 0xFFFFFFFFu || q != 0xFFFFu) continue;
+                const uint32_t a = face_vertex(s_fe, s_ev, f, 0), b = face_vertex(s_fe, s_ev, f, 1),
+                               c = face_vertex(s_fe, s_ev, f, 2);
+                if (a == v && b == r) q = c;
+                else if (b == v && c == r) q = a;
+                else if (c == v && a == r) q = b;
+            }
+            const bool boundary = s_ef[2 * e + 1] == 0xFFFFFFFFu;
+            er[i]               = e | (r << 16);
+            sq[i]               = q | ((boundary && q != 0xFFFFu) ? 0x10000u : 0u);
+        }
+    }
+    uint32_t visited = 0, pos = 0;
+    while (pos < K) {
+        // start: a startable edge first, then the smallest neighbour global id
+        uint32_t cur  = 0;
+        uint64_t best = ~0ull;
+#pragma unroll
+        for (uint32_t i = 0; i < KMAX; ++i) {
+            if (i < K && !((visited >> i) & 1u)) {
+                const uint64_t key = (uint64_t((sq[i] >> 16) ^ 1u) << 32) | s_gid[er[i] >> 16];
+                if (key < best) {
+                    best = key;
+                    cur  = i;
+                }
+            }
+        }
+        while (true) {
+            uint32_t ec = 0, qc = 0xFFFFu;
+#pragma unroll
+            for (uint32_t i = 0; i < KMAX; ++i) {
+                if (i == cur) {
+                    ec = er[i] & 0xFFFFu;
+                    qc = sq[i] & 0xFFFFu;
+                }
+            }
+            visited |= 1u << cur;
+            val[k0 + pos++] = static_cast<uint16_t>(ec);
+            if (qc == 0xFFFFu) break;
+            uint32_t nxt = KMAX;
+#pragma unroll
+            for (uint32_t i = 0; i < KMAX; ++i)
+                if (i < K && !((visited >> i) & 1u) && (er[i] >> 16) == qc) nxt = i;
+            if (nxt == KMAX) break;
+            cur = nxt;
+        }
+    }
+}
+
 }  // namespace rxm

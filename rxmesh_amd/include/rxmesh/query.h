@@ -293,14 +293,14 @@ struct Query
                 for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
                 for (uint32_t i = threadIdx.x; i < 2u * d.ne; i += T) L.ef[i] = 0xFFFFFFFFu;
                 block.sync();
-                rxm::lds_transpose<2>(2u * d.ne, d.nv, key, m_off, L.cur, m_val, L.wave, rows);
+                rxm::lds_transpose<2>(2u * d.ne, d.nv, key, m_off, L.cur, m_val, L.wave, 0u);
                 for (uint32_t i = threadIdx.x; i < 3u * d.nf; i += T) {
                     const uint32_t e = L.fe[i] >> 1, f = i / 3u;
                     if (atomicCAS(&L.ef[2u * e], 0xFFFFFFFFu, f) != 0xFFFFFFFFu) L.ef[2u * e + 1u] = f;
                 }
                 block.sync();
                 for (uint32_t v = threadIdx.x; v < rows; v += T)
-                    rxm::orient_row(v, m_off[v], m_off[v + 1], m_val, L.ev, L.fe, L.ef, L.gid);
+                    rxm::orient_row_fast<12>(v, m_off[v], m_off[v + 1], m_val, L.ev, L.fe, L.ef, L.gid);
                 if constexpr (op == Op::VV) {
                     block.sync();
                     for (uint32_t v = threadIdx.x; v < rows; v += T)

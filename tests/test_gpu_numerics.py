@@ -73,9 +73,11 @@ def test_cot_assembly_vs_fp64(name):
 
 
 @pytest.mark.parametrize("name", ["ico4", "torus"])
-def test_spmv_and_mcf_vs_fp64(name):
+@pytest.mark.parametrize("lanes", [1, 2, 4, 8])
+def test_spmv_and_mcf_vs_fp64(name, lanes):
     import torch
     V, F, m, o = _mesh(name)
+    m.set_launch_threads(5, lanes)
     V32 = V.astype(np.float32)
     dt = 1e-5
     x = _lin(m, V32)
@@ -96,6 +98,7 @@ def test_spmv_and_mcf_vs_fp64(name):
     # determinism
     out2 = m.mcf_explicit(A, x, 50)
     assert torch.equal(out, out2)
+    m.set_launch_threads(5, 8)
 
 
 @pytest.mark.parametrize("name", ["disk", "c3"])
