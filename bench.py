@@ -227,7 +227,7 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
         dt = 1e-5
         A = m.cot_assemble(x, dt)
         nnz = int(A[1].numel())
-        t_a = event_time_ms(lambda: m.cot_assemble(x, dt), max(3, reps // 5))
+        t_a = event_time_ms(lambda: m.cot_assemble(x, dt, out=A), max(10, reps))
         y = torch.empty_like(x)
         tmp = torch.empty_like(x)
         sweep = {}

@@ -32,7 +32,7 @@ typedef int rxm_status;
 enum {
     RXM_OK = 0,
     RXM_ERR_INVALID_ARG = 1,
-    RXM_ERR_NON_MANIFOLD = 2,
+    RXM_ERR_NON_MANIFOLD = 2, /* also: oriented paths on an inconsistently oriented mesh */
     RXM_ERR_PATCH_TOO_BIG = 3,
     RXM_ERR_IO = 4,
     RXM_ERR_DEVICE = 5,
@@ -69,6 +69,7 @@ typedef struct rxm_mesh_info_t {
     uint32_t num_ribbon_faces;
     uint32_t on_device;
     uint64_t device_bytes;
+    uint32_t is_consistently_oriented; /* each edge's faces traverse it in opposite directions */
 } rxm_mesh_info_t;
 
 /* Arrays exportable with rxm_mesh_get_array (host copies, uint32 unless noted). */
@@ -156,7 +157,7 @@ rxm_status rxm_query_nnz(rxm_mesh* mesh, int op, int64_t* nnz);
  * driven by the reference test lambda output(id, i) = iter[i]
  * (tests/RXMesh_test/query_kernel.cuh:13-45). Row order within a row:
  * ascending source-local order (unoriented), or the canonical CCW one-ring
- * (oriented, VV/VE only, manifold meshes). */
+ * (oriented, VV/VE only, edge-manifold and consistently oriented meshes). */
 rxm_status rxm_query_csr(rxm_mesh* mesh, int op, int oriented, int32_t* row_ptr,
                          int32_t* col, void* stream);
 
@@ -179,9 +180,11 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *             1 = one float4 per vertex;
  *   kernel 4: smoothing, 1 = two Jacobi steps per launch over the two-step
  *             plan (RXM_ARR_VV2_*), 0 = one step per launch;
- *   kernel 5: SpMV-3 / explicit MCF lanes per CSR row, 1 / 2 / 4 / 8.
- * Settings of kernels 0-4 give bit-identical results; kernel 5 changes the
- * summation order of a row (fp32 results within rounding). */
+ *   kernel 5: SpMV-3 / explicit MCF lanes per CSR row, 1 / 2 / 4 / 8;
+ *   kernel 6: oriented VV / VE workgroup size 256 / 512;
+ *   kernel 7: cotangent assembly workgroup size 256 / 512.
+ * Settings of kernels 0-4 and 6 give bit-identical results; kernel 5 changes
+ * the summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 
 /* `iters` Jacobi steps of uniform Laplacian smoothing
@@ -207,7 +210,8 @@ rxm_status rxm_cot_nnz(rxm_mesh* mesh, int64_t* nnz);
  * Replaces SparseMatrix<T>(rx, Op::VV) (include/rxmesh/matrix/sparse_matrix.h:
  * 133-242) filled by mcf_A_setup (apps/MCF/mcf_kernels.cuh:354-426); weights
  * restate edge_cotan_weight / partial_voronoi_area
- * (include/rxmesh/geometry_util.cuh:107-206). Needs an edge-manifold mesh. */
+ * (include/rxmesh/geometry_util.cuh:107-206). Needs an edge-manifold, consistently
+ * oriented mesh. */
 rxm_status rxm_cot_csr_assemble(rxm_mesh* mesh, const float* pos, float dt, int32_t* row_ptr,
                                 int32_t* col, float* val, float* mass, void* stream);
 /* y = A x for 3 SoA columns over the V vertex rows. Replaces

@@ -884,17 +884,22 @@ static double edge_cotan_weight(const double* p, const double* r, const double* 
 }
 
 // Cotangent system of explicit mean-curvature flow, restating the lambda of
-// apps/MCF/mcf_kernels.cuh:354-426 (mcf_A_setup) over the canonical oriented
-// one-ring. Output in GLOBAL vertex ids as COO (row, col, val) with the
-// diagonal first in each row, followed by the ring; mass[v] = 1 / v_weight.
-// Returns the number of entries.
+// apps/MCF/mcf_kernels.cuh:354-426 (mcf_A_setup): weights and areas are
+// evaluated over the canonical oriented one-ring (cyclically, as the lambda
+// does), and each weight lands in the column of its neighbour, as
+// A_mat(p_id, iter[v]) does. The columns of a row are the diagonal first,
+// then the neighbours in VV query order -- the order SparseMatrix fills them
+// (sparse_matrix_kernels.cuh:75-140 walks the VV iterator after the
+// diagonal). Output in GLOBAL vertex ids as COO (row, col, val);
+// mass[v] = 1 / v_weight. Returns the number of entries.
 int64_t ora_cot_assemble(const ora_mesh* h, const double* pos, double dt,
                          int32_t* rows, int32_t* cols, double* vals, double* mass)
 {
     const Mesh& m = *reinterpret_cast<const Mesh*>(h);
     int64_t n = 0;
     for (uint32_t p = 0; p < m.P; ++p) {
-        PatchQuery q = patch_query(m, p, OP_VV, true);
+        PatchQuery q  = patch_query(m, p, OP_VV, true);
+        PatchQuery qu = patch_query(m, p, OP_VV, false);
         const Patch& P = m.patches[p];
         for (uint32_t v = 0; v < P.nvo; ++v) {
             uint32_t gp = P.ltog_v[v];
@@ -902,7 +907,7 @@ int64_t ora_cot_assemble(const ora_mesh* h, const double* pos, double dt,
             for (uint32_t k = q.off[v]; k < q.off[v + 1]; ++k) ring.push_back(P.ltog_v[q.val[k]]);
             const size_t K = ring.size();
             double sum_w = 0, vw = 0;
-            int64_t diag = n++;
+            std::map<uint32_t, double> wcol;
             uint32_t gq = ring[K - 1];
             for (size_t i = 0; i < K; ++i) {
                 uint32_t gr = ring[i];
@@ -911,14 +916,18 @@ int64_t ora_cot_assemble(const ora_mesh* h, const double* pos, double dt,
                 w = (w >= 0.0) ? w : 0.0;
                 w *= dt;
                 sum_w += w;
-                rows[n] = gp; cols[n] = gr; vals[n] = -w; ++n;
+                wcol[gr] = -w;
                 double a = partial_voronoi(pos + 3 * gp, pos + 3 * gq, pos + 3 * gr);
                 vw += (a > 0) ? a : 0;
                 gq = gr;
             }
             vw = 0.5 / vw;
-            rows[diag] = gp; cols[diag] = gp; vals[diag] = 1.0 / vw + sum_w;
+            rows[n] = gp; cols[n] = gp; vals[n] = 1.0 / vw + sum_w; ++n;
             mass[gp] = 1.0 / vw;
+            for (uint32_t k = qu.off[v]; k < qu.off[v + 1]; ++k) {
+                const uint32_t gr = P.ltog_v[qu.val[k]];
+                rows[n] = gp; cols[n] = gr; vals[n] = wcol.at(gr); ++n;
+            }
         }
     }
     return n;

@@ -37,7 +37,8 @@ class MeshInfo(ctypes.Structure):
         "max_vertices_per_patch", "max_edges_per_patch", "max_faces_per_patch",
         "max_owned_vertices", "max_owned_edges", "max_owned_faces", "is_closed",
         "is_edge_manifold", "max_valence", "max_edge_incident_faces",
-        "max_face_adjacent_faces", "num_ribbon_faces", "on_device"]] + [("device_bytes", ctypes.c_uint64)]
+        "max_face_adjacent_faces", "num_ribbon_faces", "on_device"]] + [("device_bytes", ctypes.c_uint64),
+                                                                          ("is_consistently_oriented", ctypes.c_uint32)]
 
 
 _lib = None
@@ -292,14 +293,21 @@ class Mesh:
         _check(lib().rxm_cot_nnz(self._h, ctypes.byref(n)), "rxm_cot_nnz")
         return n.value
 
-    def cot_assemble(self, pos, dt, stream=None):
-        """(row_ptr, col, val, mass) of A = M + dt L on device."""
+    def cot_assemble(self, pos, dt, out=None, stream=None):
+        """(row_ptr, col, val, mass) of A = M + dt L on device; `out` reuses
+        the four tensors of an earlier call."""
         import torch
-        nnz = self.cot_nnz()
-        rp = torch.empty(self.nV + 1, dtype=torch.int32, device="cuda")
-        col = torch.empty(nnz, dtype=torch.int32, device="cuda")
-        val = torch.empty(nnz, dtype=torch.float32, device="cuda")
-        mass = torch.empty(self.nV, dtype=torch.float32, device="cuda")
+        if out is None:
+            nnz = self.cot_nnz()
+            rp = torch.empty(self.nV + 1, dtype=torch.int32, device="cuda")
+            col = torch.empty(nnz, dtype=torch.int32, device="cuda")
+            val = torch.empty(nnz, dtype=torch.float32, device="cuda")
+            mass = torch.empty(self.nV, dtype=torch.float32, device="cuda")
+        else:
+            rp, col, val, mass = out
+            nnz = self.cot_nnz()
+            if (rp.numel(), col.numel(), val.numel(), mass.numel()) != (self.nV + 1, nnz, nnz, self.nV):
+                raise ValueError("cot_assemble: out tensors do not match the system's sizes")
         _check(lib().rxm_cot_csr_assemble(self._h, _ptr(pos), float(dt), _ptr(rp), _ptr(col), _ptr(val),
                                           _ptr(mass), _stream(stream)), "rxm_cot_csr_assemble")
         return rp, col, val, mass

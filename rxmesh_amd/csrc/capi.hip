@@ -30,6 +30,7 @@ struct rxm_mesh
     uint64_t              device_bytes = 0;
     std::map<int, uint32_t*>            d_base;  // per-op patch entry bases (P+1)
     std::map<int, std::vector<uint32_t>> h_base;
+    std::map<int, uint32_t>              h_maxrow;  // largest per-patch entry count of an op
     float*                              d_partial = nullptr;  // P per-patch partial sums
     // patch-local VV plan (u16 row offsets + u16 neighbour local ids)
     const uint32_t* d_vv_rbase = nullptr;
@@ -57,7 +58,16 @@ struct rxm_mesh
     uint16_t*       d_fp_roff  = nullptr;
     uint16_t*       d_fp_vf    = nullptr;
     uint32_t        fp_max_nent = 0;
+    // cotangent assembly: closed-fan flags per linear vertex (bitmask) and the
+    // patches owning a vertex whose ring is not one closed fan
+    const uint32_t* d_cot_closed = nullptr;
+    const uint32_t* d_cot_plist  = nullptr;
+    uint32_t        cot_nplist   = 0;
+    const uint32_t* d_cot_ent    = nullptr;  // patch-local VV edge pattern (build_vv_edge_pattern)
+    const uint32_t* d_cot_rs     = nullptr;
     uint32_t        smooth_threads = 128;
+    uint32_t        ring_threads   = 256;      // oriented VV, VE
+    uint32_t        cot_threads    = 512;      // cotangent assembly
     uint32_t        smooth_aos4    = 0;        // 1: smoothing stages positions as float4
     uint32_t        grad_threads   = 256;
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
@@ -193,6 +203,17 @@ const std::vector<uint32_t>& op_base(rxm_mesh* m, int op)
         b[p + 1] = b[p] + static_cast<uint32_t>(c);
     }
     return m->h_base[op] = std::move(b);
+}
+
+// Largest number of entries one patch contributes to an op.
+uint32_t op_max_patch(rxm_mesh* m, int op)
+{
+    auto it = m->h_maxrow.find(op);
+    if (it != m->h_maxrow.end()) return it->second;
+    const auto& b  = op_base(m, op);
+    uint32_t    mx = 0;
+    for (uint32_t p = 0; p < m->h.P; ++p) mx = std::max(mx, b[p + 1] - b[p]);
+    return m->h_maxrow[op] = mx;
 }
 
 int device_base(rxm_mesh* m, int op, const uint32_t** out)
@@ -356,6 +377,43 @@ int ensure_vv2_plan(rxm_mesh* m)
     if (!rc) rc = upload(m, v.nbr, m->d_vv2_nbr);  // last: marks the plan complete
     if (rc) m->d_vv2_nbr = nullptr;
     return rc;
+}
+
+// Static part of the cotangent system, built once per mesh (the role of the
+// reference's SparseMatrix constructor): the patch-local VV edge pattern, the
+// closed-fan bitmask (per linear vertex id) and the list of patches with
+// owned vertices outside it, for the ring-row pass.
+int ensure_cot_plan(rxm_mesh* m)
+{
+    if (m->d_cot_closed) return RXM_OK;
+    const HostMesh&       h = m->h;
+    std::vector<uint32_t> ent, rs;
+    build_vv_edge_pattern(h, op_base(m, RXM_OP_VV), ent, rs);
+    int rc0 = upload(m, ent, m->d_cot_ent);
+    if (!rc0) rc0 = upload(m, rs, m->d_cot_rs);
+    if (rc0) return rc0;
+    std::vector<uint8_t> closed;
+    closed_fan_vertices(h, closed);
+    std::vector<uint32_t> bits((h.nV + 31) / 32, 0), plist;
+    for (uint32_t p = 0; p < h.P; ++p) {
+        bool ring = false;
+        for (uint32_t l = 0; l < h.nvo[p]; ++l) {
+            const uint32_t lin = h.vp[p] + l;
+            if (closed[h.ltog_v[h.v_off[p] + l]]) bits[lin >> 5] |= 1u << (lin & 31u);
+            else ring = true;
+        }
+        if (ring) plist.push_back(p);
+    }
+    const uint32_t* dl = nullptr;
+    int             rc = upload(m, plist, dl);
+    if (!rc) rc = upload(m, bits, m->d_cot_closed);  // last: marks the plan complete
+    if (rc) {
+        m->d_cot_closed = nullptr;
+        return rc;
+    }
+    m->d_cot_plist = dl;
+    m->cot_nplist  = static_cast<uint32_t>(plist.size());
+    return RXM_OK;
 }
 
 // two Jacobi steps in one launch
@@ -555,6 +613,7 @@ rxm_status rxm_mesh_info(const rxm_mesh* m, rxm_mesh_info_t* info)
     info->max_owned_faces = h.max_nfo;
     info->is_closed = h.closed;
     info->is_edge_manifold = h.manifold;
+    info->is_consistently_oriented = h.oriented;
     info->max_valence = h.max_valence;
     info->max_edge_incident_faces = h.max_edge_faces;
     info->max_face_adjacent_faces = h.max_face_adj;
@@ -688,18 +747,22 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
     if (!m || !row_ptr || !col || !valid_op(op)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
     if (oriented && op != RXM_OP_VV && op != RXM_OP_VE) return RXM_ERR_UNSUPPORTED;
-    if (oriented && !m->h.manifold) return RXM_ERR_NON_MANIFOLD;
+    if (oriented && !(m->h.manifold && m->h.oriented)) return RXM_ERR_NON_MANIFOLD;
     const HostMesh& h = m->h;
     const uint32_t* base = nullptr;
     int rc = device_base(m, op, &base);
     if (rc) return rc;
     Bump b(nullptr);
     constexpr uint32_t T = kQueryThreads;
-    if (oriented) {  // oriented VV / VE: general transpose + canonical CCW walk
-        plan_vx(b, h.max_nv, h.max_ne, h.max_nf, true);
-        if (op == RXM_OP_VV)
-            return launch_patches(k_query_vx<true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col, oriented);
-        return launch_patches(k_query_vx<false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col, oriented);
+    if (oriented) {  // oriented VV / VE: owned transpose columns + canonical CCW walk
+        plan_vxo(b, h.max_nv, h.max_ne, h.max_nf, h.max_nv, op_max_patch(m, op));
+        auto go = [&](auto kernel, uint32_t threads) {
+            return launch_patches_xcd(kernel, threads, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+        };
+        if (m->ring_threads == 512)
+            return (op == RXM_OP_VV) ? go(k_query_vx_oriented<512, true>, 512)
+                                     : go(k_query_vx_oriented<512, false>, 512);
+        return (op == RXM_OP_VV) ? go(k_query_vx_oriented<256, true>, 256) : go(k_query_vx_oriented<256, false>, 256);
     }
     switch (op) {
         case RXM_OP_VV:
@@ -805,6 +868,14 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
         m->spmv_lanes = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
+    if (kernel == 6 && (threads == 256 || threads == 512)) {
+        m->ring_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 7 && (threads == 256 || threads == 512)) {
+        m->cot_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
     return RXM_ERR_INVALID_ARG;
 }
 
@@ -866,14 +937,26 @@ rxm_status rxm_cot_csr_assemble(rxm_mesh* m, const float* pos, float dt, int32_t
 {
     if (!m || !pos || !row_ptr || !col || !val || !mass) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    if (!m->h.manifold) return RXM_ERR_NON_MANIFOLD;
+    if (!(m->h.manifold && m->h.oriented)) return RXM_ERR_NON_MANIFOLD;
     const uint32_t* base = nullptr;
     int rc = device_base(m, kOpCot, &base);
     if (rc) return rc;
-    Bump b(nullptr);
-    plan_cot(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
-    return launch_patches(k_cot_assemble, kQueryThreads, b.bytes(), stream, m, m->dm, base, pos, dt, row_ptr,
-                          col, val, mass);
+    const uint32_t* vv_base = nullptr;
+    if ((rc = device_base(m, RXM_OP_VV, &vv_base))) return rc;
+    if ((rc = ensure_cot_plan(m))) return rc;
+    const uint32_t maxrow = op_max_patch(m, kOpCot);
+    Bump           b(nullptr);
+    plan_cot(b, m->h.max_nv, m->h.max_ne, m->h.max_nvo, op_max_patch(m, RXM_OP_VV));
+    auto go = [&](auto kernel, uint32_t threads) {
+        return launch_patches_xcd(kernel, threads, b.bytes(), stream, m, m->dm, vv_base, m->d_cot_rs, m->d_cot_ent,
+                                  pos, dt, row_ptr, col, val, mass);
+    };
+    rc = (m->cot_threads == 512) ? go(k_cot_assemble<512>, 512) : go(k_cot_assemble<256>, 256);
+    if (rc || m->cot_nplist == 0) return rc;
+    Bump br(nullptr);
+    plan_cot_ring(br, m->h.max_nv, m->h.max_ne, m->h.max_nf, m->h.max_nv, maxrow);
+    return launch_grid(k_cot_ring_rows<kQueryThreads>, m->cot_nplist, kQueryThreads, br.bytes(), stream, m->dm,
+                       base, m->d_cot_plist, m->d_cot_closed, pos, dt, val, mass);
 }
 
 }  // extern "C"

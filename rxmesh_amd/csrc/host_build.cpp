@@ -145,6 +145,11 @@ int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_h
         std::vector<uint32_t> cur(m.ef_off.begin(), m.ef_off.end() - 1);
         for (uint64_t h = 0; h < nH; ++h) m.ef_val[cur[m.fe_g[h]]++] = static_cast<uint32_t>(h / 3);
     }
+    // consistent orientation: the half-edges sharing an edge with its first
+    // half-edge run the other way
+    m.oriented = true;
+    for (uint64_t h = 0; h < nH; ++h)
+        if (leader[h] != h && he_a(h) == m.ev[2ull * m.fe_g[h]]) m.oriented = false;
     // statistics (rxmesh.cpp:589-660)
     m.closed = true;
     m.manifold = true;
@@ -866,6 +871,68 @@ int build_patches(HostMesh& m)
 }
 
 // ---------------------------------------------------------------------------
+// Closed-fan vertices (host_build.h).
+void closed_fan_vertices(const HostMesh& m, std::vector<uint8_t>& closed)
+{
+    std::vector<uint32_t> valence(m.nV, 0);
+    for (uint32_t e = 0; e < m.nE; ++e) {
+        valence[m.ev[2ull * e]]++;
+        valence[m.ev[2ull * e + 1]]++;
+    }
+    closed.assign(m.nV, 0);
+    parallel_for(m.nV, [&](size_t v) {
+        const uint32_t b = m.vf_off[v], k = m.vf_off[v + 1] - b;
+        if (k == 0 || k != valence[v]) return;
+        std::vector<uint32_t> key(k), nxt(k);
+        for (uint32_t i = 0; i < k; ++i) {
+            const uint32_t f = m.vf_val[b + i];
+            uint32_t       j = 0;
+            while (j < 3 && m.fv[3ull * f + j] != v) ++j;
+            if (j == 3) return;
+            key[i] = m.fv[3ull * f + (j + 1) % 3];
+            nxt[i] = m.fv[3ull * f + (j + 2) % 3];
+        }
+        // follow the successor map from key[0]; a single cycle visits k keys
+        uint32_t cur = 0, steps = 0;
+        do {
+            uint32_t i = 0;
+            while (i < k && key[i] != nxt[cur]) ++i;
+            if (i == k) return;
+            cur = i;
+            ++steps;
+        } while (cur != 0 && steps <= k);
+        if (cur == 0 && steps == k) closed[v] = 1;
+    }, 4096);
+}
+
+// ---------------------------------------------------------------------------
+// Patch-local VV edge pattern (host_build.h).
+void build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint32_t>& ent,
+                           std::vector<uint32_t>& rs)
+{
+    ent.assign(vv_base[m.P], 0);
+    rs.assign(m.nV + 1, 0);
+    rs[m.nV] = vv_base[m.P];
+    parallel_for(m.P, [&](size_t p) {
+        const uint32_t        nvo = m.nvo[p], ne = m.ne[p], vp = m.vp[p];
+        std::vector<uint32_t> cnt(nvo + 1, 0);
+        const uint32_t*       ev = m.ev_local.data() + m.e_off[p];
+        for (uint32_t e = 0; e < ne; ++e) {
+            const uint32_t a = ev[e] & 0xFFFFu, b = ev[e] >> 16;
+            if (a < nvo) cnt[a + 1]++;
+            if (b < nvo) cnt[b + 1]++;
+        }
+        for (uint32_t l = 0; l < nvo; ++l) cnt[l + 1] += cnt[l];
+        for (uint32_t l = 0; l < nvo; ++l) rs[vp + l] = vv_base[p] + cnt[l];
+        uint32_t* out = ent.data() + vv_base[p];
+        for (uint32_t e = 0; e < ne; ++e) {
+            const uint32_t a = ev[e] & 0xFFFFu, b = ev[e] >> 16;
+            if (a < nvo) out[cnt[a]++] = e | (a << 16);
+            if (b < nvo) out[cnt[b]++] = e | (b << 16);
+        }
+    }, 16);
+}
+
 // ---------------------------------------------------------------------------
 // Two-step smoothing plan (host_build.h).
 int build_vv2_plan(const HostMesh& m, Vv2Plan& out)
@@ -992,7 +1059,7 @@ int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& o, std::v
     o = HostMesh();
     o.nV = vb[K]; o.nE = eb[K]; o.nF = fb[K]; o.P = pb[K];
     o.patch_size = parts[0]->patch_size;
-    o.closed = true; o.manifold = true;
+    o.closed = true; o.manifold = true; o.oriented = true;
     o.fv.resize(3ull * o.nF); o.fe_g.resize(3ull * o.nF); o.ev.resize(2ull * o.nE);
     o.vf_off.assign(o.nV + 1, 0); o.vf_val.resize(3ull * o.nF);
     o.ef_off.assign(o.nE + 1, 0); o.ef_val.resize(3ull * o.nF);
@@ -1010,6 +1077,7 @@ int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& o, std::v
         const HostMesh& m = *parts[k];
         o.closed = o.closed && m.closed;
         o.manifold = o.manifold && m.manifold;
+        o.oriented = o.oriented && m.oriented;
         o.max_valence = std::max(o.max_valence, m.max_valence);
         o.max_edge_faces = std::max(o.max_edge_faces, m.max_edge_faces);
         o.max_face_adj = std::max(o.max_face_adj, m.max_face_adj);

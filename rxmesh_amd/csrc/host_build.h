@@ -30,6 +30,7 @@ struct HostMesh
     std::vector<uint32_t> vf_off, vf_val;  // vertex -> incident faces, face order
 
     bool     closed = true, manifold = true;
+    bool     oriented = true;  // every edge's faces traverse it in opposite directions
     uint32_t max_valence = 0, max_edge_faces = 0, max_face_adj = 0;
 
     // patch assignment in the reference's layout (patcher.h:162-182)
@@ -125,6 +126,22 @@ struct Vv2Plan
     uint32_t              max_nx = 0, max_nent = 0;
 };
 int build_vv2_plan(const HostMesh& m, Vv2Plan& out);
+
+// Vertices whose one-ring is one closed fan: the successor map of its faces
+// (half-edge v -> a of face (v, a, b) is followed by b) is a single cycle
+// through all its neighbours. The canonical ring of such a vertex has a real
+// face between every two consecutive neighbours, so its cotangent row can be
+// built from per-face quantities. Needs a consistently oriented mesh.
+// closed[g] = 1 for such vertices (global ids).
+void closed_fan_vertices(const HostMesh& m, std::vector<uint8_t>& closed);
+
+// Patch-local pattern of the VV query: for every patch p and owned vertex l,
+// the patch's local edges incident to l in ascending local edge id (the VV
+// query's row order), as ent[vv_base[p] + k] = edge | (l << 16); rs[lin] is
+// the first entry of linear vertex lin, rs[nV] the total. vv_base is the VV
+// query's per-patch entry base (P + 1).
+void build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint32_t>& ent,
+                           std::vector<uint32_t>& rs);
 
 unsigned num_threads();
 

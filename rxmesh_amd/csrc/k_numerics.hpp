@@ -54,70 +54,361 @@ __device__ __forceinline__ float edge_cotan_weight(f3 p, f3 r, f3 q, f3 s)
 }
 
 // ---- cotangent CSR assembly ------------------------------------------------
+// One wedge (p, a, b) of a vertex's ring: the clamped cotangents of the
+// angles at a and at b (reference "cot" = dot / tri_area) and the wedge's
+// partial Voronoi area at p (geometry_util.cuh:120-172, clamped at 0). Each
+// wedge enters two edge weights and one area, so a ring of K edges costs K
+// wedges (one reciprocal square root each) instead of 3K triangle areas.
+// |cross|^2 <= FLT_MIN counts as degenerate (area 0): cotangents 0 and no
+// Voronoi share, as the reference's area <= FLT_MIN tests give.
+struct Wedge
+{
+    float ca, cb, vor;
+};
+__device__ __forceinline__ Wedge wedge(f3 p, f3 a, f3 b)
+{
+    const f3    pa = sub(a, p), pb = sub(b, p), ab = sub(b, a);
+    const f3    c  = cross(pa, pb);
+    const float cc = dot(c, c);
+    Wedge       w{0.f, 0.f, 0.f};
+    if (!(cc > FLT_MIN)) return w;
+    const float rs   = __builtin_amdgcn_rsqf(cc);
+    const float area = 0.5f * cc * rs;  // tri_area
+    const float inv  = 2.0f * rs;       // 1 / tri_area
+    const float dp = dot(pa, pb), da = -dot(pa, ab), db = dot(pb, ab);
+    w.ca = clamp_cot(da * inv);
+    w.cb = clamp_cot(db * inv);
+    if (dp < 0.f) w.vor = 0.25f * area;
+    else if (da < 0.f || db < 0.f) w.vor = 0.125f * area;
+    else w.vor = fmaxf(0.125f * (dot(pb, pb) * w.ca + dot(pa, pa) * w.cb), 0.f);
+    return w;
+}
+
+// Per-face terms of the reference weights (geometry_util.cuh:107-206) for
+// face (a0, a1, a2): cot[j] = the clamped cotangent of the angle at a_j
+// ("cot" = dot / tri_area), which enters the weight of the opposite edge
+// (face slot j + 1, a_{j+1} -> a_{j+2}); vor[j] = the partial Voronoi area of
+// a_j in the face (partial_voronoi_area(a_j, a_{j+1}, a_{j+2}) clamped at 0).
+// |cross|^2 <= FLT_MIN counts as degenerate: all terms 0, as the
+// reference's area <= FLT_MIN tests give.
+struct FaceCot
+{
+    float cot[3], vor[3];
+};
+__device__ __forceinline__ FaceCot face_cot(f3 P0, f3 P1, f3 P2)
+{
+    const f3    E0 = sub(P1, P0), E1 = sub(P2, P1), E2 = sub(P0, P2);
+    const f3    c  = cross(E0, E2);
+    const float cc = dot(c, c);
+    FaceCot     r{{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
+    if (!(cc > FLT_MIN)) return r;
+    const float rs = __builtin_amdgcn_rsqf(cc);
+    const float area = 0.5f * cc * rs, inv = 2.0f * rs;
+    const float D[3] = {-dot(E0, E2), -dot(E1, E0), -dot(E2, E1)};  // corner dots
+    const float L[3] = {dot(E0, E0), dot(E1, E1), dot(E2, E2)};     // |a_{j+1} - a_j|^2
+#pragma unroll
+    for (int j = 0; j < 3; ++j) r.cot[j] = clamp_cot(D[j] * inv);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const int j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+        if (D[j] < 0.f) r.vor[j] = 0.25f * area;
+        else if (D[j1] < 0.f || D[j2] < 0.f) r.vor[j] = 0.125f * area;
+        else r.vor[j] = fmaxf(0.125f * (L[j2] * r.cot[j1] + L[j] * r.cot[j2]), 0.f);
+    }
+    return r;
+}
+
+// LDS carve of the assembly: float4 positions and linear ids of the patch
+// vertices, the EV slab, per-edge half-edge terms he[e] = {cot(2e), vor(2e),
+// cot(2e + 1), vor(2e + 1)} (half-edge 2e + dir starts at the edge's
+// (dir ? hi16 : lo16) endpoint; cot = the clamped cotangent of the angle
+// opposite it in its face, vor = the partial Voronoi area of its start in
+// that face; zeros where no face of the patch holds it), and the patch's
+// pattern: row starts (nvo + 1) and entries (nent).
 struct PlanCOT
 {
-    PlanVX vx;
-    float *x, *y, *z;
+    float4*   pos;
+    float4*   he;
+    uint32_t *lin, *ev, *rs, *ent;
 };
-__host__ __device__ inline PlanCOT plan_cot(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+__host__ __device__ inline PlanCOT plan_cot(Bump& b, uint32_t nv, uint32_t ne, uint32_t nvo, uint32_t nent)
 {
     PlanCOT p{};
-    p.vx = plan_vx(b, nv, ne, nf, true);
-    p.x  = b.alloc<float>(nv);
-    p.y  = b.alloc<float>(nv);
-    p.z  = b.alloc<float>(nv);
+    p.pos = b.alloc<float4>(nv);
+    p.he  = b.alloc<float4>(ne);
+    p.lin = b.alloc<uint32_t>(nv);
+    p.ev  = b.alloc<uint32_t>(r4(ne));
+    p.rs  = b.alloc<uint32_t>(nvo + 1);
+    p.ent = b.alloc<uint32_t>(nent);
     return p;
 }
 
-// Rows of A = M + dt L over linear ids, restating the lambda of
-// apps/MCF/mcf_kernels.cuh:354-426 on the canonical CCW ring: the diagonal
-// first, then the ring neighbours; mass[v] = 1 / v_weight. `base` holds each
-// patch's first entry (VV base + vertex prefix).
-__global__ __launch_bounds__(kQueryThreads) void k_cot_assemble(DevMesh m, const uint32_t* __restrict__ base,
-                                                                const float* __restrict__ pos, float dt,
-                                                                int32_t* __restrict__ row_ptr,
-                                                                int32_t* __restrict__ col, float* __restrict__ val,
-                                                                float* __restrict__ mass)
+// Copy n u32 global -> LDS (any alignment), R loads of a thread in flight
+// before its stores; dst[i] = src[i] - sub.
+template <uint32_t T, uint32_t R>
+__device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst, const uint32_t* __restrict__ src,
+                                            uint32_t n, uint32_t sub = 0)
+{
+    for (uint32_t b0 = 0; b0 < n; b0 += R * T) {
+        uint32_t x[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t i = b0 + r * T + threadIdx.x;
+            x[r]             = (i < n) ? src[i] : 0u;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t i = b0 + r * T + threadIdx.x;
+            if (i < n) dst[i] = x[r] - sub;
+        }
+    }
+}
+
+// Stage, per patch vertex, its linear id (owned by prefix, ribbon from the
+// owner table) and position; all global loads of a thread are issued before
+// its LDS stores.
+template <uint32_t T>
+__device__ __forceinline__ void stage_vertices(const DevMesh& m, const PatchDesc& d, const float* __restrict__ pos,
+                                               float4* s_pos, uint32_t* s_lin)
+{
+    constexpr uint32_t R  = 2;
+    const uint32_t     N  = m.nV;
+    const uint32_t*    vl = m.v_lin + d.v_off;
+    uint32_t           g[R];
+    float              x[R], y[R], z[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t l = threadIdx.x + r * T;
+        g[r]             = (l < d.nvo) ? d.vp + l : ((l < d.nv) ? vl[l] : 0u);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        if (threadIdx.x + r * T < d.nv) {
+            x[r] = pos[g[r]];
+            y[r] = pos[N + g[r]];
+            z[r] = pos[2 * N + g[r]];
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t l = threadIdx.x + r * T;
+        if (l < d.nv) {
+            s_pos[l] = make_float4(x[r], y[r], z[r], 0.f);
+            s_lin[l] = g[r];
+        }
+    }
+    for (uint32_t l = threadIdx.x + R * T; l < d.nv; l += T) {
+        const uint32_t gg = lin_of(l, d.nvo, d.vp, vl);
+        s_pos[l]          = make_float4(pos[gg], pos[N + gg], pos[2 * N + gg], 0.f);
+        s_lin[l]          = gg;
+    }
+}
+
+// Rows of A = M + dt L over linear ids (apps/MCF/mcf_kernels.cuh:354-426,
+// mcf_A_setup): the diagonal, then the neighbours in VV query order
+// (ascending local id of the connecting edge), the column order SparseMatrix
+// fills (sparse_matrix_kernels.cuh:75-140); mass[v] = 1 / v_weight.
+// The pattern is static, like the reference's SparseMatrix: `ent` holds, in
+// VV query order, each owned row's entries of the patch as (local edge) |
+// (local row << 16), `rs` each linear vertex's first entry (patch-relative
+// offsets come from vv_base, the VV query's per-patch base; the system's
+// base is vv_base + the vertex prefix, one diagonal per owned row).
+// Every face of the patch is evaluated once and stores, for each of its
+// half-edges, the cotangent opposite it and the Voronoi term of its start
+// (plain stores: a half-edge has one face on a consistently oriented
+// manifold). An edge's weight is cot(2e) + cot(2e + 1) -- the reference's
+// part(q) + part(s), in an order floating-point addition does not see --
+// and a row's area sums the Voronoi terms of its outgoing half-edges. This
+// equals the reference's walk over the oriented ring for every vertex whose
+// ring is one closed fan; the rows of other vertices (boundary, several
+// fans) take wedges across the ring's gaps, as the reference's cyclic walk
+// does, and are rewritten afterwards by k_cot_ring_rows.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_cot_assemble(DevMesh m, const uint32_t* __restrict__ vv_base,
+                                                    const uint32_t* __restrict__ rs,
+                                                    const uint32_t* __restrict__ ent, const float* __restrict__ pos,
+                                                    float dt, int32_t* __restrict__ row_ptr,
+                                                    int32_t* __restrict__ col, float* __restrict__ val,
+                                                    float* __restrict__ mass)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
-    const PatchDesc d = load_desc(m.patch, p);
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d    = load_desc(m.patch, p);
+    const uint32_t  vb   = vv_base[p], nent = vv_base[p + 1] - vb;
+    const uint32_t  bse  = vb + d.vp;  // first entry of the patch's rows
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanCOT   L = plan_cot(b, d.nv, d.ne, d.nf);
-    stage_positions(L.x, L.y, L.z, pos, m.nV, d, m.v_lin);
-    vx_prologue(m, d, L.vx, true);
-    const uint32_t bse = base[p];
-    for (uint32_t v = threadIdx.x; v < d.nvo; v += blockDim.x) {
-        const uint32_t k0 = L.vx.off[v], k1 = L.vx.off[v + 1], K = k1 - k0;
-        const uint32_t rs = bse + k0 + v;
-        auto nb = [&](uint32_t i) {
-            const uint32_t x = L.vx.ev[L.vx.val[k0 + i]];
-            return (lo16(x) == v) ? hi16(x) : lo16(x);
-        };
-        auto P3 = [&](uint32_t l) { return f3{L.x[l], L.y[l], L.z[l]}; };
-        const f3 pp = P3(v);
-        float    sum_w = 0.f, vw = 0.f;
-        uint32_t q = nb(K - 1);
-        for (uint32_t i = 0; i < K; ++i) {
-            const uint32_t r = nb(i), s = (i == K - 1) ? nb(0) : nb(i + 1);
-            float          w = edge_cotan_weight(pp, P3(r), P3(q), P3(s));
-            w = (w >= 0.f) ? w : 0.f;
-            w *= dt;
-            sum_w += w;
-            col[rs + 1 + i] = static_cast<int32_t>(lin_of(r, d.nvo, d.vp, m.v_lin + d.v_off));
-            val[rs + 1 + i] = -w;
-            const float a = partial_voronoi_area(pp, P3(q), P3(r));
-            vw += (a > 0.f) ? a : 0.f;
-            q = r;
+    const PlanCOT   L = plan_cot(b, d.nv, d.ne, d.nvo, nent);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_words<T, 8>(L.ent, ent + vb, nent);
+    stage_words<T, 2>(L.rs, rs + d.vp, d.nvo + 1, vb);
+    stage_vertices<T>(m, d, pos, L.pos, L.lin);
+    for (uint32_t i = threadIdx.x; i < d.ne; i += T) L.he[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    __syncthreads();
+    // per face (FE read straight from the slab): the terms of its half-edges
+    const uint16_t* fe = m.fe + 3u * d.f_off;
+    float2*         hh = reinterpret_cast<float2*>(L.he);  // hh[2e + dir]
+    for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
+        uint32_t x[3], a[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) x[j] = fe[3 * f + j];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t e = L.ev[x[j] >> 1];
+            a[j]             = (x[j] & 1u) ? hi16(e) : lo16(e);
         }
-        vw = 0.5f / vw;
-        const float ms = 1.0f / vw;
-        row_ptr[d.vp + v] = static_cast<int32_t>(rs);
-        col[rs] = static_cast<int32_t>(d.vp + v);
-        val[rs] = ms + sum_w;
-        mass[d.vp + v] = ms;
+        const float4  q0 = L.pos[a[0]], q1 = L.pos[a[1]], q2 = L.pos[a[2]];
+        const FaceCot c  = face_cot(f3{q0.x, q0.y, q0.z}, f3{q1.x, q1.y, q1.z}, f3{q2.x, q2.y, q2.z});
+        // slot j runs a_j -> a_{j+1}: opposite corner a_{j+2}, start a_j
+#pragma unroll
+        for (int j = 0; j < 3; ++j) hh[x[j]] = make_float2(c.cot[(j + 2) % 3], c.vor[j]);
     }
-    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + L.vx.off[d.nvo] + d.nvo);
+    __syncthreads();
+    // rows: diagonal and mass (entries in pattern order)
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t k0 = L.rs[v], k1 = L.rs[v + 1];
+        float          sum_w = 0.f, vw = 0.f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t e = L.ent[k] & 0xFFFFu;
+            const float4   h = L.he[e];
+            const float    w = h.x + h.z;
+            sum_w += ((w >= 0.f) ? w : 0.f) * dt;
+            vw += (lo16(L.ev[e]) == v) ? h.y : h.w;
+        }
+        const float    ms = 1.0f / (0.5f / vw);
+        const uint32_t o  = bse + k0 + v;
+        row_ptr[d.vp + v] = static_cast<int32_t>(o);
+        mass[d.vp + v]    = ms;
+        col[o]            = static_cast<int32_t>(d.vp + v);
+        val[o]            = ms + sum_w;
+    }
+    if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + nent + d.nvo);
+    // off-diagonal entries, contiguous over the patch's output range
+    for (uint32_t k = threadIdx.x; k < nent; k += T) {
+        const uint32_t s = L.ent[k], e = s & 0xFFFFu, v = s >> 16, x = L.ev[e];
+        const float4   h = L.he[e];
+        const float    w = h.x + h.z;
+        const uint32_t o = bse + k + v + 1;
+        col[o]           = static_cast<int32_t>(L.lin[(lo16(x) == v) ? hi16(x) : lo16(x)]);
+        val[o]           = -(((w >= 0.f) ? w : 0.f) * dt);
+    }
+}
+
+// LDS carve of the ring-row pass: EV, FE, half-edge successors, global ids,
+// column offsets, the owned rows (sorted) and a scratch copy of one row's
+// edges per row (ordered into the canonical ring in place).
+struct PlanCOTR
+{
+    uint32_t *ev, *gid, *off, *wave, *slots, *ring;
+    uint16_t *fe, *hn;
+};
+__host__ __device__ inline PlanCOTR plan_cot_ring(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf, uint32_t nvo,
+                                                  uint32_t nrow)
+{
+    PlanCOTR p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.gid   = b.alloc<uint32_t>(nv);
+    p.off   = b.alloc<uint32_t>(nvo + 1);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.slots = b.alloc<uint32_t>(nrow);
+    p.ring  = b.alloc<uint32_t>(nrow);
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.hn    = b.alloc<uint16_t>(2 * ne);
+    return p;
+}
+
+// Rows of the vertices whose ring is not one closed fan (flag bit clear in
+// `closed`, per linear id), for the patches listed in `plist`: the
+// reference's cyclic walk over the canonical oriented ring (wedge (p, r_i,
+// r_{i+1}) for every consecutive pair, the last one wrapping to the first),
+// each weight written to its neighbour's column of the row k_cot_assemble
+// laid out (neighbours ascending by local id of the connecting edge). Positions come from global
+// memory: these rows are few.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_cot_ring_rows(DevMesh m, const uint32_t* __restrict__ base,
+                                                     const uint32_t* __restrict__ plist,
+                                                     const uint32_t* __restrict__ closed,
+                                                     const float* __restrict__ pos, float dt,
+                                                     float* __restrict__ val, float* __restrict__ mass)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p   = plist[blockIdx.x];
+    const PatchDesc d   = load_desc(m.patch, p);
+    const uint32_t  bse = base[p], nrow = base[p + 1] - bse;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanCOTR  L = plan_cot_ring(b, d.nv, d.ne, d.nf, d.nvo, nrow);
+    const uint32_t  N = m.nV;
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
+    for (uint32_t i = threadIdx.x; i < d.ne; i += T) reinterpret_cast<uint32_t*>(L.hn)[i] = 0xFFFFFFFFu;
+    __syncthreads();
+    build_half_next(L.fe, L.ev, d.nf, L.hn);
+    fill_columns<T>(
+        2 * d.ne, d.nvo,
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return (i & 1u) ? hi16(x) : lo16(x);
+        },
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
+        },
+        L.off, L.slots, L.wave);
+    if (L.off[d.nvo] + d.nvo != nrow) return;
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t lv = d.vp + v;
+        if ((closed[lv >> 5] >> (lv & 31u)) & 1u) continue;
+        const uint32_t k0 = L.off[v], K = L.off[v + 1] - k0;
+        sort_row(L.slots + k0, K, [&](uint32_t j, uint32_t s) {
+            L.slots[k0 + j] = s;
+            L.ring[k0 + j]  = s >> 16;
+        });
+        orient_ring<12, true>(v, k0, k0 + K, L.ring, L.ring, L.ev, L.hn, L.gid);
+        auto P3 = [&](uint32_t l) {
+            const uint32_t g = lin_of(l, d.nvo, d.vp, m.v_lin + d.v_off);
+            return f3{pos[g], pos[N + g], pos[2 * N + g]};
+        };
+        // column of neighbour r: its position in the sorted row
+        auto rank = [&](uint32_t r) {
+            uint32_t c = 0;
+            for (uint32_t j = 0; j < K; ++j) c = ((L.slots[k0 + j] & 0xFFFFu) == r) ? j : c;
+            return c;
+        };
+        const uint32_t o  = bse + k0 + v;
+        const f3       pp = P3(v);
+        float          sum_w = 0.f, vw = 0.f;
+        if (K > 0) {
+            // wedge i = (p, r_i, r_{i+1}); edge p-r_i takes the angle at
+            // r_{i-1} of wedge i-1 and the angle at r_{i+1} of wedge i
+            uint32_t    rc = L.ring[k0];
+            f3          Pc = P3(rc);
+            const Wedge wl = wedge(pp, P3(L.ring[k0 + K - 1]), Pc);
+            float       ca_prev = wl.ca;
+            vw                  = wl.vor;
+            for (uint32_t i = 0; i < K; ++i) {
+                uint32_t rn = 0;
+                f3       Pn{};
+                Wedge    w = wl;
+                if (i + 1 < K) {
+                    rn = L.ring[k0 + i + 1];
+                    Pn = P3(rn);
+                    w  = wedge(pp, Pc, Pn);
+                    vw += w.vor;
+                }
+                float wt = ca_prev + w.cb;
+                wt       = ((wt >= 0.f) ? wt : 0.f) * dt;
+                sum_w += wt;
+                val[o + 1 + rank(rc)] = -wt;
+                ca_prev = w.ca;
+                rc      = rn;
+                Pc      = Pn;
+            }
+        }
+        const float ms = 1.0f / (0.5f / vw);
+        mass[lv]       = ms;
+        val[o]         = ms + sum_w;
+    }
 }
 
 // ---- SpMV over 3 SoA columns, optionally fused with the explicit MCF step --

@@ -6,6 +6,7 @@
 // source elements of the patch (linear ids prefix[p] + local); entries are the
 // target elements' linear ids, i.e. always the owner's copy.
 #pragma once
+#include "lds_sort.hpp"
 #include "orient.hpp"
 
 namespace rxm {
@@ -13,25 +14,25 @@ namespace rxm {
 constexpr uint32_t kQueryThreads = 256;
 
 // ---- LDS plans (same carve on host for sizing and on device) -------------
-struct PlanVX
+// Oriented VV / VE: EV and FE slabs, the owned columns of the vertex-edge
+// transpose (nrow = entries of the patch's owned rows), the half-edge
+// successor table and the global ids that fix each ring's start.
+struct PlanVXO
 {
-    uint32_t *ev, *off, *cur, *wave, *ef, *gid;
-    uint16_t *val, *rowof, *fe;
+    uint32_t *ev, *off, *wave, *slots, *gid;
+    uint16_t *fe, *hn;
 };
-__host__ __device__ inline PlanVX plan_vx(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf, bool oriented)
+__host__ __device__ inline PlanVXO plan_vxo(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf, uint32_t nvo,
+                                            uint32_t nrow)
 {
-    PlanVX p{};
+    PlanVXO p{};
     p.ev    = b.alloc<uint32_t>(r4(ne));
-    p.off   = b.alloc<uint32_t>(nv + 1);
-    p.cur   = b.alloc<uint32_t>(nv);
+    p.off   = b.alloc<uint32_t>(nvo + 1);
     p.wave  = b.alloc<uint32_t>(20);
-    p.val   = b.alloc<uint16_t>(2 * ne);
-    p.rowof = b.alloc<uint16_t>(2 * ne);
-    if (oriented) {
-        p.fe  = b.alloc<uint16_t>(r4(3 * nf));
-        p.ef  = b.alloc<uint32_t>(2 * ne);
-        p.gid = b.alloc<uint32_t>(nv);
-    }
+    p.slots = b.alloc<uint32_t>(nrow);
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.hn    = b.alloc<uint16_t>(2 * ne);
+    p.gid   = b.alloc<uint32_t>(nv);
     return p;
 }
 
@@ -48,66 +49,54 @@ __host__ __device__ inline PlanFX plan_fx(Bump& b, uint32_t ne, uint32_t nf)
     return p;
 }
 
-// Shared prologue of VV / VE: stage EV, transpose to VE, optionally orient.
-__device__ inline void vx_prologue(const DevMesh& m, const PatchDesc& d, const PlanVX& L, bool oriented)
+// Oriented rows of the patch's owned vertices: stage EV / FE / global ids,
+// build the half-edge successor table and the owned columns of the
+// vertex-edge transpose, then order every row into its canonical CCW ring
+// (neighbour vertices if NBR, else edges) in place in L.slots. On return
+// L.off[v] is the start of row v and L.off[nvo] the total.
+template <uint32_t T, bool NBR>
+__device__ inline void oriented_rows(const DevMesh& m, const PatchDesc& d, const PlanVXO& L)
 {
-    const uint32_t T = blockDim.x;
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    if (oriented) {
-        stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
-        for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
-        for (uint32_t i = threadIdx.x; i < 2 * d.ne; i += T) L.ef[i] = 0xFFFFFFFFu;
-    }
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
+    for (uint32_t i = threadIdx.x; i < d.ne; i += T) reinterpret_cast<uint32_t*>(L.hn)[i] = 0xFFFFFFFFu;
     __syncthreads();
-    // the canonical ring does not depend on the row order: no row sort needed
-    lds_transpose<2>(2 * d.ne, d.nv,
-                     [&](uint32_t i) {
-                         const uint32_t x = L.ev[i >> 1];
-                         return (i & 1u) ? hi16(x) : lo16(x);
-                     },
-                     L.off, L.cur, L.val, L.wave, oriented ? 0u : d.nvo);
-    if (oriented) {
-        for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T) {
-            const uint32_t e = L.fe[i] >> 1, f = i / 3;
-            if (atomicCAS(&L.ef[2 * e], 0xFFFFFFFFu, f) != 0xFFFFFFFFu) L.ef[2 * e + 1] = f;
-        }
-        __syncthreads();
-        for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
-            orient_row_fast<12>(v, L.off[v], L.off[v + 1], L.val, L.ev, L.fe, L.ef, L.gid);
-        __syncthreads();
-    }
+    build_half_next(L.fe, L.ev, d.nf, L.hn);
+    fill_columns<T>(
+        2 * d.ne, d.nvo,
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return (i & 1u) ? hi16(x) : lo16(x);
+        },
+        [&](uint32_t i) { return i >> 1; }, L.off, L.slots, L.wave);
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
+        orient_ring<12, NBR>(v, L.off[v], L.off[v + 1], L.slots, L.slots, L.ev, L.hn, L.gid);
+    __syncthreads();
 }
 
-// VV (IS_VV) or VE
-template <bool IS_VV>
-__global__ __launch_bounds__(kQueryThreads) void k_query_vx(DevMesh m, const uint32_t* __restrict__ base,
-                                                            int32_t* __restrict__ row_ptr,
-                                                            int32_t* __restrict__ col, int oriented)
+// Oriented VV (IS_VV) or VE; rows leave as one contiguous range per patch.
+template <uint32_t T, bool IS_VV>
+__global__ __launch_bounds__(T) void k_query_vx_oriented(DevMesh m, const uint32_t* __restrict__ base,
+                                                         int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
-    const PatchDesc d = load_desc(m.patch, p);
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d   = load_desc(m.patch, p);
+    const uint32_t  bse = base[p], nent = base[p + 1] - bse;
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanVX    L = plan_vx(b, d.nv, d.ne, d.nf, oriented != 0);
-    vx_prologue(m, d, L, oriented != 0);
-    const uint32_t T = blockDim.x;
-    for (uint32_t v = threadIdx.x; v < d.nvo; v += T)
-        for (uint32_t k = L.off[v]; k < L.off[v + 1]; ++k) L.rowof[k] = static_cast<uint16_t>(v);
-    __syncthreads();
-    const uint32_t bse  = base[p];
-    const uint32_t nent = L.off[d.nvo];
+    const PlanVXO   L = plan_vxo(b, d.nv, d.ne, d.nf, d.nvo, nent);
+    oriented_rows<T, IS_VV>(m, d, L);
+    // the host's row counts (global valences) match the patch's owned
+    // columns on every manifold patch; anything else leaves it unwritten
+    if (L.off[d.nvo] != nent) return;
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) row_ptr[d.vp + v] = static_cast<int32_t>(bse + L.off[v]);
     if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + nent);
     for (uint32_t k = threadIdx.x; k < nent; k += T) {
-        const uint32_t e = L.val[k];
-        uint32_t       out;
-        if constexpr (IS_VV) {
-            const uint32_t x = L.ev[e], v = L.rowof[k];
-            const uint32_t w = (lo16(x) == v) ? hi16(x) : lo16(x);
-            out              = lin_of(w, d.nvo, d.vp, m.v_lin + d.v_off);
-        } else {
-            out = lin_of(e, d.neo, d.ep, m.e_lin + d.e_off);
-        }
+        const uint32_t x = L.slots[k];
+        const uint32_t out = IS_VV ? lin_of(x, d.nvo, d.vp, m.v_lin + d.v_off)
+                                   : lin_of(x, d.neo, d.ep, m.e_lin + d.e_off);
         col[bse + k] = static_cast<int32_t>(out);
     }
 }

@@ -93,8 +93,8 @@ struct QueryLDS
 {
     uint32_t *mask, *off;
     uint16_t* val;
-    uint32_t *ev, *cur, *slots, *wave, *ef, *gid, *eoff;
-    uint16_t* fe;
+    uint32_t *ev, *slots, *wave, *gid, *eoff;
+    uint16_t *fe, *hn;
     uint32_t  persist_bytes;
 };
 
@@ -120,13 +120,11 @@ __host__ __device__ inline QueryLDS plan_query(rxm::Bump& b, Op op, uint32_t nv,
         case Op::VV:
         case Op::VE:
             q.ev = b.alloc<uint32_t>(rxm::r4(ne));
+            q.slots = b.alloc<uint32_t>(2u * ne);
             if (oriented) {
                 q.fe  = b.alloc<uint16_t>(rxm::r4(3u * nf));
-                q.ef  = b.alloc<uint32_t>(2u * ne);
+                q.hn  = b.alloc<uint16_t>(2u * ne);
                 q.gid = b.alloc<uint32_t>(nv);
-                q.cur = b.alloc<uint32_t>(nv);
-            } else {
-                q.slots = b.alloc<uint32_t>(2u * ne);
             }
             break;
         case Op::VF:
@@ -289,26 +287,18 @@ struct Query
                 return (i & 1u) ? rxm::hi16(x) : rxm::lo16(x);
             };
             if (oriented) {
+                // half-edge successor table + edge columns, then each
+                // participating row ordered into its canonical CCW ring
                 rxm::stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
                 for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
-                for (uint32_t i = threadIdx.x; i < 2u * d.ne; i += T) L.ef[i] = 0xFFFFFFFFu;
+                for (uint32_t i = threadIdx.x; i < d.ne; i += T) reinterpret_cast<uint32_t*>(L.hn)[i] = 0xFFFFFFFFu;
                 block.sync();
-                rxm::lds_transpose<2>(2u * d.ne, d.nv, key, m_off, L.cur, m_val, L.wave, 0u);
-                for (uint32_t i = threadIdx.x; i < 3u * d.nf; i += T) {
-                    const uint32_t e = L.fe[i] >> 1, f = i / 3u;
-                    if (atomicCAS(&L.ef[2u * e], 0xFFFFFFFFu, f) != 0xFFFFFFFFu) L.ef[2u * e + 1u] = f;
-                }
-                block.sync();
+                rxm::build_half_next(L.fe, L.ev, d.nf, L.hn);
+                rxm::fill_columns<T>(2u * d.ne, d.nv, key, [&](uint32_t i) { return i >> 1; }, m_off, L.slots,
+                                     L.wave);
                 for (uint32_t v = threadIdx.x; v < rows; v += T)
-                    rxm::orient_row_fast<12>(v, m_off[v], m_off[v + 1], m_val, L.ev, L.fe, L.ef, L.gid);
-                if constexpr (op == Op::VV) {
-                    block.sync();
-                    for (uint32_t v = threadIdx.x; v < rows; v += T)
-                        for (uint32_t k = m_off[v]; k < m_off[v + 1]; ++k) {
-                            const uint32_t x = L.ev[m_val[k]];
-                            m_val[k] = static_cast<uint16_t>(rxm::lo16(x) == v ? rxm::hi16(x) : rxm::lo16(x));
-                        }
-                }
+                    rxm::orient_ring<12, op == Op::VV>(v, m_off[v], m_off[v + 1], L.slots, m_val, L.ev, L.hn,
+                                                       L.gid);
             } else {
                 block.sync();
                 rxm::fill_columns<T>(

@@ -94,6 +94,7 @@ class RXMeshStatic
     uint32_t get_max_face_adjacent_faces() const { return m_info.max_face_adjacent_faces; }
     bool     is_closed() const { return m_info.is_closed != 0; }
     bool     is_edge_manifold() const { return m_info.is_edge_manifold != 0; }
+    bool     is_consistently_oriented() const { return m_info.is_consistently_oriented != 0; }
     uint32_t get_num_vertices(const uint32_t p) const { return m_counts[6 * p + 0]; }
     uint32_t get_num_edges(const uint32_t p) const { return m_counts[6 * p + 1]; }
     uint32_t get_num_faces(const uint32_t p) const { return m_counts[6 * p + 2]; }
@@ -224,8 +225,9 @@ class RXMeshStatic
     {
         const uint32_t nv = m_info.max_vertices_per_patch, ne = m_info.max_edges_per_patch,
                        nf = m_info.max_faces_per_patch;
-        if (oriented && !is_edge_manifold())
-            throw std::runtime_error("prepare_launch_box: oriented queries need an edge-manifold mesh");
+        if (oriented && !(is_edge_manifold() && is_consistently_oriented()))
+            throw std::runtime_error(
+                "prepare_launch_box: oriented queries need an edge-manifold, consistently oriented mesh");
         size_t q = 0;
         for (Op o : op) {
             if (o == Op::V || o == Op::E || o == Op::F) continue;

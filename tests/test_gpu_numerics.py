@@ -56,7 +56,7 @@ def test_cot_assembly_vs_fp64(name):
     rp, col, val, mass = (t.cpu().numpy() for t in (rp, col, val, mass))
     orows, ocols, ovals, omass = o.cot_assemble(V32.astype(np.float64), dt)
     l2g = m.array("v_l2g")
-    # pattern: row = [diag, CCW ring] in the same order as the oracle
+    # pattern: row = [diag, VV query row] in the same order as the oracle
     assert rp[-1] == len(ocols) == m.cot_nnz()
     g2l = m.array("v_g2l")
     # compare entry by entry in row order (oracle rows are in linear order too)
@@ -70,6 +70,23 @@ def test_cot_assembly_vs_fp64(name):
     # row sums of L vanish: A 1 = M (up to fp32 rounding)
     rs = np.add.reduceat(val.astype(np.float64), rp[:-1])
     assert np.abs(rs - mass).max() <= 1e-5 * np.abs(val).max()
+
+
+@pytest.mark.parametrize("name", ["ico5_perm", "disk"])
+def test_cot_assembly_workgroup_sizes_agree(name):
+    """Knob 7 (256 / 512 threads per patch) changes the launch only: the
+    systems are bit-identical, and so are two calls in a row."""
+    import torch
+    V, F, m, o = _mesh(name)
+    x = _lin(m, V.astype(np.float32))
+    m.set_launch_threads(7, 256)
+    a = [t.clone() for t in m.cot_assemble(x, 1e-3)]
+    m.set_launch_threads(7, 512)
+    b = m.cot_assemble(x, 1e-3)
+    c = m.cot_assemble(x, 1e-3, out=tuple(torch.empty_like(t) for t in b))
+    torch.cuda.synchronize()
+    for u, v, w in zip(a, b, c):
+        assert torch.equal(u, v) and torch.equal(v, w)
 
 
 @pytest.mark.parametrize("name", ["ico4", "torus"])

@@ -86,14 +86,39 @@ def test_query_bit_exact(name, op):
 
 @pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "disk", "c1_kmeans"])
 @pytest.mark.parametrize("op", ["VV", "VE"])
-def test_oriented_query_bit_exact(name, op):
+@pytest.mark.parametrize("threads", [256, 512])
+def test_oriented_query_bit_exact(name, op, threads):
     import torch
     V, F, m, o = case(name)
+    m.set_launch_threads(6, threads)
     rp, col = m.query(op, oriented=True)
     torch.cuda.synchronize()
+    m.set_launch_threads(6, 256)
     orp, ocol = o.query(op, oriented=True)
     assert np.array_equal(rp.cpu().numpy(), orp)
     assert np.array_equal(col.cpu().numpy(), ocol)
+
+
+def test_oriented_paths_refuse_inconsistent_orientation():
+    """A face traversed backwards leaves half-edges with two faces: the
+    oriented query and the cotangent assembly return RXM_ERR_NON_MANIFOLD
+    while the unoriented queries still run."""
+    import rxmesh_amd as rx
+    import torch
+    V, F = mg.config_mesh("c1")
+    F = F.copy()
+    F[7] = F[7][::-1]
+    m = rx.Mesh(F)
+    for op in ("VV", "VE"):
+        with pytest.raises(rx.RXMError) as e:
+            m.query(op, oriented=True)
+        assert e.value.code == 2
+    with pytest.raises(rx.RXMError) as e:
+        m.cot_assemble(torch.zeros(3, m.nV, device="cuda"), 1e-3)
+    assert e.value.code == 2
+    rp, col = m.query("VV")
+    torch.cuda.synchronize()
+    assert int(rp[-1]) == m.query_nnz("VV")
 
 
 def test_query_set_semantics_vs_bruteforce():

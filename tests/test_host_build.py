@@ -106,7 +106,21 @@ def test_mesh_info_statistics():
     V, F = mg.config_mesh("c1")
     i = rx.Mesh(F, host_only=True).info
     assert (i["num_vertices"], i["num_edges"], i["num_faces"]) == (40962, 122880, 81920)
-    assert i["is_closed"] == 1
+    assert i["is_closed"] == 1 and i["is_consistently_oriented"] == 1
+
+
+@pytest.mark.parametrize("flip", [0, 5, 81919])
+def test_consistent_orientation_flag(flip):
+    """One face traversed backwards makes its edges run the same way in both
+    faces: the flag drops (oriented queries then need a fixed mesh); the
+    plane (boundary edges, single faces) and the icosphere keep it."""
+    V, F = obj("plane.obj")
+    assert rx.Mesh(F, host_only=True).info["is_consistently_oriented"] == 1
+    V, F = mg.config_mesh("c1")
+    F = F.copy()
+    F[flip] = F[flip][::-1]
+    i = rx.Mesh(F, host_only=True).info
+    assert i["is_edge_manifold"] == 1 and i["is_consistently_oriented"] == 0
 
 
 # ---- error handling: status codes, never exit (util/macros.h:77-89 exits) --

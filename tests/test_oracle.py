@@ -296,6 +296,23 @@ def test_cot_weights_flat_grid_closed_form():
     assert abs(A[(v, v)] - (mass[v] + 4 * 4.0)) < 1e-12
 
 
+@pytest.mark.parametrize("case", ["sphere", "disk"])
+def test_cot_rows_follow_vv_query_order(case):
+    """Row layout of the cotangent system: the diagonal, then the VV query's
+    row (the order SparseMatrix fills columns). One patch: linear ids are
+    global ids."""
+    pos, fv = mg.icosphere(2, seed=1, sigma=0.01) if case == "sphere" else mg.disk(8, seed=0)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    rows, cols, vals, mass = o.cot_assemble(pos, 1e-3)
+    rp, col = o.query("VV")
+    starts = np.flatnonzero(np.r_[True, rows[1:] != rows[:-1]])
+    assert len(starts) == len(pos)
+    for s, e in zip(starts, np.r_[starts[1:], len(rows)]):
+        v = rows[s]
+        assert cols[s] == v
+        assert np.array_equal(cols[s + 1:e], col[rp[v]:rp[v + 1]])
+
+
 @pytest.mark.parametrize("level", [2, 3])
 def test_cot_system_invariants_sphere(level):
     pos, fv = mg.icosphere(level, seed=1, sigma=0.01)
