@@ -321,6 +321,94 @@ __device__ __forceinline__ float3 s2_update(const PlanS2& L, uint32_t v, bool se
     return pv;
 }
 
+// Staging of the two-step kernel in two dependent levels of global loads,
+// each level's loads of a thread issued before it waits: (1) the plan rows
+// (u16 x 8 vectors), the owner-table ids of the ribbon vertices and the
+// extra vertices' linear ids; (2) the position gathers. Then the LDS stores.
+// Counts beyond the register rounds take plain loops.
+template <uint32_t T, bool AOS4>
+__device__ __forceinline__ void stage_s2(const PlanS2& L, const DevMesh& m, const PatchDesc& d,
+                                         const uint16_t* __restrict__ ro_src, const uint16_t* __restrict__ nb_src,
+                                         uint32_t nent, const uint32_t* __restrict__ xl, uint32_t nx,
+                                         const float* __restrict__ pos)
+{
+    constexpr uint32_t R1 = 2, RN = 4;
+    constexpr uint32_t RV = (512u / T) > 0u ? 512u / T : 1u;
+    constexpr uint32_t RX = (256u / T) > 0u ? 256u / T : 1u;
+    const uint32_t     N   = m.nV;
+    const uint32_t     nro = (d.nv + 1u + 7u) >> 3, nnb = (nent + 7u) >> 3;
+    const uint4*       rs  = reinterpret_cast<const uint4*>(ro_src);
+    const uint4*       ns  = reinterpret_cast<const uint4*>(nb_src);
+    uint4*             rd  = reinterpret_cast<uint4*>(L.ro);
+    uint4*             nd  = reinterpret_cast<uint4*>(L.nb);
+    const uint32_t*    vl  = m.v_lin + d.v_off;
+    const uint32_t     t   = threadIdx.x;
+    uint4              ra[R1], na[RN];
+    uint32_t           g[RV], gx[RX];
+    // level 1
+#pragma unroll
+    for (uint32_t r = 0; r < R1; ++r) ra[r] = (t + r * T < nro) ? rs[t + r * T] : uint4{};
+#pragma unroll
+    for (uint32_t r = 0; r < RN; ++r) na[r] = (t + r * T < nnb) ? ns[t + r * T] : uint4{};
+#pragma unroll
+    for (uint32_t r = 0; r < RV; ++r) {
+        const uint32_t l = t + r * T;
+        g[r]             = (l < d.nvo) ? d.vp + l : ((l < d.nv) ? vl[l] : 0u);
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RX; ++r) gx[r] = (t + r * T < nx) ? xl[t + r * T] : 0u;
+    // level 2
+    float x[RV], y[RV], z[RV], xx[RX], xy[RX], xz[RX];
+#pragma unroll
+    for (uint32_t r = 0; r < RV; ++r) {
+        if (t + r * T < d.nv) {
+            x[r] = pos[g[r]];
+            y[r] = pos[N + g[r]];
+            z[r] = pos[2 * N + g[r]];
+        }
+    }
+#pragma unroll
+    for (uint32_t r = 0; r < RX; ++r) {
+        if (t + r * T < nx) {
+            xx[r] = pos[gx[r]];
+            xy[r] = pos[N + gx[r]];
+            xz[r] = pos[2 * N + gx[r]];
+        }
+    }
+    auto put = [&](uint32_t l, float a, float bb, float c) {
+        if constexpr (AOS4) {
+            L.p4[l] = make_float4(a, bb, c, 0.f);
+        } else {
+            L.x[l] = a;
+            L.y[l] = bb;
+            L.z[l] = c;
+        }
+    };
+#pragma unroll
+    for (uint32_t r = 0; r < R1; ++r)
+        if (t + r * T < nro) rd[t + r * T] = ra[r];
+#pragma unroll
+    for (uint32_t r = 0; r < RN; ++r)
+        if (t + r * T < nnb) nd[t + r * T] = na[r];
+#pragma unroll
+    for (uint32_t r = 0; r < RV; ++r)
+        if (t + r * T < d.nv) put(t + r * T, x[r], y[r], z[r]);
+#pragma unroll
+    for (uint32_t r = 0; r < RX; ++r)
+        if (t + r * T < nx) put(d.nv + t + r * T, xx[r], xy[r], xz[r]);
+    // remainders of larger patches
+    for (uint32_t i = t + R1 * T; i < nro; i += T) rd[i] = rs[i];
+    for (uint32_t i = t + RN * T; i < nnb; i += T) nd[i] = ns[i];
+    for (uint32_t l = t + RV * T; l < d.nv; l += T) {
+        const uint32_t gg = lin_of(l, d.nvo, d.vp, vl);
+        put(l, pos[gg], pos[N + gg], pos[2 * N + gg]);
+    }
+    for (uint32_t i = t + RX * T; i < nx; i += T) {
+        const uint32_t gg = xl[i];
+        put(d.nv + i, pos[gg], pos[N + gg], pos[2 * N + gg]);
+    }
+}
+
 template <uint32_t T, bool AOS4>
 __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __restrict__ rbase,
                                                const uint32_t* __restrict__ nbase, const uint32_t* __restrict__ xbase,
@@ -336,20 +424,7 @@ __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __rest
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanS2    L = plan_s2(b, d.nv, nx, nent, AOS4);
     const uint32_t  N = m.nV;
-    stage_u16x8<T>(L.ro, roff + rb, d.nv + 1);
-    stage_u16x8<T>(L.nb, nbr + nb0, nent);
-    stage_positions_batched<T, 512u / T, AOS4>(L.x, L.y, L.z, L.p4, pos, N, d, m.v_lin);
-    for (uint32_t i = threadIdx.x; i < nx; i += T) {
-        const uint32_t g = xlin[xb + i];
-        const float    x = pos[g], y = pos[N + g], z = pos[2 * N + g];
-        if constexpr (AOS4) {
-            L.p4[d.nv + i] = make_float4(x, y, z, 0.f);
-        } else {
-            L.x[d.nv + i] = x;
-            L.y[d.nv + i] = y;
-            L.z[d.nv + i] = z;
-        }
-    }
+    stage_s2<T, AOS4>(L, m, d, roff + rb, nbr + nb0, nent, xlin + xb, nx, pos);
     __syncthreads();
     // step 1: every local vertex
     for (uint32_t v = threadIdx.x; v < d.nv; v += T) {
