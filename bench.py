@@ -38,12 +38,15 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=10.0)
     ap.add_argument("--profile", default=None,
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
-                         "smooth_uniform, smooth_2step (with --smooth-fused 1), vertex_normals, copy")
+                         "smooth_uniform, smooth_2step (with --smooth-fused 1), smooth_ksteps (with --smooth-steps k), "
+                         "vertex_normals, copy")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
     ap.add_argument("--smooth-fused", type=int, default=None,
                     help="1: two Jacobi steps per smoothing launch (default: autotuned)")
+    ap.add_argument("--smooth-steps", type=int, default=None,
+                    help="1-6: Jacobi steps per launch over the K-level plan, 0: off (default: autotuned)")
     ap.add_argument("--batch-count", type=int, default=256, help="meshes in the batched workload (0: skip)")
     ap.add_argument("--batch-steps", type=int, default=20)
     ap.add_argument("--batch-reps", type=int, default=3)
@@ -307,16 +310,18 @@ def main():
     m.build_vv_plan()  # static patch-local VV plan, built once per mesh
     torch.cuda.synchronize()
     from rxmesh_amd import autotune
-    if args.smooth_threads or args.smooth_layout is not None or args.smooth_fused is not None:
+    if (args.smooth_threads or args.smooth_layout is not None or args.smooth_fused is not None
+            or args.smooth_steps is not None):
         smooth_cfg = {"smooth_threads": args.smooth_threads or 128,
                       "smooth_layout": args.smooth_layout if args.smooth_layout is not None else 0,
-                      "smooth_fused": args.smooth_fused if args.smooth_fused is not None else 0}
+                      "smooth_fused": args.smooth_fused if args.smooth_fused is not None else 0,
+                      "smooth_steps": args.smooth_steps if args.smooth_steps is not None else 0}
         autotune.apply(m, smooth_cfg)
         smooth_sweep = {}
     elif args.profile:
         smooth_cfg, smooth_sweep = {}, {}  # library defaults; --smooth-threads / --smooth-layout choose
     else:
-        smooth_cfg, smooth_sweep = autotune.tune_smooth(m, x, out, use_cache=False)
+        smooth_cfg, smooth_sweep = autotune.tune_smooth(m, x, out, use_cache=False, iters=it)
     torch.cuda.synchronize()
 
     if args.profile:
@@ -324,6 +329,7 @@ def main():
                "query_vf": lambda: m.query("VF", row_ptr=vf_rp, col=vf_col),
                "smooth_uniform": lambda: m.smooth(x, 0.5, 1, out=out),
                "smooth_2step": lambda: m.smooth(x, 0.5, 2, out=out, tmp=tmp),
+               "smooth_ksteps": lambda: m.smooth(x, 0.5, max(1, args.smooth_steps or 1), out=out, tmp=tmp),
                "smooth_csr": lambda: m.smooth_csr((vv_rp, vv_col), x, 0.5, 1, out=out),
                "vertex_normals": lambda: m.vertex_normals(x, "max", out=out),
                "copy": lambda: out.copy_(x)}  # plain copy of 12 V bytes (counter calibration)
@@ -360,10 +366,16 @@ def main():
     kern["query_vf"] = (t_vf, 4 * E + 6 * F + 4 * nnz_vf + 4 * (V + 1))
     t_sm = event_time_ms(lambda: m.smooth(x, 0.5, 1, out=out), reps)
     kern["smooth_uniform"] = (t_sm, 24 * V + 4 * E)
-    fused = bool(smooth_cfg.get("smooth_fused", 0))
+    ksteps = int(smooth_cfg.get("smooth_steps", 0))
+    fused = bool(smooth_cfg.get("smooth_fused", 0)) and not ksteps
     if fused:  # one launch = two Jacobi steps: twice the entries of one step
         t_s2 = event_time_ms(lambda: m.smooth(x, 0.5, 2, out=out, tmp=tmp), reps)
         kern["smooth_uniform_2step"] = (t_s2, 2 * (24 * V + 4 * E))
+    if ksteps:  # one launch = ksteps Jacobi steps over the K-level plan
+        t_sk = event_time_ms(lambda: m.smooth(x, 0.5, ksteps, out=out, tmp=tmp), reps)
+        kern["smooth_uniform_%dstep" % ksteps] = (t_sk, ksteps * (24 * V + 4 * E))
+        t_rem = (event_time_ms(lambda: m.smooth(x, 0.5, it % ksteps, out=out, tmp=tmp), reps)
+                 if it % ksteps else 0.0)
     t_sc = event_time_ms(lambda: m.smooth_csr((vv_rp, vv_col), x, 0.5, 1, out=out), reps)
     kern["smooth_csr"] = (t_sc, 24 * V + 4 * E)
     t_vn = event_time_ms(lambda: m.vertex_normals(x, "max", out=out), reps)
@@ -372,7 +384,9 @@ def main():
                    "frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k, (t, b) in kern.items()}
     # dominant kernel of the step (by time per step)
     share = {"query_vv": t_vv, "query_vf": t_vf}
-    if fused:
+    if ksteps:
+        share["smooth_uniform_%dstep" % ksteps] = (it // ksteps) * t_sk + t_rem
+    elif fused:
         share["smooth_uniform_2step"] = (it // 2) * t_s2 + (it % 2) * t_sm
     else:
         share["smooth_uniform"] = it * t_sm

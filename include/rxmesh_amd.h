@@ -182,7 +182,10 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *             plan (RXM_ARR_VV2_*), 0 = one step per launch;
  *   kernel 5: SpMV-3 / explicit MCF lanes per CSR row, 1 / 2 / 4 / 8;
  *   kernel 6: oriented VV / VE workgroup size 256 / 512;
- *   kernel 7: cotangent assembly workgroup size 256 / 512.
+ *   kernel 7: cotangent assembly workgroup size 256 / 512;
+ *   kernel 8: smoothing, 1-6 = that many Jacobi steps per launch over the
+ *             K-level plan (patch + rings of neighbours kept in LDS; overrides
+ *             kernels 3 and 4), 0 = off.
  * Settings of kernels 0-4 and 6 give bit-identical results; kernel 5 changes
  * the summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);

@@ -60,13 +60,24 @@ SMOOTH_LAYOUTS = (0, 1)  # LDS staging of positions: 0 three float arrays, 1 flo
 
 
 SMOOTH_FUSED = (0, 1)  # 1: two Jacobi steps per launch over the two-step plan
+SMOOTH_STEPS = (2, 3, 4, 5, 6)  # Jacobi steps per launch over the K-level plan
+SMOOTH_STEPS_THREADS = (128, 256, 512)
 
 
 def smooth_configs():
     for fused in SMOOTH_FUSED:
         for st in SMOOTH_THREADS:
             for lay in SMOOTH_LAYOUTS:
-                yield {"smooth_threads": st, "smooth_layout": lay, "smooth_fused": fused}
+                yield {"smooth_threads": st, "smooth_layout": lay, "smooth_fused": fused, "smooth_steps": 0}
+    for steps in SMOOTH_STEPS:
+        for st in SMOOTH_STEPS_THREADS:
+            yield {"smooth_threads": st, "smooth_layout": 0, "smooth_fused": 0, "smooth_steps": steps}
+
+
+def config_name(cfg):
+    if cfg.get("smooth_steps", 0):
+        return "t%d/k%d" % (cfg["smooth_threads"], cfg["smooth_steps"])
+    return "t%d/l%d/f%d" % (cfg["smooth_threads"], cfg["smooth_layout"], cfg["smooth_fused"])
 
 
 def apply(mesh, cfg):
@@ -80,13 +91,16 @@ def apply(mesh, cfg):
         mesh.set_launch_threads(3, cfg["smooth_layout"])
     if "smooth_fused" in cfg:
         mesh.set_launch_threads(4, cfg["smooth_fused"])
+    if "smooth_threads" in cfg:
+        mesh.set_launch_threads(8, cfg.get("smooth_steps", 0))
 
 
-def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True):
+def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True, iters=10):
     """Pick workgroup size, LDS layout and steps per launch of uniform
-    smoothing from the time of two Jacobi iterations; returns (config,
-    timings in us per iteration)."""
-    key = _key(mesh, "smooth")
+    smoothing from the time of `iters` Jacobi iterations (the workload's
+    count: a launch of k steps only pays off over k iterations); returns
+    (config, timings in us per iteration)."""
+    key = _key(mesh, "smooth%d" % iters)
     cache = _load() if use_cache else {}
     if key in cache:
         apply(mesh, cache[key])
@@ -98,8 +112,8 @@ def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True):
     best, best_t = None, float("inf")
     for cfg in smooth_configs():
         apply(mesh, cfg)
-        t = _time_ms(lambda: mesh.smooth(x, lam, 2, out=out, tmp=tmp), reps) / 2
-        timings["t%d/l%d/f%d" % (cfg["smooth_threads"], cfg["smooth_layout"], cfg["smooth_fused"])] = round(t * 1e3, 2)
+        t = _time_ms(lambda: mesh.smooth(x, lam, iters, out=out, tmp=tmp), reps) / iters
+        timings[config_name(cfg)] = round(t * 1e3, 2)
         if t < best_t:
             best, best_t = cfg, t
     apply(mesh, best)

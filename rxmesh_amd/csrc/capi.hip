@@ -48,6 +48,16 @@ struct rxm_mesh
     const uint16_t*   d_vv2_roff  = nullptr;
     const uint16_t*   d_vv2_nbr   = nullptr;
     const uint32_t*   d_vv2_xlin  = nullptr;
+    // K-level smoothing plan (temporal blocking), built on first use
+    VvkPlan           vvk;
+    const uint32_t*   d_vvk_lvl   = nullptr;
+    const uint32_t*   d_vvk_rbase = nullptr;
+    const uint32_t*   d_vvk_nbase = nullptr;
+    const uint32_t*   d_vvk_xbase = nullptr;
+    const uint16_t*   d_vvk_roff  = nullptr;
+    const uint16_t*   d_vvk_nbr   = nullptr;
+    const uint32_t*   d_vvk_xlin  = nullptr;
+    uint32_t          smooth_steps = 0;  // > 0: Jacobi steps per launch over the K-level plan
     uint32_t          smooth_fused = 0;  // 1: two Jacobi steps per launch
     uint32_t          spmv_lanes   = 8;  // lanes per CSR row in the SpMV-3 kernels
     // patch-local face plan (FV table + VF rows of (face << 2 | corner))
@@ -416,6 +426,43 @@ int ensure_cot_plan(rxm_mesh* m)
     return RXM_OK;
 }
 
+constexpr uint32_t kSmoothLevels = 6;  // rings of the K-level plan: up to 6 steps per launch
+
+int ensure_vvk_plan(rxm_mesh* m)
+{
+    if (m->d_vvk_nbr) return RXM_OK;
+    int rc = build_vvk_plan(m->h, kSmoothLevels, m->vvk);
+    const VvkPlan& v = m->vvk;
+    if (!rc) rc = upload(m, v.lvl, m->d_vvk_lvl);
+    if (!rc) rc = upload(m, v.rbase, m->d_vvk_rbase);
+    if (!rc) rc = upload(m, v.nbase, m->d_vvk_nbase);
+    if (!rc) rc = upload(m, v.xbase, m->d_vvk_xbase);
+    if (!rc) rc = upload(m, v.roff, m->d_vvk_roff);
+    if (!rc) rc = upload(m, v.xlin, m->d_vvk_xlin);
+    if (!rc) rc = upload(m, v.nbr, m->d_vvk_nbr);  // last: marks the plan complete
+    if (rc) m->d_vvk_nbr = nullptr;
+    return rc;
+}
+
+// `steps` Jacobi steps in one launch over the K-level plan
+int smooth_steps(rxm_mesh* m, uint32_t steps, float lambda, const float* src, float* dst, void* stream)
+{
+    const VvkPlan& v = m->vvk;
+    Bump           b(nullptr);
+    plan_sk(b, v.max_cnt[steps], v.max_cnt[steps - 1], v.max_ent[steps - 1]);
+    auto go = [&](auto kernel, uint32_t T) {
+        return launch_patches_xcd(kernel, T, b.bytes(), stream, m, m->dm, v.K, m->d_vvk_lvl, m->d_vvk_rbase,
+                                  m->d_vvk_nbase, m->d_vvk_xbase, m->d_vvk_roff, m->d_vvk_nbr, m->d_vvk_xlin, steps,
+                                  lambda, src, dst);
+    };
+    switch (m->smooth_threads) {
+        case 64: return go(k_smoothk<64>, 64);
+        case 256: return go(k_smoothk<256>, 256);
+        case 512: return go(k_smoothk<512>, 512);
+        default: return go(k_smoothk<128>, 128);
+    }
+}
+
 // two Jacobi steps in one launch
 template <bool AOS4>
 int smooth_twice_t(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
@@ -767,19 +814,19 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
     switch (op) {
         case RXM_OP_VV:
             plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches(k_query2_vx<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            return launch_patches_xcd(k_query2_vx<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_VE:
             plan_q2(b, 3, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches(k_query2_vx<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            return launch_patches_xcd(k_query2_vx<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_VF:
             plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches(k_query2_vf<T>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            return launch_patches_xcd(k_query2_vf<T>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_EF:
             plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches(k_query2_ef<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            return launch_patches_xcd(k_query2_ef<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_FF:
             plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches(k_query2_ef<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            return launch_patches_xcd(k_query2_ef<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_EV:
             plan_fx(b, h.max_ne, h.max_nf);
             return launch_patches(k_query_fixed<9>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
@@ -825,7 +872,23 @@ rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float
     if (!m || !pos || !out || iters < 1 || pos == out) return RXM_ERR_INVALID_ARG;
     if (iters > 1 && (!tmp || tmp == out || tmp == pos)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    int rc = ensure_vv_plan(m, stream);
+    int rc = RXM_OK;
+    if (m->smooth_steps > 0) {  // K-level plan: launches of up to smooth_steps steps
+        if ((rc = ensure_vvk_plan(m))) return rc;
+        const uint32_t S        = m->smooth_steps;
+        const int      launches = static_cast<int>((iters + S - 1) / S);
+        const float*   src      = pos;
+        int            left     = iters;
+        for (int i = 0; i < launches; ++i) {
+            float*         dst = ((launches - 1 - i) % 2 == 0) ? out : tmp;
+            const uint32_t s   = std::min<uint32_t>(S, static_cast<uint32_t>(left));
+            if ((rc = smooth_steps(m, s, lambda, src, dst, stream))) return rc;
+            left -= static_cast<int>(s);
+            src = dst;
+        }
+        return RXM_OK;
+    }
+    rc = ensure_vv_plan(m, stream);
     if (rc) return rc;
     const bool fused = m->smooth_fused && iters > 1 && ensure_vv2_plan(m) == RXM_OK;
     // launches: pairs of steps (fused) then a single one; the last writes out
@@ -874,6 +937,10 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
     }
     if (kernel == 7 && (threads == 256 || threads == 512)) {
         m->cot_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 8 && threads >= 0 && threads <= static_cast<int>(kSmoothLevels)) {
+        m->smooth_steps = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
     return RXM_ERR_INVALID_ARG;

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 #include <atomic>
+#include <unordered_map>
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
@@ -1022,6 +1023,108 @@ int build_vv2_plan(const HostMesh& m, Vv2Plan& out)
         out.nx[p]        = static_cast<uint32_t>(px[p].size());
         out.max_nx       = std::max(out.max_nx, out.nx[p]);
         out.max_nent     = std::max(out.max_nent, static_cast<uint32_t>(pn[p].size()));
+    }
+    out.roff.assign(std::max<uint32_t>(out.rbase[P], 8), 0);
+    out.nbr.assign(std::max<uint32_t>(out.nbase[P], 8), 0);
+    out.xlin.assign(std::max<uint32_t>(out.xbase[P], 4), 0);
+    parallel_for(P, [&](size_t p) {
+        std::copy(pr[p].begin(), pr[p].end(), out.roff.begin() + out.rbase[p]);
+        std::copy(pn[p].begin(), pn[p].end(), out.nbr.begin() + out.nbase[p]);
+        std::copy(px[p].begin(), px[p].end(), out.xlin.begin() + out.xbase[p]);
+    }, 64);
+    return kOk;
+}
+
+// K-level smoothing plan (host_build.h).
+int build_vvk_plan(const HostMesh& m, uint32_t K, VvkPlan& out)
+{
+    if (K < 1 || K > 16) return kErrInvalidArg;
+    const uint32_t P = m.P, V = m.nV, W = 2 * K + 1;
+    // canonical VV rows in global ids, indexed by linear vertex id (as in
+    // build_vv2_plan): ascending owner-local edge order
+    std::vector<uint32_t> deg(V, 0);
+    for (uint64_t e = 0; e < m.nE; ++e) {
+        deg[m.ev[2 * e]]++;
+        deg[m.ev[2 * e + 1]]++;
+    }
+    std::vector<uint32_t> roff_g(V + 1, 0);
+    for (uint32_t l = 0; l < V; ++l) roff_g[l + 1] = roff_g[l] + deg[m.v_l2g[l]];
+    std::vector<uint32_t> rows(roff_g[V]);
+    parallel_for(P, [&](size_t q) {
+        const uint32_t        vo = m.v_off[q], eo = m.e_off[q], nvo = m.nvo[q], ne = m.ne[q];
+        std::vector<uint32_t> fill(nvo, 0);
+        for (uint32_t e = 0; e < ne; ++e) {
+            const uint32_t x = m.ev_local[eo + e], a = x & 0xFFFFu, b = x >> 16;
+            if (a < nvo) rows[roff_g[m.vp[q] + a] + fill[a]++] = m.ltog_v[vo + b];
+            if (b < nvo) rows[roff_g[m.vp[q] + b] + fill[b]++] = m.ltog_v[vo + a];
+        }
+    }, 1);
+    std::vector<std::vector<uint16_t>> pr(P), pn(P);
+    std::vector<std::vector<uint32_t>> px(P);
+    out.K = K;
+    out.lvl.assign(size_t(P) * W, 0);
+    std::atomic<int> err{kOk};
+    parallel_for(P, [&](size_t p) {
+        const uint32_t  vo = m.v_off[p], nvo = m.nvo[p];
+        const uint32_t* gl = &m.ltog_v[vo];
+        std::unordered_map<uint32_t, uint32_t> ext;  // global id -> extended id
+        std::vector<uint32_t>                  E(gl, gl + nvo);
+        ext.reserve(8 * nvo + 64);
+        for (uint32_t i = 0; i < nvo; ++i) ext.emplace(gl[i], i);
+        uint32_t* c = &out.lvl[size_t(p) * W];
+        c[0]        = nvo;
+        uint32_t lo = 0;
+        for (uint32_t k = 1; k <= K; ++k) {
+            std::vector<uint32_t> next;
+            for (uint32_t i = lo; i < c[k - 1]; ++i) {
+                const uint32_t lin = m.v_g2l[E[i]];
+                for (uint32_t j = roff_g[lin]; j < roff_g[lin + 1]; ++j)
+                    if (!ext.count(rows[j])) next.push_back(rows[j]);
+            }
+            std::sort(next.begin(), next.end());
+            next.erase(std::unique(next.begin(), next.end()), next.end());
+            for (uint32_t g : next) {
+                ext.emplace(g, static_cast<uint32_t>(E.size()));
+                E.push_back(g);
+            }
+            lo   = c[k - 1];
+            c[k] = static_cast<uint32_t>(E.size());
+        }
+        if (c[K] >= 0xFFFFu) {
+            err = kErrPatchTooBig;
+            return;
+        }
+        std::vector<uint16_t>& R  = pr[p];
+        std::vector<uint16_t>& Nb = pn[p];
+        const uint32_t         nr = c[K - 1];
+        R.assign(nr + 1, 0);
+        for (uint32_t i = 0; i < nr; ++i) {
+            const uint32_t lin = m.v_g2l[E[i]];
+            for (uint32_t j = roff_g[lin]; j < roff_g[lin + 1]; ++j) Nb.push_back(static_cast<uint16_t>(ext.at(rows[j])));
+            if (Nb.size() >= 0xFFFFu) {
+                err = kErrPatchTooBig;
+                return;
+            }
+            R[i + 1] = static_cast<uint16_t>(Nb.size());
+        }
+        for (uint32_t k = 0; k < K; ++k) c[K + 1 + k] = R[c[k]];
+        std::vector<uint32_t>& X = px[p];
+        X.resize(c[K] - nvo);
+        for (uint32_t i = nvo; i < c[K]; ++i) X[i - nvo] = m.v_g2l[E[i]];
+    });
+    if (err != kOk) return err;
+    out.rbase.assign(P + 1, 0);
+    out.nbase.assign(P + 1, 0);
+    out.xbase.assign(P + 1, 0);
+    out.max_cnt.assign(K + 1, 0);
+    out.max_ent.assign(K, 0);
+    for (uint32_t p = 0; p < P; ++p) {
+        out.rbase[p + 1] = out.rbase[p] + round_up(static_cast<uint32_t>(pr[p].size()), 8);
+        out.nbase[p + 1] = out.nbase[p] + round_up(static_cast<uint32_t>(pn[p].size()), 8);
+        out.xbase[p + 1] = out.xbase[p] + round_up(static_cast<uint32_t>(px[p].size()), 4);
+        const uint32_t* c = &out.lvl[size_t(p) * W];
+        for (uint32_t k = 0; k <= K; ++k) out.max_cnt[k] = std::max(out.max_cnt[k], c[k]);
+        for (uint32_t k = 0; k < K; ++k) out.max_ent[k] = std::max(out.max_ent[k], c[K + 1 + k]);
     }
     out.roff.assign(std::max<uint32_t>(out.rbase[P], 8), 0);
     out.nbr.assign(std::max<uint32_t>(out.nbase[P], 8), 0);

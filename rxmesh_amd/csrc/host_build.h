@@ -127,6 +127,27 @@ struct Vv2Plan
 };
 int build_vv2_plan(const HostMesh& m, Vv2Plan& out);
 
+// K-level smoothing plan (temporal blocking). For every patch p an extended
+// vertex space ordered by ring level around the patch's owned vertices:
+// level 0 = the owned vertices (local order), level k = the vertices first
+// reached in k VV steps, each level sorted by global id. cnt[k] = vertices of
+// levels <= k, so a launch of s <= K Jacobi steps computes, at step i, the
+// prefix [0, cnt[s - i]) -- exactly the vertices the later steps read -- and
+// the last step the owned vertices. Rows (the full one-ring in the owner's
+// canonical VV order, neighbours as extended ids) exist for [0, cnt[K - 1]).
+// lvl holds 2K + 1 u32 per patch: cnt[0..K], then ent[0..K-1] with ent[k] =
+// row entries of [0, cnt[k]).
+struct VvkPlan
+{
+    uint32_t              K = 0;
+    std::vector<uint32_t> lvl;                  // P * (2K + 1)
+    std::vector<uint32_t> rbase, nbase, xbase;  // P + 1 each: u16 / u16 / u32 offsets, runs padded to 8 / 8 / 4
+    std::vector<uint16_t> roff, nbr;            // per patch: cnt[K-1] + 1 row offsets; entries
+    std::vector<uint32_t> xlin;                 // per patch: linear ids of extended vertices [nvo, cnt[K])
+    std::vector<uint32_t> max_cnt, max_ent;     // K + 1 / K: largest cnt[k] / ent[k] over patches
+};
+int build_vvk_plan(const HostMesh& m, uint32_t K, VvkPlan& out);
+
 // Vertices whose one-ring is one closed fan: the successor map of its faces
 // (half-edge v -> a of face (v, a, b) is followed by b) is a single cycle
 // through all its neighbours. The canonical ring of such a vertex has a real

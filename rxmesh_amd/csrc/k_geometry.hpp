@@ -448,6 +448,153 @@ __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __rest
     }
 }
 
+// ---- K-step smoothing over the level plan (temporal blocking) ---------------
+// One workgroup per patch runs `steps` Jacobi steps of uniform smoothing on
+// the patch's extended vertex space (owned vertices, then the rings around
+// them by level): step i computes the prefix [0, cnt[steps - i]) from the
+// previous step's values, the last step the owned vertices, which go to HBM.
+// Each vertex update lists its neighbours in the owner's canonical VV order,
+// as k_smooth_local does, so any split of the iterations into launches gives
+// bit-identical results. lvl: 2K + 1 u32 per patch (cnt[0..K], ent[0..K-1]).
+struct PlanSK
+{
+    uint16_t *ro, *nb;
+    float *   ax, *ay, *az;  // cnt[steps]: input positions, later even steps
+    float *   bx, *by, *bz;  // cnt[steps - 1]: odd steps
+};
+__host__ __device__ inline PlanSK plan_sk(Bump& b, uint32_t nin, uint32_t ncomp, uint32_t nent)
+{
+    PlanSK p{};
+    p.ro = b.alloc<uint16_t>(((ncomp + 1u) + 7u) & ~7u);
+    p.nb = b.alloc<uint16_t>((nent + 7u) & ~7u);
+    p.ax = b.alloc<float>(nin);
+    p.ay = b.alloc<float>(nin);
+    p.az = b.alloc<float>(nin);
+    p.bx = b.alloc<float>(ncomp);
+    p.by = b.alloc<float>(ncomp);
+    p.bz = b.alloc<float>(ncomp);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_smoothk(DevMesh m, uint32_t K, const uint32_t* __restrict__ lvl,
+                                               const uint32_t* __restrict__ rbase,
+                                               const uint32_t* __restrict__ nbase,
+                                               const uint32_t* __restrict__ xbase,
+                                               const uint16_t* __restrict__ roff, const uint16_t* __restrict__ nbr,
+                                               const uint32_t* __restrict__ xlin, uint32_t steps, float lambda,
+                                               const float* __restrict__ pos, float* __restrict__ out)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d     = load_desc(m.patch, p);
+    const uint32_t* c     = lvl + size_t(p) * (2 * K + 1);
+    const uint32_t  nin   = c[steps], ncomp = c[steps - 1], nent = c[K + steps];
+    const uint32_t  N     = m.nV;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanSK    L = plan_sk(b, nin, ncomp, nent);
+    {
+        // staging in two levels of global loads: (1) rows and the extended
+        // vertices' linear ids, (2) the position gathers
+        constexpr uint32_t R1 = 2, RN = 6, RV = 4;
+        const uint32_t     t   = threadIdx.x;
+        const uint32_t     nro = (ncomp + 1u + 7u) >> 3, nnb = (nent + 7u) >> 3;
+        const uint4*       rs  = reinterpret_cast<const uint4*>(roff + rbase[p]);
+        const uint4*       ns  = reinterpret_cast<const uint4*>(nbr + nbase[p]);
+        const uint32_t*    xl  = xlin + xbase[p];
+        uint4*             rd  = reinterpret_cast<uint4*>(L.ro);
+        uint4*             nd  = reinterpret_cast<uint4*>(L.nb);
+        uint4              ra[R1], na[RN];
+        uint32_t           g[RV];
+#pragma unroll
+        for (uint32_t r = 0; r < R1; ++r) ra[r] = (t + r * T < nro) ? rs[t + r * T] : uint4{};
+#pragma unroll
+        for (uint32_t r = 0; r < RN; ++r) na[r] = (t + r * T < nnb) ? ns[t + r * T] : uint4{};
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            const uint32_t l = t + r * T;
+            g[r]             = (l < d.nvo) ? d.vp + l : ((l < nin) ? xl[l - d.nvo] : 0u);
+        }
+        float x[RV], y[RV], z[RV];
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            if (t + r * T < nin) {
+                x[r] = pos[g[r]];
+                y[r] = pos[N + g[r]];
+                z[r] = pos[2 * N + g[r]];
+            }
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < R1; ++r)
+            if (t + r * T < nro) rd[t + r * T] = ra[r];
+#pragma unroll
+        for (uint32_t r = 0; r < RN; ++r)
+            if (t + r * T < nnb) nd[t + r * T] = na[r];
+#pragma unroll
+        for (uint32_t r = 0; r < RV; ++r) {
+            const uint32_t l = t + r * T;
+            if (l < nin) {
+                L.ax[l] = x[r];
+                L.ay[l] = y[r];
+                L.az[l] = z[r];
+            }
+        }
+        for (uint32_t i = t + R1 * T; i < nro; i += T) rd[i] = rs[i];
+        for (uint32_t i = t + RN * T; i < nnb; i += T) nd[i] = ns[i];
+        for (uint32_t l = t + RV * T; l < nin; l += T) {
+            const uint32_t gg = (l < d.nvo) ? d.vp + l : xl[l - d.nvo];
+            L.ax[l]           = pos[gg];
+            L.ay[l]           = pos[N + gg];
+            L.az[l]           = pos[2 * N + gg];
+        }
+    }
+    __syncthreads();
+    const float *sx = L.ax, *sy = L.ay, *sz = L.az;
+    float *      tx = L.bx, *ty = L.by, *tz = L.bz;
+    for (uint32_t i = 1; i <= steps; ++i) {
+        const uint32_t n    = c[steps - i];
+        const bool     last = (i == steps);
+        for (uint32_t v = threadIdx.x; v < n; v += T) {
+            const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+            float          ax = 0.f, ay = 0.f, az = 0.f;
+            for (uint32_t k = k0; k < k1; ++k) {
+                const uint32_t w = L.nb[k];
+                ax += sx[w];
+                ay += sy[w];
+                az += sz[w];
+            }
+            const float px = sx[v], py = sy[v], pz = sz[v];
+            float       rx = px, ry = py, rz = pz;
+            if (k1 > k0) {
+                const float inv = 1.0f / static_cast<float>(k1 - k0);
+                rx              = px + lambda * (ax * inv - px);
+                ry              = py + lambda * (ay * inv - py);
+                rz              = pz + lambda * (az * inv - pz);
+            }
+            if (last) {
+                const uint32_t gg = d.vp + v;
+                out[gg]           = rx;
+                out[N + gg]       = ry;
+                out[2 * N + gg]   = rz;
+            } else {
+                tx[v] = rx;
+                ty[v] = ry;
+                tz[v] = rz;
+            }
+        }
+        if (!last) {
+            __syncthreads();
+            // the buffer just read becomes the next target (its prefix is large enough)
+            float* ox = const_cast<float*>(sx);
+            float* oy = const_cast<float*>(sy);
+            float* oz = const_cast<float*>(sz);
+            sx = tx; sy = ty; sz = tz;
+            tx = ox; ty = oy; tz = oz;
+        }
+    }
+}
+
 // ---- vertex normals over the patch-local face plan ----------------------------
 // Same weights and summation order as k_vertex_normals (incident faces in
 // ascending local order), but the VF rows come precomputed from the face

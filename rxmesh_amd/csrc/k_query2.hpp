@@ -39,25 +39,50 @@ __host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32
     return p;
 }
 
-// Target linear ids of all local elements (owned: prefix + l; ribbon: owner
-// table), staged once so row writes need no global gathers.
-// Loads of four rounds are issued before their LDS stores.
-__device__ __forceinline__ void stage_lin(uint32_t* dst, uint32_t n, uint32_t n_owned, uint32_t prefix,
-                                          const uint32_t* __restrict__ slab, uint32_t T)
+// One-round-trip staging of a query's inputs: the EV slab (16-byte chunks),
+// the FE slab (8-byte chunks) and the target linear ids (owned: prefix + l;
+// ribbon: owner table) -- every global load of a thread is issued before
+// its first LDS store. A null destination skips that input; counts beyond
+// the register rounds take plain loops.
+template <uint32_t T>
+__device__ __forceinline__ void stage_topology(uint32_t* ev_d, const uint32_t* __restrict__ ev_s, uint32_t ne,
+                                               uint16_t* fe_d, const uint16_t* __restrict__ fe_s, uint32_t nf,
+                                               uint32_t* lin_d, uint32_t n, uint32_t n_owned, uint32_t prefix,
+                                               const uint32_t* __restrict__ slab)
 {
-    for (uint32_t base = 0; base < n; base += 4u * T) {
-        uint32_t v[4];
+    constexpr uint32_t RE = 2, RF = 4, RL = 5;
+    const uint32_t     t   = threadIdx.x;
+    const uint32_t     n4  = ev_d ? (ne + 3u) >> 2 : 0u;
+    const uint32_t     n2  = fe_d ? (3u * nf + 3u) >> 2 : 0u;
+    const uint32_t     nl  = lin_d ? n : 0u;
+    const uint4*       es  = reinterpret_cast<const uint4*>(ev_s);
+    const uint2*       fs  = reinterpret_cast<const uint2*>(fe_s);
+    uint4*             ed  = reinterpret_cast<uint4*>(ev_d);
+    uint2*             fd  = reinterpret_cast<uint2*>(fe_d);
+    uint4              ea[RE];
+    uint2              fa[RF];
+    uint32_t           la[RL];
 #pragma unroll
-        for (uint32_t r = 0; r < 4u; ++r) {
-            const uint32_t l = base + threadIdx.x + r * T;
-            v[r]             = (l < n_owned) ? prefix + l : ((l < n) ? slab[l] : 0u);
-        }
+    for (uint32_t r = 0; r < RE; ++r) ea[r] = (t + r * T < n4) ? es[t + r * T] : uint4{};
 #pragma unroll
-        for (uint32_t r = 0; r < 4u; ++r) {
-            const uint32_t l = base + threadIdx.x + r * T;
-            if (l < n) dst[l] = v[r];
-        }
+    for (uint32_t r = 0; r < RF; ++r) fa[r] = (t + r * T < n2) ? fs[t + r * T] : uint2{};
+#pragma unroll
+    for (uint32_t r = 0; r < RL; ++r) {
+        const uint32_t l = t + r * T;
+        la[r]            = (l < n_owned) ? prefix + l : ((l < nl) ? slab[l] : 0u);
     }
+#pragma unroll
+    for (uint32_t r = 0; r < RE; ++r)
+        if (t + r * T < n4) ed[t + r * T] = ea[r];
+#pragma unroll
+    for (uint32_t r = 0; r < RF; ++r)
+        if (t + r * T < n2) fd[t + r * T] = fa[r];
+#pragma unroll
+    for (uint32_t r = 0; r < RL; ++r)
+        if (t + r * T < nl) lin_d[t + r * T] = la[r];
+    for (uint32_t i = t + RE * T; i < n4; i += T) ed[i] = es[i];
+    for (uint32_t i = t + RF * T; i < n2; i += T) fd[i] = fs[i];
+    for (uint32_t l = t + RL * T; l < nl; l += T) lin_d[l] = (l < n_owned) ? prefix + l : slab[l];
 }
 
 // Coalesced copy of n u32 from LDS to a global int32 run (thread i writes
@@ -74,13 +99,17 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
                                                  int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanQ2    L = plan_q2(b, IS_VV ? 0 : 3, d.nv, d.ne, d.nf);
-    stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    if (IS_VV) stage_lin(L.lin, d.nv, d.nvo, d.vp, m.v_lin + d.v_off, T);
-    else stage_lin(L.lin, d.ne, d.neo, d.ep, m.e_lin + d.e_off, T);
+    if (IS_VV)
+        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, L.lin, d.nv, d.nvo, d.vp,
+                          m.v_lin + d.v_off);
+    else
+        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, L.lin, d.ne, d.neo, d.ep,
+                          m.e_lin + d.e_off);
     __syncthreads();
     // entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
     fill_columns<T>(
@@ -163,8 +192,7 @@ __global__ __launch_bounds__(T) void k_query2_vf_local(DevMesh m, const uint32_t
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf);
-    stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, nullptr, 0, 0, 0, nullptr);
     __syncthreads();
     uint16_t* fo = fvl + fbase[p];
     for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T)
@@ -187,13 +215,13 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
                                                  int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf);
-    stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
-    stage_lin(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, T);
+    stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
+                      m.f_lin + d.f_off);
     __syncthreads();
     fill_columns<T>(
         3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
@@ -215,12 +243,13 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
                                                  int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanQ2    L = plan_q2(b, 2, d.nv, d.ne, d.nf);
-    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
-    stage_lin(L.lin, d.nf, d.nfo, d.fp, m.f_lin + d.f_off, T);
+    stage_topology<T>(nullptr, nullptr, 0, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
+                      m.f_lin + d.f_off);
     __syncthreads();
     fill_columns<T>(
         3 * d.nf, IS_FF ? d.ne : d.neo, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
