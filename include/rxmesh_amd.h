@@ -222,7 +222,10 @@ rxm_status rxm_cot_csr_assemble(rxm_mesh* mesh, const float* pos, float dt, int3
 rxm_status rxm_spmv3(rxm_mesh* mesh, const int32_t* row_ptr, const int32_t* col, const float* val,
                      const float* x, float* y, void* stream);
 /* `iters` explicit mean-curvature-flow steps x <- x + (M x - A x) / M with the
- * SpMV fused into the update; `tmp` is a 3V workspace when iters > 1. */
+ * SpMV fused into the update; `tmp` is a 3V workspace when iters > 1. A must
+ * be the mesh's cotangent system as rxm_cot_csr_assemble lays it out (values
+ * may be any): the steps run patch-local over the mesh's VV plan, which is
+ * that system's pattern, so `col` is not read. */
 rxm_status rxm_mcf_explicit(rxm_mesh* mesh, const int32_t* row_ptr, const int32_t* col,
                             const float* val, const float* mass, const float* x, float* out,
                             float* tmp, int iters, void* stream);

@@ -1064,14 +1064,23 @@ rxm_status rxm_mcf_explicit(rxm_mesh* m, const int32_t* row_ptr, const int32_t* 
     if (!m || !row_ptr || !col || !val || !mass || !x || !out || iters < 1 || x == out) return RXM_ERR_INVALID_ARG;
     if (iters > 1 && (!tmp || tmp == out || tmp == x)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    // patch-local steps over the VV plan (the system's pattern); ping-pong
+    // between out and tmp so that the last step writes out
+    int rc = ensure_vv_plan(m, stream);
+    if (rc) return rc;
+    const uint32_t* vv_base = nullptr;
+    if ((rc = device_base(m, RXM_OP_VV, &vv_base))) return rc;
+    Bump b(nullptr);
+    plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent, false);
+    b.alloc<float>(m->vv_max_nent + m->h.max_nvo);
     const float* src = x;
-    for (int it = 0; it < iters; ++it) {
+    for (int it = 0; it < iters && !rc; ++it) {
         float* dst = ((iters - 1 - it) % 2 == 0) ? out : tmp;
-        const int rc = launch_spmv3<true>(m, row_ptr, col, val, mass, src, dst, stream);
-        if (rc) return rc;
+        rc = launch_patches_xcd(k_mcf_patch<256>, 256, b.bytes(), stream, m, m->dm, vv_base, m->d_vv_rbase,
+                                m->d_vv_nbase, m->d_vv_roff, m->d_vv_nbr, val, mass, src, dst);
         src = dst;
     }
-    return RXM_OK;
+    return rc;
 }
 
 rxm_status rxm_symdir_rest(rxm_mesh* m, const float* rest_pos, float* Mr, void* stream)

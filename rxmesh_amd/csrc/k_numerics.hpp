@@ -459,6 +459,58 @@ __global__ __launch_bounds__(256) void k_spmv3(uint32_t n, const int32_t* __rest
     }
 }
 
+// ---- explicit MCF step, patch-local ------------------------------------------
+// y = x + (M x - A x) / M on the mesh's cotangent system (rxm_cot_csr_assemble
+// layout: each patch's rows are one contiguous run, diagonal first, then the
+// neighbours in VV query order). The pattern is the patch-local VV plan
+// (u16 row offsets and local neighbour ids, the same order), so a workgroup
+// stages its patch's positions (owned + ribbon), plan rows and the run of
+// values in LDS and the matrix's column indices are never read: a row's
+// neighbours come from LDS instead of one global gather per entry. Rows sum
+// the diagonal term first, then the neighbours in row order.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_mcf_patch(DevMesh m, const uint32_t* __restrict__ vv_base,
+                                                 const uint32_t* __restrict__ rbase,
+                                                 const uint32_t* __restrict__ nbase,
+                                                 const uint16_t* __restrict__ roff, const uint16_t* __restrict__ nbr,
+                                                 const float* __restrict__ val, const float* __restrict__ mass,
+                                                 const float* __restrict__ x, float* __restrict__ y)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d    = load_desc(m.patch, p);
+    const uint32_t  rb   = rbase[p], nbb = nbase[p], nent = nbase[p + 1] - nbb;
+    const uint32_t  bse  = vv_base[p] + d.vp;  // the patch's run in the system
+    const uint32_t  N    = m.nV;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanSL    L = plan_sl(b, d.nv, d.nvo, nent, false);
+    float*          w = b.alloc<float>(nent + d.nvo);
+    stage_u16x8<T>(L.ro, roff + rb, d.nvo + 1);
+    stage_u16x8<T>(L.nb, nbr + nbb, nent);
+    stage_words<T, 8>(reinterpret_cast<uint32_t*>(w), reinterpret_cast<const uint32_t*>(val) + bse, nent + d.nvo);
+    stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, x, N, d, m.v_lin);
+    __syncthreads();
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+        const float*   wr = w + k0 + v;  // diagonal, then the row's neighbours
+        const float    px = L.x[v], py = L.y[v], pz = L.z[v];
+        float          ax = wr[0] * px, ay = wr[0] * py, az = wr[0] * pz;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t q = L.nb[k];
+            const float    c = wr[1 + k - k0];
+            ax += c * L.x[q];
+            ay += c * L.y[q];
+            az += c * L.z[q];
+        }
+        const uint32_t g = d.vp + v;
+        const float    M = mass[g];
+        y[g]             = px + (M * px - ax) / M;
+        y[N + g]         = py + (M * py - ay) / M;
+        y[2 * N + g]     = pz + (M * pz - az) / M;
+    }
+}
+
 // ---- uniform smoothing over a materialised VV CSR ---------------------------
 // One Jacobi step x' = x + lambda (mean of row neighbours - x) over the CSR of
 // a VV query (linear ids). LANES lanes share a row; fixed xor-tree reduction.
