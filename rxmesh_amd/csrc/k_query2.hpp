@@ -110,9 +110,10 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
     else
         stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, L.lin, d.ne, d.neo, d.ep,
                           m.e_lin + d.e_off);
+    for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
     // entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
-    fill_columns<T>(
+    fill_columns<T, true>(
         2 * d.ne, d.nvo,
         [&](uint32_t i) {
             const uint32_t x = L.ev[i >> 1];
@@ -154,8 +155,9 @@ __global__ __launch_bounds__(T) void k_query2_vv_local(DevMesh m, const uint32_t
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanQ2    L = plan_q2(b, 0, d.nv, d.ne, d.nf);
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
-    fill_columns<T>(
+    fill_columns<T, true>(
         2 * d.ne, d.nvo,
         [&](uint32_t i) {
             const uint32_t x = L.ev[i >> 1];
@@ -222,8 +224,9 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
     const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf);
     stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
                       m.f_lin + d.f_off);
+    for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
-    fill_columns<T>(
+    fill_columns<T, true>(
         3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
         [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bse = base[p];
@@ -250,8 +253,9 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
     const PlanQ2    L = plan_q2(b, 2, d.nv, d.ne, d.nf);
     stage_topology<T>(nullptr, nullptr, 0, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
                       m.f_lin + d.f_off);
+    for (uint32_t c = threadIdx.x; c <= (IS_FF ? d.ne : d.neo); c += T) L.off[c] = 0;
     __syncthreads();
-    fill_columns<T>(
+    fill_columns<T, true>(
         3 * d.nf, IS_FF ? d.ne : d.neo, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
         [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bse = base[p];

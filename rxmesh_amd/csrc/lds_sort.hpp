@@ -6,6 +6,20 @@
 
 namespace rxm {
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: row
+// shifts by 1, 2, 4, 8 within each 16-lane row (lanes without a source add
+// 0), then the row totals broadcast from lanes 15 and 31 into the rows above.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
+{
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x111, 0xf, 0xf, false);  // row_shr:1
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x112, 0xf, 0xf, false);  // row_shr:2
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x114, 0xf, 0xf, false);  // row_shr:4
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x118, 0xf, 0xf, false);  // row_shr:8
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x142, 0xa, 0xf, false);  // row_bcast:15
+    x += __builtin_amdgcn_update_dpp(0u, x, 0x143, 0xc, 0xf, false);  // row_bcast:31
+    return x;
+}
+
 template <uint32_t T>
 __device__ __forceinline__ uint32_t scan_excl(uint32_t* s, uint32_t n, uint32_t* s_wave)
 {
@@ -16,12 +30,7 @@ __device__ __forceinline__ uint32_t scan_excl(uint32_t* s, uint32_t n, uint32_t*
     __syncthreads();
     uint32_t sum = 0;
     for (uint32_t i = b; i < e; ++i) sum += s[i];
-    uint32_t x = sum;
-#pragma unroll
-    for (uint32_t d = 1; d < 64u; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    const uint32_t x = wave_incl_scan(sum);
     if (lane == 63u) s_wave[wid] = x;
     __syncthreads();
     uint32_t before = 0, total = 0;
@@ -92,12 +101,16 @@ __device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
 // start of column c and off[ncols] the total.
 // When n <= R * T the keys and slot values of a thread's entries are
 // computed once and held in registers between the two passes.
-template <uint32_t T, uint32_t R = 12, typename KeyF, typename PackF>
+// ZEROED: the caller already cleared off[0..ncols] before a barrier (saves
+// this function's clearing pass and its barrier).
+template <uint32_t T, bool ZEROED = false, uint32_t R = 12, typename KeyF, typename PackF>
 __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF key, PackF pack, uint32_t* off,
                                              uint32_t* slots, uint32_t* s_wave)
 {
-    for (uint32_t c = threadIdx.x; c <= ncols; c += T) off[c] = 0;
-    __syncthreads();
+    if constexpr (!ZEROED) {
+        for (uint32_t c = threadIdx.x; c <= ncols; c += T) off[c] = 0;
+        __syncthreads();
+    }
     if (n <= R * T) {
         uint32_t kr[R], pr[R];
 #pragma unroll
