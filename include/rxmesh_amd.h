@@ -185,8 +185,13 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *   kernel 7: cotangent assembly workgroup size 256 / 512;
  *   kernel 8: smoothing, 1-6 = that many Jacobi steps per launch over the
  *             K-level plan (patch + rings of neighbours kept in LDS; overrides
- *             kernels 3 and 4), 0 = off.
- * Settings of kernels 0-4 and 6 give bit-identical results; kernel 5 changes
+ *             kernels 3 and 4), 0 = off;
+ *   kernel 9: smoothing with two steps per launch (kernel 4 = 1, float
+ *             arrays in LDS) and >= 4 iterations: 1 = the iterates between
+ *             launches are one float4 per vertex in two buffers the mesh
+ *             allocates on first use (32 V bytes; a gather then reads one
+ *             cache line instead of three), 0 = SoA between launches.
+ * Settings of kernels 0-4 and 6-9 give bit-identical results; kernel 5 changes
  * the summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 

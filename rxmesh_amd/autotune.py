@@ -68,7 +68,9 @@ def smooth_configs():
     for fused in SMOOTH_FUSED:
         for st in SMOOTH_THREADS:
             for lay in SMOOTH_LAYOUTS:
-                yield {"smooth_threads": st, "smooth_layout": lay, "smooth_fused": fused, "smooth_steps": 0}
+                for g4 in ((0, 1) if fused and lay == 0 else (0,)):
+                    yield {"smooth_threads": st, "smooth_layout": lay, "smooth_fused": fused, "smooth_steps": 0,
+                           "smooth_g4": g4}
     for steps in SMOOTH_STEPS:
         for st in SMOOTH_STEPS_THREADS:
             yield {"smooth_threads": st, "smooth_layout": 0, "smooth_fused": 0, "smooth_steps": steps}
@@ -77,7 +79,8 @@ def smooth_configs():
 def config_name(cfg):
     if cfg.get("smooth_steps", 0):
         return "t%d/k%d" % (cfg["smooth_threads"], cfg["smooth_steps"])
-    return "t%d/l%d/f%d" % (cfg["smooth_threads"], cfg["smooth_layout"], cfg["smooth_fused"])
+    return "t%d/l%d/f%d%s" % (cfg["smooth_threads"], cfg["smooth_layout"], cfg["smooth_fused"],
+                              "/g4" if cfg.get("smooth_g4", 0) else "")
 
 
 def apply(mesh, cfg):
@@ -93,6 +96,7 @@ def apply(mesh, cfg):
         mesh.set_launch_threads(4, cfg["smooth_fused"])
     if "smooth_threads" in cfg:
         mesh.set_launch_threads(8, cfg.get("smooth_steps", 0))
+        mesh.set_launch_threads(9, cfg.get("smooth_g4", 0))
 
 
 def tune_smooth(mesh, x, out, lam=0.5, reps=10, use_cache=True, iters=10):

@@ -58,6 +58,8 @@ struct rxm_mesh
     const uint16_t*   d_vvk_nbr   = nullptr;
     const uint32_t*   d_vvk_xlin  = nullptr;
     uint32_t          smooth_steps = 0;  // > 0: Jacobi steps per launch over the K-level plan
+    uint32_t          smooth_g4    = 1;  // two-step launches pass float4 iterates between them
+    float*            d_iter4      = nullptr;  // two float4 iterates (8 V floats), allocated on first use
     uint32_t          smooth_fused = 0;  // 1: two Jacobi steps per launch
     uint32_t          spmv_lanes   = 8;  // lanes per CSR row in the SpMV-3 kernels
     // patch-local face plan (FV table + VF rows of (face << 2 | corner))
@@ -482,6 +484,26 @@ int smooth_twice_t(rxm_mesh* m, float lambda, const float* src, float* dst, void
     }
 }
 
+// two Jacobi steps, SoA-float-array LDS layout, with the HBM layout of the
+// input / output positions chosen per launch (float4 iterates in between)
+template <bool GIN, bool GOUT>
+int smooth_twice_g(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
+{
+    Bump b(nullptr);
+    plan_s2(b, m->h.max_nv, m->vv2.max_nx, m->vv2.max_nent, false);
+    auto go = [&](auto kernel, uint32_t T) {
+        return launch_patches_xcd(kernel, T, b.bytes(), stream, m, m->dm, m->d_vv2_rbase, m->d_vv2_nbase,
+                                  m->d_vv2_xbase, m->d_vv2_xcnt, m->d_vv2_roff, m->d_vv2_nbr, m->d_vv2_xlin, lambda,
+                                  src, dst);
+    };
+    switch (m->smooth_threads) {
+        case 64: return go(k_smooth2<64, false, GIN, GOUT>, 64);
+        case 256: return go(k_smooth2<256, false, GIN, GOUT>, 256);
+        case 512: return go(k_smooth2<512, false, GIN, GOUT>, 512);
+        default: return go(k_smooth2<128, false, GIN, GOUT>, 128);
+    }
+}
+
 int smooth_twice(rxm_mesh* m, float lambda, const float* src, float* dst, void* stream)
 {
     return m->smooth_aos4 ? smooth_twice_t<true>(m, lambda, src, dst, stream)
@@ -891,6 +913,34 @@ rxm_status rxm_laplacian_smooth(rxm_mesh* m, const float* pos, float* out, float
     rc = ensure_vv_plan(m, stream);
     if (rc) return rc;
     const bool fused = m->smooth_fused && iters > 1 && ensure_vv2_plan(m) == RXM_OK;
+    if (fused && m->smooth_g4 && !m->smooth_aos4 && iters >= 4) {
+        // an odd step first (SoA -> SoA), then pairs whose iterates between
+        // launches are float4 rows in two mesh-owned buffers
+        const size_t n = m->h.nV;
+        if (!m->d_iter4) {
+            float* p = nullptr;
+            if (hipMalloc(&p, 8 * n * sizeof(float)) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+            m->allocs.push_back(p);
+            m->device_bytes += 8 * n * sizeof(float);
+            m->d_iter4 = p;
+        }
+        const float* src = pos;
+        if (iters % 2) {
+            if ((rc = smooth_once(m, lambda, pos, tmp, stream))) return rc;
+            src = tmp;
+        }
+        const int pairs = iters / 2;
+        float*    buf[2] = {m->d_iter4, m->d_iter4 + 4 * n};
+        for (int i = 0; i < pairs && !rc; ++i) {
+            const bool first = (i == 0), last = (i == pairs - 1);
+            float*     dst   = last ? out : buf[i & 1];
+            if (first) rc = smooth_twice_g<false, true>(m, lambda, src, dst, stream);
+            else if (last) rc = smooth_twice_g<true, false>(m, lambda, src, dst, stream);
+            else rc = smooth_twice_g<true, true>(m, lambda, src, dst, stream);
+            src = dst;
+        }
+        return rc;
+    }
     // launches: pairs of steps (fused) then a single one; the last writes out
     const int    launches = fused ? iters / 2 + iters % 2 : iters;
     const float* src      = pos;
@@ -937,6 +987,10 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
     }
     if (kernel == 7 && (threads == 256 || threads == 512)) {
         m->cot_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 9 && (threads == 0 || threads == 1)) {  // float4 iterates between two-step launches
+        m->smooth_g4 = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
     if (kernel == 8 && threads >= 0 && threads <= static_cast<int>(kSmoothLevels)) {

@@ -326,7 +326,9 @@ __device__ __forceinline__ float3 s2_update(const PlanS2& L, uint32_t v, bool se
 // (u16 x 8 vectors), the owner-table ids of the ribbon vertices and the
 // extra vertices' linear ids; (2) the position gathers. Then the LDS stores.
 // Counts beyond the register rounds take plain loops.
-template <uint32_t T, bool AOS4>
+// GIN: positions in HBM as one float4 per vertex (iterates between launches)
+// instead of SoA [3][N]; a gather then touches one cache line, not three.
+template <uint32_t T, bool AOS4, bool GIN = false>
 __device__ __forceinline__ void stage_s2(const PlanS2& L, const DevMesh& m, const PatchDesc& d,
                                          const uint16_t* __restrict__ ro_src, const uint16_t* __restrict__ nb_src,
                                          uint32_t nent, const uint32_t* __restrict__ xl, uint32_t nx,
@@ -358,23 +360,26 @@ __device__ __forceinline__ void stage_s2(const PlanS2& L, const DevMesh& m, cons
 #pragma unroll
     for (uint32_t r = 0; r < RX; ++r) gx[r] = (t + r * T < nx) ? xl[t + r * T] : 0u;
     // level 2
+    const float4* p4 = reinterpret_cast<const float4*>(pos);
+    auto          ld = [&](uint32_t gg, float& a, float& bb, float& c) {
+        if constexpr (GIN) {
+            const float4 q = p4[gg];
+            a              = q.x;
+            bb             = q.y;
+            c              = q.z;
+        } else {
+            a  = pos[gg];
+            bb = pos[N + gg];
+            c  = pos[2 * N + gg];
+        }
+    };
     float x[RV], y[RV], z[RV], xx[RX], xy[RX], xz[RX];
 #pragma unroll
-    for (uint32_t r = 0; r < RV; ++r) {
-        if (t + r * T < d.nv) {
-            x[r] = pos[g[r]];
-            y[r] = pos[N + g[r]];
-            z[r] = pos[2 * N + g[r]];
-        }
-    }
+    for (uint32_t r = 0; r < RV; ++r)
+        if (t + r * T < d.nv) ld(g[r], x[r], y[r], z[r]);
 #pragma unroll
-    for (uint32_t r = 0; r < RX; ++r) {
-        if (t + r * T < nx) {
-            xx[r] = pos[gx[r]];
-            xy[r] = pos[N + gx[r]];
-            xz[r] = pos[2 * N + gx[r]];
-        }
-    }
+    for (uint32_t r = 0; r < RX; ++r)
+        if (t + r * T < nx) ld(gx[r], xx[r], xy[r], xz[r]);
     auto put = [&](uint32_t l, float a, float bb, float c) {
         if constexpr (AOS4) {
             L.p4[l] = make_float4(a, bb, c, 0.f);
@@ -400,16 +405,20 @@ __device__ __forceinline__ void stage_s2(const PlanS2& L, const DevMesh& m, cons
     for (uint32_t i = t + R1 * T; i < nro; i += T) rd[i] = rs[i];
     for (uint32_t i = t + RN * T; i < nnb; i += T) nd[i] = ns[i];
     for (uint32_t l = t + RV * T; l < d.nv; l += T) {
-        const uint32_t gg = lin_of(l, d.nvo, d.vp, vl);
-        put(l, pos[gg], pos[N + gg], pos[2 * N + gg]);
+        float a, bb, c;
+        ld(lin_of(l, d.nvo, d.vp, vl), a, bb, c);
+        put(l, a, bb, c);
     }
     for (uint32_t i = t + RX * T; i < nx; i += T) {
-        const uint32_t gg = xl[i];
-        put(d.nv + i, pos[gg], pos[N + gg], pos[2 * N + gg]);
+        float a, bb, c;
+        ld(xl[i], a, bb, c);
+        put(d.nv + i, a, bb, c);
     }
 }
 
-template <uint32_t T, bool AOS4>
+// GIN / GOUT: input / output positions as one float4 per vertex (the
+// iterates between the launches of a multi-launch smoothing) instead of SoA.
+template <uint32_t T, bool AOS4, bool GIN = false, bool GOUT = false>
 __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __restrict__ rbase,
                                                const uint32_t* __restrict__ nbase, const uint32_t* __restrict__ xbase,
                                                const uint32_t* __restrict__ xcnt, const uint16_t* __restrict__ roff,
@@ -424,7 +433,7 @@ __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __rest
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanS2    L = plan_s2(b, d.nv, nx, nent, AOS4);
     const uint32_t  N = m.nV;
-    stage_s2<T, AOS4>(L, m, d, roff + rb, nbr + nb0, nent, xlin + xb, nx, pos);
+    stage_s2<T, AOS4, GIN>(L, m, d, roff + rb, nbr + nb0, nent, xlin + xb, nx, pos);
     __syncthreads();
     // step 1: every local vertex
     for (uint32_t v = threadIdx.x; v < d.nv; v += T) {
@@ -442,9 +451,13 @@ __global__ __launch_bounds__(T) void k_smooth2(DevMesh m, const uint32_t* __rest
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         const float3   r = s2_update<AOS4>(L, v, true, lambda);
         const uint32_t g = d.vp + v;
-        out[g]           = r.x;
-        out[N + g]       = r.y;
-        out[2 * N + g]   = r.z;
+        if constexpr (GOUT) {
+            reinterpret_cast<float4*>(out)[g] = make_float4(r.x, r.y, r.z, 0.f);
+        } else {
+            out[g]         = r.x;
+            out[N + g]     = r.y;
+            out[2 * N + g] = r.z;
+        }
     }
 }
 

@@ -187,7 +187,7 @@ def test_smoothing_launch_configs_agree():
     V, F, m, o = case("c1")
     x = lin_pos(m, V.astype(np.float32))
     from rxmesh_amd import autotune
-    for iters in (2, 3, 4):
+    for iters in (2, 3, 4, 7):
         outs = []
         for cfg in autotune.smooth_configs():
             autotune.apply(m, cfg)
