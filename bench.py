@@ -39,7 +39,7 @@ def parse():
     ap.add_argument("--profile", default=None,
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
                          "smooth_uniform, smooth_2step (with --smooth-fused 1), smooth_ksteps (with --smooth-steps k), "
-                         "vertex_normals, copy")
+                         "vertex_normals, gaussian_curvature, cot_assembly, mcf_step, copy")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
@@ -328,13 +328,17 @@ def main():
     torch.cuda.synchronize()
 
     if args.profile:
+        A_cot = m.cot_assemble(x, 1e-5) if args.profile in ("cot_assembly", "mcf_step") else None
         fns = {"query_vv": lambda: m.query("VV", row_ptr=vv_rp, col=vv_col),
+               "cot_assembly": lambda: m.cot_assemble(x, 1e-5, out=A_cot),
+               "mcf_step": lambda: m.mcf_explicit(A_cot, x, 1, out=out),
                "query_vf": lambda: m.query("VF", row_ptr=vf_rp, col=vf_col),
                "smooth_uniform": lambda: m.smooth(x, 0.5, 1, out=out),
                "smooth_2step": lambda: m.smooth(x, 0.5, 2, out=out, tmp=tmp),
                "smooth_ksteps": lambda: m.smooth(x, 0.5, max(1, args.smooth_steps or 1), out=out, tmp=tmp),
                "smooth_csr": lambda: m.smooth_csr((vv_rp, vv_col), x, 0.5, 1, out=out),
                "vertex_normals": lambda: m.vertex_normals(x, "max", out=out),
+               "gaussian_curvature": lambda: m.gaussian_curvature(x, out=(out[0], out[1])),
                "copy": lambda: out.copy_(x)}  # plain copy of 12 V bytes (counter calibration)
         fn = fns[args.profile]
         for _ in range(args.reps):
@@ -383,6 +387,10 @@ def main():
     kern["smooth_csr"] = (t_sc, 24 * V + 4 * E)
     t_vn = event_time_ms(lambda: m.vertex_normals(x, "max", out=out), reps)
     kern["vertex_normals"] = (t_vn, 24 * V + 4 * E + 6 * F)
+    gcs = (torch.empty(V, dtype=torch.float32, device="cuda"), torch.empty(V, dtype=torch.float32, device="cuda"))
+    m.gaussian_curvature(x, out=gcs)
+    t_gc = event_time_ms(lambda: m.gaussian_curvature(x, out=gcs), reps)
+    kern["gaussian_curvature"] = (t_gc, 12 * V + 4 * E + 6 * F + 8 * V)  # positions, topology, gc + mixed area
     kernels = {k: {"us": round(t * 1e3, 2), "bytes": b, "GB_s": round(b / (t * 1e-3) / 1e9, 1),
                    "frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k, (t, b) in kern.items()}
     # dominant kernel of the step (by time per step)

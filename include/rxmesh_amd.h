@@ -166,6 +166,16 @@ rxm_status rxm_query_csr(rxm_mesh* mesh, int op, int oriented, int32_t* row_ptr,
 rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float* normals,
                               void* stream);
 
+/* Gaussian curvature per vertex: gc = (2 pi - sum of the vertex's corner
+ * angles) / mixed area, and optionally the mixed (Voronoi / obtuse) area
+ * (`area_mixed` may be NULL). Replaces compute_gaussian_curvature
+ * (apps/GaussianCurvature/gaussian_curvature_kernel.cuh:10-70) followed by
+ * the for_each_vertex division (apps/GaussianCurvature/gaussian_curvature.cu
+ * :55-58); the serial reference is gaussian_curvature_ref.h:8-118. Each owned
+ * vertex sums its incident faces in a fixed order (no atomics). */
+rxm_status rxm_gaussian_curvature(rxm_mesh* mesh, const float* pos, float* gc, float* area_mixed,
+                                  void* stream);
+
 /* Build (once; later calls return at once) the mesh's patch-local VV plan:
  * the VV query evaluated by the query engine and kept in its native
  * patch-local u16 encoding for iterative consumers. rxm_laplacian_smooth

@@ -786,6 +786,56 @@ int64_t ora_global_relation(const ora_mesh* h, int op, int32_t* row_ptr, int32_t
 // ---------------------------------------------------------------------------
 // Geometry (global ids, AoS doubles).
 
+// Gaussian curvature, restating apps/GaussianCurvature/gaussian_curvature_ref.h
+// (serial reference, lines 8-118; the GPU lambda is gaussian_curvature_kernel.cuh
+// :19-62): per face, s = |(c1 - c0) x (c2 - c0)|, corner dots c[i], squared
+// edge lengths l[i] = |v_i - v_{i+1}|^2, angles atan2(s, c[i]); an obtuse face
+// gives its obtuse corner s/4 and the others s/8 of mixed area, otherwise
+// corner i gets (l[i+2] c[i+1] / s + l[i] c[i+2] / s) / 8; every corner's
+// angle is subtracted from 2 pi. gc = defect / mixed area.
+void ora_gaussian_curvature(const ora_mesh* h, const double* pos, double* gc, double* amix)
+{
+    const Mesh& m = *reinterpret_cast<const Mesh*>(h);
+    const double pi = 3.14159265358979323846;
+    for (uint32_t v = 0; v < m.nV; ++v) {
+        gc[v]   = 2.0 * pi;
+        amix[v] = 0.0;
+    }
+    for (uint32_t f = 0; f < m.nF; ++f) {
+        const double* P[3] = {pos + 3 * m.fv[f][0], pos + 3 * m.fv[f][1], pos + 3 * m.fv[f][2]};
+        auto sub = [](const double* a, const double* b, double* o) {
+            for (int k = 0; k < 3; ++k) o[k] = a[k] - b[k];
+        };
+        auto dot = [](const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; };
+        double e1[3], e2[3];
+        sub(P[1], P[0], e1);
+        sub(P[2], P[0], e2);
+        const double cx = e1[1] * e2[2] - e1[2] * e2[1], cy = e1[2] * e2[0] - e1[0] * e2[2],
+                     cz = e1[0] * e2[1] - e1[1] * e2[0];
+        const double s = std::sqrt(cx * cx + cy * cy + cz * cz);
+        double       c[3], l[3], rads[3];
+        bool         ob = false;
+        for (int i = 0; i < 3; ++i) {
+            double a[3], b[3], d[3];
+            sub(P[(i + 1) % 3], P[i], a);
+            sub(P[(i + 2) % 3], P[i], b);
+            c[i] = dot(a, b);
+            sub(P[i], P[(i + 1) % 3], d);
+            l[i]    = dot(d, d);
+            rads[i] = std::atan2(s, c[i]);
+            if (rads[i] > 0.5 * pi) ob = true;
+        }
+        for (int i = 0; i < 3; ++i) {
+            const int    i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+            const uint32_t v = m.fv[f][i];
+            if (ob) amix[v] += (rads[i] > 0.5 * pi) ? 0.25 * s : 0.125 * s;
+            else amix[v] += 0.125 * (l[i2] * (c[i1] / s) + l[i] * (c[i2] / s));
+            gc[v] -= rads[i];
+        }
+    }
+    for (uint32_t v = 0; v < m.nV; ++v) gc[v] /= amix[v];
+}
+
 // Vertex normals. mode 0 = Max (1999) weights, restating
 // apps/VertexNormal/vertex_normal_ref.h:6-85 (unnormalised sum of
 // n_f / (|e_v|^2 + |e_{v+2}|^2)); mode 1 = area weights (sum of n_f / 2).

@@ -56,6 +56,7 @@ def lib():
         L.ora_global_relation.restype = ctypes.c_int64
         L.ora_global_relation.argtypes = [vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.ora_vertex_normals.argtypes = [vp, _f64p, ctypes.c_int, _f64p]
+        L.ora_gaussian_curvature.argtypes = [vp, _f64p, _f64p, _f64p]
         L.ora_smooth_uniform.argtypes = [vp, _f64p, ctypes.c_double, ctypes.c_int, _f64p]
         L.ora_cot_assemble.restype = ctypes.c_int64
         L.ora_cot_assemble.argtypes = [vp, _f64p, ctypes.c_double, _i32p, _i32p, _f64p, _f64p]
@@ -220,6 +221,14 @@ class OracleMesh:
         out = np.zeros_like(pos)
         lib().ora_vertex_normals(self._h, pos.ravel(), 0 if mode == "max" else 1, out.reshape(-1))
         return out
+
+    def gaussian_curvature(self, pos):
+        """(gc, mixed area) per vertex, global ids."""
+        pos = _c(pos, np.float64).reshape(-1, 3)
+        gc = np.zeros(len(pos), np.float64)
+        am = np.zeros(len(pos), np.float64)
+        lib().ora_gaussian_curvature(self._h, pos.ravel(), gc, am)
+        return gc, am
 
     def smooth_uniform(self, pos, lam, iters):
         pos = _c(pos, np.float64).reshape(-1, 3)

@@ -49,6 +49,7 @@ _SIGS = {
     "rxm_mesh_create": [_vp, _u32, _u32, _u32, ctypes.c_char_p, _u32, ctypes.POINTER(_vp)],
     "rxm_mesh_create_with_patches": [_vp, _u32, _u32, _vp, _u32, _u32, ctypes.POINTER(_vp)],
     "rxm_mesh_destroy": [_vp],
+    "rxm_gaussian_curvature": [_vp, _vp, _vp, _vp, _vp],
     "rxm_mesh_info": [_vp, ctypes.POINTER(MeshInfo)],
     "rxm_mesh_array_size": [_vp, _i, ctypes.POINTER(ctypes.c_uint64)],
     "rxm_mesh_get_array": [_vp, _i, _vp, ctypes.c_uint64],
@@ -221,6 +222,17 @@ class Mesh:
         _check(lib().rxm_vertex_normals(self._h, 0 if mode == "max" else 1, _ptr(pos), _ptr(out),
                                         _stream(stream)), "rxm_vertex_normals")
         return out
+
+    def gaussian_curvature(self, pos, out=None, stream=None):
+        """(gc, mixed area) per vertex in linear order, from (3, V) positions."""
+        import torch
+        if out is None:
+            out = (torch.empty(self.nV, dtype=torch.float32, device=pos.device),
+                   torch.empty(self.nV, dtype=torch.float32, device=pos.device))
+        gc, am = out
+        _check(lib().rxm_gaussian_curvature(self._h, _ptr(pos), _ptr(gc), _ptr(am) if am is not None else None,
+                                            _stream(stream)), "rxm_gaussian_curvature")
+        return gc, am
 
     def smooth(self, pos, lam=0.5, iters=1, out=None, tmp=None, stream=None):
         import torch

@@ -872,13 +872,26 @@ rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* no
     if (prc == RXM_OK) {
         Bump b(nullptr);
         plan_vnp(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
-        return launch_patches(k_vertex_normals_plan<128>, 128, b.bytes(), stream, m, m->dm, mode, m->d_fp_fbase,
+        return launch_patches_xcd(k_vertex_normals_plan<128>, 128, b.bytes(), stream, m, m->dm, mode, m->d_fp_fbase,
                               m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, normals);
     }
     if (prc != RXM_ERR_PATCH_TOO_BIG) return prc;
     Bump b(nullptr);
     plan_vn(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
     return launch_patches(k_vertex_normals, kGeomThreads, b.bytes(), stream, m, m->dm, mode, pos, normals);
+}
+
+rxm_status rxm_gaussian_curvature(rxm_mesh* m, const float* pos, float* gc, float* area_mixed, void* stream)
+{
+    if (!m || !pos || !gc) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const int rc = ensure_face_plan(m, stream);
+    if (rc) return rc;
+    Bump b(nullptr);
+    plan_vnp(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
+    return launch_patches_xcd(k_gaussian_curvature<128>, 128, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
+                              m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, gc,
+                              area_mixed);
 }
 
 rxm_status rxm_vv_plan_build(rxm_mesh* m, void* stream)

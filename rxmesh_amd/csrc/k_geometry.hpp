@@ -640,7 +640,8 @@ __global__ __launch_bounds__(T) void k_vertex_normals_plan(DevMesh m, int mode, 
                                                            const float* __restrict__ pos, float* __restrict__ nrm)
 {
     extern __shared__ uint4 smem[];
-    const uint32_t  p = blockIdx.x;
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     const uint32_t  nb0 = nbase[p], nent = nbase[p + 1] - nb0;
     Bump            b(reinterpret_cast<char*>(smem));
@@ -676,6 +677,83 @@ __global__ __launch_bounds__(T) void k_vertex_normals_plan(DevMesh m, int mode, 
         nrm[g]           = sx;
         nrm[N + g]       = sy;
         nrm[2 * N + g]   = sz;
+    }
+}
+
+// ---- Gaussian curvature over the patch-local face plan -------------------------
+// Restates apps/GaussianCurvature/gaussian_curvature_kernel.cuh:19-62: per
+// face s = |(c1 - c0) x (c2 - c0)|, corner dots c[i], squared edge lengths
+// l[i] = |v_i - v_{i+1}|^2, angles atan2(s, c[i]); the mixed area of a corner
+// is s/4 (obtuse corner of an obtuse face), s/8 (other corners of an obtuse
+// face) or (l[i+2] c[i+1] / s + l[i] c[i+2] / s) / 8; the corner's angle is
+// subtracted from 2 pi (obtuse angle <=> negative corner dot product). The
+// reference scatters these with float atomics; here
+// each owned vertex sums its incident faces (VF row of the face plan,
+// ascending local face order) itself, so the result is deterministic and
+// needs no atomics. gc = defect / mixed area; amix (optional) the mixed area.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_gaussian_curvature(DevMesh m, const uint32_t* __restrict__ fbase,
+                                                          const uint32_t* __restrict__ rbase,
+                                                          const uint32_t* __restrict__ nbase,
+                                                          const uint16_t* __restrict__ fvl,
+                                                          const uint16_t* __restrict__ roff,
+                                                          const uint16_t* __restrict__ vf,
+                                                          const float* __restrict__ pos, float* __restrict__ gc,
+                                                          float* __restrict__ amix)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d   = load_desc(m.patch, p);
+    const uint32_t  nb0 = nbase[p], nent = nbase[p + 1] - nb0;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanVNP   L = plan_vnp(b, d.nv, d.nvo, d.nf, nent);
+    const uint32_t  N = m.nV;
+    stage_u16x8<T>(L.fvl, fvl + fbase[p], 3 * d.nf);
+    stage_u16x8<T>(L.ro, roff + rbase[p], d.nvo + 1);
+    stage_u16x8<T>(L.vf, vf + nb0, nent);
+    stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, pos, N, d, m.v_lin);
+    __syncthreads();
+    const float two_pi = 6.28318530717958647692f;
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        float g = two_pi, am = 0.f;
+        for (uint32_t k = L.ro[v]; k < L.ro[v + 1]; ++k) {
+            const uint32_t sl = L.vf[k], f = sl >> 2, i = sl & 3u;
+            uint32_t       q[3];
+            float          px[3], py[3], pz[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                q[j]  = L.fvl[3 * f + j];
+                px[j] = L.x[q[j]];
+                py[j] = L.y[q[j]];
+                pz[j] = L.z[q[j]];
+            }
+            const float e1x = px[1] - px[0], e1y = py[1] - py[0], e1z = pz[1] - pz[0];
+            const float e2x = px[2] - px[0], e2y = py[2] - py[0], e2z = pz[2] - pz[0];
+            const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+            const float s = sqrtf(cx * cx + cy * cy + cz * cz);
+            float       c[3], l[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int   j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+                const float ax = px[j1] - px[j], ay = py[j1] - py[j], az = pz[j1] - pz[j];
+                const float bx = px[j2] - px[j], by = py[j2] - py[j], bz = pz[j2] - pz[j];
+                c[j]           = ax * bx + ay * by + az * bz;
+                l[j]           = ax * ax + ay * ay + az * az;
+            }
+            // this vertex is corner i of face f; an angle exceeds pi / 2
+            // exactly when its corner dot product is negative, so only this
+            // corner's angle needs atan2 (the reference evaluates all three)
+            const uint32_t i1 = (i + 1u) % 3u, i2 = (i + 2u) % 3u;
+            const float    ri = atan2f(s, c[i]);
+            const bool     ob = (c[0] < 0.f) || (c[1] < 0.f) || (c[2] < 0.f);
+            if (ob) am += (c[i] < 0.f) ? 0.25f * s : 0.125f * s;
+            else am += 0.125f * (l[i2] * (c[i1] / s) + l[i] * (c[i2] / s));
+            g -= ri;
+        }
+        const uint32_t gg = d.vp + v;
+        gc[gg]            = g / am;
+        if (amix) amix[gg] = am;
     }
 }
 

@@ -61,9 +61,12 @@ __device__ inline void oriented_rows(const DevMesh& m, const PatchDesc& d, const
     stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
     for (uint32_t v = threadIdx.x; v < d.nv; v += T) L.gid[v] = m.ltog_v[d.v_off + v];
     for (uint32_t i = threadIdx.x; i < d.ne; i += T) reinterpret_cast<uint32_t*>(L.hn)[i] = 0xFFFFFFFFu;
+    for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
+    // the successor table is complete before the rows are ordered: the
+    // counting sort below contains block barriers
     build_half_next(L.fe, L.ev, d.nf, L.hn);
-    fill_columns<T>(
+    fill_columns<T, true>(
         2 * d.ne, d.nvo,
         [&](uint32_t i) {
             const uint32_t x = L.ev[i >> 1];

@@ -213,3 +213,30 @@ def test_symdir_flipped_face():
     torch.cuda.synchronize()
     gE = _glob(m, energy.cpu().numpy(), "f_l2g")
     assert gE[1] == np.finfo(np.float32).max and np.isfinite(gE[0])
+
+
+@pytest.mark.parametrize("name", ["ico4", "ico5_perm", "torus", "disk"])
+def test_gaussian_curvature_vs_fp64(name):
+    """Gaussian curvature against the FP64 restatement of the reference's
+    serial version. The angle defect 2 pi - sum(angles) cancels in fp32, so
+    it is compared absolutely (1e-5 rad: a handful of fp32 angle roundings);
+    the mixed area relatively (1e-5); gc = defect / area follows. Two calls
+    are bitwise identical (no atomics)."""
+    import torch
+    V, F, m, o = _mesh(name)
+    V32 = V.astype(np.float32)
+    gc, am = m.gaussian_curvature(_lin(m, V32))
+    gc2, am2 = m.gaussian_curvature(_lin(m, V32))
+    torch.cuda.synchronize()
+    assert torch.equal(gc, gc2) and torch.equal(am, am2)
+    g = _glob(m, gc.cpu().numpy().astype(np.float64))
+    a = _glob(m, am.cpu().numpy().astype(np.float64))
+    rg, ra = o.gaussian_curvature(V32.astype(np.float64))
+    assert np.abs(a - ra).max() <= 1e-5 * np.abs(ra).max()
+    assert np.abs(g * a - rg * ra).max() <= 1e-5
+    # Gauss-Bonnet on the closed meshes: the oracle's defects sum to 2 pi chi,
+    # the fp32 ones within the per-vertex bound times the vertex count
+    if name != "disk":
+        chi = 0 if name == "torus" else 2
+        assert abs((rg * ra).sum() - 2 * np.pi * chi) < 1e-9
+        assert abs((g * a).sum() - 2 * np.pi * chi) <= 1e-5 * len(g)

@@ -448,3 +448,63 @@ def test_smooth_symdir_step_is_the_composition():
     w2 = o.smooth_uniform(want, lam, 1)
     w2 = w2 - eta * o.symdir3_grad(Mr, w2)
     assert np.allclose(o.smooth_symdir_steps(Mr, x, lam, eta, 2), w2, rtol=0, atol=1e-14)
+
+
+# ---- (f) Gaussian curvature -------------------------------------------------
+def _areas(pos, fv):
+    a, b, c = pos[fv[:, 0]], pos[fv[:, 1]], pos[fv[:, 2]]
+    return 0.5 * np.linalg.norm(np.cross(b - a, c - a), axis=1)
+
+
+def test_gaussian_curvature_icosahedron_known_answer():
+    """Regular icosahedron: every vertex has five equilateral corners, so its
+    angle defect is 2 pi - 5 pi / 3 = pi / 3 and its mixed area five thirds
+    of one face."""
+    pos, fv = mg.icosphere(0)
+    gc, am = OracleMesh(fv).gaussian_curvature(pos)
+    A = _areas(pos, fv)[0]
+    assert np.allclose(am, 5 * A / 3, rtol=1e-12)
+    assert np.allclose(gc, (math.pi / 3) / (5 * A / 3), rtol=1e-12)
+
+
+def test_gaussian_curvature_obtuse_triangle():
+    """One obtuse triangle: the obtuse corner gets a quarter of |cross| (half
+    the area), the others an eighth; each defect is 2 pi minus its angle."""
+    pos = np.array([[0.0, 0.0, 0.0], [4.0, 0.0, 0.0], [2.0, 0.5, 0.0]])
+    fv = np.array([[0, 1, 2]], np.uint32)
+    gc, am = OracleMesh(fv).gaussian_curvature(pos)
+    s = 2.0 * _areas(pos, fv)[0]
+    assert np.allclose(am, [s / 8, s / 8, s / 4], rtol=1e-12)
+    ang = [math.atan2(0.5, 2.0), math.atan2(0.5, 2.0), math.pi - 2 * math.atan2(0.5, 2.0)]
+    assert np.allclose(gc * am, [2 * math.pi - a for a in ang], rtol=1e-12)
+
+
+@pytest.mark.parametrize("case", ["sphere", "torus", "disk"])
+def test_gaussian_curvature_area_partition_and_gauss_bonnet(case):
+    """Mixed areas partition the surface (each face's three corner terms sum
+    to its area, acute or obtuse); on a closed surface the angle defects sum
+    to 2 pi chi (Gauss-Bonnet), and on a flat disk interior defects vanish."""
+    if case == "sphere":
+        pos, fv = mg.icosphere(3, seed=1, sigma=0.02)
+        chi = 2
+    elif case == "torus":
+        pos, fv = mg.torus(40, 20)
+        chi = 0
+    else:
+        pos, fv = mg.disk(24, seed=0)
+        pos = pos.copy()
+        pos[:, 2] = 0.0
+        chi = None
+    gc, am = OracleMesh(fv).gaussian_curvature(pos)
+    assert abs(am.sum() - _areas(pos, fv).sum()) <= 1e-12 * _areas(pos, fv).sum()
+    defect = gc * am
+    if chi is not None:
+        assert abs(defect.sum() - 2 * math.pi * chi) < 1e-9
+    else:
+        deg = np.bincount(fv.ravel(), minlength=len(pos))
+        # interior vertices of the flat disk (as many faces as neighbours) have zero defect
+        o = OracleMesh(fv)
+        rp, col = o.global_relation("VV")
+        interior = np.array([deg[v] == rp[v + 1] - rp[v] for v in range(len(pos))])
+        assert interior.sum() > 0.5 * len(pos)
+        assert np.abs(defect[interior]).max() < 1e-12
