@@ -242,12 +242,21 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
         m.set_launch_threads(5, best)
         m.mcf_explicit(A, x, 50, out=y, tmp=tmp)  # patch-local steps over the VV plan
         t_m = event_time_ms(lambda: m.mcf_explicit(A, x, 50, out=y, tmp=tmp), max(3, reps // 10))
+        # implicit step with the reference MCF app's CG defaults (dt 10, |r|^2 < 1e-6, <= 100 iterations)
+        A10 = m.cot_assemble(x, 10.0)
+        _, cg_it, cg_r0, cg_r1 = m.mcf_implicit_cg(A10, x, 100, 1e-6, 0.0, out=y)
+        t_cg = event_time_ms(lambda: m.mcf_implicit_cg(A10, x, 100, 1e-6, 0.0, out=y, stats=False), 3)
+        cg_bytes = 8 * nnz + 4 * (V + 1) + 124 * V  # one CG iteration: A p with the p update, then x / r update
+        del A10
         out[name] = {"V": V, "F": F, "nnz": nnz,
                      "cot_csr_assembly": _entry(t_a, 12 * V + 4 * E + 6 * F + 8 * nnz + 4 * (V + 1) + 4 * V,
                                                 nnz, "Gnnz_s"),
                      "mcf_spmv3_iteration": _entry(t_m / 50, 8 * nnz + 4 * (V + 1) + 12 * V + 4 * V + 12 * V,
                                                    nnz, "Gnnz_s"),
                      "mcf_50_iterations_ms": round(t_m, 4),
+                     "mcf_implicit_cg": {"ms": round(t_cg, 4), "iterations": cg_it, "start_residual": cg_r0,
+                                         "final_residual": cg_r1,
+                                         "per_iteration": _entry(t_cg / max(cg_it + 1, 1), cg_bytes, nnz, "Gnnz_s")},
                      "spmv3_csr": _entry(sweep[best], 8 * nnz + 4 * (V + 1) + 12 * V + 12 * V, nnz, "Gnnz_s"),
                      "spmv_lanes": best,
                      "spmv_lanes_sweep_us": {str(k): round(v * 1e3, 2) for k, v in sweep.items()}}

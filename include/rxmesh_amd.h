@@ -263,6 +263,23 @@ rxm_status rxm_symdir_energy(rxm_mesh* mesh, const float* Mr, const float* uv, f
 rxm_status rxm_symdir_grad(rxm_mesh* mesh, const float* Mr, const float* uv, const float* scale,
                            float* grad, void* stream);
 
+/* Implicit mean-curvature-flow step: solves A X = M x with conjugate
+ * gradients, restating the reference's MCF CG path (apps/MCF/mcf_cg.h:13-125,
+ * CGSolver include/rxmesh/matrix/cg_solver.h:37-125, B from mcf_B_setup
+ * apps/MCF/mcf_kernels.cuh:302-352): X starts at x, the three coordinate
+ * columns share alpha and beta (dots and norms over all 3V entries), and the
+ * solve stops when |r|^2 < abs_tol or |r|^2 / |r0|^2 < rel_tol, or after
+ * max_iter iterations. A is the mesh's cotangent system (rxm_cot_csr_assemble
+ * layout; col is not read). The loop runs on the device without host
+ * synchronisation; when any of iters / start_residual / final_residual is
+ * non-NULL the call waits for the stream and reports the iterations taken and
+ * the squared residual norms, as the reference's solver does. Work vectors
+ * (4 x 3V floats) are allocated by the mesh on first use. */
+rxm_status rxm_mcf_implicit_cg(rxm_mesh* mesh, const int32_t* row_ptr, const int32_t* col,
+                               const float* val, const float* mass, const float* x, float* out,
+                               int max_iter, float abs_tol, float rel_tol, int* iters,
+                               float* start_residual, float* final_residual, void* stream);
+
 /* Symmetric Dirichlet with 3-D positions as the variable (closed surfaces
  * have no UV domain): J = [b-a, c-a] Mr^{-1} is 3x2 and |J^{-1}|^2 becomes
  * |J|^2 / det(J^T J); equals the 2-D energy for planar input. Mr from

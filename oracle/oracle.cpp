@@ -999,6 +999,56 @@ void ora_mcf_explicit(int64_t nnz, const int32_t* rows, const int32_t* cols, con
     std::copy(x.begin(), x.end(), out);
 }
 
+// Implicit MCF step with conjugate gradients, restating the reference's
+// CGSolver (include/rxmesh/matrix/cg_solver.h:37-125) on A X = B, B = M x0
+// (apps/MCF/mcf_kernels.cuh:302-352): dots and norms over all 3V entries,
+// x starts at x0, stop when |r|^2 < abs_tol or |r|^2 / |r0|^2 < rel_tol.
+// COO matrix (rows, cols, vals) as ora_cot_assemble emits it; positions AoS.
+// Returns the iterations taken; res[0] = |r0|^2, res[1] = final |r|^2.
+int ora_mcf_cg(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals, const double* mass,
+               uint32_t nV, const double* x0, int max_iter, double abs_tol, double rel_tol, double* out,
+               double* res)
+{
+    const size_t n = 3ull * nV;
+    std::vector<double> x(x0, x0 + n), r(n), p(n), s(n);
+    auto matvec = [&](const std::vector<double>& a, std::vector<double>& y) {
+        std::fill(y.begin(), y.end(), 0.0);
+        for (int64_t k = 0; k < nnz; ++k)
+            for (int c = 0; c < 3; ++c) y[3 * rows[k] + c] += vals[k] * a[3 * cols[k] + c];
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double t = 0.0;
+        for (size_t i = 0; i < n; ++i) t += a[i] * b[i];
+        return t;
+    };
+    matvec(x, s);
+    for (uint32_t v = 0; v < nV; ++v)
+        for (int c = 0; c < 3; ++c) r[3 * v + c] = mass[v] * x0[3 * v + c] - s[3 * v + c];
+    p = r;
+    double delta = dot(r, r);
+    const double start = delta;
+    int it = 0;
+    auto converged = [&](double d) { return d < abs_tol || d / start < rel_tol; };
+    if (!converged(delta)) {
+        while (it < max_iter) {
+            matvec(p, s);
+            const double alpha = delta / dot(s, p);
+            for (size_t i = 0; i < n; ++i) x[i] += alpha * p[i];
+            for (size_t i = 0; i < n; ++i) r[i] -= alpha * s[i];
+            const double delta_old = delta;
+            delta = dot(r, r);
+            if (converged(delta)) break;
+            const double beta = delta / delta_old;
+            for (size_t i = 0; i < n; ++i) p[i] = r[i] + beta * p[i];
+            ++it;
+        }
+    }
+    std::copy(x.begin(), x.end(), out);
+    res[0] = start;
+    res[1] = delta;
+    return it;
+}
+
 // Rest shapes Mr (4 per face, column-major) from 3D rest positions.
 void ora_rest_shapes(const ora_mesh* h, const double* pos, double* Mr)
 {

@@ -57,6 +57,9 @@ def lib():
         L.ora_global_relation.argtypes = [vp, ctypes.c_int, ctypes.c_void_p, ctypes.c_void_p]
         L.ora_vertex_normals.argtypes = [vp, _f64p, ctypes.c_int, _f64p]
         L.ora_gaussian_curvature.argtypes = [vp, _f64p, _f64p, _f64p]
+        L.ora_mcf_cg.restype = ctypes.c_int
+        L.ora_mcf_cg.argtypes = [ctypes.c_int64, _i32p, _i32p, _f64p, _f64p, ctypes.c_uint32, _f64p,
+                                 ctypes.c_int, ctypes.c_double, ctypes.c_double, _f64p, _f64p]
         L.ora_smooth_uniform.argtypes = [vp, _f64p, ctypes.c_double, ctypes.c_int, _f64p]
         L.ora_cot_assemble.restype = ctypes.c_int64
         L.ora_cot_assemble.argtypes = [vp, _f64p, ctypes.c_double, _i32p, _i32p, _f64p, _f64p]
@@ -254,6 +257,16 @@ class OracleMesh:
                                _c(vals, np.float64), _c(mass, np.float64), x0.shape[0],
                                x0.ravel(), iters, out.reshape(-1))
         return out
+
+    def mcf_cg(self, rows, cols, vals, mass, x0, max_iter=100, abs_tol=1e-6, rel_tol=0.0):
+        """(x, iterations, start |r|^2, final |r|^2) of the reference's CG on A x = M x0."""
+        x0 = _c(x0, np.float64).reshape(-1, 3)
+        out = np.zeros_like(x0)
+        res = np.zeros(2, np.float64)
+        it = lib().ora_mcf_cg(len(rows), _c(rows, np.int32), _c(cols, np.int32), _c(vals, np.float64),
+                              _c(mass, np.float64), x0.shape[0], x0.ravel(), int(max_iter), float(abs_tol),
+                              float(rel_tol), out.reshape(-1), res)
+        return out, it, res[0], res[1]
 
     def rest_shapes(self, pos):
         pos = _c(pos, np.float64).reshape(-1, 3)

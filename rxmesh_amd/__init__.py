@@ -50,6 +50,7 @@ _SIGS = {
     "rxm_mesh_create_with_patches": [_vp, _u32, _u32, _vp, _u32, _u32, ctypes.POINTER(_vp)],
     "rxm_mesh_destroy": [_vp],
     "rxm_gaussian_curvature": [_vp, _vp, _vp, _vp, _vp],
+    "rxm_mcf_implicit_cg": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp, _vp, _vp, _vp],
     "rxm_mesh_info": [_vp, ctypes.POINTER(MeshInfo)],
     "rxm_mesh_array_size": [_vp, _i, ctypes.POINTER(ctypes.c_uint64)],
     "rxm_mesh_get_array": [_vp, _i, _vp, ctypes.c_uint64],
@@ -332,6 +333,23 @@ class Mesh:
         _check(lib().rxm_spmv3(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(x), _ptr(out), _stream(stream)),
                "rxm_spmv3")
         return out
+
+    def mcf_implicit_cg(self, A, x, max_iter=100, abs_tol=1e-6, rel_tol=0.0, out=None, stats=True, stream=None):
+        """Implicit MCF step: CG on A X = M x from X = x. Returns (X, iterations,
+        start |r|^2, final |r|^2) -- the last three None when stats is False
+        (no host synchronisation)."""
+        import torch
+        rp, col, val, mass = A
+        if out is None:
+            out = torch.empty_like(x)
+        it, r0, r1 = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
+        refs = (ctypes.byref(it), ctypes.byref(r0), ctypes.byref(r1)) if stats else (None, None, None)
+        _check(lib().rxm_mcf_implicit_cg(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(mass), _ptr(x), _ptr(out),
+                                         int(max_iter), float(abs_tol), float(rel_tol), *refs, _stream(stream)),
+               "rxm_mcf_implicit_cg")
+        if not stats:
+            return out, None, None, None
+        return out, it.value, r0.value, r1.value
 
     def mcf_explicit(self, A, x, iters, out=None, tmp=None, stream=None):
         import torch

@@ -59,6 +59,8 @@ struct rxm_mesh
     const uint32_t*   d_vvk_xlin  = nullptr;
     uint32_t          smooth_steps = 0;  // > 0: Jacobi steps per launch over the K-level plan
     uint32_t          smooth_g4    = 1;  // two-step launches pass float4 iterates between them
+    float*            d_cg         = nullptr;  // CG work vectors: r, p (two), s (3 V floats each), partials
+    CgState*          d_cg_state   = nullptr;
     float*            d_iter4      = nullptr;  // two float4 iterates (8 V floats), allocated on first use
     uint32_t          smooth_fused = 0;  // 1: two Jacobi steps per launch
     uint32_t          spmv_lanes   = 8;  // lanes per CSR row in the SpMV-3 kernels
@@ -1148,6 +1150,69 @@ rxm_status rxm_mcf_explicit(rxm_mesh* m, const int32_t* row_ptr, const int32_t* 
         src = dst;
     }
     return rc;
+}
+
+rxm_status rxm_mcf_implicit_cg(rxm_mesh* m, const int32_t* row_ptr, const int32_t* col, const float* val,
+                               const float* mass, const float* x, float* out, int max_iter, float abs_tol,
+                               float rel_tol, int* iters, float* start_residual, float* final_residual,
+                               void* stream)
+{
+    if (!m || !row_ptr || !col || !val || !mass || !x || !out || x == out || max_iter < 0)
+        return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    int rc = ensure_vv_plan(m, stream);
+    if (rc) return rc;
+    const uint32_t* vv_base = nullptr;
+    if ((rc = device_base(m, RXM_OP_VV, &vv_base))) return rc;
+    const uint32_t n = m->h.nV, P = m->h.P, nb = (n + 255u) / 256u;
+    const size_t   v3 = size_t(3) * n, npart = std::max<size_t>(P, nb);
+    if (!m->d_cg) {
+        float* p = nullptr;
+        if (hipMalloc(&p, (4 * v3 + npart) * sizeof(float)) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(p);
+        m->device_bytes += (4 * v3 + npart) * sizeof(float);
+        CgState* st = nullptr;
+        if (hipMalloc(&st, sizeof(CgState)) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(st);
+        m->d_cg       = p;
+        m->d_cg_state = st;
+    }
+    float *    r = m->d_cg, *pv[2] = {m->d_cg + v3, m->d_cg + 2 * v3}, *s = m->d_cg + 3 * v3, *part = m->d_cg + 4 * v3;
+    CgState*   st = m->d_cg_state;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    RXM_HIP(hipMemsetAsync(pv[0], 0, v3 * sizeof(float), hs));
+    Bump b(nullptr);
+    plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent, false);
+    b.alloc<float>(m->vv_max_nent + m->h.max_nvo);
+    b.alloc<float>(256 / 64);
+    const uint32_t lds = b.bytes();
+    auto apply = [&](auto kernel, const float* xin, float* pnew, float* y) {
+        return launch_patches_xcd(kernel, 256, lds, stream, m, m->dm, vv_base, m->d_vv_rbase, m->d_vv_nbase,
+                                  m->d_vv_roff, m->d_vv_nbr, val, xin, static_cast<const float*>(r), pnew, y, part,
+                                  static_cast<const CgState*>(st));
+    };
+    // set-up: s = A x0, r = M x0 - s, x = x0, delta = <r, r>
+    if ((rc = apply(k_cg_apply<256, 0>, x, nullptr, s))) return rc;
+    hipLaunchKernelGGL((k_cg_update<0>), dim3(nb), dim3(256), 0, hs, n, mass, x, out, r, nullptr, s, part, st);
+    hipLaunchKernelGGL((k_cg_scalar<0>), dim3(1), dim3(1024), 0, hs, nb, part, st, abs_tol, rel_tol);
+    for (int it = 0; it < max_iter; ++it) {
+        float* pold = pv[it & 1];
+        float* pnew = pv[(it + 1) & 1];
+        if ((rc = apply(k_cg_apply<256, 1>, pold, pnew, s))) return rc;
+        hipLaunchKernelGGL((k_cg_scalar<1>), dim3(1), dim3(1024), 0, hs, P, part, st, abs_tol, rel_tol);
+        hipLaunchKernelGGL((k_cg_update<1>), dim3(nb), dim3(256), 0, hs, n, mass, x, out, r, pnew, s, part, st);
+        hipLaunchKernelGGL((k_cg_scalar<2>), dim3(1), dim3(1024), 0, hs, nb, part, st, abs_tol, rel_tol);
+    }
+    RXM_HIP(hipGetLastError());
+    if (iters || start_residual || final_residual) {
+        CgState h{};
+        RXM_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, hs));
+        RXM_HIP(hipStreamSynchronize(hs));
+        if (iters) *iters = static_cast<int>(h.iters);
+        if (start_residual) *start_residual = h.start;
+        if (final_residual) *final_residual = h.delta;
+    }
+    return RXM_OK;
 }
 
 rxm_status rxm_symdir_rest(rxm_mesh* m, const float* rest_pos, float* Mr, void* stream)

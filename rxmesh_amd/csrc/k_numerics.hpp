@@ -480,7 +480,9 @@ __global__ __launch_bounds__(T) void k_mcf_patch(DevMesh m, const uint32_t* __re
     const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
     if (p >= m.P) return;
     const PatchDesc d    = load_desc(m.patch, p);
-    const uint32_t  rb   = rbase[p], nbb = nbase[p], nent = nbase[p + 1] - nbb;
+    // nbase runs are padded to 8 entries; the system's run is not: nent is
+    // the patch's true VV entry count
+    const uint32_t  rb   = rbase[p], nbb = nbase[p], nent = vv_base[p + 1] - vv_base[p];
     const uint32_t  bse  = vv_base[p] + d.vp;  // the patch's run in the system
     const uint32_t  N    = m.nV;
     Bump            b(reinterpret_cast<char*>(smem));
@@ -508,6 +510,186 @@ __global__ __launch_bounds__(T) void k_mcf_patch(DevMesh m, const uint32_t* __re
         y[g]             = px + (M * px - ax) / M;
         y[N + g]         = py + (M * py - ay) / M;
         y[2 * N + g]     = pz + (M * pz - az) / M;
+    }
+}
+
+// ---- implicit MCF: conjugate gradients on the cotangent system ---------------
+// Restates the reference's CGSolver (include/rxmesh/matrix/cg_solver.h:37-125)
+// on A X = B with B = M x (apps/MCF/mcf_kernels.cuh:302-352, mcf_B_setup) for
+// the three coordinate columns at once: dot products and norms run over all
+// 3V entries, as the reference's cuBLAS calls over the whole V x 3 matrix do.
+// The loop runs on the device: the scalars live in CgState, every kernel
+// returns at once when `done` is set, and the host launches max_iter
+// iterations without reading anything back. Partial sums are per workgroup
+// in a fixed order and reduced by one workgroup in a fixed order, so a solve
+// is deterministic.
+struct CgState
+{
+    float    delta, delta_old, alpha, beta, start;
+    uint32_t iters, done, pad;
+};
+
+// y = A x (MODE 0), or, for a CG iteration (MODE 1), p_new = r + beta p_old
+// staged for the patch's local vertices, y = A p_new, p_new written for the
+// owned rows and <y, p_new> summed per workgroup into partial[p]. The
+// pattern is the patch-local VV plan (the system's pattern; col is unused).
+template <uint32_t T, int MODE>
+__global__ __launch_bounds__(T) void k_cg_apply(DevMesh m, const uint32_t* __restrict__ vv_base,
+                                                const uint32_t* __restrict__ rbase,
+                                                const uint32_t* __restrict__ nbase,
+                                                const uint16_t* __restrict__ roff, const uint16_t* __restrict__ nbr,
+                                                const float* __restrict__ val, const float* __restrict__ x,
+                                                const float* __restrict__ r, float* __restrict__ p_new,
+                                                float* __restrict__ y, float* __restrict__ partial,
+                                                const CgState* __restrict__ st)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    if (MODE == 1 && st->done) return;
+    const PatchDesc d   = load_desc(m.patch, p);
+    // true VV entry count (nbase runs are padded to 8; the system's run is not)
+    const uint32_t  rb  = rbase[p], nbb = nbase[p], nent = vv_base[p + 1] - vv_base[p];
+    const uint32_t  bse = vv_base[p] + d.vp;
+    const uint32_t  N   = m.nV;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanSL    L = plan_sl(b, d.nv, d.nvo, nent, false);
+    float*          w = b.alloc<float>(nent + d.nvo);
+    float*          red = b.alloc<float>(T / 64u);
+    stage_u16x8<T>(L.ro, roff + rb, d.nvo + 1);
+    stage_u16x8<T>(L.nb, nbr + nbb, nent);
+    stage_words<T, 8>(reinterpret_cast<uint32_t*>(w), reinterpret_cast<const uint32_t*>(val) + bse, nent + d.nvo);
+    if constexpr (MODE == 0) {
+        stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, x, N, d, m.v_lin);
+    } else {
+        // x holds p_old here; stage p_new = r + beta p_old for every local vertex
+        const float     beta = st->beta;
+        const uint32_t* vl   = m.v_lin + d.v_off;
+        for (uint32_t l = threadIdx.x; l < d.nv; l += T) {
+            const uint32_t g = lin_of(l, d.nvo, d.vp, vl);
+            L.x[l]           = r[g] + beta * x[g];
+            L.y[l]           = r[N + g] + beta * x[N + g];
+            L.z[l]           = r[2 * N + g] + beta * x[2 * N + g];
+        }
+    }
+    __syncthreads();
+    float dotp = 0.f;
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t k0 = L.ro[v], k1 = L.ro[v + 1];
+        const float*   wr = w + k0 + v;  // diagonal, then the row's neighbours
+        const float    px = L.x[v], py = L.y[v], pz = L.z[v];
+        float          ax = wr[0] * px, ay = wr[0] * py, az = wr[0] * pz;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t q = L.nb[k];
+            const float    c = wr[1 + k - k0];
+            ax += c * L.x[q];
+            ay += c * L.y[q];
+            az += c * L.z[q];
+        }
+        const uint32_t g = d.vp + v;
+        y[g]             = ax;
+        y[N + g]         = ay;
+        y[2 * N + g]     = az;
+        if constexpr (MODE == 1) {
+            p_new[g]         = px;
+            p_new[N + g]     = py;
+            p_new[2 * N + g] = pz;
+            dotp += ax * px + ay * py + az * pz;
+        }
+    }
+    if constexpr (MODE == 1) {
+        // fixed-order workgroup sum: wave shuffles, then wave totals in order
+        for (uint32_t o = 32; o > 0; o >>= 1) dotp += __shfl_xor(dotp, o, 64);
+        if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = dotp;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float t = 0.f;
+            for (uint32_t i = 0; i < T / 64u; ++i) t += red[i];
+            partial[p] = t;
+        }
+    }
+}
+
+// Per linear vertex (MODE 0, set-up): r = b - s with b = mass * x0, and the
+// workgroup's partial <r, r>. MODE 1 (iteration): x += alpha p, r -= alpha s
+// and the partial <r, r>.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_cg_update(uint32_t n, const float* __restrict__ mass,
+                                                   const float* __restrict__ x0, float* __restrict__ x,
+                                                   float* __restrict__ r, const float* __restrict__ pvec,
+                                                   const float* __restrict__ s, float* __restrict__ partial,
+                                                   const CgState* __restrict__ st)
+{
+    __shared__ float red[4];
+    if (MODE == 1 && st->done) return;
+    const uint32_t i   = blockIdx.x * 256u + threadIdx.x;
+    float          acc = 0.f;
+    if (i < n) {
+        if constexpr (MODE == 0) {
+            const float M = mass[i];
+#pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) {
+                const float v = M * x0[c * n + i] - s[c * n + i];
+                r[c * n + i]  = v;
+                x[c * n + i]  = x0[c * n + i];
+                acc += v * v;
+            }
+        } else {
+            const float a = st->alpha;
+#pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) {
+                x[c * n + i] += a * pvec[c * n + i];
+                const float v = r[c * n + i] - a * s[c * n + i];
+                r[c * n + i]  = v;
+                acc += v * v;
+            }
+        }
+    }
+    for (uint32_t o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 64);
+    if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = red[0] + red[1] + red[2] + red[3];
+}
+
+// One workgroup: fixed-order sum of the partials, then the scalar step.
+//   STEP 0 (set-up): delta = start = sum; done if already converged.
+//   STEP 1 (after the SpMV): alpha = delta / sum.
+//   STEP 2 (after the update): delta_old = delta, delta = sum; done when
+//          delta < abs_tol or delta / start < rel_tol, else beta =
+//          delta / delta_old and one more iteration taken.
+template <int STEP>
+__global__ __launch_bounds__(1024) void k_cg_scalar(uint32_t n, const float* __restrict__ partial,
+                                                    CgState* __restrict__ st, float abs_tol, float rel_tol)
+{
+    __shared__ double s[1024];
+    if (STEP != 0 && st->done) return;
+    double acc = 0.0;
+    for (uint32_t i = threadIdx.x; i < n; i += 1024) acc += static_cast<double>(partial[i]);
+    s[threadIdx.x] = acc;
+    __syncthreads();
+    for (uint32_t k = 512; k > 0; k >>= 1) {
+        if (threadIdx.x < k) s[threadIdx.x] += s[threadIdx.x + k];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    const float sum = static_cast<float>(s[0]);
+    if constexpr (STEP == 0) {
+        st->delta = st->start = sum;
+        st->delta_old = 0.f;
+        st->alpha = st->beta = 0.f;
+        st->iters = 0;
+        st->done  = (sum < abs_tol || sum / sum < rel_tol) ? 1u : 0u;
+    } else if constexpr (STEP == 1) {
+        st->alpha = st->delta / sum;
+    } else {
+        st->delta_old = st->delta;
+        st->delta     = sum;
+        if (sum < abs_tol || sum / st->start < rel_tol) {
+            st->done = 1u;
+        } else {
+            st->beta = sum / st->delta_old;
+            st->iters += 1u;
+        }
     }
 }
 

@@ -508,3 +508,35 @@ def test_gaussian_curvature_area_partition_and_gauss_bonnet(case):
         interior = np.array([deg[v] == rp[v + 1] - rp[v] for v in range(len(pos))])
         assert interior.sum() > 0.5 * len(pos)
         assert np.abs(defect[interior]).max() < 1e-12
+
+
+# ---- (f) implicit MCF: the reference's CG on A x = M x0 ----------------------
+@pytest.mark.parametrize("case,dt", [("sphere", 1e-2), ("sphere", 10.0), ("torus", 1e-2)])
+def test_mcf_cg_matches_direct_solve(case, dt):
+    """Run to a tiny tolerance, the restated CG reaches the solution of
+    A x = M x0 that a sparse direct solve gives."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    pos, fv = mg.icosphere(2, seed=1, sigma=0.02) if case == "sphere" else mg.torus(24, 12)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    rows, cols, vals, mass = o.cot_assemble(pos, dt)
+    x, it, r0, r1 = o.mcf_cg(rows, cols, vals, mass, pos, 2000, 1e-26, 0.0)
+    A = sp.coo_matrix((vals, (rows, cols)), shape=(len(pos), len(pos))).tocsc()
+    xs = sla.spsolve(A, mass[:, None] * pos)
+    assert r1 < 1e-26 and it > 0
+    assert np.abs(x - xs).max() <= 1e-9 * np.abs(xs).max()
+
+
+def test_mcf_cg_stopping_rules():
+    """dt = 0 gives A = M and b = M x0: x0 already solves it (iterations 0);
+    max_iter = 0 returns x0; a relative tolerance stops on |r|^2 / |r0|^2."""
+    pos, fv = mg.icosphere(2, seed=1, sigma=0.02)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    rows, cols, vals, mass = o.cot_assemble(pos, 0.0)
+    x, it, r0, r1 = o.mcf_cg(rows, cols, vals, mass, pos, 100, 1e-20, 0.0)
+    assert it == 0 and r0 < 1e-20 and np.array_equal(x, pos)
+    rows, cols, vals, mass = o.cot_assemble(pos, 1.0)
+    x, it, r0, r1 = o.mcf_cg(rows, cols, vals, mass, pos, 0, 0.0, 0.0)
+    assert it == 0 and np.array_equal(x, pos)
+    x, it, r0, r1 = o.mcf_cg(rows, cols, vals, mass, pos, 1000, 0.0, 1e-6)
+    assert r1 / r0 < 1e-6 and it > 0
