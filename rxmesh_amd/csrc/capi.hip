@@ -1180,7 +1180,6 @@ rxm_status rxm_mcf_implicit_cg(rxm_mesh* m, const int32_t* row_ptr, const int32_
     float *    r = m->d_cg, *pv[2] = {m->d_cg + v3, m->d_cg + 2 * v3}, *s = m->d_cg + 3 * v3, *part = m->d_cg + 4 * v3;
     CgState*   st = m->d_cg_state;
     hipStream_t hs = static_cast<hipStream_t>(stream);
-    RXM_HIP(hipMemsetAsync(pv[0], 0, v3 * sizeof(float), hs));
     Bump b(nullptr);
     plan_sl(b, m->h.max_nv, m->h.max_nvo, m->vv_max_nent, false);
     b.alloc<float>(m->vv_max_nent + m->h.max_nvo);
@@ -1198,7 +1197,9 @@ rxm_status rxm_mcf_implicit_cg(rxm_mesh* m, const int32_t* row_ptr, const int32_
     for (int it = 0; it < max_iter; ++it) {
         float* pold = pv[it & 1];
         float* pnew = pv[(it + 1) & 1];
-        if ((rc = apply(k_cg_apply<256, 1>, pold, pnew, s))) return rc;
+        // the first iteration stages p = r (p_old unread), later ones r + beta p_old
+        rc = it == 0 ? apply(k_cg_apply<256, 1>, pold, pnew, s) : apply(k_cg_apply<256, 2>, pold, pnew, s);
+        if (rc) return rc;
         hipLaunchKernelGGL((k_cg_scalar<1>), dim3(1), dim3(1024), 0, hs, P, part, st, abs_tol, rel_tol);
         hipLaunchKernelGGL((k_cg_update<1>), dim3(nb), dim3(256), 0, hs, n, mass, x, out, r, pnew, s, part, st);
         hipLaunchKernelGGL((k_cg_scalar<2>), dim3(1), dim3(1024), 0, hs, nb, part, st, abs_tol, rel_tol);

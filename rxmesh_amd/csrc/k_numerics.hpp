@@ -529,10 +529,11 @@ struct CgState
     uint32_t iters, done, pad;
 };
 
-// y = A x (MODE 0), or, for a CG iteration (MODE 1), p_new = r + beta p_old
-// staged for the patch's local vertices, y = A p_new, p_new written for the
-// owned rows and <y, p_new> summed per workgroup into partial[p]. The
-// pattern is the patch-local VV plan (the system's pattern; col is unused).
+// y = A x (MODE 0, the set-up), or, for a CG iteration, p_new = r (MODE 1,
+// the first) or r + beta p_old (MODE 2) staged for the patch's local
+// vertices, y = A p_new, p_new written for the owned rows and <y, p_new>
+// summed per workgroup into partial[p]. The pattern is the patch-local VV
+// plan (the system's pattern; col is unused).
 template <uint32_t T, int MODE>
 __global__ __launch_bounds__(T) void k_cg_apply(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                 const uint32_t* __restrict__ rbase,
@@ -546,7 +547,7 @@ __global__ __launch_bounds__(T) void k_cg_apply(DevMesh m, const uint32_t* __res
     extern __shared__ uint4 smem[];
     const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
     if (p >= m.P) return;
-    if (MODE == 1 && st->done) return;
+    if (MODE != 0 && st->done) return;
     const PatchDesc d   = load_desc(m.patch, p);
     // true VV entry count (nbase runs are padded to 8; the system's run is not)
     const uint32_t  rb  = rbase[p], nbb = nbase[p], nent = vv_base[p + 1] - vv_base[p];
@@ -562,14 +563,20 @@ __global__ __launch_bounds__(T) void k_cg_apply(DevMesh m, const uint32_t* __res
     if constexpr (MODE == 0) {
         stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, x, N, d, m.v_lin);
     } else {
-        // x holds p_old here; stage p_new = r + beta p_old for every local vertex
-        const float     beta = st->beta;
+        // x holds p_old here (unread in the first iteration)
+        const float     beta = MODE == 2 ? st->beta : 0.f;
         const uint32_t* vl   = m.v_lin + d.v_off;
         for (uint32_t l = threadIdx.x; l < d.nv; l += T) {
             const uint32_t g = lin_of(l, d.nvo, d.vp, vl);
-            L.x[l]           = r[g] + beta * x[g];
-            L.y[l]           = r[N + g] + beta * x[N + g];
-            L.z[l]           = r[2 * N + g] + beta * x[2 * N + g];
+            if constexpr (MODE == 1) {
+                L.x[l] = r[g];
+                L.y[l] = r[N + g];
+                L.z[l] = r[2 * N + g];
+            } else {
+                L.x[l] = r[g] + beta * x[g];
+                L.y[l] = r[N + g] + beta * x[N + g];
+                L.z[l] = r[2 * N + g] + beta * x[2 * N + g];
+            }
         }
     }
     __syncthreads();
@@ -590,14 +597,14 @@ __global__ __launch_bounds__(T) void k_cg_apply(DevMesh m, const uint32_t* __res
         y[g]             = ax;
         y[N + g]         = ay;
         y[2 * N + g]     = az;
-        if constexpr (MODE == 1) {
+        if constexpr (MODE != 0) {
             p_new[g]         = px;
             p_new[N + g]     = py;
             p_new[2 * N + g] = pz;
             dotp += ax * px + ay * py + az * pz;
         }
     }
-    if constexpr (MODE == 1) {
+    if constexpr (MODE != 0) {
         // fixed-order workgroup sum: wave shuffles, then wave totals in order
         for (uint32_t o = 32; o > 0; o >>= 1) dotp += __shfl_xor(dotp, o, 64);
         if ((threadIdx.x & 63u) == 0) red[threadIdx.x >> 6] = dotp;
@@ -626,6 +633,9 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t n, const float* __re
     float          acc = 0.f;
     if (i < n) {
         if constexpr (MODE == 0) {
+            // b = M x0 rounded on its own (the reference stores B, then forms
+            // B - A x): no fused multiply-subtract here
+#pragma clang fp contract(off)
             const float M = mass[i];
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {

@@ -282,3 +282,30 @@ def test_mcf_implicit_cg_vs_direct_solve(name, dt, fwd_tol):
                               V.astype(np.float32).astype(np.float64), 500, 1e-12, 0.0)
     assert abs(it - it64) <= max(3, it64 // 5)
     assert abs(r0 - s0) <= 1e-4 * s0
+
+
+def test_mcf_implicit_cg_stopping_rules():
+    """The reference solver's exits, on the device: dt = 0 makes A = M, so x
+    solves the system and no iteration runs; max_iter = 0 returns x; a
+    relative tolerance stops on |r|^2 / |r0|^2 after as many iterations as
+    the FP64 restatement takes (within 20%); a solve without statistics (no
+    host synchronisation) writes the same bits as one with them."""
+    import torch
+    V, F, m, o = _mesh("torus")
+    x = _lin(m, V.astype(np.float32))
+    X, it, r0, r1 = m.mcf_implicit_cg(m.cot_assemble(x, 0.0), x, max_iter=50, abs_tol=1e-20)
+    assert it == 0 and r0 == 0.0 and torch.equal(X, x)
+    A = m.cot_assemble(x, 1e-2)
+    X, it, r0, r1 = m.mcf_implicit_cg(A, x, max_iter=0, abs_tol=0.0)
+    assert it == 0 and torch.equal(X, x) and r1 == r0 > 0
+    X, it, r0, r1 = m.mcf_implicit_cg(A, x, max_iter=500, abs_tol=0.0, rel_tol=1e-6)
+    assert 0 < it < 500 and r1 / r0 < 1e-6
+    Y, _, _, _ = m.mcf_implicit_cg(A, x, max_iter=500, abs_tol=0.0, rel_tol=1e-6, stats=False)
+    torch.cuda.synchronize()
+    assert torch.equal(X, Y)
+    rp, col, val, mass = (t.cpu().numpy() for t in A)
+    rows = np.repeat(np.arange(len(rp) - 1), np.diff(rp))
+    l2g = m.array("v_l2g")
+    _, it64, _, _ = o.mcf_cg(l2g[rows], l2g[col], val.astype(np.float64), _glob(m, mass.astype(np.float64)),
+                             V.astype(np.float32).astype(np.float64), 500, 0.0, 1e-6)
+    assert abs(it - it64) <= max(3, it64 // 5)
