@@ -248,6 +248,20 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
         t_cg = event_time_ms(lambda: m.mcf_implicit_cg(A10, x, 100, 1e-6, 0.0, out=y, stats=False), 3)
         cg_bytes = 8 * nnz + 4 * (V + 1) + 124 * V  # one CG iteration: A p with the p update, then x / r update
         del A10
+        # the EVDiamond query and the ARAP-style terms built on it (DiffARAP pieces)
+        rp_d = torch.empty(E + 1, dtype=torch.int32, device="cuda")
+        col_d = torch.empty(4 * E, dtype=torch.int32, device="cuda")
+        m.query("EVD", row_ptr=rp_d, col=col_d)
+        t_evd = event_time_ms(lambda: m.query("EVD", row_ptr=rp_d, col=col_d), reps)
+        gen = torch.Generator(device="cuda").manual_seed(0)
+        xd = x + 1e-3 * torch.randn(x.shape, generator=gen, device="cuda")
+        w_a = m.arap_weights(x)
+        R_a = m.arap_rotations(x, xd, w_a)
+        tot_a, g_a = m.arap_energy_grad(x, xd, w_a, R_a)
+        t_aw = event_time_ms(lambda: m.arap_weights(x, out=w_a), reps)
+        t_ar = event_time_ms(lambda: m.arap_rotations(x, xd, w_a, out=R_a), reps)
+        t_ae = event_time_ms(lambda: m.arap_energy_grad(x, xd, w_a, R_a, total=tot_a, grad=g_a), reps)
+        del rp_d, col_d, xd, w_a, R_a, tot_a, g_a
         out[name] = {"V": V, "F": F, "nnz": nnz,
                      "cot_csr_assembly": _entry(t_a, 12 * V + 4 * E + 6 * F + 8 * nnz + 4 * (V + 1) + 4 * V,
                                                 nnz, "Gnnz_s"),
@@ -258,6 +272,14 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
                                          "final_residual": cg_r1,
                                          "per_iteration": _entry(t_cg / max(cg_it + 1, 1), cg_bytes, nnz, "Gnnz_s")},
                      "spmv3_csr": _entry(sweep[best], 8 * nnz + 4 * (V + 1) + 12 * V + 12 * V, nnz, "Gnnz_s"),
+                     # EV 4E + FE 6F read; row_ptr 4(E+1) + 4 ids per edge written
+                     "evd_query": _entry(t_evd, 4 * E + 6 * F + 4 * (E + 1) + 16 * E, 4 * E, "Gentries_s"),
+                     # positions 12V + EV 4E + FE 6F read, one weight per edge written
+                     "arap_weights": _entry(t_aw, 12 * V + 4 * E + 6 * F + 4 * E, E, "Gedges_s"),
+                     # rest + deformed 24V, weights 4E, EV 4E, edge pattern 8E + 4V read; 9 floats per vertex written
+                     "arap_rotations": _entry(t_ar, 24 * V + 16 * E + 4 * V + 36 * V, V, "Gverts_s"),
+                     # as above plus rotations 36V read; gradient 12V written
+                     "arap_energy_grad": _entry(t_ae, 24 * V + 16 * E + 4 * V + 36 * V + 12 * V, E, "Gedges_s"),
                      "spmv_lanes": best,
                      "spmv_lanes_sweep_us": {str(k): round(v * 1e3, 2) for k, v in sweep.items()}}
         del A, y, tmp

@@ -44,7 +44,8 @@ enum {
  * (include/rxmesh/types.h:113-129). */
 enum {
     RXM_OP_VV = 3, RXM_OP_VE = 4, RXM_OP_VF = 5, RXM_OP_FV = 6,
-    RXM_OP_FE = 7, RXM_OP_FF = 8, RXM_OP_EV = 9, RXM_OP_EF = 11
+    RXM_OP_FE = 7, RXM_OP_FF = 8, RXM_OP_EV = 9, RXM_OP_EF = 11,
+    RXM_OP_EVD = 12 /* EVDiamond: 4 per edge [v0, opposite of v0->v1, v1, opposite of v1->v0], -1 if absent */
 };
 
 /* rxm_mesh_create flags. The default patcher is a recursive proportional
@@ -175,6 +176,28 @@ rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float*
  * vertex sums its incident faces in a fixed order (no atomics). */
 rxm_status rxm_gaussian_curvature(rxm_mesh* mesh, const float* pos, float* gc, float* area_mixed,
                                   void* stream);
+
+/* ARAP-style deformation terms, replacing the pieces of the reference's
+ * DiffARAP app (apps/DiffARAP). Positions are 3 x V SoA in linear order.
+ *
+ * Edge weights per linear edge id: w = max(0, (cot_q + cot_s) / 2) over the
+ * EVDiamond opposite vertices q, s (calc_edge_weights_mat,
+ * apps/DiffARAP/arap_kernels.h:8-54, which stores them in a symmetric
+ * SparseMatrix); 0 for edges with one face. */
+rxm_status rxm_arap_weights(rxm_mesh* mesh, const float* pos, float* w, void* stream);
+/* Best rotation per vertex (calculate_rotation_matrix, arap_kernels.h:60-117):
+ * the rotation maximising tr(R S), S = sum over the vertex's neighbours j of
+ * w (P_v - P_j)(X_v - X_j)^T, i.e. V U^T from S = U Sigma V^T with the
+ * reflection fix. R is nine SoA planes: R[(3 i + j) * V + v] = R_v(i, j). */
+rxm_status rxm_arap_rotations(rxm_mesh* mesh, const float* rest, const float* x, const float* w, float* R,
+                              void* stream);
+/* Energy sum over edges (i, j) in EV order (i the larger global id) of
+ * w |(X_i - X_j) - R_i (P_i - P_j)|^2 (the term of diff_arap.cu:87-118) into
+ * *total (device float), and its gradient with respect to X with the
+ * rotations held fixed, as the reference's term reads them, into grad
+ * (3 x V). The gradient is gathered per vertex without atomics. */
+rxm_status rxm_arap_energy_grad(rxm_mesh* mesh, const float* rest, const float* x, const float* w,
+                                const float* R, float* total, float* grad, void* stream);
 
 /* Build (once; later calls return at once) the mesh's patch-local VV plan:
  * the VV query evaluated by the query engine and kept in its native

@@ -309,7 +309,7 @@ void build_linear(Mesh& m)
 // the same fill gives rows in ascending source-local order, which is the
 // order restated here (stable counting sort).
 // Op numbering follows include/rxmesh/types.h:113-129.
-enum OpId { OP_VV = 3, OP_VE = 4, OP_VF = 5, OP_FV = 6, OP_FE = 7, OP_FF = 8, OP_EV = 9, OP_EF = 11 };
+enum OpId { OP_VV = 3, OP_VE = 4, OP_VF = 5, OP_FV = 6, OP_FE = 7, OP_FF = 8, OP_EV = 9, OP_EF = 11, OP_EVD = 12 };
 
 // stable transpose of a fixed-width row matrix (num_rows x width) of column ids
 void stable_transpose(uint32_t num_rows, uint32_t width, uint32_t num_cols,
@@ -344,14 +344,14 @@ int src_kind(int op)
 {
     switch (op) {
         case OP_VV: case OP_VE: case OP_VF: return 0;
-        case OP_EV: case OP_EF: return 1;
+        case OP_EV: case OP_EF: case OP_EVD: return 1;
         default: return 2;
     }
 }
 int dst_kind(int op)
 {
     switch (op) {
-        case OP_VV: case OP_FV: case OP_EV: return 0;
+        case OP_VV: case OP_FV: case OP_EV: case OP_EVD: return 0;
         case OP_VE: case OP_FE: return 1;
         default: return 2;
     }
@@ -453,6 +453,29 @@ PatchQuery patch_query(const Mesh& m, uint32_t p, int op, bool oriented)
     };
     switch (op) {
         case OP_EV: fixed(P.neo, 2, P.ev); break;
+        case OP_EVD: {
+            // reference e_v_diamond (rxmesh_queries.cuh:198-265): per edge
+            // [v0, opposite through the face running v0 -> v1, v1, opposite
+            // through the face running v1 -> v0]; a face slot's start vertex
+            // is opposite the face's next slot, and lands in position 1 + 2 dir
+            // of that slot's edge. Missing sides stay INVALID (0xFFFF); the
+            // serial face order makes the last face win where two faces run an
+            // edge the same way.
+            std::vector<uint16_t> out(4 * ne, 0xFFFF);
+            for (uint32_t e = 0; e < ne; ++e) {
+                out[4 * e]     = P.ev[2 * e];
+                out[4 * e + 2] = P.ev[2 * e + 1];
+            }
+            for (uint32_t f = 0; f < nf; ++f)
+                for (uint32_t k = 0; k < 3; ++k) {
+                    const uint16_t x  = P.fe[3 * f + k];
+                    const uint16_t vi = out[4 * (x >> 1) + 2 * (x & 1)];
+                    const uint16_t y  = P.fe[3 * f + (k + 1) % 3];
+                    out[4 * (y >> 1) + 1 + 2 * (y & 1)] = vi;
+                }
+            fixed(P.neo, 4, out);
+            break;
+        }
         case OP_FV: fixed(P.nfo, 3, fvl); break;
         case OP_FE: fixed(P.nfo, 3, fe_e); break;
         case OP_VE:
@@ -733,7 +756,7 @@ int ora_query(const ora_mesh* h, int op, int oriented, int32_t* row_ptr, int32_t
         for (uint32_t s = 0; s < nsrc; ++s) {
             row_ptr[prefix[p] + s] = static_cast<int32_t>(base + q.off[s]);
             for (uint32_t k = q.off[s]; k < q.off[s + 1]; ++k)
-                col[base + k] = static_cast<int32_t>(to_linear(m, p, dk, q.val[k]));
+                col[base + k] = (q.val[k] == 0xFFFFu) ? -1 : static_cast<int32_t>(to_linear(m, p, dk, q.val[k]));
         }
         base += q.off[nsrc];
     }
@@ -764,6 +787,18 @@ int64_t ora_global_relation(const ora_mesh* h, int op, int32_t* row_ptr, int32_t
             if (op == OP_FE) rel[f].push_back(e);
             if (op == OP_EF) rel[e].push_back(f);
         }
+    if (op == OP_EVD) {
+        // [a, c, b, d] per edge with a the larger global id: c closes the
+        // face holding half-edge a -> b, d the face holding b -> a (-1 when
+        // there is none; the last such face in face order otherwise)
+        for (const auto& kv : m.edges_map)
+            rel[kv.second] = {kv.first.first, INV32, kv.first.second, INV32};
+        for (uint32_t f = 0; f < m.nF; ++f)
+            for (int i = 0; i < 3; ++i) {
+                const uint32_t x = m.fv[f][i], y = m.fv[f][(i + 1) % 3], z = m.fv[f][(i + 2) % 3];
+                rel[m.edge_id(x, y)][x > y ? 1 : 3] = z;
+            }
+    }
     if (op == OP_FF)
         for (uint32_t e = 0; e < m.nE; ++e)
             for (uint32_t f0 : m.edge_faces[e])
@@ -771,7 +806,7 @@ int64_t ora_global_relation(const ora_mesh* h, int op, int32_t* row_ptr, int32_t
                     if (f0 != f1) rel[f0].push_back(f1);
     int64_t n = 0;
     for (uint32_t s = 0; s < nsrc; ++s) {
-        if (op != OP_FV && op != OP_FE && op != OP_EV)
+        if (op != OP_FV && op != OP_FE && op != OP_EV && op != OP_EVD)
             std::sort(rel[s].begin(), rel[s].end());
         if (row_ptr) row_ptr[s] = static_cast<int32_t>(n);
         for (uint32_t x : rel[s]) {

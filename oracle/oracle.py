@@ -14,7 +14,7 @@ _LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
 _lib = None
 
 OP = {"V": 0, "E": 1, "F": 2, "VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7,
-      "FF": 8, "EV": 9, "EE": 10, "EF": 11}
+      "FF": 8, "EV": 9, "EE": 10, "EF": 11, "EVD": 12}
 
 _u32p = np.ctypeslib.ndpointer(np.uint32, flags="C")
 _u16p = np.ctypeslib.ndpointer(np.uint16, flags="C")
@@ -308,6 +308,80 @@ def _smooth_symdir_steps(self, Mr, x, lam, eta, steps):
 OracleMesh.symdir3_energy = _symdir3
 OracleMesh.symdir3_grad = _symdir3_grad
 OracleMesh.smooth_symdir_steps = _smooth_symdir_steps
+
+
+# -- ARAP-style deformation (reference apps/DiffARAP), FP64, global ids -------
+def _arap_weights(self, pos):
+    """Per global edge: max(0, (cot_q + cot_s) / 2) over the EVDiamond
+    [p, q, r, s] (arap_kernels.h:8-54), cot at v = dot(p - v, r - v) /
+    |cross(p - v, r - v)|; 0 for edges lacking a side, which the reference
+    skips and leaves at the matrix's reset value."""
+    P = np.asarray(pos, np.float64).reshape(-1, 3)
+    _, col = self.global_relation("EVD")
+    d = col.reshape(-1, 4).astype(np.int64)
+    ok = (d[:, 1] >= 0) & (d[:, 3] >= 0)
+    dd = np.where(d < 0, 0, d)
+    p, q, r, s = P[dd[:, 0]], P[dd[:, 1]], P[dd[:, 2]], P[dd[:, 3]]
+
+    def cot(v):
+        u0, u1 = p - v, r - v
+        return (u0 * u1).sum(1) / np.linalg.norm(np.cross(u0, u1), axis=1)
+
+    with np.errstate(divide="ignore", invalid="ignore"):
+        w = (cot(q) + cot(s)) / 2
+    return np.where(ok, np.maximum(0.0, np.nan_to_num(w, nan=0.0)), 0.0)
+
+
+def _arap_covariances(self, rest, x, w):
+    """S_v = sum over edges (v, j) of w (P_v - P_j)(X_v - X_j)^T."""
+    P = np.asarray(rest, np.float64).reshape(-1, 3)
+    X = np.asarray(x, np.float64).reshape(-1, 3)
+    ev = self.ev().astype(np.int64)
+    e, ep = P[ev[:, 0]] - P[ev[:, 1]], X[ev[:, 0]] - X[ev[:, 1]]
+    outer = np.asarray(w, np.float64)[:, None, None] * e[:, :, None] * ep[:, None, :]
+    S = np.zeros((len(P), 3, 3))
+    np.add.at(S, ev[:, 0], outer)
+    np.add.at(S, ev[:, 1], outer)  # (P_j - P_v)(X_j - X_v)^T is the same product
+    return S
+
+
+def _arap_rotations(self, rest, x, w):
+    """Per vertex R = V U^T from S = U Sigma V^T (LAPACK SVD), with the sign
+    of U's column of the smallest singular value flipped when det < 0
+    (arap_kernels.h:60-117): the rotation maximising tr(R S). (V, 3, 3)."""
+    S = _arap_covariances(self, rest, x, w)
+    U, sig, Vt = np.linalg.svd(S)
+    R = np.einsum("vji,vkj->vik", Vt, U)  # V U^T
+    neg = np.linalg.det(R) < 0
+    if neg.any():
+        U2 = U[neg].copy()
+        U2[:, :, 2] *= -1.0  # numpy orders singular values descending
+        R[neg] = np.einsum("vji,vkj->vik", Vt[neg], U2)
+    return R
+
+
+def _arap_energy_grad(self, rest, x, w, R):
+    """E = sum over edges (i, j) = EV order (i the larger global id) of
+    w |(X_i - X_j) - R_i (P_i - P_j)|^2 (diff_arap.cu:87-118) and dE/dX
+    with the rotations held fixed."""
+    P = np.asarray(rest, np.float64).reshape(-1, 3)
+    X = np.asarray(x, np.float64).reshape(-1, 3)
+    R = np.asarray(R, np.float64).reshape(-1, 3, 3)
+    w = np.asarray(w, np.float64)
+    ev = self.ev().astype(np.int64)  # first-seen orientation; the EV query puts the larger id first
+    i, j = ev.max(1), ev.min(1)
+    d = (X[i] - X[j]) - np.einsum("eab,eb->ea", R[i], P[i] - P[j])
+    E = float((w * (d * d).sum(1)).sum())
+    g = np.zeros_like(X)
+    np.add.at(g, i, 2 * w[:, None] * d)
+    np.add.at(g, j, -2 * w[:, None] * d)
+    return E, g
+
+
+OracleMesh.arap_weights = _arap_weights
+OracleMesh.arap_covariances = _arap_covariances
+OracleMesh.arap_rotations = _arap_rotations
+OracleMesh.arap_energy_grad = _arap_energy_grad
 
 
 def read_patch_archive(path):

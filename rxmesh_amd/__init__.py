@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "_lib", "librxmesh_amd.so")
 
-OP = {"VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7, "FF": 8, "EV": 9, "EF": 11}
+OP = {"VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7, "FF": 8, "EV": 9, "EF": 11, "EVD": 12}
 STATUS = {0: "ok", 1: "invalid argument", 2: "non-manifold mesh", 3: "patch too big for LDS/16-bit ids",
           4: "I/O error", 5: "device error", 6: "unsupported", 7: "out of memory"}
 ARR = {"ev": 0, "face_patch": 1, "vertex_patch": 2, "edge_patch": 3, "patches_val": 4,
@@ -50,6 +50,9 @@ _SIGS = {
     "rxm_mesh_create_with_patches": [_vp, _u32, _u32, _vp, _u32, _u32, ctypes.POINTER(_vp)],
     "rxm_mesh_destroy": [_vp],
     "rxm_gaussian_curvature": [_vp, _vp, _vp, _vp, _vp],
+    "rxm_arap_weights": [_vp, _vp, _vp, _vp],
+    "rxm_arap_rotations": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "rxm_arap_energy_grad": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_mcf_implicit_cg": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp, _vp, _vp, _vp],
     "rxm_mesh_info": [_vp, ctypes.POINTER(MeshInfo)],
     "rxm_mesh_array_size": [_vp, _i, ctypes.POINTER(ctypes.c_uint64)],
@@ -234,6 +237,35 @@ class Mesh:
         _check(lib().rxm_gaussian_curvature(self._h, _ptr(pos), _ptr(gc), _ptr(am) if am is not None else None,
                                             _stream(stream)), "rxm_gaussian_curvature")
         return gc, am
+
+    # -- ARAP-style deformation ---------------------------------------------
+    def arap_weights(self, pos, out=None, stream=None):
+        """Cotangent edge weights (E,) in linear edge order, 0 on boundary edges."""
+        import torch
+        if out is None:
+            out = torch.empty(self.nE, dtype=torch.float32, device=pos.device)
+        _check(lib().rxm_arap_weights(self._h, _ptr(pos), _ptr(out), _stream(stream)), "rxm_arap_weights")
+        return out
+
+    def arap_rotations(self, rest, x, w, out=None, stream=None):
+        """Best rotation per vertex as (9, V): row-major R_v in column v."""
+        import torch
+        if out is None:
+            out = torch.empty(9, self.nV, dtype=torch.float32, device=x.device)
+        _check(lib().rxm_arap_rotations(self._h, _ptr(rest), _ptr(x), _ptr(w), _ptr(out), _stream(stream)),
+               "rxm_arap_rotations")
+        return out
+
+    def arap_energy_grad(self, rest, x, w, R, total=None, grad=None, stream=None):
+        """(energy as a 1-element device tensor, dE/dX as (3, V)), rotations fixed."""
+        import torch
+        if total is None:
+            total = torch.empty(1, dtype=torch.float32, device=x.device)
+        if grad is None:
+            grad = torch.empty_like(x)
+        _check(lib().rxm_arap_energy_grad(self._h, _ptr(rest), _ptr(x), _ptr(w), _ptr(R), _ptr(total), _ptr(grad),
+                                          _stream(stream)), "rxm_arap_energy_grad")
+        return total, grad
 
     def smooth(self, pos, lam=0.5, iters=1, out=None, tmp=None, stream=None):
         import torch

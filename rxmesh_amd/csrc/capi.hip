@@ -14,6 +14,7 @@
 #include "../../include/rxmesh_amd.h"
 #include "host_build.h"
 #include "k_geometry.hpp"
+#include "k_arap.hpp"
 #include "k_numerics.hpp"
 #include "k_query.hpp"
 #include "k_query2.hpp"
@@ -209,6 +210,7 @@ const std::vector<uint32_t>& op_base(rxm_mesh* m, int op)
                 }
                 break;
             case RXM_OP_EV: c = 2ull * h.neo[p]; break;
+            case RXM_OP_EVD: c = 4ull * h.neo[p]; break;
             case kOpCot:
                 for (uint32_t l = 0; l < h.nvo[p]; ++l) c += vval[h.ltog_v[h.v_off[p] + l]] + 1;
                 break;
@@ -521,7 +523,7 @@ int smooth_once(rxm_mesh* m, float lambda, const float* src, float* dst, void* s
 bool valid_op(int op)
 {
     return op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF || op == RXM_OP_FV ||
-           op == RXM_OP_FE || op == RXM_OP_FF || op == RXM_OP_EV || op == RXM_OP_EF;
+           op == RXM_OP_FE || op == RXM_OP_FF || op == RXM_OP_EV || op == RXM_OP_EF || op == RXM_OP_EVD;
 }
 
 }  // namespace
@@ -860,6 +862,9 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
         case RXM_OP_FE:
             plan_fx(b, h.max_ne, h.max_nf);
             return launch_patches(k_query_fixed<7>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
+        case RXM_OP_EVD:
+            plan_evd(b, h.max_ne, h.max_nf, h.max_neo);
+            return launch_patches_xcd(k_query_evd<T>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
         default: return RXM_ERR_INVALID_ARG;
     }
 }
@@ -894,6 +899,68 @@ rxm_status rxm_gaussian_curvature(rxm_mesh* m, const float* pos, float* gc, floa
     return launch_patches_xcd(k_gaussian_curvature<128>, 128, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
                               m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, gc,
                               area_mixed);
+}
+
+// ---- ARAP-style deformation -------------------------------------------------
+rxm_status rxm_arap_weights(rxm_mesh* m, const float* pos, float* w, void* stream)
+{
+    if (!m || !pos || !w) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    Bump b(nullptr);
+    plan_aw(b, m->h.max_nv, m->h.max_ne, m->h.max_nf, m->h.max_neo);
+    return launch_patches_xcd(k_arap_weights<256>, 256, b.bytes(), stream, m, m->dm, pos, w);
+}
+
+namespace {
+// the owned rows' edges (the VV edge pattern the cotangent assembly caches)
+int arap_rows(rxm_mesh* m, const uint32_t** vv_base, uint32_t* lds, bool rot)
+{
+    int rc = ensure_cot_plan(m);
+    if (!rc) rc = device_base(m, RXM_OP_VV, vv_base);
+    if (rc) return rc;
+    Bump b(nullptr);
+    plan_ar(b, m->h.max_nv, m->h.max_ne, m->h.max_nvo, op_max_patch(m, RXM_OP_VV), rot);
+    *lds = b.bytes();
+    return RXM_OK;
+}
+}  // namespace
+
+rxm_status rxm_arap_rotations(rxm_mesh* m, const float* rest, const float* x, const float* w, float* R,
+                              void* stream)
+{
+    if (!m || !rest || !x || !w || !R) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const uint32_t* vv_base = nullptr;
+    uint32_t        lds     = 0;
+    const int       rc      = arap_rows(m, &vv_base, &lds, false);
+    if (rc) return rc;
+    return launch_patches_xcd(k_arap_rotations<256>, 256, lds, stream, m, m->dm, vv_base,
+                              static_cast<const uint32_t*>(m->d_cot_rs), static_cast<const uint32_t*>(m->d_cot_ent),
+                              rest, x, w, R);
+}
+
+rxm_status rxm_arap_energy_grad(rxm_mesh* m, const float* rest, const float* x, const float* w, const float* R,
+                                float* total, float* grad, void* stream)
+{
+    if (!m || !rest || !x || !w || !R || !total || !grad) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const uint32_t* vv_base = nullptr;
+    uint32_t        lds     = 0;
+    int             rc      = arap_rows(m, &vv_base, &lds, true);
+    if (rc) return rc;
+    if (!m->d_partial) {
+        if (hipMalloc(&m->d_partial, sizeof(float) * m->h.P) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(m->d_partial);
+        m->device_bytes += sizeof(float) * m->h.P;
+    }
+    rc = launch_patches_xcd(k_arap_energy_grad<256>, 256, lds, stream, m, m->dm, vv_base,
+                            static_cast<const uint32_t*>(m->d_cot_rs), static_cast<const uint32_t*>(m->d_cot_ent),
+                            rest, x, w, R, m->d_partial, grad);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), m->h.P,
+                       static_cast<const float*>(m->d_partial), total);
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
 }
 
 rxm_status rxm_vv_plan_build(rxm_mesh* m, void* stream)

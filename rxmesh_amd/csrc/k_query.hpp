@@ -139,4 +139,59 @@ __global__ __launch_bounds__(kQueryThreads) void k_query_fixed(DevMesh m, int32_
     }
 }
 
+// EVDiamond (reference e_v_diamond, rxmesh_queries.cuh:198-265): per owned
+// edge [v0, the vertex closing the face that runs v0 -> v1, v1, the vertex
+// closing the face that runs v1 -> v0] as linear vertex ids (v0 = lo16 of EV,
+// the larger global id), -1 where the edge has no such face. Every local face
+// slot names its start vertex as the opposite of the face's next slot, which
+// lands in position 1 + 2 dir of that slot's edge. Owned edges' slots take a
+// u32 LDS atomicMax over (face + 1) << 16 | vertex: where two faces run an
+// edge the same way (non-manifold input) the larger local face index wins,
+// as in a serial pass over the faces, independent of scheduling.
+struct PlanEVD
+{
+    uint32_t* ev;
+    uint16_t* fe;
+    uint32_t* opp;
+};
+__host__ __device__ inline PlanEVD plan_evd(Bump& b, uint32_t ne, uint32_t nf, uint32_t neo)
+{
+    PlanEVD p{};
+    p.ev  = b.alloc<uint32_t>(r4(ne));
+    p.fe  = b.alloc<uint16_t>(r4(3 * nf));
+    p.opp = b.alloc<uint32_t>(2 * neo);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query_evd(DevMesh m, int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanEVD   L = plan_evd(b, d.ne, d.nf, d.neo);
+    for (uint32_t i = threadIdx.x; i < 2 * d.neo; i += T) L.opp[i] = 0u;
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i < 3 * d.nf; i += T) {
+        const uint32_t f = i / 3u, k = i - 3u * f;
+        const uint32_t y = L.fe[3u * f + (k == 2u ? 0u : k + 1u)];
+        if ((y >> 1) < d.neo) atomicMax(&L.opp[y], ((f + 1u) << 16) | face_vertex(L.fe, L.ev, f, k));
+    }
+    __syncthreads();
+    const uint32_t* vl = m.v_lin + d.v_off;
+    for (uint32_t e = threadIdx.x; e < d.neo; e += T) {
+        const uint32_t x = L.ev[e], r = d.ep + e, a = L.opp[2 * e], c = L.opp[2 * e + 1];
+        row_ptr[r]     = static_cast<int32_t>(4 * r);
+        col[4 * r]     = static_cast<int32_t>(lin_of(lo16(x), d.nvo, d.vp, vl));
+        col[4 * r + 1] = a ? static_cast<int32_t>(lin_of(a & 0xFFFFu, d.nvo, d.vp, vl)) : -1;
+        col[4 * r + 2] = static_cast<int32_t>(lin_of(hi16(x), d.nvo, d.vp, vl));
+        col[4 * r + 3] = c ? static_cast<int32_t>(lin_of(c & 0xFFFFu, d.nvo, d.vp, vl)) : -1;
+    }
+    if (p + 1 == m.P && threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(4 * (d.ep + d.neo));
+}
+
 }  // namespace rxm

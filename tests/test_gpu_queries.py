@@ -73,7 +73,7 @@ NAMES = ["sphere3", "torus", "plane", "cube", "bunnyhead", "c0", "c1", "c1_p128"
 
 
 @pytest.mark.parametrize("name", NAMES)
-@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("op", OPS + ["EVD"])
 def test_query_bit_exact(name, op):
     import torch
     V, F, m, o = case(name)
@@ -119,6 +119,24 @@ def test_oriented_paths_refuse_inconsistent_orientation():
     rp, col = m.query("VV")
     torch.cuda.synchronize()
     assert int(rp[-1]) == m.query_nnz("VV")
+
+
+def test_evdiamond_inconsistent_orientation():
+    """With one face traversed backwards, two faces run some edges the same
+    way: the EVDiamond slot takes the later face (serial order of the
+    reference's face pass), on the device as in the oracle."""
+    import rxmesh_amd as rx
+    import torch
+    V, F = mg.config_mesh("c1")
+    F = F.copy()
+    F[7] = F[7][::-1]
+    m = rx.Mesh(F, patch_size=512)
+    o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    rp, col = m.query("EVD")
+    torch.cuda.synchronize()
+    orp, ocol = o.query("EVD")
+    assert np.array_equal(rp.cpu().numpy(), orp)
+    assert np.array_equal(col.cpu().numpy(), ocol)
 
 
 def test_query_set_semantics_vs_bruteforce():

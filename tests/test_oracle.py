@@ -170,6 +170,56 @@ def test_queries_vs_bruteforce(name, op):
             assert sorted(got) == exp
 
 
+def _evd_rows(o, rp, col):
+    src, dst = o.linear_to_global(1), o.linear_to_global(0)
+    return {int(src[l]): [int(dst[c]) if c >= 0 else -1 for c in col[rp[l]:rp[l + 1]]]
+            for l in range(len(rp) - 1)}
+
+
+@pytest.mark.parametrize("name", ["sphere3", "torus", "disk"])
+def test_evdiamond_vs_bruteforce(name):
+    """EVDiamond (reference e_v_diamond, rxmesh_queries.cuh:198-265): per
+    edge [a, c, b, d], a the larger global id, (a, b, c) and (b, a, d) faces
+    of the input in their own orientation, -1 where the edge has no such
+    face. The patch restatement equals a one-pass ground truth over fv; the
+    ground truth itself is checked against the faces and the boundary."""
+    if name == "disk":
+        import rxmesh_amd as rx
+        V, F = mg.disk(24, seed=3)
+        m = rx.Mesh(F, patch_size=64, host_only=True)
+        o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+        assert m.P > 1
+    else:
+        V, F, a, o = oracle_from_archive(name)
+    rp, col = o.query("EVD")
+    assert np.array_equal(rp, 4 * np.arange(len(rp)))
+    got = _evd_rows(o, rp, col)
+    grp, gcol = o.global_relation("EVD")
+    assert len(got) == len(grp) - 1
+    for e, row in got.items():
+        assert row == gcol[grp[e]:grp[e + 1]].tolist()
+    faces = {tuple(np.roll(f, -i)) for f in F.tolist() for i in range(3)}
+    ev = o.ev()
+    nbnd = 0
+    for e, (a_, c, b, d) in got.items():
+        assert a_ == max(ev[e]) and b == min(ev[e])
+        assert c == -1 or (a_, b, c) in faces
+        assert d == -1 or (b, a_, d) in faces
+        assert not (c == -1 and d == -1)
+        nbnd += (c == -1) + (d == -1)
+    exp = 0 if name != "disk" else sum(1 for k in _edge_face_counts(F).values() if k == 1)
+    assert nbnd == exp and (name != "disk" or exp > 0)
+
+
+def _edge_face_counts(F):
+    cnt = {}
+    for f in F.tolist():
+        for i in range(3):
+            k = tuple(sorted((f[i], f[(i + 1) % 3])))
+            cnt[k] = cnt.get(k, 0) + 1
+    return cnt
+
+
 def test_fv_fe_ev_exact_sequences():
     V, F, a, o = oracle_from_archive("sphere3")
     ev = o.ev()
@@ -540,3 +590,111 @@ def test_mcf_cg_stopping_rules():
     assert it == 0 and np.array_equal(x, pos)
     x, it, r0, r1 = o.mcf_cg(rows, cols, vals, mass, pos, 1000, 0.0, 1e-6)
     assert r1 / r0 < 1e-6 and it > 0
+
+
+def _rotation(axis, angle):
+    a = np.asarray(axis, np.float64) / np.linalg.norm(axis)
+    K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
+    return np.eye(3) + math.sin(angle) * K + (1 - math.cos(angle)) * K @ K
+
+
+def test_arap_weights_match_cot_system():
+    """ARAP edge weights (cot_q + cot_s) / 2, clamped at 0 (DiffARAP
+    arap_kernels.h:8-54), against the separately pinned cotangent system,
+    whose off-diagonals are -dt (dot/area at q + dot/area at s) = -4 dt x
+    the same half-sum (area = |cross| / 2), away from its +-19.1 clamp."""
+    pos, fv = mg.icosphere(2, seed=1, sigma=0.03)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    w = o.arap_weights(pos)
+    rows, cols, vals, mass = o.cot_assemble(pos, 1.0)
+    off = {(int(r), int(c)): v for r, c, v in zip(rows, cols, vals) if r != c}
+    ev = o.ev()
+    for e, (a, b) in enumerate(ev.tolist()):
+        ref = -off[(a, b)] / 4.0
+        assert abs(w[e] - max(0.0, ref)) <= 1e-12 * max(1.0, abs(ref))
+    assert (w > 0).mean() > 0.9
+
+
+def test_arap_weights_zero_on_boundary():
+    pos, fv = mg.disk(12, seed=2)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    w = o.arap_weights(pos)
+    cnt = _edge_face_counts(fv)
+    ev = o.ev()
+    bnd = np.array([cnt[tuple(sorted(e))] == 1 for e in ev.tolist()])
+    assert bnd.any() and np.all(w[bnd] == 0.0) and np.all(w >= 0.0)
+
+
+def test_arap_rotations_and_energy_of_a_rigid_motion():
+    """X = Q P + t: the fitted rotation of every vertex is Q and the energy
+    and its gradient vanish; X = P gives the identity."""
+    pos, fv = mg.icosphere(2, seed=4, sigma=0.02)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    w = o.arap_weights(pos)
+    Q = _rotation([0.3, -1.0, 0.5], 0.9)
+    X = pos @ Q.T + np.array([0.2, -0.1, 0.4])
+    R = o.arap_rotations(pos, X, w)
+    assert np.abs(R - Q).max() <= 1e-10
+    E, g = o.arap_energy_grad(pos, X, w, R)
+    assert E <= 1e-20 and np.abs(g).max() <= 1e-10
+    R0 = o.arap_rotations(pos, pos, w)
+    assert np.abs(R0 - np.eye(3)).max() <= 1e-10
+
+
+def test_arap_rotations_are_the_best_rotations():
+    """For a non-rigid X, R is a proper rotation and no nearby rotation
+    (R times small rotations about the axes) increases tr(R S)."""
+    pos, fv = mg.icosphere(1, seed=5, sigma=0.02)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    w = o.arap_weights(pos)
+    rng = np.random.default_rng(0)
+    X = pos + 0.1 * rng.standard_normal(pos.shape)
+    R = o.arap_rotations(pos, X, w)
+    S = o.arap_covariances(pos, X, w)
+    assert np.abs(np.einsum("vij,vkj->vik", R, R) - np.eye(3)).max() <= 1e-12
+    assert np.all(np.linalg.det(R) > 0.999999)
+    best = np.einsum("vij,vji->v", R, S)
+    for ax in np.eye(3):
+        for t in (1e-3, -1e-3):
+            D = _rotation(ax, t)
+            assert np.all(np.einsum("vij,vji->v", D @ R, S) <= best + 1e-12)
+
+
+def test_arap_energy_takes_the_rotation_of_the_larger_id():
+    """The EV term's R_i is that of iter[0] of the EV query, the endpoint
+    with the larger global id (EV query rows are [larger, smaller]):
+    vertex V-1 is the larger end of all its edges, so turning its frame
+    changes the energy; vertex 0 is never the larger end, so turning its
+    frame changes nothing."""
+    pos, fv = mg.icosphere(1, seed=3, sigma=0.02)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    rp, col = o.query("EV")
+    l2g = o.linear_to_global(0)
+    pairs = l2g[col].reshape(-1, 2)
+    assert np.all(pairs[:, 0] > pairs[:, 1])
+    w = o.arap_weights(pos)
+    R = np.tile(np.eye(3), (len(pos), 1, 1))
+    E0, _ = o.arap_energy_grad(pos, pos, w, R)
+    for v, changes in ((len(pos) - 1, True), (0, False)):
+        R2 = R.copy()
+        R2[v] = _rotation([1.0, 0.0, 0.0], 0.5)
+        E1, _ = o.arap_energy_grad(pos, pos, w, R2)
+        assert (E1 - E0 > 1e-6) == changes
+    assert E0 <= 1e-24
+
+
+def test_arap_gradient_central_differences():
+    pos, fv = mg.icosphere(1, seed=6, sigma=0.02)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    w = o.arap_weights(pos)
+    rng = np.random.default_rng(1)
+    X = pos + 0.05 * rng.standard_normal(pos.shape)
+    R = o.arap_rotations(pos, X, w)
+    E, g = o.arap_energy_grad(pos, X, w, R)
+    h = 1e-6
+    for _ in range(5):
+        D = rng.standard_normal(pos.shape)
+        ep, _ = o.arap_energy_grad(pos, X + h * D, w, R)
+        em, _ = o.arap_energy_grad(pos, X - h * D, w, R)
+        fd = (ep - em) / (2 * h)
+        assert abs(fd - (g * D).sum()) <= 1e-6 * max(1.0, abs(fd))
