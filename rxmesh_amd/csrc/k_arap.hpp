@@ -140,6 +140,11 @@ __device__ inline void best_rotation(const float (&S)[3][3], float (&R)[3][3])
 #pragma unroll
             for (int j = 0; j < 4; ++j) a[i][j] *= inv;
         for (int sweep = 0; sweep < 8; ++sweep) {
+            // quadratic convergence: stop once the off-diagonal mass is below
+            // fp32 resolution of the unit-scaled matrix (3-4 sweeps typically)
+            const float off = a[0][1] * a[0][1] + a[0][2] * a[0][2] + a[0][3] * a[0][3] + a[1][2] * a[1][2] +
+                              a[1][3] * a[1][3] + a[2][3] * a[2][3];
+            if (off < 1e-15f) break;
             jacobi4<0, 1>(a, v);
             jacobi4<0, 2>(a, v);
             jacobi4<0, 3>(a, v);
