@@ -89,6 +89,20 @@ _SIGS = {
 }
 
 
+def _init_torch_runtime_first():
+    """torch carries its own copy of the HIP runtime next to the system one
+    this library links. With both in one process, torch finds no device if
+    the system runtime initialised first; torch first, both work. So when a
+    device is present, torch's runtime is initialised before the library is
+    loaded (device_count() itself initialises nothing)."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.device_count() > 0:
+        torch.cuda.init()
+
+
 def lib():
     """Load librxmesh_amd.so (raises if it has not been built)."""
     global _lib
@@ -96,6 +110,7 @@ def lib():
         if not os.path.exists(LIB_PATH):
             raise ImportError("librxmesh_amd.so not built: run `python -m rxmesh_amd.build` "
                               "(no CPU fallback exists)")
+        _init_torch_runtime_first()
         L = ctypes.CDLL(LIB_PATH)
         for name, argt in _SIGS.items():
             fn = getattr(L, name)

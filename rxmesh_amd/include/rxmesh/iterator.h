@@ -30,6 +30,7 @@ struct Iterator
     {
         if (i >= m_size) return HandleT();
         const uint16_t l = m_row[i];
+        if (l == INVALID16) return HandleT();  // e.g. the missing side of an EVDiamond row
         if (l < m_owned) return HandleT(m_patch, LocalT(l));
         const uint32_t q = m_own[l];
         return HandleT(q, LocalT(static_cast<uint16_t>(m_lin[l] - m_prefix[q])));

@@ -30,23 +30,25 @@ namespace detail {
 __host__ __device__ constexpr int op_src_kind(Op op)
 {
     return (op == Op::VV || op == Op::VE || op == Op::VF || op == Op::V) ? 0
-           : (op == Op::EV || op == Op::EF || op == Op::E)                ? 1
+           : (op == Op::EV || op == Op::EF || op == Op::EVDiamond || op == Op::E) ? 1
                                                                           : 2;
 }
 __host__ __device__ constexpr int op_dst_kind(Op op)
 {
-    return (op == Op::VV || op == Op::EV || op == Op::FV) ? 0 : (op == Op::VE || op == Op::FE) ? 1 : 2;
+    return (op == Op::VV || op == Op::EV || op == Op::FV || op == Op::EVDiamond) ? 0
+           : (op == Op::VE || op == Op::FE)                                        ? 1
+                                                                                    : 2;
 }
 // entries per source of the fixed-width relations (reference fixed_offset,
 // query_dispatcher.cuh:238-242); 0 = variable (CSR offsets in LDS)
 __host__ __device__ constexpr uint32_t op_fixed_width(Op op)
 {
-    return (op == Op::EV) ? 2u : ((op == Op::FV || op == Op::FE) ? 3u : 0u);
+    return (op == Op::EV) ? 2u : ((op == Op::FV || op == Op::FE) ? 3u : ((op == Op::EVDiamond) ? 4u : 0u));
 }
 __host__ __device__ constexpr bool op_supported(Op op)
 {
     return op == Op::VV || op == Op::VE || op == Op::VF || op == Op::FV || op == Op::FE || op == Op::FF ||
-           op == Op::EV || op == Op::EF;
+           op == Op::EV || op == Op::EF || op == Op::EVDiamond;
 }
 
 template <typename HandleT>
@@ -112,6 +114,7 @@ __host__ __device__ inline QueryLDS plan_query(rxm::Bump& b, Op op, uint32_t nv,
         case Op::VE: nval = 2u * ne; break;
         case Op::EV: nval = 2u * rxm::r4(ne); break;  // staged as whole 16-byte chunks of EV
         case Op::FF: nval = 3u * nf * (ef_max > 2u ? ef_max - 1u : 1u); break;
+        case Op::EVDiamond: nval = 4u * ne; break;
         default: nval = 3u * nf; break;
     }
     q.val           = b.alloc<uint16_t>(rxm::r4(nval));
@@ -144,6 +147,11 @@ __host__ __device__ inline QueryLDS plan_query(rxm::Bump& b, Op op, uint32_t nv,
             q.fe    = b.alloc<uint16_t>(rxm::r4(3u * nf));
             q.eoff  = b.alloc<uint32_t>(ne + 1u);
             q.slots = b.alloc<uint32_t>(3u * nf);
+            break;
+        case Op::EVDiamond:
+            q.ev    = b.alloc<uint32_t>(rxm::r4(ne));
+            q.fe    = b.alloc<uint16_t>(rxm::r4(3u * nf));
+            q.slots = b.alloc<uint32_t>(2u * ne);
             break;
         default: break;
     }
@@ -349,6 +357,28 @@ struct Query
                 const uint32_t o = m_off[e];
                 rxm::sort_row(L.slots + o, m_off[e + 1] - o,
                               [&](uint32_t j, uint32_t s) { m_val[o + j] = static_cast<uint16_t>(s); });
+            }
+        } else if constexpr (op == Op::EVDiamond) {
+            // [v0, opposite of v0 -> v1, v1, opposite of v1 -> v0] per local
+            // edge, INVALID16 for a missing side; the opposite slots take the
+            // u32 atomicMax of (face + 1) << 16 | vertex, so the later face
+            // wins where two run an edge the same way (as rxm_query_csr)
+            rxm::stage_u32(L.ev, m.ev + d.e_off, d.ne);
+            rxm::stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);
+            for (uint32_t i = threadIdx.x; i < 2u * d.ne; i += T) L.slots[i] = 0u;
+            block.sync();
+            for (uint32_t i = threadIdx.x; i < 3u * d.nf; i += T) {
+                const uint32_t f = i / 3u, k = i - 3u * f;
+                const uint32_t y = L.fe[3u * f + (k == 2u ? 0u : k + 1u)];
+                atomicMax(&L.slots[y], ((f + 1u) << 16) | rxm::face_vertex(L.fe, L.ev, f, k));
+            }
+            block.sync();
+            for (uint32_t e = threadIdx.x; e < d.ne; e += T) {
+                const uint32_t x = L.ev[e], a = L.slots[2u * e], c = L.slots[2u * e + 1u];
+                m_val[4u * e]      = static_cast<uint16_t>(rxm::lo16(x));
+                m_val[4u * e + 1u] = a ? static_cast<uint16_t>(a & 0xFFFFu) : INVALID16;
+                m_val[4u * e + 2u] = static_cast<uint16_t>(rxm::hi16(x));
+                m_val[4u * e + 3u] = c ? static_cast<uint16_t>(c & 0xFFFFu) : INVALID16;
             }
         } else if constexpr (op == Op::FF) {
             rxm::stage_fe(L.fe, m.fe + 3u * d.f_off, d.nf);

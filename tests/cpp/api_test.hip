@@ -48,6 +48,39 @@ __global__ void compute_vertex_normal(const rxmesh::Context context, rxmesh::Ver
     query.dispatch<Op::FV>(block, shrd_alloc, vn_lambda);
 }
 
+// the DiffARAP edge-weight kernel as the reference app spells it
+// (apps/DiffARAP/arap_kernels.h:8-54), writing one weight per linear edge id
+// instead of into a SparseMatrix
+template <typename T, uint32_t blockThreads>
+__global__ void calc_edge_weights(const rxmesh::Context context, const rxmesh::VertexAttribute<T> coords,
+                                  T* edge_weights)
+{
+    using namespace rxmesh;
+    auto calc_weights = [&](EdgeHandle edge_id, VertexIterator& vv) {
+        const VertexHandle p_id = vv[0];
+        const VertexHandle r_id = vv[2];
+        const VertexHandle q_id = vv[1];
+        const VertexHandle s_id = vv[3];
+        if (!p_id.is_valid() || !r_id.is_valid() || !q_id.is_valid() || !s_id.is_valid()) {
+            return;
+        }
+        const vec3<T> p = coords.to_glm<3>(p_id);
+        const vec3<T> r = coords.to_glm<3>(r_id);
+        const vec3<T> q = coords.to_glm<3>(q_id);
+        const vec3<T> s = coords.to_glm<3>(s_id);
+        T weight = 0;
+        weight += dot((p - q), (r - q)) / length(cross(p - q, r - q));
+        weight += dot((p - s), (r - s)) / length(cross(p - s, r - s));
+        weight /= 2;
+        weight = std::max(T(0), weight);
+        edge_weights[context.linear_id(edge_id)] = weight;
+    };
+    auto                block = cooperative_groups::this_thread_block();
+    Query<blockThreads> query(context);
+    ShmemAllocator      shrd_alloc;
+    query.dispatch<Op::EVDiamond>(block, shrd_alloc, calc_weights);
+}
+
 template <uint32_t blockThreads>
 __global__ void valence_kernel(const Context context, uint32_t* out)
 {
@@ -105,7 +138,7 @@ int run_query(RXMeshStatic& rx, bool oriented, int active_mod, int32_t* h_row, i
     const int32_t* rp   = cnt;
     auto           fill = [=] __device__(SrcH h, IterT & it) {
         const int32_t b = rp[ctx.linear_id(h)];
-        for (uint16_t i = 0; i < it.size(); ++i) col[b + i] = int32_t(ctx.linear_id(it[i]));
+        for (uint16_t i = 0; i < it.size(); ++i) col[b + i] = it[i].is_valid() ? int32_t(ctx.linear_id(it[i])) : -1;
     };
     if (active_mod > 1) {
         LaunchBox<256> lb;
@@ -151,6 +184,8 @@ int api_query(const uint32_t* fv, uint32_t nF, uint32_t patch_size, int op, int 
             case 8: return run_query<Op::FF, FaceHandle, FaceIterator>(rx, o, active_mod, row_ptr, col, col_cap);
             case 9: return run_query<Op::EV, EdgeHandle, VertexIterator>(rx, o, active_mod, row_ptr, col, col_cap);
             case 11: return run_query<Op::EF, EdgeHandle, FaceIterator>(rx, o, active_mod, row_ptr, col, col_cap);
+            case 12:
+                return run_query<Op::EVDiamond, EdgeHandle, VertexIterator>(rx, o, active_mod, row_ptr, col, col_cap);
             default: return 1;
         }
     } catch (const std::exception& e) {
@@ -186,6 +221,35 @@ int api_vertex_normals(const uint32_t* fv, uint32_t nF, const float* xyz, uint32
         return 0;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "api_vertex_normals: %s\n", e.what());
+        return 6;
+    }
+}
+
+// ARAP edge weights through the reference-style kernel above, per linear
+// edge id (0 where the kernel skips the edge).
+int api_arap_weights(const uint32_t* fv, uint32_t nF, const float* xyz, uint32_t nV, uint32_t patch_size,
+                     float* out)
+{
+    try {
+        auto                            f = to_fv(fv, nF);
+        RXMeshStatic                    rx(f, "", patch_size);
+        std::vector<std::vector<float>> verts(nV, std::vector<float>(3));
+        for (uint32_t v = 0; v < nV; ++v)
+            for (int i = 0; i < 3; ++i) verts[v][i] = xyz[3 * v + i];
+        rx.add_vertex_coordinates(verts);
+        auto           coords = rx.get_input_vertex_coordinates();
+        const uint32_t nE     = rx.get_num_edges();
+        float*         w      = nullptr;
+        if (hipMalloc(&w, nE * sizeof(float)) != hipSuccess) return 5;
+        if (hipMemset(w, 0, nE * sizeof(float)) != hipSuccess) return 5;
+        LaunchBox<256> lb;
+        rx.prepare_launch_box({Op::EVDiamond}, lb, (void*)calc_edge_weights<float, 256>);
+        calc_edge_weights<float, 256><<<lb.blocks, lb.num_threads, lb.smem_bytes_dyn>>>(rx.get_context(), *coords, w);
+        int rc = (hipMemcpy(out, w, nE * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess) ? 0 : 5;
+        (void)hipFree(w);
+        return rc;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "api_arap_weights: %s\n", e.what());
         return 6;
     }
 }

@@ -18,7 +18,7 @@ from rxmesh_amd import meshgen as mg
 pytestmark = pytest.mark.gpu
 
 API_LIB = os.path.join(ROOT, "tests", "cpp", "_build", "libapi_test.so")
-OPS = {"VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7, "FF": 8, "EV": 9, "EF": 11}
+OPS = {"VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7, "FF": 8, "EV": 9, "EF": 11, "EVD": 12}
 
 _api = None
 
@@ -33,6 +33,7 @@ def api():
         L.api_query.argtypes = [vp, u32, u32, i, i, i, vp, vp, i64]
         L.api_vertex_normals.argtypes = [vp, u32, vp, u32, u32, vp]
         L.api_misc.argtypes = [vp, u32, u32, ctypes.c_char_p, i]
+        L.api_arap_weights.argtypes = [vp, u32, vp, u32, u32, vp]
         _api = L
     return _api
 
@@ -120,6 +121,25 @@ def test_api_vertex_normal_kernel(case, ps):
     # fp32 with atomics in any order vs FP64: 1e-5 of the largest normal
     err = np.abs(out - ref).max() / np.abs(ref).max()
     assert err < 1e-5, err
+
+
+@pytest.mark.parametrize("case,ps", [("sphere3.obj", 128), ("disk", 96), ("torus", 64), ("ico", 128)])
+def test_api_arap_weights_kernel(case, ps):
+    """The DiffARAP edge-weight kernel, spelled as the reference app writes
+    it over Query::dispatch<Op::EVDiamond>, equals rxm_arap_weights per edge
+    (same fp32 formula; 1e-6 relative for contraction differences)."""
+    import torch
+    V, F = _case(case)
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    F = np.ascontiguousarray(F, dtype=np.uint32)
+    m = rx.Mesh(F, patch_size=ps)
+    out = np.zeros(m.nE, np.float32)
+    rc = api().api_arap_weights(F.ctypes.data, len(F), V.ctypes.data, len(V), ps, out.ctypes.data)
+    assert rc == 0
+    l2g = m.array("v_l2g")
+    w = m.arap_weights(torch.from_numpy(np.ascontiguousarray(V[l2g].T)).cuda()).cpu().numpy()
+    assert np.all(np.abs(out - w) <= 1e-6 * np.maximum(1.0, np.abs(w)))
+    assert (w > 0).any()
 
 
 @pytest.mark.parametrize("case,ps", [("sphere3.obj", 128), ("disk", 96), ("torus", 64)])
