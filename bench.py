@@ -291,6 +291,13 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
     m = rx.Mesh(fv)
     x = lin(m, pos)
     mcf(m, x, "c2t_torus_2048x1024")
+    # connectivity build on the device (edge ids as the host build numbers them)
+    fv_t = torch.from_numpy(np.ascontiguousarray(fv, dtype=np.int32)).cuda()
+    rx.build_edges_device(fv_t, len(pos))
+    t_eb = event_time_ms(lambda: rx.build_edges_device(fv_t, len(pos)), 3)
+    out["c2t_torus_2048x1024"]["edge_build_device"] = {"ms": round(t_eb, 3), "Gfaces_s": round(len(fv) / (t_eb * 1e-3) / 1e9, 4),
+                                                       "note": "whole call: work buffers, sort, scans, sync"}
+    del fv_t
     del x
     m.close()
     log("c2t done")

@@ -199,6 +199,18 @@ rxm_status rxm_arap_rotations(rxm_mesh* mesh, const float* rest, const float* x,
 rxm_status rxm_arap_energy_grad(rxm_mesh* mesh, const float* rest, const float* x, const float* w,
                                 const float* R, float* total, float* grad, void* stream);
 
+/* Connectivity build on the device, the edge step of the mesh build
+ * (build_supporting_structures, include/rxmesh/rxmesh.cpp:518-681) as
+ * sort-based deduplication: edge ids in order of first appearance over the
+ * half-edges (f, i) -> (fv[3f+i], fv[3f+(i+1)%3]), EV in the first
+ * half-edge's orientation, and every half-edge's edge id -- the same ids as
+ * the host build. fv (3F), ev (capacity 2 x 3F) and fe (3F) are device
+ * pointers; the call waits for the stream and sets *num_edges. Allocates its
+ * work buffers. RXM_ERR_INVALID_ARG for a vertex id >= num_vertices or a
+ * degenerate face. */
+rxm_status rxm_build_edges_device(const uint32_t* fv, uint32_t num_faces, uint32_t num_vertices, uint32_t* ev,
+                                  uint32_t* fe, uint32_t* num_edges, void* stream);
+
 /* Build (once; later calls return at once) the mesh's patch-local VV plan:
  * the VV query evaluated by the query engine and kept in its native
  * patch-local u16 encoding for iterative consumers. rxm_laplacian_smooth

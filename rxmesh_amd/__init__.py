@@ -51,6 +51,7 @@ _SIGS = {
     "rxm_mesh_destroy": [_vp],
     "rxm_gaussian_curvature": [_vp, _vp, _vp, _vp, _vp],
     "rxm_arap_weights": [_vp, _vp, _vp, _vp],
+    "rxm_build_edges_device": [_vp, _u32, _u32, _vp, _vp, _vp, _vp],
     "rxm_arap_rotations": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_arap_energy_grad": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_mcf_implicit_cg": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp, _vp, _vp, _vp],
@@ -136,6 +137,21 @@ def _stream(stream):
 
 def _ptr(t):
     return ctypes.c_void_p(t.data_ptr())
+
+
+def build_edges_device(fv, num_vertices, stream=None):
+    """Edges of a triangle list on the device, numbered as the host build
+    numbers them (first appearance over the half-edges). fv: (F, 3) int32 /
+    uint32 device tensor. Returns (ev (E, 2) int32, fe (3F,) int32)."""
+    import torch
+    fv = fv.contiguous()
+    nF = fv.shape[0]
+    ev = torch.empty(2 * 3 * nF, dtype=torch.int32, device=fv.device)
+    fe = torch.empty(3 * nF, dtype=torch.int32, device=fv.device)
+    ne = ctypes.c_uint32()
+    _check(lib().rxm_build_edges_device(_ptr(fv), nF, int(num_vertices), _ptr(ev), _ptr(fe), ctypes.byref(ne),
+                                        _stream(stream)), "rxm_build_edges_device")
+    return ev[: 2 * ne.value].view(-1, 2), fe
 
 
 class Mesh:

@@ -1,0 +1,67 @@
+"""The device edge build (rxm_build_edges_device) against the oracle's
+restatement of the reference's edge numbering (first appearance over the
+half-edges, first-seen orientation; itself pinned to the reference's patch
+archives and a brute force in test_oracle.py), bit for bit, and against the
+product's host build."""
+import numpy as np
+import pytest
+
+from conftest import obj
+from oracle.oracle import OracleMesh
+from rxmesh_amd import meshgen as mg
+
+pytestmark = pytest.mark.gpu
+
+
+def _faces(name):
+    if name.endswith(".obj"):
+        return obj(name)[1]
+    if name == "ico_perm":
+        return mg.permute_mesh(*mg.icosphere(4, seed=2, sigma=0.005), seed=3)[1]
+    if name == "torus":
+        return mg.torus(40, 24)[1]
+    if name == "disk":
+        return mg.disk(30, seed=5)[1]
+    if name == "c1":
+        return mg.config_mesh("c1")[1]
+    if name == "flipped":
+        F = mg.config_mesh("c1")[1].copy()
+        F[7] = F[7][::-1]
+        return F
+    if name == "fin":  # an edge with three faces
+        return np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4], [2, 1, 5]], np.uint32)
+    raise KeyError(name)
+
+
+@pytest.mark.parametrize("name", ["plane.obj", "cube.obj", "sphere3.obj", "bunnyhead.obj", "ico_perm", "torus",
+                                  "disk", "c1", "flipped", "fin"])
+def test_device_edges_equal_host_numbering(name):
+    import torch
+    import rxmesh_amd as rx
+    F = np.ascontiguousarray(_faces(name), dtype=np.uint32)
+    nV = int(F.max()) + 1
+    print("case", name, len(F), flush=True)
+    ev, fe = rx.build_edges_device(torch.from_numpy(F.astype(np.int32)).cuda(), nV)
+    torch.cuda.synchronize()
+    o = OracleMesh(F)
+    ref = o.ev().astype(np.int64)
+    assert np.array_equal(ev.cpu().numpy().astype(np.int64), ref)
+    ids = {tuple(sorted(e)): i for i, e in enumerate(ref.tolist())}
+    exp = [ids[tuple(sorted((f[i], f[(i + 1) % 3])))] for f in F.tolist() for i in range(3)]
+    assert np.array_equal(fe.cpu().numpy(), np.array(exp))
+    if name != "fin":
+        m = rx.Mesh(F, host_only=True)
+        assert np.array_equal(m.array("ev").reshape(-1, 2).astype(np.int64), ref)
+
+
+def test_device_edges_reject_bad_faces():
+    import torch
+    import rxmesh_amd as rx
+    F = np.array([[0, 1, 2], [1, 3, 2]], np.int32)
+    with pytest.raises(rx.RXMError) as e:
+        rx.build_edges_device(torch.from_numpy(F).cuda(), 3)  # vertex 3 out of range
+    assert e.value.code == 1
+    F = np.array([[0, 1, 2], [1, 1, 2]], np.int32)
+    with pytest.raises(rx.RXMError) as e:
+        rx.build_edges_device(torch.from_numpy(F).cuda(), 3)  # degenerate face
+    assert e.value.code == 1
