@@ -39,7 +39,8 @@ def parse():
     ap.add_argument("--profile", default=None,
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
                          "smooth_uniform, smooth_2step (with --smooth-fused 1), smooth_ksteps (with --smooth-steps k), "
-                         "vertex_normals, gaussian_curvature, cot_assembly, mcf_step, mcf_cg (20 CG iterations, dt 10), copy")
+                         "vertex_normals, gaussian_curvature, cot_assembly, mcf_step, mcf_cg (20 CG iterations, dt 10), "
+                         "query_evd, arap_weights, arap_rotations, arap_energy_grad, copy")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
@@ -368,10 +369,22 @@ def main():
     if args.profile:
         A_cot = m.cot_assemble(x, 1e-5) if args.profile in ("cot_assembly", "mcf_step") else None
         A_cg = m.cot_assemble(x, 10.0) if args.profile == "mcf_cg" else None
+        if args.profile.startswith("arap") or args.profile == "query_evd":
+            evd_rp = torch.empty(E + 1, dtype=torch.int32, device="cuda")
+            evd_col = torch.empty(4 * E, dtype=torch.int32, device="cuda")
+            gen = torch.Generator(device="cuda").manual_seed(0)
+            xd = x + 1e-3 * torch.randn(x.shape, generator=gen, device="cuda")
+            w_a = m.arap_weights(x)
+            R_a = m.arap_rotations(x, xd, w_a)
+            tot_a, g_a = m.arap_energy_grad(x, xd, w_a, R_a)
         fns = {"query_vv": lambda: m.query("VV", row_ptr=vv_rp, col=vv_col),
                "cot_assembly": lambda: m.cot_assemble(x, 1e-5, out=A_cot),
                "mcf_step": lambda: m.mcf_explicit(A_cot, x, 1, out=out),
                "mcf_cg": lambda: m.mcf_implicit_cg(A_cg, x, 20, 1e-6, 0.0, out=out, stats=False),
+               "query_evd": lambda: m.query("EVD", row_ptr=evd_rp, col=evd_col),
+               "arap_weights": lambda: m.arap_weights(x, out=w_a),
+               "arap_rotations": lambda: m.arap_rotations(x, xd, w_a, out=R_a),
+               "arap_energy_grad": lambda: m.arap_energy_grad(x, xd, w_a, R_a, total=tot_a, grad=g_a),
                "query_vf": lambda: m.query("VF", row_ptr=vf_rp, col=vf_col),
                "smooth_uniform": lambda: m.smooth(x, 0.5, 1, out=out),
                "smooth_2step": lambda: m.smooth(x, 0.5, 2, out=out, tmp=tmp),

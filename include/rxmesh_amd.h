@@ -188,7 +188,8 @@ rxm_status rxm_arap_weights(rxm_mesh* mesh, const float* pos, float* w, void* st
 /* Best rotation per vertex (calculate_rotation_matrix, arap_kernels.h:60-117):
  * the rotation maximising tr(R S), S = sum over the vertex's neighbours j of
  * w (P_v - P_j)(X_v - X_j)^T, i.e. V U^T from S = U Sigma V^T with the
- * reflection fix. R is nine SoA planes: R[(3 i + j) * V + v] = R_v(i, j). */
+ * reflection fix. R holds one row-major record per vertex (linear order):
+ * R[9 v + 3 i + j] = R_v(i, j), the reference's rotations(v, 3 i + j). */
 rxm_status rxm_arap_rotations(rxm_mesh* mesh, const float* rest, const float* x, const float* w, float* R,
                               void* stream);
 /* Energy sum over edges (i, j) in EV order (i the larger global id) of

@@ -279,10 +279,10 @@ class Mesh:
         return out
 
     def arap_rotations(self, rest, x, w, out=None, stream=None):
-        """Best rotation per vertex as (9, V): row-major R_v in column v."""
+        """Best rotation per vertex as (V, 9): row v is R_v, row-major."""
         import torch
         if out is None:
-            out = torch.empty(9, self.nV, dtype=torch.float32, device=x.device)
+            out = torch.empty(self.nV, 9, dtype=torch.float32, device=x.device)
         _check(lib().rxm_arap_rotations(self._h, _ptr(rest), _ptr(x), _ptr(w), _ptr(out), _stream(stream)),
                "rxm_arap_rotations")
         return out

@@ -223,19 +223,22 @@ __device__ __forceinline__ void stage_arap(const DevMesh& m, const PatchDesc& d,
     for (uint32_t e = threadIdx.x; e < d.ne; e += T) L.w[e] = w[lin_of(e, d.neo, d.ep, el)];
     if (R) {
         const uint32_t* vl = m.v_lin + d.v_off;
-        const uint32_t  N  = m.nV;
         for (uint32_t l = threadIdx.x; l < d.nv; l += T) {
-            const uint32_t g = lin_of(l, d.nvo, d.vp, vl);
+            const float* r = R + 9ull * lin_of(l, d.nvo, d.vp, vl);  // one 36-byte record per vertex
+            float        t[9];
 #pragma unroll
-            for (uint32_t c = 0; c < 9; ++c) L.rot[9 * l + c] = R[c * N + g];
+            for (uint32_t c = 0; c < 9; ++c) t[c] = r[c];
+#pragma unroll
+            for (uint32_t c = 0; c < 9; ++c) L.rot[9 * l + c] = t[c];
         }
     }
 }
 
 // Best rotation per owned vertex (arap_kernels.h:60-117): S = sum over the
-// vertex's VV row of w (P_v - P_j)(X_v - X_j)^T, R from best_rotation. R is
-// nine SoA planes, R[(3 i + j) V + v] = R_v(i, j) (the reference's
-// rotations(v, i * 3 + j)).
+// vertex's VV row of w (P_v - P_j)(X_v - X_j)^T, R from best_rotation. R
+// holds one row-major 3 x 3 record per vertex, R[9 v + 3 i + j] = R_v(i, j)
+// (the reference's rotations(v, i * 3 + j)): a ribbon vertex's rotation is
+// one or two cache lines for the energy kernel's gather, not nine.
 template <uint32_t T>
 __global__ __launch_bounds__(T) void k_arap_rotations(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                       const uint32_t* __restrict__ rs,
@@ -252,7 +255,6 @@ __global__ __launch_bounds__(T) void k_arap_rotations(DevMesh m, const uint32_t*
     const PlanAR    L = plan_ar(b, d.nv, d.ne, d.nvo, nent, false);
     stage_arap<T>(m, d, L, rs, ent, vb, nent, rest, x, w, nullptr);
     __syncthreads();
-    const uint32_t N = m.nV;
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         float S[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
         for (uint32_t k = L.rs[v]; k < L.rs[v + 1]; ++k) {
@@ -268,11 +270,11 @@ __global__ __launch_bounds__(T) void k_arap_rotations(DevMesh m, const uint32_t*
         }
         float Rv[3][3];
         best_rotation(S, Rv);
-        const uint32_t g = d.vp + v;
+        float* out = R + 9ull * (d.vp + v);
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
-            for (int s = 0; s < 3; ++s) R[(3 * r + s) * N + g] = Rv[r][s];
+            for (int s = 0; s < 3; ++s) out[3 * r + s] = Rv[r][s];
     }
 }
 

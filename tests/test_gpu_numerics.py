@@ -354,7 +354,7 @@ def test_arap_rotations_energy_grad_vs_fp64(name):
     torch.cuda.synchronize()
     w64 = o.arap_weights(V32.astype(np.float64))
     R64 = o.arap_rotations(V32.astype(np.float64), X32.astype(np.float64), w64)
-    Rg = _glob(m, R.cpu().numpy().astype(np.float64)).T.reshape(-1, 3, 3)
+    Rg = _glob(m, R.cpu().numpy().astype(np.float64).T).T.reshape(-1, 3, 3)
     S = o.arap_covariances(V32.astype(np.float64), X32.astype(np.float64), w64)
     sig = np.linalg.svd(S, compute_uv=False)
     gap = sig[:, 1] + np.sign(np.linalg.det(S)) * sig[:, 2]
@@ -384,7 +384,7 @@ def test_arap_rigid_motion():
     R = m.arap_rotations(P, Xd, w)
     E, g = m.arap_energy_grad(P, Xd, w, R)
     torch.cuda.synchronize()
-    Rg = R.cpu().numpy().T.reshape(-1, 3, 3)
+    Rg = R.cpu().numpy().reshape(-1, 3, 3)
     assert np.abs(Rg - Q).max() <= 1e-5
     _, g0 = m.arap_energy_grad(P, P, w, torch.zeros_like(R))  # R = 0: the gradient of sum w |X_i - X_j|^2
     scale = float(torch.abs(g0).max())
