@@ -401,19 +401,27 @@ def main():
                           "copy_bytes": 12 * V}), flush=True)
         return
 
-    def step():
+    def step():  # VV and VF from one launch (rxm_query_vv_vf), then the smoothing iterations
+        m.query_vv_vf(vv=(vv_rp, vv_col), vf=(vf_rp, vf_col))
+        m.smooth(x, 0.5, it, out=out, tmp=tmp)
+
+    def step_separate():  # the same work with the two queries launched separately
         m.query("VV", row_ptr=vv_rp, col=vv_col)
         m.query("VF", row_ptr=vf_rp, col=vf_col)
         m.smooth(x, 0.5, it, out=out, tmp=tmp)
 
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step()
-    torch.cuda.synchronize()
-    ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    def wall_ms(fn):
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            fn()
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / args.steps
+
+    ms_separate = wall_ms(step_separate)
+    ms = wall_ms(step)
     entries = nnz_vv + nnz_vf + it * nnz_vv
     value = entries / (ms * 1e-3) / 1e9
 
@@ -424,6 +432,8 @@ def main():
     kern["query_vv"] = (t_vv, 4 * E + 4 * nnz_vv + 4 * (V + 1))
     t_vf = event_time_ms(lambda: m.query("VF", row_ptr=vf_rp, col=vf_col), reps)
     kern["query_vf"] = (t_vf, 4 * E + 6 * F + 4 * nnz_vf + 4 * (V + 1))
+    t_vvf = event_time_ms(lambda: m.query_vv_vf(vv=(vv_rp, vv_col), vf=(vf_rp, vf_col)), reps)
+    kern["query_vv_vf"] = (t_vvf, 4 * E + 6 * F + 4 * nnz_vv + 4 * nnz_vf + 8 * (V + 1))
     t_sm = event_time_ms(lambda: m.smooth(x, 0.5, 1, out=out), reps)
     kern["smooth_uniform"] = (t_sm, 24 * V + 4 * E)
     ksteps = int(smooth_cfg.get("smooth_steps", 0))
@@ -447,7 +457,7 @@ def main():
     kernels = {k: {"us": round(t * 1e3, 2), "bytes": b, "GB_s": round(b / (t * 1e-3) / 1e9, 1),
                    "frac": round(b / (t * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)} for k, (t, b) in kern.items()}
     # dominant kernel of the step (by time per step)
-    share = {"query_vv": t_vv, "query_vf": t_vf}
+    share = {"query_vv_vf": t_vvf}
     if ksteps:
         share["smooth_uniform_%dstep" % ksteps] = (it // ksteps) * t_sk + t_rem
     elif fused:
@@ -461,8 +471,9 @@ def main():
         "metric": "VV-query Gentries/s + %HBM peak; batched grad meshes/s vs plain launches",
         "value": round(value, 4), "unit": "Gentries/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(ms, 4), "higher_is_better": True, "scaling": "weak",
+        "ms_per_step_separate_queries": round(ms_separate, 4),
         "vs_baseline": None, "dtype": "fp32+u16/i32", "data": "synthetic",
-        "config": {"workload": "VV+VF query CSR + %d uniform smoothing iters, icosphere L9 permuted "
+        "config": {"workload": "VV+VF query CSR (one launch) + %d uniform smoothing iters, icosphere L9 permuted "
                                "(%d V / %d F), patch size %d" % (it, V, F, args.patch_size),
                    "mesh": args.workload, "num_patches": P, "patcher": args.patcher,
                    "entries_per_step": entries},

@@ -162,6 +162,13 @@ rxm_status rxm_query_nnz(rxm_mesh* mesh, int op, int64_t* nnz);
 rxm_status rxm_query_csr(rxm_mesh* mesh, int op, int oriented, int32_t* row_ptr,
                          int32_t* col, void* stream);
 
+/* VV and VF (unoriented) in one launch: the same two CSRs as two
+ * rxm_query_csr calls (row_ptr V + 1, col sized by rxm_query_nnz), with the
+ * patch topology staged once for both. A fusion of two reference queries
+ * (rxmesh_queries.cuh v_v and v_f); no single reference counterpart. */
+rxm_status rxm_query_vv_vf(rxm_mesh* mesh, int32_t* vv_row_ptr, int32_t* vv_col, int32_t* vf_row_ptr,
+                           int32_t* vf_col, void* stream);
+
 /* Vertex normals, unnormalised. mode RXM_NORMALS_MAX restates the reference
  * VertexNormal app (apps/VertexNormal/vertex_normal_kernel.cuh:10-42). */
 rxm_status rxm_vertex_normals(rxm_mesh* mesh, int mode, const float* pos, float* normals,

@@ -61,6 +61,7 @@ _SIGS = {
     "rxm_mesh_save_patches": [_vp, ctypes.c_char_p],
     "rxm_query_nnz": [_vp, _i, ctypes.POINTER(ctypes.c_int64)],
     "rxm_query_csr": [_vp, _i, _i, _vp, _vp, _vp],
+    "rxm_query_vv_vf": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_vertex_normals": [_vp, _i, _vp, _vp, _vp],
     "rxm_laplacian_smooth": [_vp, _vp, _vp, _vp, _f, _i, _vp],
     "rxm_to_linear": [_vp, _i, _i, _vp, _vp, _vp],
@@ -234,6 +235,19 @@ class Mesh:
         n = ctypes.c_int64()
         _check(lib().rxm_query_nnz(self._h, OP[op], ctypes.byref(n)), "rxm_query_nnz")
         return n.value
+
+    def query_vv_vf(self, vv=None, vf=None, stream=None):
+        """VV and VF CSRs (same as query("VV") and query("VF")) from one launch."""
+        import torch
+        if vv is None:
+            vv = (torch.empty(self.nV + 1, dtype=torch.int32, device="cuda"),
+                  torch.empty(max(self.query_nnz("VV"), 1), dtype=torch.int32, device="cuda"))
+        if vf is None:
+            vf = (torch.empty(self.nV + 1, dtype=torch.int32, device="cuda"),
+                  torch.empty(max(self.query_nnz("VF"), 1), dtype=torch.int32, device="cuda"))
+        _check(lib().rxm_query_vv_vf(self._h, _ptr(vv[0]), _ptr(vv[1]), _ptr(vf[0]), _ptr(vf[1]), _stream(stream)),
+               "rxm_query_vv_vf")
+        return vv, vf
 
     def num_src(self, op):
         return {"V": self.nV, "E": self.nE, "F": self.nF}[op[0]]

@@ -869,6 +869,22 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
     }
 }
 
+rxm_status rxm_query_vv_vf(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col, int32_t* vf_row_ptr, int32_t* vf_col,
+                           void* stream)
+{
+    if (!m || !vv_row_ptr || !vv_col || !vf_row_ptr || !vf_col) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const uint32_t *vv_base = nullptr, *vf_base = nullptr;
+    int             rc = device_base(m, RXM_OP_VV, &vv_base);
+    if (!rc) rc = device_base(m, RXM_OP_VF, &vf_base);
+    if (rc) return rc;
+    Bump b(nullptr);
+    plan_vvf(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
+    constexpr uint32_t T = kQueryThreads;
+    return launch_patches_xcd(k_query2_vv_vf<T>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
+                              vv_col, vf_row_ptr, vf_col);
+}
+
 rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* normals, void* stream)
 {
     if (!m || !pos || !normals || (mode != RXM_NORMALS_MAX && mode != RXM_NORMALS_AREA)) return RXM_ERR_INVALID_ARG;

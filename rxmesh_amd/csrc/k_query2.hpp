@@ -240,6 +240,103 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
     copy_out<T>(col + bse, L.slots, L.off[d.nvo]);
 }
 
+// VV and VF over linear ids in one launch: the two relations of a vertex
+// share their staged topology (EV, FE and both linear-id tables arrive in
+// one round trip), then the VV transpose and the VF transpose run in turn
+// in one slot array; each CSR is written exactly as k_query2_vx<T, true> and
+// k_query2_vf<T> write it. (One counting sort over a combined column space
+// for both relations was measured slower: 118 against 84 us on C2s.)
+struct PlanVVF
+{
+    uint32_t *ev, *off, *slots, *wave, *lin_v, *lin_f;
+    uint16_t* fe;
+};
+__host__ __device__ inline PlanVVF plan_vvf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+{
+    PlanVVF p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.off   = b.alloc<uint32_t>(nv + 1);
+    p.slots = b.alloc<uint32_t>(2 * ne > 3 * nf ? 2 * ne : 3 * nf);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.lin_v = b.alloc<uint32_t>(nv);
+    p.lin_f = b.alloc<uint32_t>(nf);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* __restrict__ vv_base,
+                                                    const uint32_t* __restrict__ vf_base,
+                                                    int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
+                                                    int32_t* __restrict__ vf_row, int32_t* __restrict__ vf_col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanVVF   L = plan_vvf(b, d.nv, d.ne, d.nf);
+    {
+        // face linear ids are loaded ahead of the staging's LDS stores
+        constexpr uint32_t RL = 3;
+        const uint32_t*    fs = m.f_lin + d.f_off;
+        uint32_t           fl[RL];
+#pragma unroll
+        for (uint32_t r = 0; r < RL; ++r) {
+            const uint32_t l = threadIdx.x + r * T;
+            fl[r]            = (l < d.nfo) ? d.fp + l : ((l < d.nf) ? fs[l] : 0u);
+        }
+        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin_v, d.nv, d.nvo,
+                          d.vp, m.v_lin + d.v_off);
+#pragma unroll
+        for (uint32_t r = 0; r < RL; ++r)
+            if (threadIdx.x + r * T < d.nf) L.lin_f[threadIdx.x + r * T] = fl[r];
+        for (uint32_t l = threadIdx.x + RL * T; l < d.nf; l += T) L.lin_f[l] = (l < d.nfo) ? d.fp + l : fs[l];
+    }
+    for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
+    __syncthreads();
+    // VV: entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
+    fill_columns<T, true>(
+        2 * d.ne, d.nvo,
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return (i & 1u) ? hi16(x) : lo16(x);
+        },
+        [&](uint32_t i) {
+            const uint32_t x = L.ev[i >> 1];
+            return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
+        },
+        L.off, L.slots, L.wave);
+    {
+        const uint32_t bse = vv_base[p];
+        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+            const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+            vv_row[d.vp + v] = static_cast<int32_t>(bse + o);
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = L.lin_v[lo16(s)]; });
+        }
+        if (threadIdx.x == 0) vv_row[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+        __syncthreads();
+        copy_out<T>(vv_col + bse, L.slots, L.off[d.nvo]);
+    }
+    __syncthreads();  // the VV slots and offsets are read out before VF reuses them
+    for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
+    __syncthreads();
+    fill_columns<T, true>(
+        3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+        [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
+    {
+        const uint32_t bse = vf_base[p];
+        for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+            const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+            vf_row[d.vp + v] = static_cast<int32_t>(bse + o);
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = L.lin_f[s]; });
+        }
+        if (threadIdx.x == 0) vf_row[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
+        __syncthreads();
+        copy_out<T>(vf_col + bse, L.slots, L.off[d.nvo]);
+    }
+}
+
 // EF (IS_FF == false) or FF.
 template <uint32_t T, bool IS_FF>
 __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __restrict__ base,

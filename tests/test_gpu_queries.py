@@ -121,6 +121,21 @@ def test_oriented_paths_refuse_inconsistent_orientation():
     assert int(rp[-1]) == m.query_nnz("VV")
 
 
+@pytest.mark.parametrize("name", ["sphere3", "torus", "plane", "cube", "bunnyhead", "c1", "c1_p128", "disk",
+                                  "c1_kmeans"])
+def test_query_vv_vf_one_launch(name):
+    """The fused VV + VF launch writes both CSRs exactly as the oracle (and
+    the separate queries) do."""
+    import torch
+    V, F, m, o = case(name)
+    (vrp, vcol), (frp, fcol) = m.query_vv_vf()
+    torch.cuda.synchronize()
+    for op, rp, col in (("VV", vrp, vcol), ("VF", frp, fcol)):
+        orp, ocol = o.query(op)
+        assert np.array_equal(rp.cpu().numpy(), orp)
+        assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
+
+
 def test_evdiamond_inconsistent_orientation():
     """With one face traversed backwards, two faces run some edges the same
     way: the EVDiamond slot takes the later face (serial order of the
