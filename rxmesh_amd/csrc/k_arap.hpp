@@ -76,15 +76,21 @@ __global__ __launch_bounds__(T) void k_arap_weights(DevMesh m, const float* __re
 
 // ---- best rotation -------------------------------------------------------------
 // One Jacobi rotation of the symmetric 4x4 a in the (P, Q) plane, zeroing
-// a[P][Q]; v accumulates the eigenvectors as columns.
+// a[P][Q]; v accumulates the eigenvectors as columns. The rotation only has
+// to be orthogonal, not exact (an angle off by an ulp leaves a tiny a[P][Q]
+// that later sweeps remove): c = rsqrt(1 + t^2) and s = t c keep c^2 + s^2 = 1
+// to an ulp, so the hardware reciprocal and reciprocal square root serve.
 template <int P, int Q>
 __device__ __forceinline__ void jacobi4(float (&a)[4][4], float (&v)[4][4])
 {
     const float apq = a[P][Q];
-    if (apq == 0.f) return;
-    const float theta = (a[Q][Q] - a[P][P]) / (2.f * apq);
-    const float t     = copysignf(1.f, theta) / (fabsf(theta) + sqrtf(theta * theta + 1.f));
-    const float c = 1.f / sqrtf(t * t + 1.f), s = t * c;
+    if (fabsf(apq) < 1e-30f) return;  // negligible in the unit-scaled matrix; keeps rcp away from denormals
+    const float theta = 0.5f * (a[Q][Q] - a[P][P]) * __builtin_amdgcn_rcpf(apq);
+    const float at    = fabsf(theta);
+    // t = sign(theta) / (|theta| + sqrt(theta^2 + 1)); ~1 / (2 theta) once theta^2 would overflow
+    const float den = (at < 1e18f) ? at + (at * at + 1.f) * __builtin_amdgcn_rsqf(at * at + 1.f) : 2.f * at;
+    const float t   = copysignf(__builtin_amdgcn_rcpf(den), theta);
+    const float c = __builtin_amdgcn_rsqf(t * t + 1.f), s = t * c;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
         const float akp = a[k][P], akq = a[k][Q];
