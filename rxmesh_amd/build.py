@@ -15,7 +15,7 @@ ARCH = "gfx950"
 # packed-FP32 instructions are disabled for every HIP translation unit
 NO_PK_FP32 = ["-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 
-SOURCES_HIP = ["capi.hip", "batch.hip", "cpp_api.hip", "gpu_build.hip"]
+SOURCES_HIP = ["capi.hip", "gpu_build.hip"]
 SOURCES_CPP = ["host_build.cpp"]
 
 
