@@ -40,7 +40,7 @@ def parse():
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
                          "smooth_uniform, smooth_2step (with --smooth-fused 1), smooth_ksteps (with --smooth-steps k), "
                          "vertex_normals, gaussian_curvature, cot_assembly, mcf_step, mcf_cg (20 CG iterations, dt 10), "
-                         "query_evd, arap_weights, arap_rotations, arap_energy_grad, copy")
+                         "query_evd, query_vv_vf, arap_weights, arap_rotations, arap_energy_grad, copy")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
@@ -382,6 +382,7 @@ def main():
                "mcf_step": lambda: m.mcf_explicit(A_cot, x, 1, out=out),
                "mcf_cg": lambda: m.mcf_implicit_cg(A_cg, x, 20, 1e-6, 0.0, out=out, stats=False),
                "query_evd": lambda: m.query("EVD", row_ptr=evd_rp, col=evd_col),
+               "query_vv_vf": lambda: m.query_vv_vf(vv=(vv_rp, vv_col), vf=(vf_rp, vf_col)),
                "arap_weights": lambda: m.arap_weights(x, out=w_a),
                "arap_rotations": lambda: m.arap_rotations(x, xd, w_a, out=R_a),
                "arap_energy_grad": lambda: m.arap_energy_grad(x, xd, w_a, R_a, total=tot_a, grad=g_a),
