@@ -136,6 +136,45 @@ def test_query_vv_vf_one_launch(name):
         assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
 
 
+def test_largest_patch_that_fits():
+    """One patch holding a whole 5,120-face sphere (7,680 edges: the VV
+    transpose needs about 111 KB of LDS, beyond the 64 KB default): every
+    query is bit-exact and the normals and smoothing match FP64, or an entry
+    point reports RXM_ERR_PATCH_TOO_BIG cleanly."""
+    import torch
+    import rxmesh_amd as rx
+    V, F = mg.icosphere(4, seed=1, sigma=0.01)
+    m = rx.Mesh(F, patch_size=8192)
+    assert m.P == 1
+    o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    refused = []
+    for op in OPS + ["EVD"]:
+        print("op", op, flush=True)
+        try:
+            rp, col = m.query(op)
+        except rx.RXMError as e:
+            assert e.code == 3
+            refused.append(op)
+            continue
+        torch.cuda.synchronize()
+        orp, ocol = o.query(op)
+        assert np.array_equal(rp.cpu().numpy(), orp)
+        assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
+    # EF / FF stage FE, both offset tables and 3F slots: just over 160 KB here
+    assert set(refused) <= {"EF", "FF"}
+    V32 = V.astype(np.float32)
+    x = lin_pos(m, V32)
+    n = m.vertex_normals(x, "max")
+    s = m.smooth(x, 0.5, 4)
+    torch.cuda.synchronize()
+    got = to_global_np(m, n.cpu().numpy()).T
+    ref = o.vertex_normals(V32.astype(np.float64), "max")
+    assert (np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)).max() <= 1e-5
+    got = to_global_np(m, s.cpu().numpy()).T
+    ref = o.smooth_uniform(V32.astype(np.float64), 0.5, 4)
+    assert (np.linalg.norm(got - ref, axis=1) / np.linalg.norm(ref, axis=1)).max() <= 1e-5
+
+
 def test_evdiamond_inconsistent_orientation():
     """With one face traversed backwards, two faces run some edges the same
     way: the EVDiamond slot takes the later face (serial order of the

@@ -146,6 +146,9 @@ def test_errors_are_status_codes(tmp_path):
     assert e.value.code == 1
     with pytest.raises(rx.RXMError):
         rx.Mesh(F, patch_size=4, host_only=True)
+    with pytest.raises(rx.RXMError) as e:  # one patch of 81,920 faces: beyond 16-bit local ids
+        rx.Mesh(mg.config_mesh("c1")[1], patch_size=1 << 20, host_only=True)
+    assert e.value.code == 3
     m = rx.Mesh(F, host_only=True)
     # device entry points on a host-only mesh report "unsupported"
     rc = rx.lib().rxm_query_csr(m._h, 3, 0, ctypes.c_void_p(16), ctypes.c_void_p(16), None)
