@@ -667,7 +667,7 @@ __global__ __launch_bounds__(T) void k_vertex_normals_plan(DevMesh m, int mode, 
                 const float l1 = vx * vx + vy * vy + vz * vz;  // b-c
                 const float l2 = wx * wx + wy * wy + wz * wz;  // c-a
                 const float den = (c == 0u) ? (l0 + l2) : ((c == 1u) ? (l1 + l0) : (l2 + l1));
-                w = 1.0f / den;
+                w               = __builtin_amdgcn_rcpf(den);  // 1 ulp: well inside the weights' use
             }
             sx += nx * w;
             sy += ny * w;
