@@ -45,7 +45,8 @@ enum {
 enum {
     RXM_OP_VV = 3, RXM_OP_VE = 4, RXM_OP_VF = 5, RXM_OP_FV = 6,
     RXM_OP_FE = 7, RXM_OP_FF = 8, RXM_OP_EV = 9, RXM_OP_EF = 11,
-    RXM_OP_EVD = 12 /* EVDiamond: 4 per edge [v0, opposite of v0->v1, v1, opposite of v1->v0], -1 if absent */
+    RXM_OP_EVD = 12 /* EVDiamond (rxmesh_queries.cuh:198-265): 4 per edge [v0, opposite of v0->v1, v1,
+                       opposite of v1->v0], -1 if absent */
 };
 
 /* rxm_mesh_create flags. The default patcher is a recursive proportional
@@ -164,8 +165,9 @@ rxm_status rxm_query_csr(rxm_mesh* mesh, int op, int oriented, int32_t* row_ptr,
 
 /* VV and VF (unoriented) in one launch: the same two CSRs as two
  * rxm_query_csr calls (row_ptr V + 1, col sized by rxm_query_nnz), with the
- * patch topology staged once for both. A fusion of two reference queries
- * (rxmesh_queries.cuh v_v and v_f); no single reference counterpart. */
+ * patch topology staged once for both. Replaces a v_v and a v_f query of
+ * the reference (include/rxmesh/kernels/rxmesh_queries.cuh:502-858) issued
+ * back to back; no single reference call does both. */
 rxm_status rxm_query_vv_vf(rxm_mesh* mesh, int32_t* vv_row_ptr, int32_t* vv_col, int32_t* vf_row_ptr,
                            int32_t* vf_col, void* stream);
 
