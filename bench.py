@@ -337,7 +337,9 @@ def main():
 
     torch.cuda.set_device(0)
     pos, fv = mg.config_mesh(args.workload)
-    m = rx.Mesh(fv, patch_size=args.patch_size, patcher=args.patcher)
+    t_build = time.perf_counter()
+    m = rx.Mesh(fv, patch_size=args.patch_size, patcher=args.patcher)  # host build + upload
+    t_build = time.perf_counter() - t_build
     V, E, F, P = m.nV, m.nE, m.nF, m.P
     l2g = m.array("v_l2g")
     x = torch.from_numpy(np.ascontiguousarray(pos[l2g].T.astype(np.float32))).cuda()
@@ -477,7 +479,8 @@ def main():
         "config": {"workload": "VV+VF query CSR (one launch) + %d uniform smoothing iters, icosphere L9 permuted "
                                "(%d V / %d F), patch size %d" % (it, V, F, args.patch_size),
                    "mesh": args.workload, "num_patches": P, "patcher": args.patcher,
-                   "entries_per_step": entries},
+                   "entries_per_step": entries,
+                   "mesh_build_s": round(t_build, 3)},
         "roofline": {"bound": "hbm", "kernel": dom, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
                      "traffic": (measured_traffic(args.workload, args.patch_size, dom)

@@ -851,7 +851,7 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
             plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
             return launch_patches_xcd(k_query2_ef<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_FF:
-            plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
+            plan_q2(b, 4, h.max_nv, h.max_ne, h.max_nf);
             return launch_patches_xcd(k_query2_ef<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
         case RXM_OP_EV:
             plan_fx(b, h.max_ne, h.max_nf);

@@ -20,19 +20,20 @@ struct PlanQ2
     uint32_t *ev, *off, *cur, *slots, *wave, *lin;
     uint16_t *fe;
 };
-// kind: 0 = VV, 3 = VE (EV only), 1 = VF (EV + FE), 2 = EF/FF (FE only).
+// kind: 0 = VV, 3 = VE (EV only), 1 = VF (EV + FE), 2 = EF (FE only), 4 = FF
+// (FE only, plus per owned face entry counts).
 // Every array is sized for its kind, so a workgroup holds no more LDS than
 // its relation needs (occupancy is LDS-bound).
 __host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32_t ne, uint32_t nf)
 {
     PlanQ2 p{};
     const bool     vx    = (kind == 0 || kind == 3);
-    const uint32_t ncols = (kind == 2) ? ne : nv;
+    const uint32_t ncols = (kind == 2 || kind == 4) ? ne : nv;
     const uint32_t n     = vx ? 2 * ne : 3 * nf;
-    if (kind != 2) p.ev = b.alloc<uint32_t>(r4(ne));
-    if (kind == 1 || kind == 2) p.fe = b.alloc<uint16_t>(r4(3 * nf));
+    if (kind != 2 && kind != 4) p.ev = b.alloc<uint32_t>(r4(ne));
+    if (kind == 1 || kind == 2 || kind == 4) p.fe = b.alloc<uint16_t>(r4(3 * nf));
     p.off   = b.alloc<uint32_t>(ncols + 1);
-    if (kind == 2) p.cur = b.alloc<uint32_t>(nf + 1);  // FF: per owned face entry counts
+    if (kind == 4) p.cur = b.alloc<uint32_t>(nf + 1);  // FF: per owned face entry counts
     p.slots = b.alloc<uint32_t>(n);
     p.wave  = b.alloc<uint32_t>(20);
     p.lin   = b.alloc<uint32_t>((kind == 0) ? nv : ((kind == 3) ? ne : nf));
@@ -347,7 +348,7 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanQ2    L = plan_q2(b, 2, d.nv, d.ne, d.nf);
+    const PlanQ2    L = plan_q2(b, IS_FF ? 4 : 2, d.nv, d.ne, d.nf);
     stage_topology<T>(nullptr, nullptr, 0, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
                       m.f_lin + d.f_off);
     for (uint32_t c = threadIdx.x; c <= (IS_FF ? d.ne : d.neo); c += T) L.off[c] = 0;
