@@ -408,21 +408,25 @@ class Bisector
                 best_key = key;
             }
         }
-        for (uint32_t f : faces) side_[f] = 0;
-        A.clear();
-        B.clear();
-        A.reserve(n1);
-        B.reserve(n - n1);
-        for (uint32_t i = 0; i < n; ++i) (i < n1 ? A : B).push_back(uint32_t(best_key[i].second));
-        std::vector<uint32_t>().swap(faces);
+        // sides of the best cut (1 = A, 2 = B); each half is rebuilt in
+        // ascending face order by one pass over `faces` (sorted on entry)
+        // instead of sorting it
+        for (uint32_t i = 0; i < n; ++i) side_[uint32_t(best_key[i].second)] = (i < n1) ? 1u : 2u;
+        std::vector<std::pair<int64_t, uint64_t>>().swap(best_key);
+        auto collect = [&] {
+            A.clear();
+            B.clear();
+            for (uint32_t f : faces) (side_[f] == 1u ? A : B).push_back(f);
+        };
+        collect();
         // keep both halves connected: pieces cut off from the larger
         // component of one half border the other half, so they move there
-        std::sort(A.begin(), A.end());
-        reconnect(A, B);
-        std::sort(B.begin(), B.end());
-        reconnect(B, A);
-        std::sort(A.begin(), A.end());
-        std::sort(B.begin(), B.end());
+        move_cut_off(A, 2u);
+        collect();
+        move_cut_off(B, 1u);
+        collect();
+        for (uint32_t f : faces) side_[f] = 0;
+        std::vector<uint32_t>().swap(faces);
         kA = std::max<uint32_t>(k1, static_cast<uint32_t>((A.size() + cap - 1) / cap));
         kB = std::max<uint32_t>(k2, static_cast<uint32_t>((B.size() + cap - 1) / cap));
     }
@@ -463,13 +467,12 @@ class Bisector
         return out;
     }
 
-    // move every component of `from` but the largest into `to`
-    void reconnect(std::vector<uint32_t>& from, std::vector<uint32_t>& to)
+    // every component of the (sorted) set but the largest changes side
+    void move_cut_off(const std::vector<uint32_t>& set, uint32_t to_side)
     {
-        auto comps = components(from);
-        if (comps.size() <= 1) return;
-        from = std::move(comps[0]);
-        for (size_t i = 1; i < comps.size(); ++i) to.insert(to.end(), comps[i].begin(), comps[i].end());
+        auto comps = components(set);
+        for (size_t i = 1; i < comps.size(); ++i)
+            for (uint32_t f : comps[i]) side_[f] = to_side;
     }
 
     // BFS inside the set tagged `tag`; fills dist_ for its faces
