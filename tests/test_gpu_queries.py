@@ -136,6 +136,45 @@ def test_query_vv_vf_one_launch(name):
         assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
 
 
+@pytest.mark.parametrize("seed", range(8))
+def test_random_meshes_and_patch_sizes(seed):
+    """Seeded sweep: a permuted jittered sphere, disk or torus with a random
+    patch size (64..512) and patcher; every query, oriented VV / VE and the
+    fused VV + VF launch equal the oracle bit for bit."""
+    import torch
+    import rxmesh_amd as rx
+    rng = np.random.default_rng(1000 + seed)
+    kind = ("sphere", "disk", "torus")[seed % 3]
+    if kind == "sphere":
+        V, F = mg.icosphere(int(rng.integers(3, 5)), seed=seed, sigma=0.01)
+    elif kind == "disk":
+        V, F = mg.disk(int(rng.integers(24, 60)), seed=seed)
+    else:
+        V, F = mg.torus(int(rng.integers(20, 60)), int(rng.integers(12, 30)))
+    V, F = mg.permute_mesh(V, F, seed=seed)
+    ps = int(rng.integers(64, 513))
+    patcher = "kmeans" if seed % 2 else "bisect"
+    print("case", kind, len(F), ps, patcher, flush=True)
+    m = rx.Mesh(F, patch_size=ps, patcher=patcher)
+    o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    for op in OPS + ["EVD"]:
+        rp, col = m.query(op)
+        torch.cuda.synchronize()
+        orp, ocol = o.query(op)
+        assert np.array_equal(rp.cpu().numpy(), orp), op
+        assert np.array_equal(col.cpu().numpy()[: len(ocol)], ocol), op
+    for op in ("VV", "VE"):
+        rp, col = m.query(op, oriented=True)
+        torch.cuda.synchronize()
+        orp, ocol = o.query(op, oriented=True)
+        assert np.array_equal(rp.cpu().numpy(), orp) and np.array_equal(col.cpu().numpy(), ocol), op
+    (vrp, vcol), (frp, fcol) = m.query_vv_vf()
+    torch.cuda.synchronize()
+    for op, rp, col in (("VV", vrp, vcol), ("VF", frp, fcol)):
+        orp, ocol = o.query(op)
+        assert np.array_equal(rp.cpu().numpy(), orp) and np.array_equal(col.cpu().numpy()[: len(ocol)], ocol)
+
+
 def test_largest_patch_that_fits():
     """One patch holding a whole 5,120-face sphere (7,680 edges: the VV
     transpose needs about 111 KB of LDS, beyond the 64 KB default): every
