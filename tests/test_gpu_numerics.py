@@ -390,3 +390,49 @@ def test_arap_rigid_motion():
     scale = float(torch.abs(g0).max())
     assert float(torch.abs(g).max()) <= 1e-4 * scale
     assert float(E.cpu()[0]) <= 1e-9 * float(m.arap_energy_grad(P, P, w, torch.zeros_like(R))[0].cpu()[0])
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_random_meshes_numerics_vs_fp64(seed):
+    """Seeded sweep over permuted meshes with random patch sizes: cotangent
+    system (pattern exact, values 1e-5 of the largest), vertex normals and
+    four smoothing iterations (1e-5 per vertex, norm-wise), Gaussian
+    curvature (defect 1e-5 rad, mixed area 1e-5 relative) and ARAP weights
+    (1e-5 x max(1, |w|)) against the FP64 oracle."""
+    import torch
+    import rxmesh_amd as rx
+    rng = np.random.default_rng(2000 + seed)
+    if seed % 2 == 0:
+        V, F = mg.icosphere(int(rng.integers(3, 5)), seed=seed, sigma=0.01)
+    else:
+        V, F = mg.disk(int(rng.integers(24, 48)), seed=seed)
+    V, F = mg.permute_mesh(V, F, seed=seed)
+    ps = int(rng.integers(96, 513))
+    print("case", seed, len(F), ps, flush=True)
+    m = rx.Mesh(F, patch_size=ps)
+    o = OracleMesh(F).set_patches(m.array("face_patch"), m.array("patches_val"), m.array("patches_offset"))
+    V32 = V.astype(np.float32)
+    V64 = V32.astype(np.float64)
+    x = _lin(m, V32)
+    rp, col, val, mass = (t.cpu().numpy() for t in m.cot_assemble(x, 1e-3))
+    orows, ocols, ovals, omass = o.cot_assemble(V64, 1e-3)
+    g2l = m.array("v_g2l")
+    order = np.argsort(g2l[orows], kind="stable")
+    assert np.array_equal(col, g2l[ocols[order]])
+    assert np.abs(val - ovals[order]).max() <= 1e-5 * np.abs(ovals).max()
+    assert np.abs(_glob(m, mass) - omass).max() <= 1e-5 * np.abs(omass).max()
+
+    def rel(got, ref):
+        return (np.linalg.norm(got - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)).max()
+
+    n = _glob(m, m.vertex_normals(x, "max").cpu().numpy()).T
+    assert rel(n, o.vertex_normals(V64, "max")) <= 1e-5
+    s = _glob(m, m.smooth(x, 0.5, 4).cpu().numpy()).T
+    assert rel(s, o.smooth_uniform(V64, 0.5, 4)) <= 1e-5
+    gc, am = (t.cpu().numpy() for t in m.gaussian_curvature(x))
+    rg, ra = o.gaussian_curvature(V64)
+    assert np.abs(_glob(m, gc * am) - rg * ra).max() <= 1e-5
+    assert np.abs(_glob(m, am) - ra).max() <= 1e-5 * np.abs(ra).max()
+    w = _glob(m, m.arap_weights(x).cpu().numpy().astype(np.float64), "e_l2g")
+    wr = o.arap_weights(V64)
+    assert np.all(np.abs(w - wr) <= 1e-5 * np.maximum(1.0, np.abs(wr)))
