@@ -12,6 +12,7 @@
 #include <vector>
 
 #include "rxmesh/rxmesh_static.h"
+#include "rxmesh/matrix/sparse_matrix.h"
 
 using namespace rxmesh;
 
@@ -74,6 +75,40 @@ __global__ void calc_edge_weights(const rxmesh::Context context, const rxmesh::V
         weight /= 2;
         weight = std::max(T(0), weight);
         edge_weights[context.linear_id(edge_id)] = weight;
+    };
+    auto                block = cooperative_groups::this_thread_block();
+    Query<blockThreads> query(context);
+    ShmemAllocator      shrd_alloc;
+    query.dispatch<Op::EVDiamond>(block, shrd_alloc, calc_weights);
+}
+
+// the same kernel with the reference's SparseMatrix output, unchanged from
+// apps/DiffARAP/arap_kernels.h:8-54 but for the name
+template <typename T, uint32_t blockThreads>
+__global__ static void calc_edge_weights_mat(const rxmesh::Context            context,
+                                             const rxmesh::VertexAttribute<T> coords,
+                                             rxmesh::SparseMatrix<T>          edge_weights)
+{
+    using namespace rxmesh;
+    auto calc_weights = [&](EdgeHandle edge_id, VertexIterator& vv) {
+        const VertexHandle p_id = vv[0];
+        const VertexHandle r_id = vv[2];
+        const VertexHandle q_id = vv[1];
+        const VertexHandle s_id = vv[3];
+        if (!p_id.is_valid() || !r_id.is_valid() || !q_id.is_valid() || !s_id.is_valid()) {
+            return;
+        }
+        const vec3<T> p = coords.to_glm<3>(p_id);
+        const vec3<T> r = coords.to_glm<3>(r_id);
+        const vec3<T> q = coords.to_glm<3>(q_id);
+        const vec3<T> s = coords.to_glm<3>(s_id);
+        T weight = 0;
+        weight += dot((p - q), (r - q)) / length(cross(p - q, r - q));
+        weight += dot((p - s), (r - s)) / length(cross(p - s, r - s));
+        weight /= 2;
+        weight = std::max(0.f, weight);
+        edge_weights(p_id, r_id) = weight;
+        edge_weights(r_id, p_id) = weight;
     };
     auto                block = cooperative_groups::this_thread_block();
     Query<blockThreads> query(context);
@@ -250,6 +285,45 @@ int api_arap_weights(const uint32_t* fv, uint32_t nF, const float* xyz, uint32_t
         return rc;
     } catch (const std::exception& e) {
         std::fprintf(stderr, "api_arap_weights: %s\n", e.what());
+        return 6;
+    }
+}
+
+// ARAP edge weights through the SparseMatrix form of the kernel: out[2 e]
+// and out[2 e + 1] are the (v0, v1) and (v1, v0) entries of linear edge e,
+// read back through SparseMatrix::operator() in an EV pass.
+int api_arap_weights_mat(const uint32_t* fv, uint32_t nF, const float* xyz, uint32_t nV, uint32_t patch_size,
+                         float* out)
+{
+    try {
+        auto                            f = to_fv(fv, nF);
+        RXMeshStatic                    rx(f, "", patch_size);
+        std::vector<std::vector<float>> verts(nV, std::vector<float>(3));
+        for (uint32_t v = 0; v < nV; ++v)
+            for (int i = 0; i < 3; ++i) verts[v][i] = xyz[3 * v + i];
+        rx.add_vertex_coordinates(verts);
+        auto                 coords = rx.get_input_vertex_coordinates();
+        SparseMatrix<float>  weight_matrix(rx);
+        weight_matrix.reset(0.f, LOCATION_ALL);
+        LaunchBox<256> lb;
+        rx.prepare_launch_box({Op::EVDiamond}, lb, (void*)calc_edge_weights_mat<float, 256>);
+        calc_edge_weights_mat<float, 256>
+            <<<lb.blocks, lb.num_threads, lb.smem_bytes_dyn>>>(rx.get_context(), *coords, weight_matrix);
+        const uint32_t nE  = rx.get_num_edges();
+        float*         d   = nullptr;
+        const Context  ctx = rx.get_context();
+        if (hipMalloc(&d, 2 * nE * sizeof(float)) != hipSuccess) return 5;
+        rx.for_each<Op::EV, 256>([=] __device__(EdgeHandle eh, VertexIterator & ev) {
+            const uint32_t e = ctx.linear_id(eh);
+            d[2 * e]         = weight_matrix(ev[0], ev[1]);
+            d[2 * e + 1]     = weight_matrix(ev[1], ev[0]);
+        });
+        int rc = (hipMemcpy(out, d, 2 * nE * sizeof(float), hipMemcpyDeviceToHost) == hipSuccess) ? 0 : 5;
+        (void)hipFree(d);
+        weight_matrix.release();
+        return rc;
+    } catch (const std::exception& e) {
+        std::fprintf(stderr, "api_arap_weights_mat: %s\n", e.what());
         return 6;
     }
 }

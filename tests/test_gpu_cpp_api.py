@@ -34,6 +34,7 @@ def api():
         L.api_vertex_normals.argtypes = [vp, u32, vp, u32, u32, vp]
         L.api_misc.argtypes = [vp, u32, u32, ctypes.c_char_p, i]
         L.api_arap_weights.argtypes = [vp, u32, vp, u32, u32, vp]
+        L.api_arap_weights_mat.argtypes = [vp, u32, vp, u32, u32, vp]
         _api = L
     return _api
 
@@ -139,6 +140,26 @@ def test_api_arap_weights_kernel(case, ps):
     l2g = m.array("v_l2g")
     w = m.arap_weights(torch.from_numpy(np.ascontiguousarray(V[l2g].T)).cuda()).cpu().numpy()
     assert np.all(np.abs(out - w) <= 1e-6 * np.maximum(1.0, np.abs(w)))
+    assert (w > 0).any()
+
+
+@pytest.mark.parametrize("case,ps", [("sphere3.obj", 128), ("disk", 96), ("ico", 128)])
+def test_api_sparse_matrix_arap_weights(case, ps):
+    """The reference's calc_edge_weights_mat, unchanged, writing into the
+    SparseMatrix mirror: both symmetric entries of every edge, read back
+    through SparseMatrix::operator(), equal rxm_arap_weights."""
+    import torch
+    V, F = _case(case)
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    F = np.ascontiguousarray(F, dtype=np.uint32)
+    m = rx.Mesh(F, patch_size=ps)
+    out = np.zeros((m.nE, 2), np.float32)
+    rc = api().api_arap_weights_mat(F.ctypes.data, len(F), V.ctypes.data, len(V), ps, out.ctypes.data)
+    assert rc == 0
+    l2g = m.array("v_l2g")
+    w = m.arap_weights(torch.from_numpy(np.ascontiguousarray(V[l2g].T)).cuda()).cpu().numpy()
+    for k in range(2):
+        assert np.all(np.abs(out[:, k] - w) <= 1e-6 * np.maximum(1.0, np.abs(w)))
     assert (w > 0).any()
 
 
