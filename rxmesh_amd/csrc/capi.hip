@@ -895,7 +895,7 @@ rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* no
     if (prc == RXM_OK) {
         Bump b(nullptr);
         plan_vnp(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
-        return launch_patches_xcd(k_vertex_normals_plan<128>, 128, b.bytes(), stream, m, m->dm, mode, m->d_fp_fbase,
+        return launch_patches_xcd(k_vertex_normals_plan<256>, 256, b.bytes(), stream, m, m->dm, mode, m->d_fp_fbase,
                               m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, normals);
     }
     if (prc != RXM_ERR_PATCH_TOO_BIG) return prc;
@@ -912,7 +912,7 @@ rxm_status rxm_gaussian_curvature(rxm_mesh* m, const float* pos, float* gc, floa
     if (rc) return rc;
     Bump b(nullptr);
     plan_vnp(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
-    return launch_patches_xcd(k_gaussian_curvature<128>, 128, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
+    return launch_patches_xcd(k_gaussian_curvature<256>, 256, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
                               m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, gc,
                               area_mixed);
 }
