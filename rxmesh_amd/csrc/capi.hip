@@ -950,7 +950,7 @@ rxm_status rxm_arap_rotations(rxm_mesh* m, const float* rest, const float* x, co
     uint32_t        lds     = 0;
     const int       rc      = arap_rows(m, &vv_base, &lds, false);
     if (rc) return rc;
-    return launch_patches_xcd(k_arap_rotations<256>, 256, lds, stream, m, m->dm, vv_base,
+    return launch_patches_xcd(k_arap_rotations<512>, 512, lds, stream, m, m->dm, vv_base,
                               static_cast<const uint32_t*>(m->d_cot_rs), static_cast<const uint32_t*>(m->d_cot_ent),
                               rest, x, w, R);
 }
@@ -969,7 +969,7 @@ rxm_status rxm_arap_energy_grad(rxm_mesh* m, const float* rest, const float* x, 
         m->allocs.push_back(m->d_partial);
         m->device_bytes += sizeof(float) * m->h.P;
     }
-    rc = launch_patches_xcd(k_arap_energy_grad<256>, 256, lds, stream, m, m->dm, vv_base,
+    rc = launch_patches_xcd(k_arap_energy_grad<512>, 512, lds, stream, m, m->dm, vv_base,
                             static_cast<const uint32_t*>(m->d_cot_rs), static_cast<const uint32_t*>(m->d_cot_ent),
                             rest, x, w, R, m->d_partial, grad);
     if (rc) return rc;
