@@ -335,6 +335,12 @@ rxm_status rxm_symdir3_energy(rxm_mesh* mesh, const float* Mr, const float* x, f
  * without atomics (x, x_out: 3 x V SoA, distinct). */
 rxm_status rxm_symdir3_grad_step(rxm_mesh* mesh, const float* Mr, const float* x, float eta,
                                  float* x_out, void* stream);
+/* grad (3 x V SoA) = d(sum_f E_f)/dx of the 3-D energy above, the same
+ * adjoint as rxm_symdir3_grad_step (per-face Jacobians recomputed, gathered
+ * per vertex over VF without atomics) written out instead of applied.
+ * Replaces the gradient DiffScalarProblem::eval_terms_grad_only returns
+ * (include/rxmesh/diff/diff_scalar_problem.h:327-335). x and grad distinct. */
+rxm_status rxm_symdir3_grad(rxm_mesh* mesh, const float* Mr, const float* x, float* grad, void* stream);
 /* `steps` iterations of { x1 = one uniform smoothing step of x (lambda);
  * x = x1 - eta grad E(x1) } in place on x, `tmp` a 3V workspace. Two launches
  * per step over all patches (all meshes of a batch). */

@@ -192,8 +192,29 @@ void assign_patch(Mesh& m)
     }
 }
 
+// Owned edges / vertices of every patch in ascending id order (bucketed once,
+// so the safeguard loops of build_ltog visit exactly the elements the
+// reference's full scans over all edges / vertices would accept, in the same
+// order, without O(P (V + E)) work).
+struct OwnedLists
+{
+    std::vector<uint32_t> e_off, e_val, v_off, v_val;
+};
+static void bucket(const std::vector<uint32_t>& owner, uint32_t P, std::vector<uint32_t>& off,
+                   std::vector<uint32_t>& val)
+{
+    off.assign(P + 1, 0);
+    for (uint32_t o : owner)
+        if (o < P) off[o + 1]++;
+    for (uint32_t p = 0; p < P; ++p) off[p + 1] += off[p];
+    val.assign(off[P], 0);
+    std::vector<uint32_t> cur(off.begin(), off.end() - 1);
+    for (uint32_t i = 0; i < owner.size(); ++i)
+        if (owner[i] < P) val[cur[owner[i]]++] = i;
+}
+
 // reference: include/rxmesh/rxmesh.cpp:754-870 (build_single_patch_ltog)
-void build_ltog(Mesh& m, uint32_t p)
+void build_ltog(Mesh& m, uint32_t p, const OwnedLists& own)
 {
     Patch&   P  = m.patches[p];
     uint32_t ps = (p == 0) ? 0 : m.patches_offset[p - 1];
@@ -217,15 +238,18 @@ void build_ltog(Mesh& m, uint32_t p)
     for (uint32_t k = rs; k < re; ++k) add_face(m.ribbon_val[k]);
     // Safeguard loops of the reference (owned elements not reached through a
     // face). With the reference's ownership rule they never add anything.
-    for (uint32_t e = 0; e < m.nE; ++e)
-        if (m.edge_patch[e] == p && eadded.insert(e).second) {
+    for (uint32_t k = own.e_off[p]; k < own.e_off[p + 1]; ++k) {
+        const uint32_t e = own.e_val[k];  // edge_patch[e] == p, ascending e
+        if (eadded.insert(e).second) {
             P.ltog_e.push_back(e);
             for (uint32_t i = 0; i < 2; ++i)
                 if (!vadded.count(m.ev[e][i])) P.ltog_v.push_back(m.ev[e][i]);
         }
-    for (uint32_t v = 0; v < m.nV; ++v)
-        if (m.vertex_patch[v] == p && vadded.insert(v).second)
-            P.ltog_v.push_back(v);
+    }
+    for (uint32_t k = own.v_off[p]; k < own.v_off[p + 1]; ++k) {
+        const uint32_t v = own.v_val[k];  // vertex_patch[v] == p, ascending v
+        if (vadded.insert(v).second) P.ltog_v.push_back(v);
+    }
 
     auto uniq = [&](std::vector<uint32_t>& l, const std::vector<uint32_t>& owner) {
         std::sort(l.begin(), l.end());
@@ -627,7 +651,10 @@ int ora_set_patches(ora_mesh* h, const uint32_t* face_patch, const uint32_t* pat
     extract_ribbons(m);
     assign_patch(m);
     m.patches.assign(P, Patch());
-    for (uint32_t p = 0; p < P; ++p) build_ltog(m, p);
+    OwnedLists own;
+    bucket(m.edge_patch, P, own.e_off, own.e_val);
+    bucket(m.vertex_patch, P, own.v_off, own.v_val);
+    for (uint32_t p = 0; p < P; ++p) build_ltog(m, p, own);
     for (uint32_t p = 0; p < P; ++p) {
         if (m.patches[p].ltog_v.size() >= INV16 || m.patches[p].ltog_e.size() >= INV16 ||
             m.patches[p].ltog_f.size() >= INV16)

@@ -75,6 +75,7 @@ _SIGS = {
     "rxm_batch_offsets": [_vp, _i, _vp, _u32],
     "rxm_symdir3_energy": [_vp, _vp, _vp, _vp, _vp],
     "rxm_symdir3_grad_step": [_vp, _vp, _vp, _f, _vp, _vp],
+    "rxm_symdir3_grad": [_vp, _vp, _vp, _vp, _vp],
     "rxm_smooth_symdir_steps": [_vp, _vp, _vp, _vp, _f, _f, _i, _vp],
     "rxm_smooth_symdir_steps_each": [_vp, _vp, _vp, _vp, _u32, _f, _f, _i, _vp],
     "rxm_graph_capture_smooth_symdir": [_vp, _vp, _vp, _vp, _f, _f, _i, ctypes.POINTER(_vp)],
@@ -136,22 +137,52 @@ def _stream(stream):
     return ctypes.c_void_p(stream)
 
 
-def _ptr(t):
+def _dev(t, name, shape=None, dtype="float32", min_numel=None):
+    """ctypes pointer to a tensor the C-ABI reads or writes, after checking
+    what the C-ABI cannot see through a raw pointer: the dtype, the exact
+    shape (or, for buffers sized by a count, a minimum element count),
+    contiguity and that the tensor lives on the GPU. Raises ValueError before
+    anything is launched, so a transposed view, a (V, 3) array where (3, V)
+    SoA is expected, an undersized buffer or a float16 tensor never reaches a
+    kernel."""
+    import torch
+    if not isinstance(t, torch.Tensor):
+        raise ValueError("%s: expected a torch tensor, got %s" % (name, type(t).__name__))
+    want = {"float32": torch.float32, "int32": torch.int32}[dtype]
+    if t.dtype != want:
+        raise ValueError("%s: expected dtype %s, got %s" % (name, want, t.dtype))
+    if shape is not None and tuple(t.shape) != tuple(int(n) for n in shape):
+        raise ValueError("%s: expected shape %s, got %s" % (name, tuple(int(n) for n in shape), tuple(t.shape)))
+    if min_numel is not None and (t.dim() != 1 or t.numel() < min_numel):
+        raise ValueError("%s: expected a 1-D tensor of at least %d elements, got shape %s"
+                         % (name, min_numel, tuple(t.shape)))
+    if not t.is_contiguous():
+        raise ValueError("%s: tensor must be contiguous (call .contiguous())" % name)
+    if not t.is_cuda:
+        raise ValueError("%s: tensor must be on the GPU" % name)
     return ctypes.c_void_p(t.data_ptr())
+
+
+def _opt(t, *args, **kw):
+    """_dev for an optional tensor (None passes a null pointer)."""
+    return None if t is None else _dev(t, *args, **kw)
 
 
 def build_edges_device(fv, num_vertices, stream=None):
     """Edges of a triangle list on the device, numbered as the host build
-    numbers them (first appearance over the half-edges). fv: (F, 3) int32 /
-    uint32 device tensor. Returns (ev (E, 2) int32, fe (3F,) int32)."""
+    numbers them (first appearance over the half-edges). fv: (F, 3) int32
+    device tensor. Returns (ev (E, 2) int32, fe (3F,) int32)."""
     import torch
-    fv = fv.contiguous()
+    if not isinstance(fv, torch.Tensor) or fv.dim() != 2 or fv.shape[1] != 3:
+        raise ValueError("fv: expected an (F, 3) tensor")
     nF = fv.shape[0]
+    pf = _dev(fv, "fv", (nF, 3), "int32")
     ev = torch.empty(2 * 3 * nF, dtype=torch.int32, device=fv.device)
     fe = torch.empty(3 * nF, dtype=torch.int32, device=fv.device)
     ne = ctypes.c_uint32()
-    _check(lib().rxm_build_edges_device(_ptr(fv), nF, int(num_vertices), _ptr(ev), _ptr(fe), ctypes.byref(ne),
-                                        _stream(stream)), "rxm_build_edges_device")
+    _check(lib().rxm_build_edges_device(pf, nF, int(num_vertices), ctypes.c_void_p(ev.data_ptr()),
+                                        ctypes.c_void_p(fe.data_ptr()), ctypes.byref(ne), _stream(stream)),
+           "rxm_build_edges_device")
     return ev[: 2 * ne.value].view(-1, 2), fe
 
 
@@ -230,6 +261,15 @@ class Mesh:
     def save_patches(self, path):
         _check(lib().rxm_mesh_save_patches(self._h, path.encode()), "rxm_mesh_save_patches")
 
+    # -- argument shapes ------------------------------------------------
+    def _v3(self, t, name):
+        """(3, V) float32 SoA positions / vertex vectors over linear ids."""
+        return _dev(t, name, (3, self.nV))
+
+    def _mr(self, Mr):
+        """(4, F) float32 rest shapes (rxm_symdir_rest)."""
+        return _dev(Mr, "Mr", (4, self.nF))
+
     # -- queries --------------------------------------------------------
     def query_nnz(self, op):
         n = ctypes.c_int64()
@@ -239,14 +279,16 @@ class Mesh:
     def query_vv_vf(self, vv=None, vf=None, stream=None):
         """VV and VF CSRs (same as query("VV") and query("VF")) from one launch."""
         import torch
+        nvv, nvf = self.query_nnz("VV"), self.query_nnz("VF")
         if vv is None:
             vv = (torch.empty(self.nV + 1, dtype=torch.int32, device="cuda"),
-                  torch.empty(max(self.query_nnz("VV"), 1), dtype=torch.int32, device="cuda"))
+                  torch.empty(max(nvv, 1), dtype=torch.int32, device="cuda"))
         if vf is None:
             vf = (torch.empty(self.nV + 1, dtype=torch.int32, device="cuda"),
-                  torch.empty(max(self.query_nnz("VF"), 1), dtype=torch.int32, device="cuda"))
-        _check(lib().rxm_query_vv_vf(self._h, _ptr(vv[0]), _ptr(vv[1]), _ptr(vf[0]), _ptr(vf[1]), _stream(stream)),
-               "rxm_query_vv_vf")
+                  torch.empty(max(nvf, 1), dtype=torch.int32, device="cuda"))
+        args = (_dev(vv[0], "vv row_ptr", (self.nV + 1,), "int32"), _dev(vv[1], "vv col", None, "int32", nvv),
+                _dev(vf[0], "vf row_ptr", (self.nV + 1,), "int32"), _dev(vf[1], "vf col", None, "int32", nvf))
+        _check(lib().rxm_query_vv_vf(self._h, *args, _stream(stream)), "rxm_query_vv_vf")
         return vv, vf
 
     def num_src(self, op):
@@ -255,12 +297,14 @@ class Mesh:
     def query(self, op, oriented=False, row_ptr=None, col=None, stream=None):
         """CSR (row_ptr, col) of `op` over linear ids, as int32 device tensors."""
         import torch
+        nnz = self.query_nnz(op)
         if row_ptr is None:
             row_ptr = torch.empty(self.num_src(op) + 1, dtype=torch.int32, device="cuda")
         if col is None:
-            col = torch.empty(max(self.query_nnz(op), 1), dtype=torch.int32, device="cuda")
-        _check(lib().rxm_query_csr(self._h, OP[op], int(oriented), _ptr(row_ptr), _ptr(col), _stream(stream)),
-               "rxm_query_csr")
+            col = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
+        prp = _dev(row_ptr, "row_ptr", (self.num_src(op) + 1,), "int32")
+        pcol = _dev(col, "col", None, "int32", nnz)
+        _check(lib().rxm_query_csr(self._h, OP[op], int(oriented), prp, pcol, _stream(stream)), "rxm_query_csr")
         return row_ptr, col
 
     # -- geometry -------------------------------------------------------
@@ -268,8 +312,8 @@ class Mesh:
         import torch
         if out is None:
             out = torch.empty_like(pos)
-        _check(lib().rxm_vertex_normals(self._h, 0 if mode == "max" else 1, _ptr(pos), _ptr(out),
-                                        _stream(stream)), "rxm_vertex_normals")
+        _check(lib().rxm_vertex_normals(self._h, 0 if mode == "max" else 1, self._v3(pos, "pos"),
+                                        self._v3(out, "out"), _stream(stream)), "rxm_vertex_normals")
         return out
 
     def gaussian_curvature(self, pos, out=None, stream=None):
@@ -279,8 +323,9 @@ class Mesh:
             out = (torch.empty(self.nV, dtype=torch.float32, device=pos.device),
                    torch.empty(self.nV, dtype=torch.float32, device=pos.device))
         gc, am = out
-        _check(lib().rxm_gaussian_curvature(self._h, _ptr(pos), _ptr(gc), _ptr(am) if am is not None else None,
-                                            _stream(stream)), "rxm_gaussian_curvature")
+        _check(lib().rxm_gaussian_curvature(self._h, self._v3(pos, "pos"), _dev(gc, "gc", (self.nV,)),
+                                            _opt(am, "area_mixed", (self.nV,)), _stream(stream)),
+               "rxm_gaussian_curvature")
         return gc, am
 
     # -- ARAP-style deformation ---------------------------------------------
@@ -289,7 +334,8 @@ class Mesh:
         import torch
         if out is None:
             out = torch.empty(self.nE, dtype=torch.float32, device=pos.device)
-        _check(lib().rxm_arap_weights(self._h, _ptr(pos), _ptr(out), _stream(stream)), "rxm_arap_weights")
+        _check(lib().rxm_arap_weights(self._h, self._v3(pos, "pos"), _dev(out, "w", (self.nE,)), _stream(stream)),
+               "rxm_arap_weights")
         return out
 
     def arap_rotations(self, rest, x, w, out=None, stream=None):
@@ -297,7 +343,8 @@ class Mesh:
         import torch
         if out is None:
             out = torch.empty(self.nV, 9, dtype=torch.float32, device=x.device)
-        _check(lib().rxm_arap_rotations(self._h, _ptr(rest), _ptr(x), _ptr(w), _ptr(out), _stream(stream)),
+        _check(lib().rxm_arap_rotations(self._h, self._v3(rest, "rest"), self._v3(x, "x"),
+                                        _dev(w, "w", (self.nE,)), _dev(out, "R", (self.nV, 9)), _stream(stream)),
                "rxm_arap_rotations")
         return out
 
@@ -308,7 +355,9 @@ class Mesh:
             total = torch.empty(1, dtype=torch.float32, device=x.device)
         if grad is None:
             grad = torch.empty_like(x)
-        _check(lib().rxm_arap_energy_grad(self._h, _ptr(rest), _ptr(x), _ptr(w), _ptr(R), _ptr(total), _ptr(grad),
+        _check(lib().rxm_arap_energy_grad(self._h, self._v3(rest, "rest"), self._v3(x, "x"),
+                                          _dev(w, "w", (self.nE,)), _dev(R, "R", (self.nV, 9)),
+                                          _dev(total.view(-1), "total", None, "float32", 1), self._v3(grad, "grad"),
                                           _stream(stream)), "rxm_arap_energy_grad")
         return total, grad
 
@@ -318,8 +367,9 @@ class Mesh:
             out = torch.empty_like(pos)
         if tmp is None and iters > 1:
             tmp = torch.empty_like(pos)
-        _check(lib().rxm_laplacian_smooth(self._h, _ptr(pos), _ptr(out), _ptr(tmp) if tmp is not None else None,
-                                          float(lam), int(iters), _stream(stream)), "rxm_laplacian_smooth")
+        _check(lib().rxm_laplacian_smooth(self._h, self._v3(pos, "pos"), self._v3(out, "out"),
+                                          None if tmp is None else self._v3(tmp, "tmp"), float(lam), int(iters),
+                                          _stream(stream)), "rxm_laplacian_smooth")
         return out
 
     def build_vv_plan(self, stream=None):
@@ -335,8 +385,10 @@ class Mesh:
             out = torch.empty_like(pos)
         if tmp is None and iters > 1:
             tmp = torch.empty_like(pos)
-        _check(lib().rxm_laplacian_smooth_csr(self._h, _ptr(vv[0]), _ptr(vv[1]), _ptr(pos), _ptr(out),
-                                              _ptr(tmp) if tmp is not None else None, float(lam), int(iters),
+        _check(lib().rxm_laplacian_smooth_csr(self._h, _dev(vv[0], "vv row_ptr", (self.nV + 1,), "int32"),
+                                              _dev(vv[1], "vv col", None, "int32", self.query_nnz("VV")),
+                                              self._v3(pos, "pos"), self._v3(out, "out"),
+                                              None if tmp is None else self._v3(tmp, "tmp"), float(lam), int(iters),
                                               _stream(stream)), "rxm_laplacian_smooth_csr")
         return out
 
@@ -357,21 +409,31 @@ class Mesh:
         import torch
         if out is None:
             out = torch.empty(self.nF, dtype=torch.float32, device="cuda")
-        _check(lib().rxm_symdir3_energy(self._h, _ptr(Mr), _ptr(x), _ptr(out), _stream(stream)),
-               "rxm_symdir3_energy")
+        _check(lib().rxm_symdir3_energy(self._h, self._mr(Mr), self._v3(x, "x"), _dev(out, "energy", (self.nF,)),
+                                        _stream(stream)), "rxm_symdir3_energy")
         return out
 
     def symdir3_grad_step(self, Mr, x, eta, out=None, stream=None):
         import torch
         if out is None:
             out = torch.empty_like(x)
-        _check(lib().rxm_symdir3_grad_step(self._h, _ptr(Mr), _ptr(x), float(eta), _ptr(out), _stream(stream)),
-               "rxm_symdir3_grad_step")
+        _check(lib().rxm_symdir3_grad_step(self._h, self._mr(Mr), self._v3(x, "x"), float(eta),
+                                           self._v3(out, "out"), _stream(stream)), "rxm_symdir3_grad_step")
+        return out
+
+    def symdir3_grad(self, Mr, x, out=None, stream=None):
+        """dE/dx (3, V) of the 3-D symmetric Dirichlet energy (rxm_symdir3_grad)."""
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        _check(lib().rxm_symdir3_grad(self._h, self._mr(Mr), self._v3(x, "x"), self._v3(out, "grad"),
+                                      _stream(stream)), "rxm_symdir3_grad")
         return out
 
     def smooth_symdir_steps(self, Mr, x, tmp, lam, eta, steps, stream=None):
-        _check(lib().rxm_smooth_symdir_steps(self._h, _ptr(Mr), _ptr(x), _ptr(tmp), float(lam), float(eta),
-                                             int(steps), _stream(stream)), "rxm_smooth_symdir_steps")
+        _check(lib().rxm_smooth_symdir_steps(self._h, self._mr(Mr), self._v3(x, "x"), self._v3(tmp, "tmp"),
+                                             float(lam), float(eta), int(steps), _stream(stream)),
+               "rxm_smooth_symdir_steps")
         return x
 
     def capture_smooth_symdir(self, Mr, x, tmp, lam, eta, steps):
@@ -383,32 +445,40 @@ class Mesh:
         _check(lib().rxm_cot_nnz(self._h, ctypes.byref(n)), "rxm_cot_nnz")
         return n.value
 
+    def _system(self, A):
+        """Pointers of a cotangent system (row_ptr, col, val, mass) as
+        rxm_cot_csr_assemble lays it out, every array checked against the
+        mesh's sizes (the patch-local kernels read val by the mesh's own row
+        layout)."""
+        rp, col, val, mass = A[:4]
+        nnz = self.cot_nnz()
+        pv, pm = _dev(val, "val", (nnz,)), _dev(mass, "mass", (self.nV,))  # the arrays the kernels read
+        return _dev(rp, "row_ptr", (self.nV + 1,), "int32"), _dev(col, "col", (nnz,), "int32"), pv, pm
+
     def cot_assemble(self, pos, dt, out=None, stream=None):
         """(row_ptr, col, val, mass) of A = M + dt L on device; `out` reuses
         the four tensors of an earlier call."""
         import torch
         if out is None:
             nnz = self.cot_nnz()
-            rp = torch.empty(self.nV + 1, dtype=torch.int32, device="cuda")
-            col = torch.empty(nnz, dtype=torch.int32, device="cuda")
-            val = torch.empty(nnz, dtype=torch.float32, device="cuda")
-            mass = torch.empty(self.nV, dtype=torch.float32, device="cuda")
-        else:
-            rp, col, val, mass = out
-            nnz = self.cot_nnz()
-            if (rp.numel(), col.numel(), val.numel(), mass.numel()) != (self.nV + 1, nnz, nnz, self.nV):
-                raise ValueError("cot_assemble: out tensors do not match the system's sizes")
-        _check(lib().rxm_cot_csr_assemble(self._h, _ptr(pos), float(dt), _ptr(rp), _ptr(col), _ptr(val),
-                                          _ptr(mass), _stream(stream)), "rxm_cot_csr_assemble")
-        return rp, col, val, mass
+            out = (torch.empty(self.nV + 1, dtype=torch.int32, device="cuda"),
+                   torch.empty(nnz, dtype=torch.int32, device="cuda"),
+                   torch.empty(nnz, dtype=torch.float32, device="cuda"),
+                   torch.empty(self.nV, dtype=torch.float32, device="cuda"))
+        _check(lib().rxm_cot_csr_assemble(self._h, self._v3(pos, "pos"), float(dt), *self._system(out),
+                                          _stream(stream)), "rxm_cot_csr_assemble")
+        return tuple(out)
 
     def spmv3(self, A, x, out=None, stream=None):
+        """y = A x over the mesh's V rows for a CSR A = (row_ptr, col, val, ...)."""
         import torch
         rp, col, val = A[:3]
         if out is None:
             out = torch.empty_like(x)
-        _check(lib().rxm_spmv3(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(x), _ptr(out), _stream(stream)),
-               "rxm_spmv3")
+        pc = _dev(col, "col", (col.numel(),), "int32")
+        pv = _dev(val, "val", (col.numel(),))
+        _check(lib().rxm_spmv3(self._h, _dev(rp, "row_ptr", (self.nV + 1,), "int32"), pc, pv, self._v3(x, "x"),
+                               self._v3(out, "out"), _stream(stream)), "rxm_spmv3")
         return out
 
     def mcf_implicit_cg(self, A, x, max_iter=100, abs_tol=1e-6, rel_tol=0.0, out=None, stats=True, stream=None):
@@ -416,12 +486,11 @@ class Mesh:
         start |r|^2, final |r|^2) -- the last three None when stats is False
         (no host synchronisation)."""
         import torch
-        rp, col, val, mass = A
         if out is None:
             out = torch.empty_like(x)
         it, r0, r1 = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
         refs = (ctypes.byref(it), ctypes.byref(r0), ctypes.byref(r1)) if stats else (None, None, None)
-        _check(lib().rxm_mcf_implicit_cg(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(mass), _ptr(x), _ptr(out),
+        _check(lib().rxm_mcf_implicit_cg(self._h, *self._system(A), self._v3(x, "x"), self._v3(out, "out"),
                                          int(max_iter), float(abs_tol), float(rel_tol), *refs, _stream(stream)),
                "rxm_mcf_implicit_cg")
         if not stats:
@@ -430,21 +499,21 @@ class Mesh:
 
     def mcf_explicit(self, A, x, iters, out=None, tmp=None, stream=None):
         import torch
-        rp, col, val, mass = A
         if out is None:
             out = torch.empty_like(x)
         if tmp is None and iters > 1:
             tmp = torch.empty_like(x)
-        _check(lib().rxm_mcf_explicit(self._h, _ptr(rp), _ptr(col), _ptr(val), _ptr(mass), _ptr(x), _ptr(out),
-                                      _ptr(tmp) if tmp is not None else None, int(iters), _stream(stream)),
+        _check(lib().rxm_mcf_explicit(self._h, *self._system(A), self._v3(x, "x"), self._v3(out, "out"),
+                                      None if tmp is None else self._v3(tmp, "tmp"), int(iters), _stream(stream)),
                "rxm_mcf_explicit")
         return out
 
     # -- symmetric Dirichlet --------------------------------------------
     def symdir_rest(self, rest_pos, stream=None):
         import torch
+        pr = self._v3(rest_pos, "rest_pos")
         Mr = torch.empty(4, self.nF, dtype=torch.float32, device="cuda")
-        _check(lib().rxm_symdir_rest(self._h, _ptr(rest_pos), _ptr(Mr), _stream(stream)), "rxm_symdir_rest")
+        _check(lib().rxm_symdir_rest(self._h, pr, self._mr(Mr), _stream(stream)), "rxm_symdir_rest")
         return Mr
 
     def symdir_energy(self, Mr, uv, energy=None, total=None, stream=None):
@@ -453,7 +522,9 @@ class Mesh:
             energy = torch.empty(self.nF, dtype=torch.float32, device="cuda")
         if total is None:
             total = torch.empty((), dtype=torch.float32, device="cuda")
-        _check(lib().rxm_symdir_energy(self._h, _ptr(Mr), _ptr(uv), _ptr(energy), _ptr(total), _stream(stream)),
+        _check(lib().rxm_symdir_energy(self._h, self._mr(Mr), _dev(uv, "uv", (2, self.nV)),
+                                       _dev(energy, "energy", (self.nF,)),
+                                       _dev(total.view(-1), "total", None, "float32", 1), _stream(stream)),
                "rxm_symdir_energy")
         return total, energy
 
@@ -461,25 +532,31 @@ class Mesh:
         import torch
         if out is None:
             out = torch.empty_like(uv)
-        _check(lib().rxm_symdir_grad(self._h, _ptr(Mr), _ptr(uv), _ptr(scale) if scale is not None else None,
-                                     _ptr(out), _stream(stream)), "rxm_symdir_grad")
+        ps = None if scale is None else _dev(scale.view(-1), "scale", None, "float32", 1)
+        _check(lib().rxm_symdir_grad(self._h, self._mr(Mr), _dev(uv, "uv", (2, self.nV)), ps,
+                                     _dev(out, "grad", (2, self.nV)), _stream(stream)), "rxm_symdir_grad")
         return out
 
+    def _perm_args(self, x, out, kind):
+        n = (self.nV, self.nE, self.nF)[kind]
+        shape = (n,) if x.dim() == 1 else (x.shape[0], n)
+        return x.shape[0] if x.dim() == 2 else 1, _dev(x, "x", shape), _dev(out, "out", shape)
+
     def to_linear(self, x, kind=0, out=None, stream=None):
-        """x: (ncomp, n) float32 device tensor in global order -> linear order."""
+        """x: (ncomp, n) or (n,) float32 device tensor in global order -> linear order."""
         import torch
         if out is None:
             out = torch.empty_like(x)
-        ncomp = x.shape[0] if x.dim() == 2 else 1
-        _check(lib().rxm_to_linear(self._h, kind, ncomp, _ptr(x), _ptr(out), _stream(stream)), "rxm_to_linear")
+        ncomp, px, po = self._perm_args(x, out, kind)
+        _check(lib().rxm_to_linear(self._h, kind, ncomp, px, po, _stream(stream)), "rxm_to_linear")
         return out
 
     def to_global(self, x, kind=0, out=None, stream=None):
         import torch
         if out is None:
             out = torch.empty_like(x)
-        ncomp = x.shape[0] if x.dim() == 2 else 1
-        _check(lib().rxm_to_global(self._h, kind, ncomp, _ptr(x), _ptr(out), _stream(stream)), "rxm_to_global")
+        ncomp, px, po = self._perm_args(x, out, kind)
+        _check(lib().rxm_to_global(self._h, kind, ncomp, px, po, _stream(stream)), "rxm_to_global")
         return out
 
 
@@ -490,8 +567,9 @@ class Graph:
     def __init__(self, mesh, Mr, x, tmp, lam, eta, steps):
         self._keep = (mesh, Mr, x, tmp)
         g = ctypes.c_void_p()
-        _check(lib().rxm_graph_capture_smooth_symdir(mesh._h, _ptr(Mr), _ptr(x), _ptr(tmp), float(lam), float(eta),
-                                                     int(steps), ctypes.byref(g)), "rxm_graph_capture_smooth_symdir")
+        _check(lib().rxm_graph_capture_smooth_symdir(mesh._h, mesh._mr(Mr), mesh._v3(x, "x"), mesh._v3(tmp, "tmp"),
+                                                     float(lam), float(eta), int(steps), ctypes.byref(g)),
+               "rxm_graph_capture_smooth_symdir")
         self._g = g
 
     def launch(self, stream=None):
@@ -512,9 +590,13 @@ class Graph:
 def smooth_symdir_steps_each(meshes, Mrs, xs, tmps, lam, eta, steps, stream=None):
     """Plain per-mesh launches of the step loop over separate meshes."""
     n = len(meshes)
-    arr = lambda items: (ctypes.c_void_p * n)(*items)  # noqa: E731
-    _check(lib().rxm_smooth_symdir_steps_each(arr([m._h.value for m in meshes]), arr([t.data_ptr() for t in Mrs]),
-                                              arr([t.data_ptr() for t in xs]), arr([t.data_ptr() for t in tmps]),
+    if not (len(Mrs) == len(xs) == len(tmps) == n):
+        raise ValueError("smooth_symdir_steps_each: one Mr, x and tmp per mesh")
+    arr = lambda items: (ctypes.c_void_p * n)(*[p.value for p in items])  # noqa: E731
+    _check(lib().rxm_smooth_symdir_steps_each((ctypes.c_void_p * n)(*[m._h.value for m in meshes]),
+                                              arr([m._mr(t) for m, t in zip(meshes, Mrs)]),
+                                              arr([m._v3(t, "x") for m, t in zip(meshes, xs)]),
+                                              arr([m._v3(t, "tmp") for m, t in zip(meshes, tmps)]),
                                               n, float(lam), float(eta), int(steps), _stream(stream)),
            "rxm_smooth_symdir_steps_each")
 
@@ -536,7 +618,9 @@ class SymmetricDirichlet:
             class _SymDir(torch.autograd.Function):
                 @staticmethod
                 def forward(ctx, uv_, mesh_, Mr_):
-                    total, _ = mesh_.symdir_energy(Mr_, uv_.detach())
+                    uv_ = uv_.detach().contiguous()
+                    Mr_ = Mr_.contiguous()
+                    total, _ = mesh_.symdir_energy(Mr_, uv_)
                     ctx.mesh = mesh_
                     ctx.Mr = Mr_
                     ctx.save_for_backward(uv_)
@@ -546,7 +630,7 @@ class SymmetricDirichlet:
                 def backward(ctx, g):
                     (uv_,) = ctx.saved_tensors
                     g = g.detach().to(torch.float32).contiguous()
-                    return ctx.mesh.symdir_grad(ctx.Mr, uv_.detach(), scale=g), None, None
+                    return ctx.mesh.symdir_grad(ctx.Mr, uv_, scale=g), None, None
 
             cls._fn = _SymDir
         return cls._fn.apply(uv, mesh, Mr)

@@ -87,6 +87,7 @@ struct rxm_mesh
     uint32_t        grad_threads   = 256;
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
     uint32_t*       d_queue        = nullptr;  // work-queue ticket counter
+    bool            system_runs_ok = false;    // check_system_runs passed
     int             num_cus        = 0;
     // batch (disjoint union of meshes): per-mesh id / patch offsets
     std::vector<uint32_t> b_v, b_e, b_f, b_p;
@@ -230,6 +231,28 @@ uint32_t op_max_patch(rxm_mesh* m, int op)
     uint32_t    mx = 0;
     for (uint32_t p = 0; p < m->h.P; ++p) mx = std::max(mx, b[p + 1] - b[p]);
     return m->h_maxrow[op] = mx;
+}
+
+// The patch-local MCF and CG kernels stage patch p's run of the cotangent
+// system's values as [vv_base[p] + vp[p], vv_base[p + 1] + vp[p + 1]): the
+// VV entries of its owned rows plus one diagonal per row. Every such run
+// must start where the system's row layout puts the patch and end at or
+// before rxm_cot_nnz, so no workgroup reads past the caller's value array.
+// Checked once per mesh on the host.
+int check_system_runs(rxm_mesh* m)
+{
+    if (m->system_runs_ok) return RXM_OK;
+    const auto&     vb = op_base(m, RXM_OP_VV);
+    const auto&     cb = op_base(m, kOpCot);
+    const HostMesh& h  = m->h;
+    const uint64_t  nnz = cb.back();
+    for (uint32_t p = 0; p < h.P; ++p) {
+        const uint64_t start = uint64_t(vb[p]) + h.vp[p];
+        const uint64_t end   = uint64_t(vb[p + 1]) + h.vp[p] + h.nvo[p];
+        if (start != cb[p] || end != cb[p + 1] || end > nnz) return RXM_ERR_INVALID_ARG;
+    }
+    m->system_runs_ok = true;
+    return RXM_OK;
 }
 
 int device_base(rxm_mesh* m, int op, const uint32_t** out)
@@ -1218,7 +1241,8 @@ rxm_status rxm_mcf_explicit(rxm_mesh* m, const int32_t* row_ptr, const int32_t* 
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
     // patch-local steps over the VV plan (the system's pattern); ping-pong
     // between out and tmp so that the last step writes out
-    int rc = ensure_vv_plan(m, stream);
+    int rc = check_system_runs(m);
+    if (!rc) rc = ensure_vv_plan(m, stream);
     if (rc) return rc;
     const uint32_t* vv_base = nullptr;
     if ((rc = device_base(m, RXM_OP_VV, &vv_base))) return rc;
@@ -1243,7 +1267,8 @@ rxm_status rxm_mcf_implicit_cg(rxm_mesh* m, const int32_t* row_ptr, const int32_
     if (!m || !row_ptr || !col || !val || !mass || !x || !out || x == out || max_iter < 0)
         return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    int rc = ensure_vv_plan(m, stream);
+    int rc = check_system_runs(m);
+    if (!rc) rc = ensure_vv_plan(m, stream);
     if (rc) return rc;
     const uint32_t* vv_base = nullptr;
     if ((rc = device_base(m, RXM_OP_VV, &vv_base))) return rc;
@@ -1399,6 +1424,20 @@ rxm_status rxm_symdir3_grad_step(rxm_mesh* m, const float* Mr, const float* x, f
     int rc = prepare_steps(m, stream);
     if (rc) return rc;
     return launch_symdir3_step(m, Mr, eta, x, x_out, stream);
+}
+
+rxm_status rxm_symdir3_grad(rxm_mesh* m, const float* Mr, const float* x, float* grad, void* stream)
+{
+    if (!m || !Mr || !x || !grad || x == grad) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    int rc = ensure_face_plan(m, stream);
+    if (rc) return rc;
+    Bump b(nullptr);
+    plan_s3p(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
+    return launch_patches(k_symdir3_grad_plan<256>, 256, b.bytes(), stream, m, m->dm, m->d_fp_fbase, m->d_fp_rbase,
+                          m->d_fp_nbase, static_cast<const uint16_t*>(m->d_fp_fvl),
+                          static_cast<const uint16_t*>(m->d_fp_roff), static_cast<const uint16_t*>(m->d_fp_vf), Mr,
+                          x, grad);
 }
 
 // The batched workload step: x1 = smooth(x), x = x1 - eta grad E_symdir3(x1).

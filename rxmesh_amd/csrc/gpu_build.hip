@@ -100,19 +100,24 @@ extern "C" rxm_status rxm_build_edges_device(const uint32_t* fv, uint32_t num_fa
         return RXM_ERR_DEVICE;
     const size_t tmp_bytes = std::max(sort_bytes, std::max(scan_bytes, max_bytes));
     const size_t n         = nH;
-    const size_t total     = 2 * 8 * n + 6 * 4 * n + 256 + tmp_bytes + 256;
+    // every buffer starts on a 256-byte boundary (hipMalloc's own alignment),
+    // the alignment the library expects of its temporary storage
+    auto         up        = [](size_t x) { return (x + 255) & ~size_t(255); };
+    const size_t o_k1 = up(8 * n), o_v0 = o_k1 + up(8 * n), o_v1 = o_v0 + up(4 * n), o_flag = o_v1 + up(4 * n);
+    const size_t o_rank = o_flag + up(4 * n), o_hpos = o_rank + up(4 * n), o_head = o_hpos + up(4 * n);
+    const size_t o_bad = o_head + up(4 * n), o_tmp = o_bad + 256, total = o_tmp + up(tmp_bytes);
     char*        base      = nullptr;
     if (hipMalloc(&base, total) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
     uint64_t* k0   = reinterpret_cast<uint64_t*>(base);
-    uint64_t* k1   = k0 + n;
-    uint32_t* v0   = reinterpret_cast<uint32_t*>(k1 + n);
-    uint32_t* v1   = v0 + n;
-    uint32_t* flag = v1 + n;
-    uint32_t* rank = flag + n;
-    uint32_t* hpos = rank + n;
-    uint32_t* head = hpos + n;
-    uint32_t* bad  = head + n;
-    void*     tmp  = reinterpret_cast<char*>(bad) + 256;
+    uint64_t* k1   = reinterpret_cast<uint64_t*>(base + o_k1);
+    uint32_t* v0   = reinterpret_cast<uint32_t*>(base + o_v0);
+    uint32_t* v1   = reinterpret_cast<uint32_t*>(base + o_v1);
+    uint32_t* flag = reinterpret_cast<uint32_t*>(base + o_flag);
+    uint32_t* rank = reinterpret_cast<uint32_t*>(base + o_rank);
+    uint32_t* hpos = reinterpret_cast<uint32_t*>(base + o_hpos);
+    uint32_t* head = reinterpret_cast<uint32_t*>(base + o_head);
+    uint32_t* bad  = reinterpret_cast<uint32_t*>(base + o_bad);
+    void*     tmp  = base + o_tmp;
 
     int            rc   = RXM_OK;
     const uint32_t grid = (nH + kT - 1) / kT;

@@ -1068,10 +1068,11 @@ __host__ __device__ inline PlanS3P plan_s3p(Bump& b, uint32_t nv, uint32_t nvo, 
     return p;
 }
 
-// x_out = x - eta * dE/dx over the patch-local face plan: positions, rest
-// shapes and plan rows are staged into LDS; each owned vertex recomputes the
-// corner gradient of every incident face in the plan's ascending order.
-template <uint32_t T>
+// x_out = x - eta * dE/dx over the patch-local face plan (GRAD: x_out =
+// dE/dx itself, eta unused): positions, rest shapes and plan rows are staged
+// into LDS; each owned vertex recomputes the corner gradient of every
+// incident face in the plan's ascending order.
+template <uint32_t T, bool GRAD = false>
 __device__ __forceinline__ void symdir3_plan_patch(const DevMesh& m, uint32_t p, const uint32_t* __restrict__ fbase,
                                                    const uint32_t* __restrict__ rbase,
                                                    const uint32_t* __restrict__ nbase,
@@ -1110,9 +1111,15 @@ __device__ __forceinline__ void symdir3_plan_patch(const DevMesh& m, uint32_t p,
             gz += g.z;
         }
         const uint32_t g = d.vp + v;
-        xo[g]            = L.x[v] - eta * gx;
-        xo[N + g]        = L.y[v] - eta * gy;
-        xo[2 * N + g]    = L.z[v] - eta * gz;
+        if constexpr (GRAD) {
+            xo[g]         = gx;
+            xo[N + g]     = gy;
+            xo[2 * N + g] = gz;
+        } else {
+            xo[g]         = L.x[v] - eta * gx;
+            xo[N + g]     = L.y[v] - eta * gy;
+            xo[2 * N + g] = L.z[v] - eta * gz;
+        }
     }
 }
 
@@ -1128,6 +1135,21 @@ __global__ __launch_bounds__(T) void k_symdir3_step_plan(DevMesh m, const uint32
     extern __shared__ uint4 smem[];
     symdir3_plan_patch<T>(m, blockIdx.x, fbase, rbase, nbase, fvl, roff, vf, Mr, eta, x, xo,
                           reinterpret_cast<char*>(smem));
+}
+
+// grad = dE/dx (3 x V SoA) over the face plan, one workgroup per patch.
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_symdir3_grad_plan(DevMesh m, const uint32_t* __restrict__ fbase,
+                                                         const uint32_t* __restrict__ rbase,
+                                                         const uint32_t* __restrict__ nbase,
+                                                         const uint16_t* __restrict__ fvl,
+                                                         const uint16_t* __restrict__ roff,
+                                                         const uint16_t* __restrict__ vf, const float* __restrict__ Mr,
+                                                         const float* __restrict__ x, float* __restrict__ grad)
+{
+    extern __shared__ uint4 smem[];
+    symdir3_plan_patch<T, true>(m, blockIdx.x, fbase, rbase, nbase, fvl, roff, vf, Mr, 0.f, x, grad,
+                                reinterpret_cast<char*>(smem));
 }
 
 // Persistent form: a grid sized to the machine pulls patch ids from a global
