@@ -245,8 +245,13 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *             arrays in LDS) and >= 4 iterations: 1 = the iterates between
  *             launches are one float4 per vertex in two buffers the mesh
  *             allocates on first use (32 V bytes; a gather then reads one
- *             cache line instead of three), 0 = SoA between launches.
- * Settings of kernels 0-4 and 6-9 give bit-identical results; kernel 5 changes
+ *             cache line instead of three), 0 = SoA between launches;
+ *   kernel 10: unoriented VV / VE / VF / EF / FF queries (rxm_query_csr,
+ *             rxm_query_vv_vf): 1 (default) = atomic-free stable counting
+ *             sort (wave ballots + mbcnt ranks + one block scan; rows come
+ *             out in order, no row sort), 0 = LDS-atomic histogram and slot
+ *             fill followed by a sort of every row.
+ * Settings of kernels 0-4 and 6-10 give bit-identical results; kernel 5 changes
  * the summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);
 

@@ -88,6 +88,7 @@ struct rxm_mesh
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
     uint32_t*       d_queue        = nullptr;  // work-queue ticket counter
     bool            system_runs_ok = false;    // check_system_runs passed
+    uint32_t        query_stable   = 1;        // 1: atomic-free stable counting sort in the queries
     int             num_cus        = 0;
     // batch (disjoint union of meshes): per-mesh id / patch offsets
     std::vector<uint32_t> b_v, b_e, b_f, b_p;
@@ -860,22 +861,30 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
                                      : go(k_query_vx_oriented<512, false>, 512);
         return (op == RXM_OP_VV) ? go(k_query_vx_oriented<256, true>, 256) : go(k_query_vx_oriented<256, false>, 256);
     }
+    // transposed relations: atomic-free stable counting sort, or the LDS
+    // atomic histogram + row sort (knob 10); identical rows
+    const bool     st = m->query_stable != 0;
+    const uint32_t hm = st ? T / 64u : 1u;
+    auto q2 = [&](auto k_atomic, auto k_stable) {
+        return st ? launch_patches_xcd(k_stable, T, b.bytes(), stream, m, m->dm, base, row_ptr, col)
+                  : launch_patches_xcd(k_atomic, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+    };
     switch (op) {
         case RXM_OP_VV:
-            plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches_xcd(k_query2_vx<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf, hm);
+            return q2(k_query2_vx<T, true, false>, k_query2_vx<T, true, true>);
         case RXM_OP_VE:
-            plan_q2(b, 3, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches_xcd(k_query2_vx<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            plan_q2(b, 3, h.max_nv, h.max_ne, h.max_nf, hm);
+            return q2(k_query2_vx<T, false, false>, k_query2_vx<T, false, true>);
         case RXM_OP_VF:
-            plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches_xcd(k_query2_vf<T>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf, hm);
+            return q2(k_query2_vf<T, false>, k_query2_vf<T, true>);
         case RXM_OP_EF:
-            plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches_xcd(k_query2_ef<T, false>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf, hm);
+            return q2(k_query2_ef<T, false, false>, k_query2_ef<T, false, true>);
         case RXM_OP_FF:
-            plan_q2(b, 4, h.max_nv, h.max_ne, h.max_nf);
-            return launch_patches_xcd(k_query2_ef<T, true>, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+            plan_q2(b, 4, h.max_nv, h.max_ne, h.max_nf, hm);
+            return q2(k_query2_ef<T, true, false>, k_query2_ef<T, true, true>);
         case RXM_OP_EV:
             plan_fx(b, h.max_ne, h.max_nf);
             return launch_patches(k_query_fixed<9>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
@@ -901,10 +910,14 @@ rxm_status rxm_query_vv_vf(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col, in
     int             rc = device_base(m, RXM_OP_VV, &vv_base);
     if (!rc) rc = device_base(m, RXM_OP_VF, &vf_base);
     if (rc) return rc;
-    Bump b(nullptr);
-    plan_vvf(b, m->h.max_nv, m->h.max_ne, m->h.max_nf);
-    constexpr uint32_t T = kQueryThreads;
-    return launch_patches_xcd(k_query2_vv_vf<T>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
+    Bump               b(nullptr);
+    constexpr uint32_t T  = kQueryThreads;
+    const bool         st = m->query_stable != 0;
+    plan_vvf(b, m->h.max_nv, m->h.max_ne, m->h.max_nf, st ? T / 64u : 1u);
+    if (st)
+        return launch_patches_xcd(k_query2_vv_vf<T, true>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base,
+                                  vv_row_ptr, vv_col, vf_row_ptr, vf_col);
+    return launch_patches_xcd(k_query2_vv_vf<T, false>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                               vv_col, vf_row_ptr, vf_col);
 }
 
@@ -1104,6 +1117,10 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
     }
     if (kernel == 6 && (threads == 256 || threads == 512)) {
         m->ring_threads = static_cast<uint32_t>(threads);
+        return RXM_OK;
+    }
+    if (kernel == 10 && (threads == 0 || threads == 1)) {  // query transposes: 1 stable counting sort, 0 atomic
+        m->query_stable = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
     if (kernel == 7 && (threads == 256 || threads == 512)) {
@@ -1333,6 +1350,26 @@ rxm_status rxm_symdir_rest(rxm_mesh* m, const float* rest_pos, float* Mr, void* 
     return launch_patches(k_symdir_rest, kGeomThreads, b.bytes(), stream, m, m->dm, rest_pos, Mr);
 }
 
+}  // extern "C"
+
+// Per-face energies, per-patch partials into `partial` (P floats, every
+// entry written), then their fixed-order sum into total[0].
+static int symdir_energy_launch(rxm_mesh* m, const float* Mr, const float* uv, float* energy, float* total,
+                                float* partial, void* stream)
+{
+    Bump b(nullptr);
+    plan_fv(b, m->h.max_ne, m->h.max_nf);
+    b.alloc<float>(kGeomThreads);
+    int rc = launch_patches(k_symdir_energy, kGeomThreads, b.bytes(), stream, m, m->dm, Mr, uv, energy, partial);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), m->h.P,
+                       static_cast<const float*>(partial), total);
+    RXM_HIP(hipGetLastError());
+    return RXM_OK;
+}
+
+extern "C" {
+
 rxm_status rxm_symdir_energy(rxm_mesh* m, const float* Mr, const float* uv, float* energy, float* total,
                              void* stream)
 {
@@ -1343,16 +1380,7 @@ rxm_status rxm_symdir_energy(rxm_mesh* m, const float* Mr, const float* uv, floa
         m->allocs.push_back(m->d_partial);
         m->device_bytes += sizeof(float) * m->h.P;
     }
-    Bump b(nullptr);
-    plan_fv(b, m->h.max_ne, m->h.max_nf);
-    b.alloc<float>(kGeomThreads);
-    int rc = launch_patches(k_symdir_energy, kGeomThreads, b.bytes(), stream, m, m->dm, Mr, uv, energy,
-                            m->d_partial);
-    if (rc) return rc;
-    hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), m->h.P,
-                       static_cast<const float*>(m->d_partial), total);
-    RXM_HIP(hipGetLastError());
-    return RXM_OK;
+    return symdir_energy_launch(m, Mr, uv, energy, total, m->d_partial, stream);
 }
 
 rxm_status rxm_symdir_grad(rxm_mesh* m, const float* Mr, const float* uv, const float* scale, float* grad,

@@ -24,7 +24,10 @@ struct PlanQ2
 // (FE only, plus per owned face entry counts).
 // Every array is sized for its kind, so a workgroup holds no more LDS than
 // its relation needs (occupancy is LDS-bound).
-__host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32_t ne, uint32_t nf)
+// hist_mul = 1: offsets of the atomic sort (ncols + 1); hist_mul = waves per
+// workgroup: the (key, wave) table of the stable sort (ncols * waves + 1).
+__host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32_t ne, uint32_t nf,
+                                          uint32_t hist_mul = 1)
 {
     PlanQ2 p{};
     const bool     vx    = (kind == 0 || kind == 3);
@@ -32,7 +35,7 @@ __host__ __device__ inline PlanQ2 plan_q2(Bump& b, int kind, uint32_t nv, uint32
     const uint32_t n     = vx ? 2 * ne : 3 * nf;
     if (kind != 2 && kind != 4) p.ev = b.alloc<uint32_t>(r4(ne));
     if (kind == 1 || kind == 2 || kind == 4) p.fe = b.alloc<uint16_t>(r4(3 * nf));
-    p.off   = b.alloc<uint32_t>(ncols + 1);
+    p.off   = b.alloc<uint32_t>(ncols * hist_mul + 1);
     if (kind == 4) p.cur = b.alloc<uint32_t>(nf + 1);  // FF: per owned face entry counts
     p.slots = b.alloc<uint32_t>(n);
     p.wave  = b.alloc<uint32_t>(20);
@@ -94,8 +97,24 @@ __device__ __forceinline__ void copy_out(int32_t* __restrict__ dst, const uint32
     for (uint32_t i = threadIdx.x; i < n; i += T) dst[i] = static_cast<int32_t>(src[i]);
 }
 
-// VV (IS_VV) or VE over linear ids.
-template <uint32_t T, bool IS_VV>
+// Key of VV / VE entry i = (edge i >> 1, endpoint i & 1): the endpoint's
+// local vertex.
+__device__ __forceinline__ uint32_t ev_end(const uint32_t* ev, uint32_t i)
+{
+    const uint32_t x = ev[i >> 1];
+    return (i & 1u) ? hi16(x) : lo16(x);
+}
+__device__ __forceinline__ uint32_t ev_other(const uint32_t* ev, uint32_t i)
+{
+    const uint32_t x = ev[i >> 1];
+    return (i & 1u) ? lo16(x) : hi16(x);
+}
+
+// VV (IS_VV) or VE over linear ids. STABLE: atomic-free counting sort
+// (sort_columns_stable), rows in entry order with no row sort; otherwise the
+// LDS-atomic histogram and slot fill followed by a sort of every row. Both
+// give the same rows: ascending local edge id.
+template <uint32_t T, bool IS_VV, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __restrict__ base,
                                                  int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
@@ -104,13 +123,24 @@ __global__ __launch_bounds__(T) void k_query2_vx(DevMesh m, const uint32_t* __re
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanQ2    L = plan_q2(b, IS_VV ? 0 : 3, d.nv, d.ne, d.nf);
+    const PlanQ2    L = plan_q2(b, IS_VV ? 0 : 3, d.nv, d.ne, d.nf, STABLE ? T / 64u : 1u);
     if (IS_VV)
         stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, L.lin, d.nv, d.nvo, d.vp,
                           m.v_lin + d.v_off);
     else
         stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, L.lin, d.ne, d.neo, d.ep,
                           m.e_lin + d.e_off);
+    if constexpr (STABLE) {
+        const uint32_t bse = base[p];
+        const uint32_t tot = sort_columns_stable<T>(
+            2 * d.ne, d.nvo, [&](uint32_t i) { return ev_end(L.ev, i); },
+            [&](uint32_t i) { return IS_VV ? L.lin[ev_other(L.ev, i)] : L.lin[i >> 1]; },
+            [&](uint32_t c, uint32_t st) { row_ptr[d.vp + c] = static_cast<int32_t>(bse + st); }, L.off, L.slots,
+            L.wave);
+        if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + tot);
+        copy_out<T>(col + bse, L.slots, tot);
+        return;
+    }
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
     // entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
@@ -213,7 +243,8 @@ __global__ __launch_bounds__(T) void k_query2_vf_local(DevMesh m, const uint32_t
 }
 
 // VF: FV from FE + EV, transposed; slot = face (the sort key itself).
-template <uint32_t T>
+// STABLE as in k_query2_vx.
+template <uint32_t T, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __restrict__ base,
                                                  int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
@@ -222,9 +253,20 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf);
+    const PlanQ2    L = plan_q2(b, 1, d.nv, d.ne, d.nf, STABLE ? T / 64u : 1u);
     stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
                       m.f_lin + d.f_off);
+    if constexpr (STABLE) {
+        const uint32_t bse = base[p];
+        const uint32_t tot = sort_columns_stable<T>(
+            3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+            [&](uint32_t i) { return L.lin[i / 3u]; },
+            [&](uint32_t c, uint32_t st) { row_ptr[d.vp + c] = static_cast<int32_t>(bse + st); }, L.off, L.slots,
+            L.wave);
+        if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + tot);
+        copy_out<T>(col + bse, L.slots, tot);
+        return;
+    }
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
     fill_columns<T, true>(
@@ -252,12 +294,12 @@ struct PlanVVF
     uint32_t *ev, *off, *slots, *wave, *lin_v, *lin_f;
     uint16_t* fe;
 };
-__host__ __device__ inline PlanVVF plan_vvf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf)
+__host__ __device__ inline PlanVVF plan_vvf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nf, uint32_t hist_mul = 1)
 {
     PlanVVF p{};
     p.ev    = b.alloc<uint32_t>(r4(ne));
     p.fe    = b.alloc<uint16_t>(r4(3 * nf));
-    p.off   = b.alloc<uint32_t>(nv + 1);
+    p.off   = b.alloc<uint32_t>(nv * hist_mul + 1);
     p.slots = b.alloc<uint32_t>(2 * ne > 3 * nf ? 2 * ne : 3 * nf);
     p.wave  = b.alloc<uint32_t>(20);
     p.lin_v = b.alloc<uint32_t>(nv);
@@ -265,7 +307,7 @@ __host__ __device__ inline PlanVVF plan_vvf(Bump& b, uint32_t nv, uint32_t ne, u
     return p;
 }
 
-template <uint32_t T>
+template <uint32_t T, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                     const uint32_t* __restrict__ vf_base,
                                                     int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
@@ -276,7 +318,7 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanVVF   L = plan_vvf(b, d.nv, d.ne, d.nf);
+    const PlanVVF   L = plan_vvf(b, d.nv, d.ne, d.nf, STABLE ? T / 64u : 1u);
     {
         // face linear ids are loaded ahead of the staging's LDS stores
         constexpr uint32_t RL = 3;
@@ -293,6 +335,30 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         for (uint32_t r = 0; r < RL; ++r)
             if (threadIdx.x + r * T < d.nf) L.lin_f[threadIdx.x + r * T] = fl[r];
         for (uint32_t l = threadIdx.x + RL * T; l < d.nf; l += T) L.lin_f[l] = (l < d.nfo) ? d.fp + l : fs[l];
+    }
+    if constexpr (STABLE) {
+        {
+            const uint32_t bse = vv_base[p];
+            const uint32_t tot = sort_columns_stable<T>(
+                2 * d.ne, d.nvo, [&](uint32_t i) { return ev_end(L.ev, i); },
+                [&](uint32_t i) { return L.lin_v[ev_other(L.ev, i)]; },
+                [&](uint32_t c, uint32_t st) { vv_row[d.vp + c] = static_cast<int32_t>(bse + st); }, L.off,
+                L.slots, L.wave);
+            if (threadIdx.x == 0) vv_row[d.vp + d.nvo] = static_cast<int32_t>(bse + tot);
+            copy_out<T>(vv_col + bse, L.slots, tot);
+        }
+        __syncthreads();  // the VV slots are read out before VF reuses them
+        {
+            const uint32_t bse = vf_base[p];
+            const uint32_t tot = sort_columns_stable<T>(
+                3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
+                [&](uint32_t i) { return L.lin_f[i / 3u]; },
+                [&](uint32_t c, uint32_t st) { vf_row[d.vp + c] = static_cast<int32_t>(bse + st); }, L.off,
+                L.slots, L.wave);
+            if (threadIdx.x == 0) vf_row[d.vp + d.nvo] = static_cast<int32_t>(bse + tot);
+            copy_out<T>(vf_col + bse, L.slots, tot);
+        }
+        return;
     }
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
@@ -338,8 +404,8 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
     }
 }
 
-// EF (IS_FF == false) or FF.
-template <uint32_t T, bool IS_FF>
+// EF (IS_FF == false) or FF. STABLE as in k_query2_vx.
+template <uint32_t T, bool IS_FF, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __restrict__ base,
                                                  int32_t* __restrict__ row_ptr, int32_t* __restrict__ col)
 {
@@ -348,9 +414,50 @@ __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __re
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanQ2    L = plan_q2(b, IS_FF ? 4 : 2, d.nv, d.ne, d.nf);
+    const PlanQ2    L = plan_q2(b, IS_FF ? 4 : 2, d.nv, d.ne, d.nf, STABLE ? T / 64u : 1u);
     stage_topology<T>(nullptr, nullptr, 0, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin, d.nf, d.nfo, d.fp,
                       m.f_lin + d.f_off);
+    if constexpr (STABLE) {
+        constexpr uint32_t NW  = T / 64u;
+        const uint32_t     bse = base[p];
+        const uint32_t ncols = IS_FF ? d.ne : d.neo;
+        const uint32_t tot   = sort_columns_stable<T>(
+            3 * d.nf, ncols, [&](uint32_t i) { return static_cast<uint32_t>(L.fe[i] >> 1); },
+            [&](uint32_t i) { return IS_FF ? i / 3u : L.lin[i / 3u]; },
+            [&](uint32_t c, uint32_t st) {
+                if (!IS_FF) row_ptr[d.ep + c] = static_cast<int32_t>(bse + st);
+            },
+            L.off, L.slots, L.wave);
+        if constexpr (!IS_FF) {
+            if (threadIdx.x == 0) row_ptr[d.ep + d.neo] = static_cast<int32_t>(bse + tot);
+            copy_out<T>(col + bse, L.slots, tot);
+        } else {
+            // after the sort, table cell e * NW - 1 holds the start of edge e's row
+            auto rs = [&](uint32_t e) { return e ? L.off[e * NW - 1u] : 0u; };
+            for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
+                uint32_t c = 0;
+                for (uint32_t i = 0; i < 3; ++i) {
+                    const uint32_t e = L.fe[3 * f + i] >> 1;
+                    c += rs(e + 1) - rs(e) - 1;
+                }
+                L.cur[f] = c;
+            }
+            const uint32_t total = scan_excl<T>(L.cur, d.nfo, L.wave);
+            for (uint32_t f = threadIdx.x; f < d.nfo; f += T) {
+                uint32_t k        = bse + L.cur[f];
+                row_ptr[d.fp + f] = static_cast<int32_t>(k);
+                for (uint32_t i = 0; i < 3; ++i) {
+                    const uint32_t e = L.fe[3 * f + i] >> 1;
+                    for (uint32_t j = rs(e); j < rs(e + 1); ++j) {
+                        const uint32_t g = L.slots[j];
+                        if (g != f) col[k++] = static_cast<int32_t>(L.lin[g]);
+                    }
+                }
+            }
+            if (threadIdx.x == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(bse + total);
+        }
+        return;
+    }
     for (uint32_t c = threadIdx.x; c <= (IS_FF ? d.ne : d.neo); c += T) L.off[c] = 0;
     __syncthreads();
     fill_columns<T, true>(

@@ -140,4 +140,108 @@ __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF ke
     __syncthreads();
 }
 
+// ---- atomic-free stable counting sort ------------------------------------
+// Bits needed to tell keys 0 .. ncols - 1 apart.
+__device__ __forceinline__ uint32_t key_bits(uint32_t ncols)
+{
+    return ncols > 1u ? 32u - __builtin_clz(ncols - 1u) : 0u;
+}
+
+// Lanes of this wave whose key equals this lane's key, among the lanes in
+// `valid`: one ballot per key bit. Wave-uniform control flow required.
+__device__ __forceinline__ uint64_t match_key(uint32_t k, uint64_t valid, uint32_t nbits)
+{
+    uint64_t m = valid;
+    for (uint32_t b = 0; b < nbits; ++b) {
+        const bool     bit = (k >> b) & 1u;
+        const uint64_t bb  = __builtin_amdgcn_ballot_w64(bit);
+        m &= bit ? bb : ~bb;
+    }
+    return m;
+}
+
+// Number of set bits of m below this lane (mbcnt).
+__device__ __forceinline__ uint32_t rank_below(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi(static_cast<uint32_t>(m >> 32),
+                                     __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(m), 0u));
+}
+
+// Stable counting sort of n entries into ncols columns with no LDS atomics
+// and no per-row sort: column c receives its entries in ascending entry
+// index i. Wave w of the NW = T / 64 waves owns the contiguous entry block
+// [w C, (w + 1) C) and walks it in rounds of 64; in each round the lanes
+// holding the same key are matched with one ballot per key bit, the match's
+// lowest lane adds the match size to hist[key * NW + w] (one writer per cell
+// and round, so a plain read-modify-write), and every lane's rank inside its
+// match comes from mbcnt. An exclusive scan of hist in key-major order then
+// gives, for every (key, wave), where that wave's entries of the key start,
+// and a second walk stores each entry's payload at start + rank, advancing
+// the cell by the match size after each round. Entries whose key is >= ncols
+// are dropped. hist holds ncols * NW u32 (cleared here); on return
+// row_start(c) for c < ncols was passed to `rows(c, start)` right after the
+// scan (before the cells advance), and the total is returned.
+template <uint32_t T, uint32_t R = 12, typename KeyF, typename PayF, typename RowF>
+__device__ __forceinline__ uint32_t sort_columns_stable(uint32_t n, uint32_t ncols, KeyF key, PayF payload,
+                                                        RowF rows, uint32_t* hist, uint32_t* slots,
+                                                        uint32_t* s_wave)
+{
+    constexpr uint32_t NW   = T / 64u;
+    const uint32_t     lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint32_t     C    = ((n + 64u * NW - 1u) / (64u * NW)) * 64u;  // entries per wave
+    const uint32_t     b0 = min(n, w * C), b1 = min(n, b0 + C);
+    const uint32_t     nr = (b1 - b0 + 63u) >> 6;  // rounds of this wave (wave-uniform)
+    const uint32_t     nbits = key_bits(ncols);
+    for (uint32_t c = threadIdx.x; c < ncols * NW; c += T) hist[c] = 0u;
+    __syncthreads();
+    uint64_t mk[R];
+    uint32_t kk[R];
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        if (r < nr) {
+            const uint32_t i = b0 + (r << 6) + lane;
+            const uint32_t k = (i < b1) ? key(i) : ncols;
+            const uint64_t m = match_key(k, __builtin_amdgcn_ballot_w64(k < ncols), nbits);
+            kk[r]            = k;
+            mk[r]            = m;
+            if (k < ncols && rank_below(m) == 0u) hist[k * NW + w] += static_cast<uint32_t>(__popcll(m));
+        }
+    }
+    for (uint32_t r = R; r < nr; ++r) {
+        const uint32_t i = b0 + (r << 6) + lane;
+        const uint32_t k = (i < b1) ? key(i) : ncols;
+        const uint64_t m = match_key(k, __builtin_amdgcn_ballot_w64(k < ncols), nbits);
+        if (k < ncols && rank_below(m) == 0u) hist[k * NW + w] += static_cast<uint32_t>(__popcll(m));
+    }
+    const uint32_t total = scan_excl<T>(hist, ncols * NW, s_wave);
+    for (uint32_t c = threadIdx.x; c < ncols; c += T) rows(c, hist[c * NW]);
+    __syncthreads();  // row starts read before the cells advance
+#pragma unroll
+    for (uint32_t r = 0; r < R; ++r) {
+        if (r < nr) {
+            const uint32_t k = kk[r];
+            const uint64_t m = mk[r];
+            if (k < ncols) {
+                const uint32_t rk = rank_below(m);
+                const uint32_t at = hist[k * NW + w];
+                slots[at + rk]    = payload(b0 + (r << 6) + lane);
+                if (rk == 0u) hist[k * NW + w] = at + static_cast<uint32_t>(__popcll(m));
+            }
+        }
+    }
+    for (uint32_t r = R; r < nr; ++r) {
+        const uint32_t i = b0 + (r << 6) + lane;
+        const uint32_t k = (i < b1) ? key(i) : ncols;
+        const uint64_t m = match_key(k, __builtin_amdgcn_ballot_w64(k < ncols), nbits);
+        if (k < ncols) {
+            const uint32_t rk = rank_below(m);
+            const uint32_t at = hist[k * NW + w];
+            slots[at + rk]    = payload(i);
+            if (rk == 0u) hist[k * NW + w] = at + static_cast<uint32_t>(__popcll(m));
+        }
+    }
+    __syncthreads();
+    return total;
+}
+
 }  // namespace rxm

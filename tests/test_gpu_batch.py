@@ -86,6 +86,31 @@ def test_symdir3_step_vs_fp64():
     assert np.abs(gE - E64).max() <= 1e-5 * np.abs(E64).max()
 
 
+def test_symdir3_grad_vs_fp64_every_mesh():
+    """The gradient itself (rxm_symdir3_grad) on the deformed batch: within
+    1e-5 of each mesh's largest FP64 gradient entry, on every mesh, and equal
+    to what the gradient step applies."""
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    Xd = (X.astype(np.float64) * 1.05 + 0.01 * np.sin(5 * X)).astype(np.float32)
+    xd = lin(b, Xd)
+    g = b.symdir3_grad(Mr, xd)
+    step = b.symdir3_grad_step(Mr, xd, ETA)
+    torch.cuda.synchronize()
+    # the step kernel may contract x - eta g into one fma: equal to an ulp or two
+    assert torch.allclose(step, xd - ETA * g, rtol=0, atol=2.5e-7 * float(xd.abs().max()))
+    fv_all = np.concatenate([f + ov[k] for k, (_, f) in enumerate(ms)])
+    o = OracleMesh(fv_all).set_patches(b.array("face_patch"), b.array("patches_val"), b.array("patches_offset"))
+    gMr = np.empty((4, b.nF), np.float32)
+    gMr[:, b.array("f_l2g")] = Mr.cpu().numpy()
+    g64 = o.symdir3_grad(gMr.T.astype(np.float64), Xd.astype(np.float64))
+    got = glob(b, g.cpu().numpy()).T
+    for k in range(len(ms)):
+        sl = slice(ov[k], ov[k + 1])
+        assert np.abs(got[sl] - g64[sl]).max() <= 1e-5 * np.abs(g64[sl]).max(), k
+
+
 def test_batched_steps_vs_fp64_oracle():
     import torch
     ms, b, sep, ov, X, x = setup()

@@ -28,13 +28,15 @@ def _faces(name):
         F = mg.config_mesh("c1")[1].copy()
         F[7] = F[7][::-1]
         return F
+    if name == "ico_odd":  # an odd face count: 3F half-edges is odd
+        return mg.icosphere(4, seed=2, sigma=0.005)[1][:-1]
     if name == "fin":  # an edge with three faces
         return np.array([[0, 1, 2], [1, 0, 3], [0, 1, 4], [2, 1, 5]], np.uint32)
     raise KeyError(name)
 
 
 @pytest.mark.parametrize("name", ["plane.obj", "cube.obj", "sphere3.obj", "bunnyhead.obj", "ico_perm", "torus",
-                                  "disk", "c1", "flipped", "fin"])
+                                  "disk", "c1", "flipped", "fin", "ico_odd"])
 def test_device_edges_equal_host_numbering(name):
     import torch
     import rxmesh_amd as rx
