@@ -17,6 +17,13 @@
  *   - `stream` is a hipStream_t passed as void* (NULL = default stream).
  *
  * Thread compatibility: a mesh may be used from one host thread at a time.
+ * Streams: a mesh owns scratch buffers that several entry points share (the
+ * energy partial sums of rxm_arap_energy_grad and rxm_symdir_energy, the CG
+ * state of rxm_mcf_implicit_cg, the work-queue counter of the batched steps).
+ * Nothing orders their use across streams, so issue all calls on one mesh on
+ * one stream (a captured graph of a mesh's calls included): calls on one
+ * mesh from two streams that overlap in time may overwrite each other's
+ * partials.
  */
 #ifndef RXMESH_AMD_H
 #define RXMESH_AMD_H
@@ -356,6 +363,24 @@ rxm_status rxm_smooth_symdir_steps(rxm_mesh* mesh, const float* Mr, float* x, fl
 rxm_status rxm_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* const* Mr,
                                         float* const* x, float* const* tmp, uint32_t count,
                                         float lambda, float eta, int steps, void* stream);
+/* The same step loop with the VV and VF queries at the start of every step:
+ * each step launches rxm_query_vv_vf into the caller's CSR buffers (sized as
+ * for rxm_query_vv_vf), then the smoothing and gradient steps -- the
+ * reference's per-step query dispatch (Query<N>::dispatch, query.h:21-75,
+ * inside each kernel of a step) made explicit. Allocates nothing inside the
+ * loop (plans and query bases are prepared first), so it can be captured. */
+rxm_status rxm_query_smooth_symdir_steps(rxm_mesh* mesh, const float* Mr, float* x, float* tmp,
+                                         float lambda, float eta, int steps, int32_t* vv_row_ptr,
+                                         int32_t* vv_col, int32_t* vf_row_ptr, int32_t* vf_col,
+                                         void* stream);
+/* Plain per-mesh launches of the loop above over `count` separate meshes,
+ * one set of CSR buffers per mesh. */
+rxm_status rxm_query_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* const* Mr,
+                                              float* const* x, float* const* tmp,
+                                              int32_t* const* vv_row_ptr, int32_t* const* vv_col,
+                                              int32_t* const* vf_row_ptr, int32_t* const* vf_col,
+                                              uint32_t count, float lambda, float eta, int steps,
+                                              void* stream);
 /* Capture rxm_smooth_symdir_steps into a hipGraph (no memset nodes; the VV
  * plan and the work-queue counter are prepared before capture) and replay
  * it with rxm_graph_launch. Pointers are baked into the graph. The

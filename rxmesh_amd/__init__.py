@@ -78,6 +78,8 @@ _SIGS = {
     "rxm_symdir3_grad": [_vp, _vp, _vp, _vp, _vp],
     "rxm_smooth_symdir_steps": [_vp, _vp, _vp, _vp, _f, _f, _i, _vp],
     "rxm_smooth_symdir_steps_each": [_vp, _vp, _vp, _vp, _u32, _f, _f, _i, _vp],
+    "rxm_query_smooth_symdir_steps": [_vp, _vp, _vp, _vp, _f, _f, _i, _vp, _vp, _vp, _vp, _vp],
+    "rxm_query_smooth_symdir_steps_each": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _u32, _f, _f, _i, _vp],
     "rxm_graph_capture_smooth_symdir": [_vp, _vp, _vp, _vp, _f, _f, _i, ctypes.POINTER(_vp)],
     "rxm_graph_launch": [_vp, _vp],
     "rxm_graph_destroy": [_vp],
@@ -436,6 +438,26 @@ class Mesh:
                "rxm_smooth_symdir_steps")
         return x
 
+    def vv_vf_buffers(self):
+        """Device buffers ((vv_row_ptr, vv_col), (vf_row_ptr, vf_col)) sized for query_vv_vf."""
+        import torch
+        i32 = dict(dtype=torch.int32, device="cuda")
+        return ((torch.empty(self.nV + 1, **i32), torch.empty(max(self.query_nnz("VV"), 1), **i32)),
+                (torch.empty(self.nV + 1, **i32), torch.empty(max(self.query_nnz("VF"), 1), **i32)))
+
+    def _vv_vf_ptrs(self, vv, vf):
+        return (_dev(vv[0], "vv_row_ptr", (self.nV + 1,), "int32"),
+                _dev(vv[1], "vv_col", None, "int32", self.query_nnz("VV")),
+                _dev(vf[0], "vf_row_ptr", (self.nV + 1,), "int32"),
+                _dev(vf[1], "vf_col", None, "int32", self.query_nnz("VF")))
+
+    def query_smooth_symdir_steps(self, Mr, x, tmp, lam, eta, steps, vv, vf, stream=None):
+        """smooth_symdir_steps with the VV and VF CSRs (into vv, vf) queried at every step."""
+        _check(lib().rxm_query_smooth_symdir_steps(self._h, self._mr(Mr), self._v3(x, "x"), self._v3(tmp, "tmp"),
+                                                   float(lam), float(eta), int(steps), *self._vv_vf_ptrs(vv, vf),
+                                                   _stream(stream)), "rxm_query_smooth_symdir_steps")
+        return x
+
     def capture_smooth_symdir(self, Mr, x, tmp, lam, eta, steps):
         return Graph(self, Mr, x, tmp, lam, eta, steps)
 
@@ -599,6 +621,23 @@ def smooth_symdir_steps_each(meshes, Mrs, xs, tmps, lam, eta, steps, stream=None
                                               arr([m._v3(t, "tmp") for m, t in zip(meshes, tmps)]),
                                               n, float(lam), float(eta), int(steps), _stream(stream)),
            "rxm_smooth_symdir_steps_each")
+
+
+def query_smooth_symdir_steps_each(meshes, Mrs, xs, tmps, bufs, lam, eta, steps, stream=None):
+    """Plain per-mesh launches of the step loop with the per-step VV + VF
+    query; bufs[k] = (vv, vf) CSR buffers of mesh k (Mesh.vv_vf_buffers)."""
+    n = len(meshes)
+    if not (len(Mrs) == len(xs) == len(tmps) == len(bufs) == n):
+        raise ValueError("query_smooth_symdir_steps_each: one Mr, x, tmp and buffer set per mesh")
+    arr = lambda items: (ctypes.c_void_p * n)(*[p.value for p in items])  # noqa: E731
+    ptrs = [m._vv_vf_ptrs(*b) for m, b in zip(meshes, bufs)]
+    _check(lib().rxm_query_smooth_symdir_steps_each((ctypes.c_void_p * n)(*[m._h.value for m in meshes]),
+                                                    arr([m._mr(t) for m, t in zip(meshes, Mrs)]),
+                                                    arr([m._v3(t, "x") for m, t in zip(meshes, xs)]),
+                                                    arr([m._v3(t, "tmp") for m, t in zip(meshes, tmps)]),
+                                                    *[arr([p[j] for p in ptrs]) for j in range(4)],
+                                                    n, float(lam), float(eta), int(steps), _stream(stream)),
+           "rxm_query_smooth_symdir_steps_each")
 
 
 class SymmetricDirichlet:

@@ -153,17 +153,49 @@ def tune_smooth_symdir(mesh, Mr, x, tmp, lam, eta, steps=2, reps=3, use_cache=Tr
     return best, timings
 
 
-def tune_patch_size(fv, make_inputs, run, sizes=(256, 512, 1024)):
+PATCH_SIZES = (256, 512, 1024)
+
+
+def tune_patch_size(fv, make_inputs, run, sizes=PATCH_SIZES, batch=False):
     """Rebuild the mesh for each candidate patch size and time run(mesh,
-    *make_inputs(mesh)); returns (best size, {size: ms})."""
+    *make_inputs(mesh)); returns (best size, {size: ms}). With batch=True, fv
+    is a list of face arrays built as one batch. A size whose build or whose
+    run is refused (a patch beyond what the workload's kernels hold in LDS or
+    in 16-bit local ids) is skipped."""
     import rxmesh_amd as rx
     out = {}
     for ps in sizes:
         try:
-            m = rx.Mesh(fv, patch_size=ps)
+            m = rx.Mesh(None, batch=fv, patch_size=ps) if batch else rx.Mesh(fv, patch_size=ps)
+            args = make_inputs(m)
+            out[ps] = _time_ms(lambda: run(m, *args), 3)
         except rx.RXMError:
             continue
-        args = make_inputs(m)
-        out[ps] = _time_ms(lambda: run(m, *args), 3)
+        finally:
+            m = None
+    if not out:
+        raise RuntimeError("tune_patch_size: no candidate patch size runs this workload")
     best = min(out, key=out.get)
     return best, out
+
+
+def tune_batch_patch_size(fvs, positions, lam, eta, steps=2, sizes=PATCH_SIZES):
+    """Patch size for the batched step loop (per-step VV + VF query,
+    smoothing, symmetric-Dirichlet gradient step), timed on the meshes
+    `fvs` (with vertex positions `positions`, (V, 3) arrays) built as one
+    batch; returns (best size, {size: ms per step})."""
+    import numpy as np
+    import torch
+
+    def make_inputs(m):
+        X = np.concatenate(positions).astype(np.float32)
+        x = torch.from_numpy(np.ascontiguousarray(X[m.array("v_l2g")].T)).cuda()
+        Mr = m.symdir_rest(x)
+        vv, vf = m.vv_vf_buffers()
+        return Mr, x, torch.empty_like(x), vv, vf
+
+    def run(m, Mr, x, tmp, vv, vf):
+        m.query_smooth_symdir_steps(Mr, x, tmp, lam, eta, steps, vv, vf)
+
+    best, times = tune_patch_size(fvs, make_inputs, run, sizes, batch=True)
+    return best, {k: round(v / steps, 4) for k, v in times.items()}

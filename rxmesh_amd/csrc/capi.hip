@@ -901,11 +901,12 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
     }
 }
 
-rxm_status rxm_query_vv_vf(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col, int32_t* vf_row_ptr, int32_t* vf_col,
-                           void* stream)
+}  // extern "C"
+
+// The VV and VF CSRs from one launch (rxm_query_vv_vf without its checks).
+static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col, int32_t* vf_row_ptr,
+                              int32_t* vf_col, void* stream)
 {
-    if (!m || !vv_row_ptr || !vv_col || !vf_row_ptr || !vf_col) return RXM_ERR_INVALID_ARG;
-    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
     const uint32_t *vv_base = nullptr, *vf_base = nullptr;
     int             rc = device_base(m, RXM_OP_VV, &vv_base);
     if (!rc) rc = device_base(m, RXM_OP_VF, &vf_base);
@@ -919,6 +920,16 @@ rxm_status rxm_query_vv_vf(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col, in
                                   vv_row_ptr, vv_col, vf_row_ptr, vf_col);
     return launch_patches_xcd(k_query2_vv_vf<T, false>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                               vv_col, vf_row_ptr, vf_col);
+}
+
+extern "C" {
+
+rxm_status rxm_query_vv_vf(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col, int32_t* vf_row_ptr, int32_t* vf_col,
+                           void* stream)
+{
+    if (!m || !vv_row_ptr || !vv_col || !vf_row_ptr || !vf_col) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    return query_vv_vf_launch(m, vv_row_ptr, vv_col, vf_row_ptr, vf_col, stream);
 }
 
 rxm_status rxm_vertex_normals(rxm_mesh* m, int mode, const float* pos, float* normals, void* stream)
@@ -1469,15 +1480,38 @@ rxm_status rxm_symdir3_grad(rxm_mesh* m, const float* Mr, const float* x, float*
 }
 
 // The batched workload step: x1 = smooth(x), x = x1 - eta grad E_symdir3(x1).
+// Per-step query outputs of the step loop (all four null: no query).
+struct StepQuery
+{
+    int32_t *vv_row_ptr = nullptr, *vv_col = nullptr, *vf_row_ptr = nullptr, *vf_col = nullptr;
+    bool     on() const { return vv_row_ptr != nullptr; }
+    bool     valid() const
+    {
+        return (vv_row_ptr && vv_col && vf_row_ptr && vf_col) || (!vv_row_ptr && !vv_col && !vf_row_ptr && !vf_col);
+    }
+};
+
 static int smooth_symdir_launches(rxm_mesh* m, const float* Mr, float* x, float* tmp, float lambda, float eta,
-                                  int steps, void* stream)
+                                  int steps, void* stream, const StepQuery& q = StepQuery{})
 {
     for (int s = 0; s < steps; ++s) {
-        int rc = smooth_once(m, lambda, x, tmp, stream);
+        int rc = q.on() ? query_vv_vf_launch(m, q.vv_row_ptr, q.vv_col, q.vf_row_ptr, q.vf_col, stream) : RXM_OK;
+        if (!rc) rc = smooth_once(m, lambda, x, tmp, stream);
         if (!rc) rc = launch_symdir3_step(m, Mr, eta, tmp, x, stream);
         if (rc) return rc;
     }
     return RXM_OK;
+}
+
+// Everything the step loop's launches need that is built or allocated once
+// per mesh (plans, query bases), so that the loop itself allocates nothing.
+static int prepare_query_steps(rxm_mesh* m, void* stream)
+{
+    int rc = prepare_steps(m, stream);
+    const uint32_t* b = nullptr;
+    if (!rc) rc = device_base(m, RXM_OP_VV, &b);
+    if (!rc) rc = device_base(m, RXM_OP_VF, &b);
+    return rc;
 }
 
 rxm_status rxm_smooth_symdir_steps(rxm_mesh* m, const float* Mr, float* x, float* tmp, float lambda, float eta,
@@ -1504,6 +1538,44 @@ rxm_status rxm_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* co
     for (int s = 0; s < steps; ++s)
         for (uint32_t k = 0; k < count; ++k) {
             int rc = smooth_symdir_launches(meshes[k], Mr[k], x[k], tmp[k], lambda, eta, 1, stream);
+            if (rc) return rc;
+        }
+    return RXM_OK;
+}
+
+rxm_status rxm_query_smooth_symdir_steps(rxm_mesh* m, const float* Mr, float* x, float* tmp, float lambda,
+                                         float eta, int steps, int32_t* vv_row_ptr, int32_t* vv_col,
+                                         int32_t* vf_row_ptr, int32_t* vf_col, void* stream)
+{
+    const StepQuery q{vv_row_ptr, vv_col, vf_row_ptr, vf_col};
+    if (!m || !Mr || !x || !tmp || x == tmp || steps < 1 || !q.on() || !q.valid()) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    int rc = prepare_query_steps(m, stream);
+    if (rc) return rc;
+    return smooth_symdir_launches(m, Mr, x, tmp, lambda, eta, steps, stream, q);
+}
+
+rxm_status rxm_query_smooth_symdir_steps_each(rxm_mesh* const* meshes, const float* const* Mr, float* const* x,
+                                              float* const* tmp, int32_t* const* vv_row_ptr, int32_t* const* vv_col,
+                                              int32_t* const* vf_row_ptr, int32_t* const* vf_col, uint32_t count,
+                                              float lambda, float eta, int steps, void* stream)
+{
+    if (!meshes || !Mr || !x || !tmp || !vv_row_ptr || !vv_col || !vf_row_ptr || !vf_col || count == 0 ||
+        steps < 1)
+        return RXM_ERR_INVALID_ARG;
+    for (uint32_t k = 0; k < count; ++k) {
+        const StepQuery q{vv_row_ptr[k], vv_col[k], vf_row_ptr[k], vf_col[k]};
+        if (!meshes[k] || !meshes[k]->on_device || !q.on() || !q.valid() || !Mr[k] || !x[k] || !tmp[k] ||
+            x[k] == tmp[k])
+            return RXM_ERR_INVALID_ARG;
+        int rc = prepare_query_steps(meshes[k], stream);
+        if (rc) return rc;
+    }
+    // plain launches: every mesh gets its own three launches per step
+    for (int s = 0; s < steps; ++s)
+        for (uint32_t k = 0; k < count; ++k) {
+            const StepQuery q{vv_row_ptr[k], vv_col[k], vf_row_ptr[k], vf_col[k]};
+            int rc = smooth_symdir_launches(meshes[k], Mr[k], x[k], tmp[k], lambda, eta, 1, stream, q);
             if (rc) return rc;
         }
     return RXM_OK;
