@@ -337,6 +337,26 @@ rxm_status rxm_mcf_implicit_cg(rxm_mesh* mesh, const int32_t* row_ptr, const int
                                int max_iter, float abs_tol, float rel_tol, int* iters,
                                float* start_residual, float* final_residual, void* stream);
 
+/* Implicit MCF step with the system never assembled: CG (precond = 0) or
+ * Jacobi-preconditioned CG (precond = 1) on A X = B, where every product
+ * recomputes the weights from the fixed input coordinates `coords` (3 x V
+ * SoA): A = M + dt L with cotangent weights max(0, cot + cot) and Voronoi
+ * mass (uniform = 0), or unit weights and mass = valence (uniform = 1); B =
+ * M x (the reference's init_B). x is the initial X (3 x V SoA; the reference
+ * starts from the input coordinates), the solution goes to `out`. Stopping
+ * rules, residuals (|r|^2, or <r, r / diag> with the preconditioner) and
+ * iteration counts as rxm_mcf_implicit_cg. Cotangent weights need an
+ * edge-manifold, consistently oriented mesh whose every vertex ring is one
+ * closed fan (else RXM_ERR_NON_MANIFOLD / RXM_ERR_UNSUPPORTED; the reference
+ * app takes closed meshes only). Replaces mcf_cg_mat_free / mcf_pcg_mat_free
+ * (apps/MCF/mcf_cg_mat_free.h:148-240) with the matvec / precond_matvec
+ * kernels (apps/MCF/mcf_kernels.cuh:114-299) under CGMatFreeAttrSolver /
+ * PCGMatFreeAttrSolver (include/rxmesh/matrix/cg_mat_free_attr_solver.h,
+ * pcg_mat_free_attr_solver.h). */
+rxm_status rxm_mcf_matfree_cg(rxm_mesh* mesh, const float* coords, const float* x, float* out,
+                              float dt, int uniform, int precond, int max_iter, float abs_tol,
+                              float rel_tol, int* iters, float* start_residual,
+                              float* final_residual, void* stream);
 /* Symmetric Dirichlet with 3-D positions as the variable (closed surfaces
  * have no UV domain): J = [b-a, c-a] Mr^{-1} is 3x2 and |J^{-1}|^2 becomes
  * |J|^2 / det(J^T J); equals the 2-D energy for planar input. Mr from

@@ -1111,6 +1111,98 @@ int ora_mcf_cg(int64_t nnz, const int32_t* rows, const int32_t* cols, const doub
     return it;
 }
 
+// The uniform-weight system of the MCF app (use_uniform_laplace, apps/MCF/
+// mcf_kernels.cuh:354-426 and the matvec of :114-213): every edge weight is
+// 1 (times dt), v_weight = 1 / valence, so A(p, p) = 1 / v_weight + dt *
+// valence and A(p, r) = -dt; the right-hand side scale mass[p] = valence
+// (init_B, :57-108). Same COO layout as ora_cot_assemble (diagonal, then the
+// VV query row). Returns the number of entries.
+int64_t ora_uniform_assemble(const ora_mesh* h, double dt, int32_t* rows, int32_t* cols, double* vals,
+                             double* mass)
+{
+    const Mesh& m = *reinterpret_cast<const Mesh*>(h);
+    int64_t n = 0;
+    for (uint32_t p = 0; p < m.P; ++p) {
+        PatchQuery   q = patch_query(m, p, OP_VV, false);
+        const Patch& P = m.patches[p];
+        for (uint32_t v = 0; v < P.nvo; ++v) {
+            const uint32_t gp = P.ltog_v[v];
+            const uint32_t K  = q.off[v + 1] - q.off[v];
+            double sum_w = 0, vw = 0;
+            for (uint32_t k = 0; k < K; ++k) {
+                sum_w += 1.0 * dt;
+                vw += 1.0;
+            }
+            vw = 1.0 / vw;
+            rows[n] = gp; cols[n] = gp; vals[n] = 1.0 / vw + sum_w; ++n;
+            mass[gp] = static_cast<double>(K);
+            for (uint32_t k = q.off[v]; k < q.off[v + 1]; ++k) {
+                rows[n] = gp; cols[n] = P.ltog_v[q.val[k]]; vals[n] = -dt; ++n;
+            }
+        }
+    }
+    return n;
+}
+
+// Jacobi-preconditioned CG, restating PCGMatFreeAttrSolver
+// (include/rxmesh/matrix/pcg_mat_free_attr_solver.h:39-125) with the
+// preconditioner of precond_matvec (apps/MCF/mcf_kernels.cuh:219-299): z =
+// r / diag(A). B = M x0, x starts at x0; delta = <r, z>; stop when delta <
+// abs_tol or delta / delta0 < rel_tol. A start residual already below the
+// tolerance ends with zero iterations, as CGSolver does (the reference's
+// matrix-free solvers always take one iteration, which divides 0 by 0 when
+// the start residual is exactly zero). Same arguments and results as
+// ora_mcf_cg.
+int ora_mcf_pcg(int64_t nnz, const int32_t* rows, const int32_t* cols, const double* vals, const double* mass,
+                uint32_t nV, const double* x0, int max_iter, double abs_tol, double rel_tol, double* out,
+                double* res)
+{
+    const size_t n = 3ull * nV;
+    std::vector<double> x(x0, x0 + n), r(n), p(n), s(n), z(n), diag(nV, 0.0);
+    for (int64_t k = 0; k < nnz; ++k)
+        if (rows[k] == cols[k]) diag[rows[k]] += vals[k];
+    auto matvec = [&](const std::vector<double>& a, std::vector<double>& y) {
+        std::fill(y.begin(), y.end(), 0.0);
+        for (int64_t k = 0; k < nnz; ++k)
+            for (int c = 0; c < 3; ++c) y[3 * rows[k] + c] += vals[k] * a[3 * cols[k] + c];
+    };
+    auto dot = [&](const std::vector<double>& a, const std::vector<double>& b) {
+        double t = 0.0;
+        for (size_t i = 0; i < n; ++i) t += a[i] * b[i];
+        return t;
+    };
+    auto precond = [&](const std::vector<double>& a, std::vector<double>& y) {
+        for (size_t i = 0; i < n; ++i) y[i] = a[i] / diag[i / 3];
+    };
+    matvec(x, s);
+    for (uint32_t v = 0; v < nV; ++v)
+        for (int c = 0; c < 3; ++c) r[3 * v + c] = mass[v] * x0[3 * v + c] - s[3 * v + c];
+    precond(r, p);
+    double delta = std::abs(dot(r, p));
+    const double start = delta;
+    int it = 0;
+    auto converged = [&](double d) { return d < abs_tol || d / start < rel_tol; };
+    if (!converged(delta)) {
+        while (it < max_iter) {
+            matvec(p, s);
+            const double alpha = delta / dot(s, p);
+            for (size_t i = 0; i < n; ++i) x[i] += alpha * p[i];
+            for (size_t i = 0; i < n; ++i) r[i] -= alpha * s[i];
+            precond(r, z);
+            const double delta_old = delta;
+            delta = dot(r, z);
+            if (converged(delta)) break;
+            const double beta = delta / delta_old;
+            for (size_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+            ++it;
+        }
+    }
+    std::copy(x.begin(), x.end(), out);
+    res[0] = start;
+    res[1] = delta;
+    return it;
+}
+
 // Rest shapes Mr (4 per face, column-major) from 3D rest positions.
 void ora_rest_shapes(const ora_mesh* h, const double* pos, double* Mr)
 {

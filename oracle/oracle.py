@@ -61,6 +61,10 @@ def lib():
         L.ora_mcf_cg.argtypes = [ctypes.c_int64, _i32p, _i32p, _f64p, _f64p, ctypes.c_uint32, _f64p,
                                  ctypes.c_int, ctypes.c_double, ctypes.c_double, _f64p, _f64p]
         L.ora_smooth_uniform.argtypes = [vp, _f64p, ctypes.c_double, ctypes.c_int, _f64p]
+        L.ora_mcf_pcg.restype = ctypes.c_int
+        L.ora_mcf_pcg.argtypes = L.ora_mcf_cg.argtypes
+        L.ora_uniform_assemble.restype = ctypes.c_int64
+        L.ora_uniform_assemble.argtypes = [vp, ctypes.c_double, _i32p, _i32p, _f64p, _f64p]
         L.ora_cot_assemble.restype = ctypes.c_int64
         L.ora_cot_assemble.argtypes = [vp, _f64p, ctypes.c_double, _i32p, _i32p, _f64p, _f64p]
         L.ora_mcf_explicit.argtypes = [ctypes.c_int64, _i32p, _i32p, _f64p, _f64p,
@@ -258,15 +262,30 @@ class OracleMesh:
                                x0.ravel(), iters, out.reshape(-1))
         return out
 
-    def mcf_cg(self, rows, cols, vals, mass, x0, max_iter=100, abs_tol=1e-6, rel_tol=0.0):
-        """(x, iterations, start |r|^2, final |r|^2) of the reference's CG on A x = M x0."""
+    def mcf_cg(self, rows, cols, vals, mass, x0, max_iter=100, abs_tol=1e-6, rel_tol=0.0, precond=False):
+        """(x, iterations, start residual, final residual) of the reference's CG
+        (|r|^2) or, with precond, its Jacobi-preconditioned CG (<r, r / diag A>)
+        on A x = M x0."""
         x0 = _c(x0, np.float64).reshape(-1, 3)
         out = np.zeros_like(x0)
         res = np.zeros(2, np.float64)
-        it = lib().ora_mcf_cg(len(rows), _c(rows, np.int32), _c(cols, np.int32), _c(vals, np.float64),
-                              _c(mass, np.float64), x0.shape[0], x0.ravel(), int(max_iter), float(abs_tol),
-                              float(rel_tol), out.reshape(-1), res)
+        fn = lib().ora_mcf_pcg if precond else lib().ora_mcf_cg
+        it = fn(len(rows), _c(rows, np.int32), _c(cols, np.int32), _c(vals, np.float64),
+                _c(mass, np.float64), x0.shape[0], x0.ravel(), int(max_iter), float(abs_tol),
+                float(rel_tol), out.reshape(-1), res)
         return out, it, res[0], res[1]
+
+    def uniform_assemble(self, dt):
+        """COO (rows, cols, vals) and right-hand-side scale (valence) of the MCF
+        app's uniform-weight system, laid out as cot_assemble's."""
+        c = self.counts()
+        nnz = c["nV"] + 2 * c["nE"]
+        rows = np.zeros(nnz, np.int32)
+        cols = np.zeros(nnz, np.int32)
+        vals = np.zeros(nnz, np.float64)
+        mass = np.zeros(c["nV"], np.float64)
+        n = lib().ora_uniform_assemble(self._h, float(dt), rows, cols, vals, mass)
+        return rows[:n], cols[:n], vals[:n], mass
 
     def rest_shapes(self, pos):
         pos = _c(pos, np.float64).reshape(-1, 3)

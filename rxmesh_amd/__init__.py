@@ -55,6 +55,7 @@ _SIGS = {
     "rxm_arap_rotations": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_arap_energy_grad": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_mcf_implicit_cg": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _f, _f, _vp, _vp, _vp, _vp],
+    "rxm_mcf_matfree_cg": [_vp, _vp, _vp, _vp, _f, _i, _i, _i, _f, _f, _vp, _vp, _vp, _vp],
     "rxm_mesh_info": [_vp, ctypes.POINTER(MeshInfo)],
     "rxm_mesh_array_size": [_vp, _i, ctypes.POINTER(ctypes.c_uint64)],
     "rxm_mesh_get_array": [_vp, _i, _vp, ctypes.c_uint64],
@@ -515,6 +516,25 @@ class Mesh:
         _check(lib().rxm_mcf_implicit_cg(self._h, *self._system(A), self._v3(x, "x"), self._v3(out, "out"),
                                          int(max_iter), float(abs_tol), float(rel_tol), *refs, _stream(stream)),
                "rxm_mcf_implicit_cg")
+        if not stats:
+            return out, None, None, None
+        return out, it.value, r0.value, r1.value
+
+    def mcf_matfree_cg(self, coords, x, dt, uniform=False, precond=False, max_iter=100, abs_tol=1e-6, rel_tol=0.0,
+                       out=None, stats=True, stream=None):
+        """Implicit MCF step with the system never assembled (weights recomputed
+        from `coords` in every product): CG, or Jacobi-preconditioned CG, on
+        A X = M x from X = x. Returns (X, iterations, start residual, final
+        residual) -- the last three None when stats is False."""
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        it, r0, r1 = ctypes.c_int(), ctypes.c_float(), ctypes.c_float()
+        refs = (ctypes.byref(it), ctypes.byref(r0), ctypes.byref(r1)) if stats else (None, None, None)
+        _check(lib().rxm_mcf_matfree_cg(self._h, self._v3(coords, "coords"), self._v3(x, "x"), self._v3(out, "out"),
+                                        float(dt), int(bool(uniform)), int(bool(precond)), int(max_iter),
+                                        float(abs_tol), float(rel_tol), *refs, _stream(stream)),
+               "rxm_mcf_matfree_cg")
         if not stats:
             return out, None, None, None
         return out, it.value, r0.value, r1.value

@@ -62,6 +62,7 @@ struct rxm_mesh
     uint32_t          smooth_g4    = 1;  // two-step launches pass float4 iterates between them
     float*            d_cg         = nullptr;  // CG work vectors: r, p (two), s (3 V floats each), partials
     CgState*          d_cg_state   = nullptr;
+    float*            d_mf         = nullptr;  // matrix-free CG: r, p (two), s (3 V each), mass, diag (V), partials
     float*            d_iter4      = nullptr;  // two float4 iterates (8 V floats), allocated on first use
     uint32_t          smooth_fused = 0;  // 1: two Jacobi steps per launch
     uint32_t          spmv_lanes   = 8;  // lanes per CSR row in the SpMV-3 kernels
@@ -271,10 +272,12 @@ int device_base(rxm_mesh* m, int op, const uint32_t** out)
 }
 
 // Launch a one-workgroup-per-patch kernel with `lds` bytes of dynamic LDS.
+constexpr uint32_t kMaxLdsBytes = 160u * 1024u;  // LDS of one gfx950 CU
+
 template <typename K, typename... Args>
 int launch_grid(K kernel, uint32_t grid, uint32_t threads, uint32_t lds, void* stream, Args... args)
 {
-    if (lds > 160u * 1024u) return RXM_ERR_PATCH_TOO_BIG;
+    if (lds > kMaxLdsBytes) return RXM_ERR_PATCH_TOO_BIG;
     if (lds > 64u * 1024u)
         RXM_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
                                     hipFuncAttributeMaxDynamicSharedMemorySize, static_cast<int>(lds)));
@@ -862,29 +865,23 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
         return (op == RXM_OP_VV) ? go(k_query_vx_oriented<256, true>, 256) : go(k_query_vx_oriented<256, false>, 256);
     }
     // transposed relations: atomic-free stable counting sort, or the LDS
-    // atomic histogram + row sort (knob 10); identical rows
-    const bool     st = m->query_stable != 0;
-    const uint32_t hm = st ? T / 64u : 1u;
-    auto q2 = [&](auto k_atomic, auto k_stable) {
-        return st ? launch_patches_xcd(k_stable, T, b.bytes(), stream, m, m->dm, base, row_ptr, col)
-                  : launch_patches_xcd(k_atomic, T, b.bytes(), stream, m, m->dm, base, row_ptr, col);
+    // atomic histogram + row sort (knob 10); identical rows. The stable
+    // sort's (key, wave) table takes T / 64 times the histogram's LDS, so a
+    // mesh whose largest patch does not fit that plan takes the atomic form.
+    auto q2 = [&](int kind, auto k_atomic, auto k_stable) {
+        Bump bs(nullptr), ba(nullptr);
+        plan_q2(bs, kind, h.max_nv, h.max_ne, h.max_nf, T / 64u);
+        plan_q2(ba, kind, h.max_nv, h.max_ne, h.max_nf, 1u);
+        if (m->query_stable != 0 && bs.bytes() <= kMaxLdsBytes)
+            return launch_patches_xcd(k_stable, T, bs.bytes(), stream, m, m->dm, base, row_ptr, col);
+        return launch_patches_xcd(k_atomic, T, ba.bytes(), stream, m, m->dm, base, row_ptr, col);
     };
     switch (op) {
-        case RXM_OP_VV:
-            plan_q2(b, 0, h.max_nv, h.max_ne, h.max_nf, hm);
-            return q2(k_query2_vx<T, true, false>, k_query2_vx<T, true, true>);
-        case RXM_OP_VE:
-            plan_q2(b, 3, h.max_nv, h.max_ne, h.max_nf, hm);
-            return q2(k_query2_vx<T, false, false>, k_query2_vx<T, false, true>);
-        case RXM_OP_VF:
-            plan_q2(b, 1, h.max_nv, h.max_ne, h.max_nf, hm);
-            return q2(k_query2_vf<T, false>, k_query2_vf<T, true>);
-        case RXM_OP_EF:
-            plan_q2(b, 2, h.max_nv, h.max_ne, h.max_nf, hm);
-            return q2(k_query2_ef<T, false, false>, k_query2_ef<T, false, true>);
-        case RXM_OP_FF:
-            plan_q2(b, 4, h.max_nv, h.max_ne, h.max_nf, hm);
-            return q2(k_query2_ef<T, true, false>, k_query2_ef<T, true, true>);
+        case RXM_OP_VV: return q2(0, k_query2_vx<T, true, false>, k_query2_vx<T, true, true>);
+        case RXM_OP_VE: return q2(3, k_query2_vx<T, false, false>, k_query2_vx<T, false, true>);
+        case RXM_OP_VF: return q2(1, k_query2_vf<T, false>, k_query2_vf<T, true>);
+        case RXM_OP_EF: return q2(2, k_query2_ef<T, false, false>, k_query2_ef<T, false, true>);
+        case RXM_OP_FF: return q2(4, k_query2_ef<T, true, false>, k_query2_ef<T, true, true>);
         case RXM_OP_EV:
             plan_fx(b, h.max_ne, h.max_nf);
             return launch_patches(k_query_fixed<9>, T, b.bytes(), stream, m, m->dm, row_ptr, col);
@@ -911,14 +908,14 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     int             rc = device_base(m, RXM_OP_VV, &vv_base);
     if (!rc) rc = device_base(m, RXM_OP_VF, &vf_base);
     if (rc) return rc;
-    Bump               b(nullptr);
-    constexpr uint32_t T  = kQueryThreads;
-    const bool         st = m->query_stable != 0;
-    plan_vvf(b, m->h.max_nv, m->h.max_ne, m->h.max_nf, st ? T / 64u : 1u);
-    if (st)
-        return launch_patches_xcd(k_query2_vv_vf<T, true>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base,
+    constexpr uint32_t T = kQueryThreads;
+    Bump               bs(nullptr), ba(nullptr);  // stable / atomic plans, as in rxm_query_csr
+    plan_vvf(bs, m->h.max_nv, m->h.max_ne, m->h.max_nf, T / 64u);
+    plan_vvf(ba, m->h.max_nv, m->h.max_ne, m->h.max_nf, 1u);
+    if (m->query_stable != 0 && bs.bytes() <= kMaxLdsBytes)
+        return launch_patches_xcd(k_query2_vv_vf<T, true>, T, bs.bytes(), stream, m, m->dm, vv_base, vf_base,
                                   vv_row_ptr, vv_col, vf_row_ptr, vf_col);
-    return launch_patches_xcd(k_query2_vv_vf<T, false>, T, b.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
+    return launch_patches_xcd(k_query2_vv_vf<T, false>, T, ba.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                               vv_col, vf_row_ptr, vf_col);
 }
 
@@ -1338,6 +1335,91 @@ rxm_status rxm_mcf_implicit_cg(rxm_mesh* m, const int32_t* row_ptr, const int32_
         if (rc) return rc;
         hipLaunchKernelGGL((k_cg_scalar<1>), dim3(1), dim3(1024), 0, hs, P, part, st, abs_tol, rel_tol);
         hipLaunchKernelGGL((k_cg_update<1>), dim3(nb), dim3(256), 0, hs, n, mass, x, out, r, pnew, s, part, st);
+        hipLaunchKernelGGL((k_cg_scalar<2>), dim3(1), dim3(1024), 0, hs, nb, part, st, abs_tol, rel_tol);
+    }
+    RXM_HIP(hipGetLastError());
+    if (iters || start_residual || final_residual) {
+        CgState h{};
+        RXM_HIP(hipMemcpyAsync(&h, st, sizeof(CgState), hipMemcpyDeviceToHost, hs));
+        RXM_HIP(hipStreamSynchronize(hs));
+        if (iters) *iters = static_cast<int>(h.iters);
+        if (start_residual) *start_residual = h.start;
+        if (final_residual) *final_residual = h.delta;
+    }
+    return RXM_OK;
+}
+
+rxm_status rxm_mcf_matfree_cg(rxm_mesh* m, const float* coords, const float* x, float* out, float dt, int uniform,
+                              int precond, int max_iter, float abs_tol, float rel_tol, int* iters,
+                              float* start_residual, float* final_residual, void* stream)
+{
+    if (!m || !coords || !x || !out || x == out || coords == out || max_iter < 0) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    const bool uni = uniform != 0, pre = precond != 0;
+    int        rc  = RXM_OK;
+    if (!uni) {
+        if (!(m->h.manifold && m->h.oriented)) return RXM_ERR_NON_MANIFOLD;
+        if ((rc = ensure_cot_plan(m))) return rc;
+        // the face pass gives a vertex its reference weights only when its
+        // ring is one closed fan (a closed vertex-manifold mesh)
+        if (m->cot_nplist != 0) return RXM_ERR_UNSUPPORTED;
+    } else if ((rc = ensure_cot_plan(m))) {
+        return rc;
+    }
+    const uint32_t* vv_base = nullptr;
+    if ((rc = device_base(m, RXM_OP_VV, &vv_base))) return rc;
+    const uint32_t n = m->h.nV, P = m->h.P, nb = (n + 255u) / 256u;
+    const size_t   v3 = size_t(3) * n, npart = std::max<size_t>(P, nb);
+    if (!m->d_mf) {
+        float* p = nullptr;
+        const size_t cnt = 4 * v3 + 2 * size_t(n) + npart;
+        if (hipMalloc(&p, cnt * sizeof(float)) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(p);
+        m->device_bytes += cnt * sizeof(float);
+        m->d_mf = p;
+    }
+    if (!m->d_cg_state) {
+        CgState* st = nullptr;
+        if (hipMalloc(&st, sizeof(CgState)) != hipSuccess) return RXM_ERR_OUT_OF_MEMORY;
+        m->allocs.push_back(st);
+        m->d_cg_state = st;
+    }
+    float *r = m->d_mf, *pv[2] = {m->d_mf + v3, m->d_mf + 2 * v3}, *s = m->d_mf + 3 * v3;
+    float *mass = m->d_mf + 4 * v3, *diag = mass + n, *part = diag + n;
+    CgState*    st = m->d_cg_state;
+    hipStream_t hs = static_cast<hipStream_t>(stream);
+    constexpr uint32_t T = 256;
+    Bump b(nullptr);
+    plan_mf(b, m->h.max_nv, m->h.max_ne, m->h.max_nvo, op_max_patch(m, RXM_OP_VV), uni, T);
+    const uint32_t lds = b.bytes();
+    float* dg = pre ? diag : nullptr;
+    auto apply = [&](auto kernel, const float* xin, float* pnew, float* y) {
+        return launch_patches_xcd(kernel, T, lds, stream, m, m->dm, vv_base, m->d_cot_rs, m->d_cot_ent, coords, dt,
+                                  xin, static_cast<const float*>(r), dg, mass, pnew, y, part,
+                                  static_cast<const CgState*>(st));
+    };
+    // the kernel for (mode, weights, preconditioner)
+    auto launch = [&](int mode, const float* xin, float* pnew, float* y) {
+#define RXM_MF(MODE)                                                                         \
+    (uni ? (pre ? apply(k_mf_apply<T, MODE, true, true>, xin, pnew, y)                       \
+                : apply(k_mf_apply<T, MODE, true, false>, xin, pnew, y))                     \
+         : (pre ? apply(k_mf_apply<T, MODE, false, true>, xin, pnew, y)                      \
+                : apply(k_mf_apply<T, MODE, false, false>, xin, pnew, y)))
+        return mode == 0 ? RXM_MF(0) : (mode == 1 ? RXM_MF(1) : RXM_MF(2));
+#undef RXM_MF
+    };
+    // set-up: s = A x0, mass (and diag), r = B - s with B = mass * x0, x = x0, delta
+    if ((rc = launch(0, x, nullptr, s))) return rc;
+    hipLaunchKernelGGL((k_cg_update<0>), dim3(nb), dim3(256), 0, hs, n, static_cast<const float*>(mass), x, out, r,
+                       nullptr, s, part, st, static_cast<const float*>(dg));
+    hipLaunchKernelGGL((k_cg_scalar<0>), dim3(1), dim3(1024), 0, hs, nb, part, st, abs_tol, rel_tol);
+    for (int it = 0; it < max_iter; ++it) {
+        float* pold = pv[it & 1];
+        float* pnew = pv[(it + 1) & 1];
+        if ((rc = launch(it == 0 ? 1 : 2, pold, pnew, s))) return rc;
+        hipLaunchKernelGGL((k_cg_scalar<1>), dim3(1), dim3(1024), 0, hs, P, part, st, abs_tol, rel_tol);
+        hipLaunchKernelGGL((k_cg_update<1>), dim3(nb), dim3(256), 0, hs, n, static_cast<const float*>(mass), x, out,
+                           r, pnew, s, part, st, static_cast<const float*>(dg));
         hipLaunchKernelGGL((k_cg_scalar<2>), dim3(1), dim3(1024), 0, hs, nb, part, st, abs_tol, rel_tol);
     }
     RXM_HIP(hipGetLastError());

@@ -617,15 +617,180 @@ __global__ __launch_bounds__(T) void k_cg_apply(DevMesh m, const uint32_t* __res
     }
 }
 
+// ---- matrix-free implicit MCF product -----------------------------------------
+// LDS carve of the matrix-free product: the assembly's carve (positions and
+// half-edge terms only for cotangent weights) plus the staged input vector.
+struct PlanMF
+{
+    float4*   pos;
+    float4*   he;
+    uint32_t *lin, *ev, *rs, *ent;
+    float *   vx, *vy, *vz, *red;
+};
+__host__ __device__ inline PlanMF plan_mf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nvo, uint32_t nent,
+                                          bool uniform, uint32_t threads)
+{
+    PlanMF p{};
+    if (!uniform) {
+        p.pos = b.alloc<float4>(nv);
+        p.he  = b.alloc<float4>(ne);
+        p.lin = b.alloc<uint32_t>(nv);
+    }
+    p.ev  = b.alloc<uint32_t>(r4(ne));
+    p.rs  = b.alloc<uint32_t>(nvo + 1);
+    p.ent = b.alloc<uint32_t>(nent);
+    p.vx  = b.alloc<float>(nv);
+    p.vy  = b.alloc<float>(nv);
+    p.vz  = b.alloc<float>(nv);
+    p.red = b.alloc<float>(threads / 64u);
+    return p;
+}
+
+// y = A v for the implicit MCF system A = M + dt L with the weights
+// recomputed from the fixed coordinates in every product (the reference's
+// matvec, apps/MCF/mcf_kernels.cuh:114-213): per owned row, in the order of
+// the reference's ring loop, acc -= (dt w_e) v[r_e] over the row's edges,
+// then y = acc + diag v[p] with diag = 1 / v_weight + sum_e dt w_e.
+// Cotangent weights (UNIFORM false): w_e = max(0, cot + cot) of the edge and
+// v_weight = 0.5 / (the vertex's Voronoi area), from one evaluation of every
+// face of the patch (the terms k_cot_assemble computes); every vertex ring
+// must be one closed fan (checked on the host). Uniform weights: w_e = 1,
+// v_weight = 1 / valence, no orientation needed.
+// MODE 0 (set-up): v = x; also writes mass[g] = 1 / v_weight (the
+// reference's init_B scale: valence for uniform weights) and, if diag is not
+// null, diag[g] for the preconditioner. MODE 1 / 2 (CG iteration): v = the
+// new direction p = z (MODE 1) or z + beta p_old (MODE 2), z = r, or r / diag
+// (Jacobi, the reference's precond_matvec, :219-299) when PRECOND; p is
+// written for the owned rows and <y, p> summed per workgroup into partial[p].
+template <uint32_t T, int MODE, bool UNIFORM, bool PRECOND>
+__global__ __launch_bounds__(T) void k_mf_apply(DevMesh m, const uint32_t* __restrict__ vv_base,
+                                                const uint32_t* __restrict__ rs, const uint32_t* __restrict__ ent,
+                                                const float* __restrict__ coords, float dt,
+                                                const float* __restrict__ x, const float* __restrict__ r,
+                                                float* __restrict__ diag, float* __restrict__ mass,
+                                                float* __restrict__ p_new, float* __restrict__ y,
+                                                float* __restrict__ partial, const CgState* __restrict__ st)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    if (MODE != 0 && st->done) return;
+    const PatchDesc d  = load_desc(m.patch, p);
+    const uint32_t  vb = vv_base[p], nent = vv_base[p + 1] - vb;
+    const uint32_t  N  = m.nV;
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanMF    L = plan_mf(b, d.nv, d.ne, d.nvo, nent, UNIFORM, T);
+    stage_u32(L.ev, m.ev + d.e_off, d.ne);
+    stage_words<T, 8>(L.ent, ent + vb, nent);
+    stage_words<T, 2>(L.rs, rs + d.vp, d.nvo + 1, vb);
+    if constexpr (!UNIFORM) {
+        stage_vertices<T>(m, d, coords, L.pos, L.lin);
+        for (uint32_t i = threadIdx.x; i < d.ne; i += T) L.he[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    }
+    {
+        const uint32_t* vl   = m.v_lin + d.v_off;
+        const float     beta = MODE == 2 ? st->beta : 0.f;
+        for (uint32_t l = threadIdx.x; l < d.nv; l += T) {
+            const uint32_t g = lin_of(l, d.nvo, d.vp, vl);
+            float          a[3];
+#pragma unroll
+            for (uint32_t c = 0; c < 3; ++c) {
+                if constexpr (MODE == 0) {
+                    a[c] = x[c * N + g];
+                } else {
+                    const float z = PRECOND ? r[c * N + g] / diag[g] : r[c * N + g];
+                    a[c]          = (MODE == 2) ? z + beta * x[c * N + g] : z;  // x holds p_old
+                }
+            }
+            L.vx[l] = a[0];
+            L.vy[l] = a[1];
+            L.vz[l] = a[2];
+        }
+    }
+    __syncthreads();
+    if constexpr (!UNIFORM) {
+        const uint16_t* fe = m.fe + 3u * d.f_off;
+        float2*         hh = reinterpret_cast<float2*>(L.he);  // hh[2e + dir]
+        for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
+            uint32_t xs[3], a[3];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) xs[j] = fe[3 * f + j];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const uint32_t e = L.ev[xs[j] >> 1];
+                a[j]             = (xs[j] & 1u) ? hi16(e) : lo16(e);
+            }
+            const float4  q0 = L.pos[a[0]], q1 = L.pos[a[1]], q2 = L.pos[a[2]];
+            const FaceCot c  = face_cot(f3{q0.x, q0.y, q0.z}, f3{q1.x, q1.y, q1.z}, f3{q2.x, q2.y, q2.z});
+#pragma unroll
+            for (int j = 0; j < 3; ++j) hh[xs[j]] = make_float2(c.cot[(j + 2) % 3], c.vor[j]);
+        }
+        __syncthreads();
+    }
+    float dotp = 0.f;
+    for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
+        const uint32_t k0 = L.rs[v], k1 = L.rs[v + 1];
+        float          sum_w = 0.f, vw = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
+        for (uint32_t k = k0; k < k1; ++k) {
+            const uint32_t e = L.ent[k] & 0xFFFFu, xe = L.ev[e];
+            const uint32_t q = (lo16(xe) == v) ? hi16(xe) : lo16(xe);
+            float          w = 1.f;
+            if constexpr (!UNIFORM) {
+                const float4 h = L.he[e];
+                w              = h.x + h.z;
+                w              = (w >= 0.f) ? w : 0.f;
+                vw += (lo16(xe) == v) ? h.y : h.w;
+            }
+            const float wd = w * dt;
+            sum_w += wd;
+            ax -= wd * L.vx[q];
+            ay -= wd * L.vy[q];
+            az -= wd * L.vz[q];
+        }
+        const float vwt = UNIFORM ? 1.0f / static_cast<float>(k1 - k0) : 0.5f / vw;
+        const float dg  = 1.0f / vwt + sum_w;
+        const float px = L.vx[v], py = L.vy[v], pz = L.vz[v];
+        ax += dg * px;
+        ay += dg * py;
+        az += dg * pz;
+        const uint32_t g = d.vp + v;
+        y[g]             = ax;
+        y[N + g]         = ay;
+        y[2 * N + g]     = az;
+        if constexpr (MODE == 0) {
+            // the reference's init_B: x * valence (uniform) or x / v_weight
+            mass[g] = UNIFORM ? static_cast<float>(k1 - k0) : 1.0f / vwt;
+            if (diag) diag[g] = dg;
+        } else {
+            p_new[g]         = px;
+            p_new[N + g]     = py;
+            p_new[2 * N + g] = pz;
+            dotp += ax * px + ay * py + az * pz;
+        }
+    }
+    if constexpr (MODE != 0) {
+        for (uint32_t o = 32; o > 0; o >>= 1) dotp += __shfl_xor(dotp, o, 64);
+        if ((threadIdx.x & 63u) == 0) L.red[threadIdx.x >> 6] = dotp;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float t = 0.f;
+            for (uint32_t i = 0; i < T / 64u; ++i) t += L.red[i];
+            partial[p] = t;
+        }
+    }
+}
+
 // Per linear vertex (MODE 0, set-up): r = b - s with b = mass * x0, and the
-// workgroup's partial <r, r>. MODE 1 (iteration): x += alpha p, r -= alpha s
-// and the partial <r, r>.
+// workgroup's partial <r, r> (<r, r / diag> with the Jacobi preconditioner,
+// diag non-null). MODE 1 (iteration): x += alpha p, r -= alpha s and the
+// partial <r, r> (<r, r / diag>).
 template <int MODE>
 __global__ __launch_bounds__(256) void k_cg_update(uint32_t n, const float* __restrict__ mass,
                                                    const float* __restrict__ x0, float* __restrict__ x,
                                                    float* __restrict__ r, const float* __restrict__ pvec,
                                                    const float* __restrict__ s, float* __restrict__ partial,
-                                                   const CgState* __restrict__ st)
+                                                   const CgState* __restrict__ st,
+                                                   const float* __restrict__ diag = nullptr)
 {
     __shared__ float red[4];
     if (MODE == 1 && st->done) return;
@@ -637,21 +802,23 @@ __global__ __launch_bounds__(256) void k_cg_update(uint32_t n, const float* __re
             // B - A x): no fused multiply-subtract here
 #pragma clang fp contract(off)
             const float M = mass[i];
+            const float D = diag ? diag[i] : 1.f;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 const float v = M * x0[c * n + i] - s[c * n + i];
                 r[c * n + i]  = v;
                 x[c * n + i]  = x0[c * n + i];
-                acc += v * v;
+                acc += diag ? v * (v / D) : v * v;
             }
         } else {
             const float a = st->alpha;
+            const float D = diag ? diag[i] : 1.f;
 #pragma unroll
             for (uint32_t c = 0; c < 3; ++c) {
                 x[c * n + i] += a * pvec[c * n + i];
                 const float v = r[c * n + i] - a * s[c * n + i];
                 r[c * n + i]  = v;
-                acc += v * v;
+                acc += diag ? v * (v / D) : v * v;
             }
         }
     }

@@ -201,3 +201,72 @@ def test_work_queue_mode_equals_grid_mode():
         assert torch.equal(xg, outs[0]), rep
     g.close()
     b.set_launch_threads(2, 0)
+
+
+def test_query_step_loop_equals_plan_loop_and_queries():
+    """The step loop with the per-step VV + VF query leaves the same positions
+    as the loop without it, its CSRs equal rxm_query_vv_vf's, and the plain
+    per-mesh form equals the batched one bit for bit."""
+    import torch
+    import rxmesh_amd as rx
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    xq, tq = x.clone(), torch.empty_like(x)
+    vv, vf = b.vv_vf_buffers()
+    b.query_smooth_symdir_steps(Mr, xq, tq, LAM, ETA, 3, vv, vf)
+    xp, tp = x.clone(), torch.empty_like(x)
+    b.smooth_symdir_steps(Mr, xp, tp, LAM, ETA, 3)
+    rvv, rvf = b.vv_vf_buffers()
+    b.query_vv_vf(vv=rvv, vf=rvf)
+    torch.cuda.synchronize()
+    assert torch.equal(xq, xp)
+    for got, ref in ((vv, rvv), (vf, rvf)):
+        assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1])
+    xs, ts, Mrs, bufs = [], [], [], []
+    for k, m in enumerate(sep):
+        xk = lin(m, X[ov[k]:ov[k + 1]])
+        Mrs.append(m.symdir_rest(xk))
+        xs.append(xk.clone())
+        ts.append(torch.empty_like(xk))
+        bufs.append(m.vv_vf_buffers())
+    rx.query_smooth_symdir_steps_each(sep, Mrs, xs, ts, bufs, LAM, ETA, 3)
+    torch.cuda.synchronize()
+    gb = glob(b, xq.cpu().numpy())
+    for k, m in enumerate(sep):
+        assert np.array_equal(gb[:, ov[k]:ov[k + 1]], glob(m, xs[k].cpu().numpy()))
+        mvv, mvf = m.query_vv_vf()
+        for got, ref in zip(bufs[k], (mvv, mvf)):
+            assert torch.equal(got[0], ref[0]) and torch.equal(got[1][: ref[1].numel()], ref[1])
+
+
+def test_query_step_loop_graph_replays_match_eager():
+    """The per-step-query loop captured in a HIP graph (torch's capture of the
+    C-ABI launches): every replay equals the eager loop, with unrelated eager
+    work and synchronisations between replays."""
+    import torch
+    ms, b, sep, ov, X, x = setup()
+    Mr = b.symdir_rest(x)
+    xe, te = x.clone(), torch.empty_like(x)
+    xg, tg = x.clone(), torch.empty_like(x)
+    qe, qg = b.vv_vf_buffers(), b.vv_vf_buffers()
+    run = lambda: b.query_smooth_symdir_steps(Mr, xg, tg, LAM, ETA, 2, *qg)  # noqa: E731
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        run()
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    xg.copy_(x)
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, capture_error_mode="thread_local"):
+        run()
+    torch.cuda.synchronize()
+    for rep in range(4):
+        b.query_smooth_symdir_steps(Mr, xe, te, LAM, ETA, 2, *qe)
+        g.replay()
+        torch.cuda.synchronize()
+        assert torch.equal(xg, xe), rep
+        for got, ref in zip(qg, qe):
+            assert torch.equal(got[0], ref[0]) and torch.equal(got[1], ref[1]), rep
+        _ = b.query("VF")  # unrelated eager work between replays
+        torch.cuda.synchronize()

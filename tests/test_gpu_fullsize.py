@@ -25,7 +25,10 @@ Numerical scope, stated per result:
     from the FP64 iterates within a few iterations; there the tests check
     what is defined: determinism, the iteration count the FP64 restatement
     takes (the cap), the starting residual within 1e-3 (r0 = M x - A x
-    cancels to about four fp32 digits) and finite output.
+    cancels to about four fp32 digits) and finite output;
+  - matrix-free CG and PCG (weights recomputed in every product): at dt =
+    1e-4 run to a residual ratio below 1e-10, within 1e-5 of the FP64 CG on
+    the FP64 system, as the assembled CG is.
 """
 import numpy as np
 import pytest
@@ -135,3 +138,20 @@ def test_fullsize_implicit_cg(full):
     _, it64, s0, _ = o.mcf_cg(*_coo(A, l2g), V64, 100, 1e-6, 0.0)
     assert it == it64 == 100
     assert abs(r0 - s0) <= 1e-3 * s0
+
+
+def test_fullsize_matfree_cg(full):
+    import torch
+    name, m, o, x, V64, l2g = full
+    dt = 1e-4
+    orows, ocols, ovals, omass = o.cot_assemble(V64, dt)
+    X64, _, _, _ = o.mcf_cg(orows, ocols, ovals, omass, V64, 5000, 0.0, 1e-14)
+    for precond in (False, True):
+        print("matrix-free", name, "precond" if precond else "cg", flush=True)
+        X, it, r0, r1 = m.mcf_matfree_cg(x, x, dt, precond=precond, max_iter=3000, abs_tol=0.0, rel_tol=1e-10)
+        torch.cuda.synchronize()
+        assert 0 < it < 3000 and r1 / r0 < 1e-10
+        got = _glob(l2g, X.cpu().numpy()).T
+        assert np.isfinite(got).all()
+        assert np.abs(got - X64).max() <= 1e-5 * np.abs(X64).max()
+        del X

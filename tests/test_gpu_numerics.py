@@ -311,6 +311,70 @@ def test_mcf_implicit_cg_stopping_rules():
     assert abs(it - it64) <= max(3, it64 // 5)
 
 
+@pytest.mark.parametrize("name,dt,uniform,precond,fwd_tol", [
+    ("ico4", 1e-2, False, False, 1e-5), ("ico4", 1e-2, False, True, 1e-5), ("torus", 1e-2, False, True, 1e-5),
+    ("ico4", 10.0, False, False, 2e-4), ("ico5_perm", 1e-2, True, False, 1e-5), ("disk", 1.0, True, True, 1e-5)])
+def test_mcf_matfree_cg_vs_direct_solve(name, dt, uniform, precond, fwd_tol):
+    """The matrix-free implicit step (weights recomputed from the coordinates
+    in every product; the reference's matvec / precond_matvec under its CG /
+    PCG) solves the FP64 system of the same weights -- cotangent or uniform
+    -- to fp32 accuracy: backward error below 1e-5, within fwd_tol of the
+    FP64 direct solution (the tolerances of the assembled CG), about as many
+    iterations as the FP64 restatement of the same (P)CG, two solves
+    bit-identical; for cotangent weights it agrees with the assembled-system
+    CG to the same tolerance."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    import torch
+    V, F, m, o = _mesh(name)
+    x = _lin(m, V.astype(np.float32))
+    kw = dict(uniform=uniform, precond=precond, max_iter=500, abs_tol=1e-12)
+    X, it, r0, r1 = m.mcf_matfree_cg(x, x, dt, **kw)
+    X2, it2, _, r2 = m.mcf_matfree_cg(x, x, dt, **kw)
+    torch.cuda.synchronize()
+    assert torch.equal(X, X2) and it == it2 and r1 == r2
+    V64 = V.astype(np.float32).astype(np.float64)
+    rows, cols, vals, mass = o.uniform_assemble(dt) if uniform else o.cot_assemble(V64, dt)
+    n = len(V)
+    As = sp.coo_matrix((vals, (rows, cols)), shape=(n, n)).tocsr()
+    b = mass[:, None] * V64
+    xs = sla.spsolve(As.tocsc(), b)
+    got = _glob(m, X.cpu().numpy()).T.astype(np.float64)
+    assert r1 < 1e-12 and 0 < it < 500
+    assert np.linalg.norm(As @ got - b) <= 1e-5 * np.linalg.norm(abs(As) @ np.abs(got))
+    assert np.abs(got - xs).max() <= fwd_tol * np.abs(xs).max()
+    _, it64, s0, _ = o.mcf_cg(rows, cols, vals, mass, V64, 500, 1e-12, 0.0, precond=precond)
+    assert abs(it - it64) <= max(3, it64 // 5)
+    assert abs(r0 - s0) <= 1e-4 * s0
+    if not uniform and not precond:
+        Xa, ita, _, _ = m.mcf_implicit_cg(m.cot_assemble(x, dt), x, max_iter=500, abs_tol=1e-12)
+        torch.cuda.synchronize()
+        assert abs(ita - it) <= max(2, it // 10)
+        assert float((Xa - X).abs().max()) <= fwd_tol * float(Xa.abs().max())
+
+
+def test_mcf_matfree_cg_stopping_rules_and_refusals():
+    """dt = 0 with cotangent weights gives A = M, so x solves A X = M x
+    exactly and no iteration runs; max_iter = 0 returns x; a relative
+    tolerance stops on the residual ratio; cotangent weights on a mesh with
+    boundary rings are refused (unsupported) before any launch."""
+    import torch
+    import rxmesh_amd as rx
+    V, F, m, o = _mesh("torus")
+    x = _lin(m, V.astype(np.float32))
+    X, it, r0, r1 = m.mcf_matfree_cg(x, x, 0.0, max_iter=50, abs_tol=1e-20)
+    assert it == 0 and r0 == 0.0 and torch.equal(X, x)
+    X, it, r0, r1 = m.mcf_matfree_cg(x, x, 1e-2, precond=True, max_iter=0, abs_tol=0.0)
+    assert it == 0 and torch.equal(X, x) and r1 == r0 > 0
+    X, it, r0, r1 = m.mcf_matfree_cg(x, x, 1e-2, precond=True, max_iter=500, abs_tol=0.0, rel_tol=1e-6)
+    assert 0 < it < 500 and r1 / r0 < 1e-6
+    V, F, m, o = _mesh("disk")
+    x = _lin(m, V.astype(np.float32))
+    with pytest.raises(rx.RXMError) as e:
+        m.mcf_matfree_cg(x, x, 1e-2)
+    assert e.value.code == 6
+
+
 def _rot(axis, angle):
     import math
     a = np.asarray(axis, np.float64) / np.linalg.norm(axis)

@@ -603,6 +603,79 @@ def test_mcf_cg_stopping_rules():
     assert r1 / r0 < 1e-6 and it > 0
 
 
+# ---- (f) matrix-free implicit MCF: uniform system and the Jacobi PCG ----------
+def test_uniform_system_against_adjacency():
+    """The uniform-weight system equals (1 + dt) diag(valence) - dt Adj with
+    Adj from the brute-force global VV relation (an independent code path),
+    is symmetric, and A 1 = M 1 (row sums = valence = the B scale)."""
+    import scipy.sparse as sp
+    for pos, fv in (mg.icosphere(2, seed=1, sigma=0.02), mg.disk(12, seed=2), mg.torus(16, 8)):
+        o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+        dt = 0.37
+        rows, cols, vals, mass = o.uniform_assemble(dt)
+        nV = len(pos)
+        A = sp.coo_matrix((vals, (rows, cols)), shape=(nV, nV)).toarray()
+        rp, col = o.global_relation("VV")
+        adj = np.zeros((nV, nV))
+        for v in range(nV):
+            adj[v, col[rp[v]:rp[v + 1]]] = 1.0
+        val = adj.sum(1)
+        assert np.array_equal(mass, val)
+        assert np.allclose(A, (1 + dt) * np.diag(val) - dt * adj, rtol=0, atol=1e-12)
+        assert np.array_equal(A, A.T)
+        assert np.allclose(A @ np.ones(nV), mass, rtol=0, atol=1e-12)
+        # the row layout is cot_assemble's: diagonal, then the VV query row
+        crow, ccol, _, _ = o.cot_assemble(pos, dt)
+        assert np.array_equal(crow, rows) and np.array_equal(ccol, cols)
+
+
+@pytest.mark.parametrize("kind,dt", [("cot", 1e-2), ("cot", 10.0), ("uniform", 1.0)])
+def test_mcf_pcg_matches_direct_solve(kind, dt):
+    """Run to a tiny tolerance, the restated Jacobi-preconditioned CG reaches
+    the sparse direct solve of A x = M x0, in no more iterations than CG."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as sla
+    pos, fv = mg.icosphere(2, seed=1, sigma=0.02)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    rows, cols, vals, mass = o.cot_assemble(pos, dt) if kind == "cot" else o.uniform_assemble(dt)
+    x, it, r0, r1 = o.mcf_cg(rows, cols, vals, mass, pos, 2000, 1e-28, 0.0, precond=True)
+    A = sp.coo_matrix((vals, (rows, cols)), shape=(len(pos), len(pos))).tocsc()
+    xs = sla.spsolve(A, mass[:, None] * pos)
+    assert r1 < 1e-28 and it > 0
+    assert np.abs(x - xs).max() <= 1e-9 * np.abs(xs).max()
+    _, it_cg, _, _ = o.mcf_cg(rows, cols, vals, mass, pos, 2000, 1e-28, 0.0)
+    assert it <= it_cg
+
+
+def test_mcf_pcg_with_constant_diagonal_is_cg():
+    """On a torus grid whose quads all split along the same diagonal every
+    vertex has valence 6, so the uniform system's diagonal is the constant
+    c = 6 (1 + dt): Jacobi preconditioning only scales, PCG takes CG's
+    iterates (x equal to rounding) and its residual is CG's divided by c, so
+    a relative tolerance stops both at the same step."""
+    pos, _ = mg.torus(16, 8)
+    nu, nw = 16, 8
+    i, j = np.meshgrid(np.arange(nu), np.arange(nw), indexing="ij")
+    vid = lambda a, b: (a % nu) * nw + (b % nw)  # noqa: E731
+    a, b = vid(i.ravel(), j.ravel()), vid(i.ravel() + 1, j.ravel())
+    c, d = vid(i.ravel(), j.ravel() + 1), vid(i.ravel() + 1, j.ravel() + 1)
+    fv = np.concatenate([np.stack([a, b, d], 1), np.stack([a, d, c], 1)]).astype(np.uint32)
+    assert np.all(np.bincount(fv.ravel()) == 6)
+    o = OracleMesh(fv).set_patches(np.zeros(len(fv), np.uint32))
+    dt = 0.5
+    rows, cols, vals, mass = o.uniform_assemble(dt)
+    c = 6 * (1 + dt)
+    for iters in (1, 3, 7):
+        xc, itc, r0c, r1c = o.mcf_cg(rows, cols, vals, mass, pos, iters, 0.0, 0.0)
+        xp, itp, r0p, r1p = o.mcf_cg(rows, cols, vals, mass, pos, iters, 0.0, 0.0, precond=True)
+        assert itc == itp == iters
+        assert np.abs(xp - xc).max() <= 1e-12 * np.abs(xc).max()
+        assert abs(r0p - r0c / c) <= 1e-12 * r0c and abs(r1p - r1c / c) <= 1e-9 * r0c
+    _, itc, _, _ = o.mcf_cg(rows, cols, vals, mass, pos, 500, 0.0, 1e-10)
+    _, itp, _, _ = o.mcf_cg(rows, cols, vals, mass, pos, 500, 0.0, 1e-10, precond=True)
+    assert itc == itp
+
+
 def _rotation(axis, angle):
     a = np.asarray(axis, np.float64) / np.linalg.norm(axis)
     K = np.array([[0, -a[2], a[1]], [a[2], 0, -a[0]], [-a[1], a[0], 0]])
