@@ -44,7 +44,8 @@ enum {
     RXM_ERR_IO = 4,
     RXM_ERR_DEVICE = 5,
     RXM_ERR_UNSUPPORTED = 6,
-    RXM_ERR_OUT_OF_MEMORY = 7
+    RXM_ERR_OUT_OF_MEMORY = 7,
+    RXM_ERR_SCRATCH_TOO_SMALL = 8 /* input beyond the fixed scratch of an allocation-free entry point */
 };
 
 /* Query operations; values follow the reference enum Op
@@ -312,6 +313,18 @@ rxm_status rxm_symdir_rest(rxm_mesh* mesh, const float* rest_pos, float* Mr, voi
  * plus ReduceHandle (reduce_handle.h:23-210). */
 rxm_status rxm_symdir_energy(rxm_mesh* mesh, const float* Mr, const float* uv, float* energy,
                              float* total, void* stream);
+/* Same results as rxm_symdir_energy (same kernels), allocating nothing: no
+ * hipMalloc, hipFree or allocating library call on any call, the first
+ * included. No counterpart in the reference: it is for callers that cannot
+ * allocate (code under HIP graph capture, tight serving loops). The per-patch
+ * partial sums live in a fixed library-owned device array of 2^18 floats
+ * (1 MiB per device, held until the process ends), so meshes with more than
+ * 262,144 patches are refused with RXM_ERR_SCRATCH_TOO_SMALL. Every call on a
+ * device shares that device's array: calls that overlap in time on one device
+ * (two streams, two threads) race and may return wrong totals. The current
+ * device must be the mesh's (else RXM_ERR_INVALID_ARG). */
+rxm_status rxm_symdir_energy_noalloc(rxm_mesh* mesh, const float* Mr, const float* uv, float* energy,
+                                     float* total, void* stream);
 /* grad (2 x V SoA) = scale[0] * d(sum_f E_f)/d uv (scale may be NULL = 1):
  * hand-written adjoint recomputing each face's Jacobian, gathered per vertex
  * over VF without atomics. Replaces the forward-mode AD gradient of

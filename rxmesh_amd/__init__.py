@@ -15,7 +15,8 @@ LIB_PATH = os.path.join(_HERE, "_lib", "librxmesh_amd.so")
 
 OP = {"VV": 3, "VE": 4, "VF": 5, "FV": 6, "FE": 7, "FF": 8, "EV": 9, "EF": 11, "EVD": 12}
 STATUS = {0: "ok", 1: "invalid argument", 2: "non-manifold mesh", 3: "patch too big for LDS/16-bit ids",
-          4: "I/O error", 5: "device error", 6: "unsupported", 7: "out of memory"}
+          4: "I/O error", 5: "device error", 6: "unsupported", 7: "out of memory",
+          8: "beyond the fixed scratch of an allocation-free entry point"}
 ARR = {"ev": 0, "face_patch": 1, "vertex_patch": 2, "edge_patch": 3, "patches_val": 4,
        "patches_offset": 5, "ribbon_val": 6, "ribbon_offset": 7, "v_l2g": 8, "e_l2g": 9,
        "f_l2g": 10, "v_g2l": 11, "e_g2l": 12, "f_g2l": 13, "patch_counts": 14,
@@ -90,6 +91,7 @@ _SIGS = {
     "rxm_mcf_explicit": [_vp, _vp, _vp, _vp, _vp, _vp, _vp, _vp, _i, _vp],
     "rxm_symdir_rest": [_vp, _vp, _vp, _vp],
     "rxm_symdir_energy": [_vp, _vp, _vp, _vp, _vp, _vp],
+    "rxm_symdir_energy_noalloc": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_symdir_grad": [_vp, _vp, _vp, _vp, _vp, _vp],
     "rxm_to_global": [_vp, _i, _i, _vp, _vp, _vp],
 }
@@ -558,16 +560,19 @@ class Mesh:
         _check(lib().rxm_symdir_rest(self._h, pr, self._mr(Mr), _stream(stream)), "rxm_symdir_rest")
         return Mr
 
-    def symdir_energy(self, Mr, uv, energy=None, total=None, stream=None):
+    def symdir_energy(self, Mr, uv, energy=None, total=None, stream=None, noalloc=False):
+        """(total, per-face energies). noalloc=True calls the allocation-free
+        entry point (rxm_symdir_energy_noalloc): same kernels and results, its
+        partial sums in the library's fixed per-device scratch."""
         import torch
         if energy is None:
             energy = torch.empty(self.nF, dtype=torch.float32, device="cuda")
         if total is None:
             total = torch.empty((), dtype=torch.float32, device="cuda")
-        _check(lib().rxm_symdir_energy(self._h, self._mr(Mr), _dev(uv, "uv", (2, self.nV)),
-                                       _dev(energy, "energy", (self.nF,)),
-                                       _dev(total.view(-1), "total", None, "float32", 1), _stream(stream)),
-               "rxm_symdir_energy")
+        name = "rxm_symdir_energy_noalloc" if noalloc else "rxm_symdir_energy"
+        _check(getattr(lib(), name)(self._h, self._mr(Mr), _dev(uv, "uv", (2, self.nV)),
+                                    _dev(energy, "energy", (self.nF,)),
+                                    _dev(total.view(-1), "total", None, "float32", 1), _stream(stream)), name)
         return total, energy
 
     def symdir_grad(self, Mr, uv, scale=None, out=None, stream=None):

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -1474,6 +1475,55 @@ rxm_status rxm_symdir_energy(rxm_mesh* m, const float* Mr, const float* uv, floa
         m->device_bytes += sizeof(float) * m->h.P;
     }
     return symdir_energy_launch(m, Mr, uv, energy, total, m->d_partial, stream);
+}
+
+}  // extern "C"
+
+// Scratch of rxm_symdir_energy_noalloc: the per-patch partial sums, one float
+// per patch, every entry rewritten by each call (nothing needs clearing).
+// Each device holds its own copy for the life of the process.
+static constexpr uint32_t kNoAllocMaxPatches = 1u << 18;  // 1 MiB
+static __device__ __align__(256) float scratch[kNoAllocMaxPatches];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex    g_scratch_mu;
+float*        g_scratch[kMaxDevices] = {};
+
+// Device address of `scratch` on the current device, resolved once per
+// device (the array's name is only a host-side stand-in).
+int scratch_on_current_device(float** out)
+{
+    int dev = -1;
+    RXM_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return RXM_ERR_DEVICE;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void* p = nullptr;
+        RXM_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return RXM_ERR_DEVICE;
+        g_scratch[dev] = static_cast<float*>(p);
+    }
+    *out = g_scratch[dev];
+    return RXM_OK;
+}
+}  // namespace
+
+extern "C" {
+
+rxm_status rxm_symdir_energy_noalloc(rxm_mesh* m, const float* Mr, const float* uv, float* energy, float* total,
+                                     void* stream)
+{
+    if (!m || !Mr || !uv || !energy || !total) return RXM_ERR_INVALID_ARG;
+    if (!m->on_device) return RXM_ERR_UNSUPPORTED;
+    if (m->h.P > kNoAllocMaxPatches) return RXM_ERR_SCRATCH_TOO_SMALL;
+    int dev = -1;
+    RXM_HIP(hipGetDevice(&dev));
+    if (dev != m->device) return RXM_ERR_INVALID_ARG;  // the mesh lives on another device
+    float* part = nullptr;
+    const int rc = scratch_on_current_device(&part);
+    if (rc) return rc;
+    return symdir_energy_launch(m, Mr, uv, energy, total, part, stream);
 }
 
 rxm_status rxm_symdir_grad(rxm_mesh* m, const float* Mr, const float* uv, const float* scale, float* grad,
