@@ -42,6 +42,8 @@ def parse():
                          "vertex_normals, gaussian_curvature, cot_assembly, mcf_step, mcf_cg (20 CG iterations, dt 10), "
                          "query_evd, query_vv_vf, arap_weights, arap_rotations, arap_energy_grad, copy")
     ap.add_argument("--reps", type=int, default=20)
+    ap.add_argument("--query-sort", type=int, default=None,
+                    help="--profile: counting-sort form of the queries (0 LDS atomics, 1 atomic-free; default: library's)")
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
     ap.add_argument("--smooth-fused", type=int, default=None,
@@ -436,9 +438,12 @@ def main():
         smooth_cfg, smooth_sweep = {}, {}  # library defaults; --smooth-threads / --smooth-layout choose
     else:
         smooth_cfg, smooth_sweep = autotune.tune_smooth(m, x, out, use_cache=False, iters=it)
+    query_sort, query_sort_us = autotune.tune_query_sort(m) if not args.profile else (None, {})
     torch.cuda.synchronize()
 
     if args.profile:
+        if args.query_sort is not None:
+            m.set_launch_threads(10, args.query_sort)
         A_cot = m.cot_assemble(x, 1e-5) if args.profile in ("cot_assembly", "mcf_step") else None
         A_cg = m.cot_assemble(x, 10.0) if args.profile == "mcf_cg" else None
         if args.profile.startswith("arap") or args.profile == "query_evd":
@@ -570,6 +575,9 @@ def main():
         "queries": queries,
         "kernels": kernels,
         "smooth_autotune": {"chosen": smooth_cfg, "us": smooth_sweep},
+        # counting-sort form of the transposed queries: 0 LDS-atomic histogram + row sort,
+        # 1 atomic-free ballot/mbcnt sort (identical rows); us of one VV + VF launch
+        "query_sort_autotune": {"chosen": query_sort, "us": query_sort_us},
     }
     cpu = None
     if not args.no_cpu:  # before the other workloads free the mesh: the CPU port runs on the same mesh

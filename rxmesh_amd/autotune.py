@@ -2,8 +2,8 @@
 
 Times the candidate launch configurations of a mesh (workgroup sizes of the
 smoothing and symmetric-Dirichlet kernels, grid vs persistent work-queue
-launch of the gradient step) on the live GPU with HIP events and keeps the
-fastest. Results are cached in a JSON file keyed by the mesh's shape (patch
+launch of the gradient step, the counting-sort form of the queries) on the
+live GPU with HIP events and keeps the fastest. Results are cached in a JSON file keyed by the mesh's shape (patch
 count and largest patch), so a later run on a mesh of the same shape reuses
 them without timing. Patch size is a build parameter: tune_patch_size()
 rebuilds the mesh for each candidate and times the same workload.
@@ -151,6 +151,23 @@ def tune_smooth_symdir(mesh, Mr, x, tmp, lam, eta, steps=2, reps=3, use_cache=Tr
     if use_cache:
         _store(cache)
     return best, timings
+
+
+QUERY_SORTS = (0, 1)  # transposes: 0 LDS-atomic histogram + row sort, 1 atomic-free ballot/mbcnt sort
+
+
+def tune_query_sort(mesh, reps=20):
+    """Pick the counting-sort form of the transposed queries (both give the
+    same rows) from the time of one VV + VF launch; returns (form, {form:
+    us})."""
+    vv, vf = mesh.vv_vf_buffers()
+    times = {}
+    for form in QUERY_SORTS:
+        mesh.set_launch_threads(10, form)
+        times[form] = round(_time_ms(lambda: mesh.query_vv_vf(vv=vv, vf=vf), reps) * 1e3, 2)
+    best = min(times, key=times.get)
+    mesh.set_launch_threads(10, best)
+    return best, times
 
 
 PATCH_SIZES = (256, 512, 1024)

@@ -90,7 +90,7 @@ struct rxm_mesh
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
     uint32_t*       d_queue        = nullptr;  // work-queue ticket counter
     bool            system_runs_ok = false;    // check_system_runs passed
-    uint32_t        query_stable   = 1;        // 1: atomic-free stable counting sort in the queries
+    uint32_t        query_stable   = 0;        // 1: atomic-free stable counting sort in the queries, 0: LDS atomics + row sort
     int             num_cus        = 0;
     // batch (disjoint union of meshes): per-mesh id / patch offsets
     std::vector<uint32_t> b_v, b_e, b_f, b_p;
@@ -565,7 +565,9 @@ rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_
     *out = nullptr;
     rxm_mesh* m = new (std::nothrow) rxm_mesh();
     if (!m) return RXM_ERR_OUT_OF_MEMORY;
+    const auto t0 = std::chrono::steady_clock::now();
     int rc = build_supporting(m->h, fv, num_faces, num_vertices);
+    const auto t1 = std::chrono::steady_clock::now();
     if (rc == RXM_OK) {
         if (patch_file && patch_file[0]) {
             std::vector<uint32_t> fp, pv, po;
@@ -582,7 +584,14 @@ rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_
         delete m;
         return rc;
     }
-    return finish_create(m, flags, out);
+    const auto t2 = std::chrono::steady_clock::now();
+    rc = finish_create(m, flags, out);
+    if (std::getenv("RXM_TIMING")) {
+        using sec = std::chrono::duration<double>;
+        fprintf(stderr, "mesh_create: supporting %.3f s, patcher %.3f s, patches + upload %.3f s\n",
+                sec(t1 - t0).count(), sec(t2 - t1).count(), sec(std::chrono::steady_clock::now() - t2).count());
+    }
+    return rc;
 }
 
 rxm_status rxm_mesh_create_with_patches(const uint32_t* fv, uint32_t num_faces, uint32_t num_vertices,
@@ -912,7 +921,7 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     constexpr uint32_t T = kQueryThreads;
     Bump               bs(nullptr), ba(nullptr);  // stable / atomic plans, as in rxm_query_csr
     plan_vvf(bs, m->h.max_nv, m->h.max_ne, m->h.max_nf, T / 64u);
-    plan_vvf(ba, m->h.max_nv, m->h.max_ne, m->h.max_nf, 1u);
+    plan_vvf16(ba, m->h.max_nv, m->h.max_nvo, m->h.max_ne, m->h.max_nf);
     if (m->query_stable != 0 && bs.bytes() <= kMaxLdsBytes)
         return launch_patches_xcd(k_query2_vv_vf<T, true>, T, bs.bytes(), stream, m, m->dm, vv_base, vf_base,
                                   vv_row_ptr, vv_col, vf_row_ptr, vf_col);

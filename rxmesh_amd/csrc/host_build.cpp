@@ -300,9 +300,13 @@ class Bisector
    public:
     using Leaves = std::vector<std::vector<uint32_t>>;
 
-    Bisector(const FaceGraph& g, uint32_t nF)
-        : g_(g), tag_(new std::atomic<uint32_t>[nF]), dist_(nF, kInvalid32), side_(nF, 0)
+    // orig: the caller's id of each face of g (null: g is numbered as the
+    // caller's faces); every tie breaks on these ids, so a relabelled graph
+    // gives the patches of the original numbering
+    Bisector(const FaceGraph& g, uint32_t nF, const std::vector<uint32_t>* orig = nullptr)
+        : g_(g), orig_(orig), tag_(new std::atomic<uint32_t>[nF]), dist_(nF, kInvalid32), side_(nF, 0)
     {
+        for (auto& cs : cside_) cs.assign(nF, 0);
         for (uint32_t f = 0; f < nF; ++f) tag_[f].store(kInvalid32, std::memory_order_relaxed);
     }
 
@@ -360,17 +364,20 @@ class Bisector
         // one whose split cuts the fewest face adjacencies wins (keeps cuts
         // short on meshes with handles, e.g. around a torus tube)
         std::vector<uint32_t> q;
-        const uint32_t        a = farthest(faces, faces[0], tag, q);
+        uint32_t              start = faces[0];  // the set's smallest caller id
+        for (uint32_t f : faces)
+            if (oid(f) < oid(start)) start = f;
+        const uint32_t a = farthest(faces, start, tag, q);
         std::vector<uint32_t> d[3];
         const uint32_t        b = farthest(faces, a, tag, q);
         d[0]                    = dists(faces);
         bfs(faces, b, tag, q);
         d[1]           = dists(faces);
-        uint32_t c     = faces[0];
+        uint32_t c     = start;
         uint32_t cbest = 0;
         for (uint32_t i = 0; i < n; ++i) {
             const uint32_t m = std::min(d[0][i], d[1][i]);
-            if (m != kInvalid32 && m > cbest) {
+            if (m != kInvalid32 && (m > cbest || (m == cbest && m > 0 && oid(faces[i]) < oid(c)))) {
                 cbest = m;
                 c     = faces[i];
             }
@@ -381,38 +388,35 @@ class Bisector
         // are ordered by the distance to the third face z, so the median
         // splits a tie band into two connected runs, then by face id
         const int pairs[3][3] = {{0, 1, 2}, {0, 2, 1}, {1, 2, 0}};
-        std::vector<std::pair<int64_t, uint64_t>> key(n), best_key;
-        uint64_t best_cut = ~0ull;
-        for (const auto& pr : pairs) {
-            const std::vector<uint32_t>& du = d[pr[0]];
-            const std::vector<uint32_t>& dw = d[pr[1]];
-            const std::vector<uint32_t>& dz = d[pr[2]];
-            for (uint32_t i = 0; i < n; ++i) {
-                // faces not connected to u / w (another component) sort last
-                const int64_t x = (du[i] == kInvalid32) ? (int64_t(1) << 40) : int64_t(du[i]);
-                const int64_t y = (dw[i] == kInvalid32) ? 0 : int64_t(dw[i]);
-                key[i] = {x - y, (uint64_t(dz[i]) << 32) | faces[i]};
-            }
-            std::nth_element(key.begin(), key.begin() + n1, key.end());
-            for (uint32_t i = 0; i < n; ++i) side_[uint32_t(key[i].second)] = (i < n1) ? 1u : 2u;
+        uint64_t  cuts[3];
+        auto      evaluate = [&](int c) {
+            const auto& pr = pairs[c];
+            select_side(faces, n1, d[pr[0]], d[pr[1]], d[pr[2]], cside_[c]);
             uint64_t cut = 0;
-            for (uint32_t i = 0; i < n1; ++i) {
-                const uint32_t f = uint32_t(key[i].second);
+            for (uint32_t f : faces) {
+                if (cside_[c][f] != 1u) continue;
                 for (uint32_t j = g_.off[f]; j < g_.off[f + 1]; ++j) {
                     const uint32_t o = g_.val[j];
-                    cut += (in(o, tag) && side_[o] == 2u);
+                    cut += (in(o, tag) && cside_[c][o] == 2u);
                 }
             }
-            if (cut < best_cut) {
-                best_cut = cut;
-                best_key = key;
-            }
+            cuts[c] = cut;
+        };
+        if (n >= kParallelMin * 8) {  // large sets: the three candidates on their own threads
+            std::thread t1(evaluate, 1), t2(evaluate, 2);
+            evaluate(0);
+            t1.join();
+            t2.join();
+        } else {
+            for (int c = 0; c < 3; ++c) evaluate(c);
         }
+        int best = 0;
+        for (int c = 1; c < 3; ++c)
+            if (cuts[c] < cuts[best]) best = c;
         // sides of the best cut (1 = A, 2 = B); each half is rebuilt in
         // ascending face order by one pass over `faces` (sorted on entry)
         // instead of sorting it
-        for (uint32_t i = 0; i < n; ++i) side_[uint32_t(best_key[i].second)] = (i < n1) ? 1u : 2u;
-        std::vector<std::pair<int64_t, uint64_t>>().swap(best_key);
+        for (uint32_t f : faces) side_[f] = cside_[best][f];
         auto collect = [&] {
             A.clear();
             B.clear();
@@ -432,6 +436,59 @@ class Bisector
     }
 
     bool in(uint32_t f, uint32_t tag) const { return tag_[f].load(std::memory_order_relaxed) == tag; }
+    uint32_t oid(uint32_t f) const { return orig_ ? (*orig_)[f] : f; }
+
+    // Side 1 for the n1 faces first in the order (du - dw, dz, face id), side
+    // 2 for the rest, written to side[f]. The key is an integer of small
+    // range, so a histogram finds the key value the median falls on and only
+    // the faces tied on it are ordered (by dz, then face id): the same
+    // partition a full sort gives. Faces not reached from u (another
+    // component) sort last; faces not reached from w count distance 0.
+    void select_side(const std::vector<uint32_t>& faces, uint32_t n1, const std::vector<uint32_t>& du,
+                                const std::vector<uint32_t>& dw, const std::vector<uint32_t>& dz,
+                                std::vector<uint8_t>& side)
+    {
+        const uint32_t n = static_cast<uint32_t>(faces.size());
+        int64_t        lo = INT64_MAX, hi = INT64_MIN;
+        auto           key = [&](uint32_t i) {
+            const int64_t x = (du[i] == kInvalid32) ? (int64_t(1) << 40) : int64_t(du[i]);
+            const int64_t y = (dw[i] == kInvalid32) ? 0 : int64_t(dw[i]);
+            return x - y;
+        };
+        bool far_keys = false;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (du[i] == kInvalid32) {
+                far_keys = true;
+                continue;
+            }
+            const int64_t k = key(i);
+            lo              = std::min(lo, k);
+            hi              = std::max(hi, k);
+        }
+        if (far_keys || n == 0) {  // general path: one selection over all keys
+            std::vector<std::pair<std::pair<int64_t, uint64_t>, uint32_t>> kv(n);
+            for (uint32_t i = 0; i < n; ++i) kv[i] = {{key(i), (uint64_t(dz[i]) << 32) | oid(faces[i])}, faces[i]};
+            std::nth_element(kv.begin(), kv.begin() + n1, kv.end());
+            for (uint32_t i = 0; i < n; ++i) side[kv[i].second] = (i < n1) ? 1u : 2u;
+            return;
+        }
+        std::vector<uint32_t> hist(static_cast<size_t>(hi - lo + 1), 0);
+        for (uint32_t i = 0; i < n; ++i) hist[static_cast<size_t>(key(i) - lo)]++;
+        size_t   kb    = 0;
+        uint32_t below = 0;
+        while (below + hist[kb] <= n1 && kb + 1 < hist.size()) below += hist[kb++];
+        const int64_t         K = lo + static_cast<int64_t>(kb);
+        std::vector<std::pair<uint64_t, uint32_t>> tie;
+        for (uint32_t i = 0; i < n; ++i) {
+            const int64_t k = key(i);
+            if (k < K) side[faces[i]] = 1u;
+            else if (k > K) side[faces[i]] = 2u;
+            else tie.push_back({(uint64_t(dz[i]) << 32) | oid(faces[i]), faces[i]});
+        }
+        const uint32_t take = n1 - below;  // tied faces that go to side 1
+        if (take > 0 && take < tie.size()) std::nth_element(tie.begin(), tie.begin() + take, tie.end());
+        for (size_t t = 0; t < tie.size(); ++t) side[tie[t].second] = (t < take) ? 1u : 2u;
+    }
 
     uint32_t mark(const std::vector<uint32_t>& faces)
     {
@@ -462,9 +519,22 @@ class Bisector
             }
             out.push_back(std::move(c));
         }
-        std::stable_sort(out.begin(), out.end(),
-                         [](const std::vector<uint32_t>& x, const std::vector<uint32_t>& y) { return x.size() > y.size(); });
-        return out;
+        // largest first; equal sizes in the order of their smallest caller id
+        std::vector<uint32_t> first(out.size());
+        for (size_t k = 0; k < out.size(); ++k) {
+            uint32_t mn = kInvalid32;
+            for (uint32_t f : out[k]) mn = std::min(mn, oid(f));
+            first[k] = mn;
+        }
+        std::vector<size_t> ord(out.size());
+        std::iota(ord.begin(), ord.end(), size_t(0));
+        std::sort(ord.begin(), ord.end(), [&](size_t x, size_t y) {
+            return out[x].size() != out[y].size() ? out[x].size() > out[y].size() : first[x] < first[y];
+        });
+        Leaves sorted;
+        sorted.reserve(out.size());
+        for (size_t k : ord) sorted.push_back(std::move(out[k]));
+        return sorted;
     }
 
     // every component of the (sorted) set but the largest changes side
@@ -508,8 +578,10 @@ class Bisector
     }
 
     const FaceGraph&                         g_;
+    const std::vector<uint32_t>*             orig_;
     std::unique_ptr<std::atomic<uint32_t>[]> tag_;
     std::vector<uint32_t>                    dist_, side_;
+    std::vector<uint8_t>                     cside_[3];  // sides of the three candidate cuts
     std::atomic<uint32_t>                    next_tag_{0};
     std::atomic<uint32_t>                    workers_{0};
 };
@@ -590,9 +662,23 @@ int run_patcher(HostMesh& m, uint32_t patch_size, int mode)
         }
         nP = static_cast<uint32_t>(seeds.size());
     } else {
+        // bisection over the faces relabelled in BFS order: neighbouring
+        // faces get nearby ids, so its many BFS passes stay in cache however
+        // the input is numbered; the patches map back to the input ids
+        FaceGraph gr;
+        gr.off.assign(nF + 1, 0);
+        for (uint32_t i = 0; i < nF; ++i) gr.off[i + 1] = gr.off[i] + (g.off[order[i] + 1] - g.off[order[i]]);
+        gr.val.resize(g.val.size());
+        parallel_for(nF, [&](size_t i) {
+            const uint32_t f = order[i];
+            uint32_t       k = gr.off[i];
+            for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) gr.val[k++] = rank[g.val[j]];
+        }, 8192);
+        std::vector<uint32_t> fpr(nF, kInvalid32);
+        Bisector bs(gr, nF, &order);
+        nP = bs.run_all(nF, patch_size, fpr);
         fp.assign(nF, kInvalid32);
-        Bisector bs(g, nF);
-        nP = bs.run_all(nF, patch_size, fp);
+        for (uint32_t i = 0; i < nF; ++i) fp[order[i]] = fpr[i];
     }
     // recursive bisection of oversize patches
     std::vector<std::vector<uint32_t>> members(nP);

@@ -307,6 +307,50 @@ __host__ __device__ inline PlanVVF plan_vvf(Bump& b, uint32_t nv, uint32_t ne, u
     return p;
 }
 
+// LDS carve of the fused launch's LDS-atomic form: u16 slots (an entry index
+// for VV, a local face for VF: both below 2^16), the histogram over owned
+// vertices only. Sorted rows are turned into linear ids as they are copied
+// out, so no 32-bit staging is needed and more workgroups fit on a CU.
+struct PlanVVF16
+{
+    uint32_t *ev, *off, *wave, *lin_v, *lin_f;
+    uint16_t *fe, *slots;
+};
+__host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf)
+{
+    PlanVVF16 p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.off   = b.alloc<uint32_t>(nvo + 1);
+    p.slots = b.alloc<uint16_t>(r4(2 * ne > 3 * nf ? 2 * ne : 3 * nf));
+    p.wave  = b.alloc<uint32_t>(20);
+    p.lin_v = b.alloc<uint32_t>(nv);
+    p.lin_f = b.alloc<uint32_t>(nf);
+    return p;
+}
+
+// Staging of the fused launch: EV, FE, vertex and face linear ids, the face
+// ids loaded ahead of the staging's LDS stores.
+template <uint32_t T>
+__device__ __forceinline__ void stage_vv_vf(const DevMesh& m, const PatchDesc& d, uint32_t* ev, uint16_t* fe,
+                                            uint32_t* lin_v, uint32_t* lin_f)
+{
+    constexpr uint32_t RL = 3;
+    const uint32_t*    fs = m.f_lin + d.f_off;
+    uint32_t           fl[RL];
+#pragma unroll
+    for (uint32_t r = 0; r < RL; ++r) {
+        const uint32_t l = threadIdx.x + r * T;
+        fl[r]            = (l < d.nfo) ? d.fp + l : ((l < d.nf) ? fs[l] : 0u);
+    }
+    stage_topology<T>(ev, m.ev + d.e_off, d.ne, fe, m.fe + 3u * d.f_off, d.nf, lin_v, d.nv, d.nvo, d.vp,
+                      m.v_lin + d.v_off);
+#pragma unroll
+    for (uint32_t r = 0; r < RL; ++r)
+        if (threadIdx.x + r * T < d.nf) lin_f[threadIdx.x + r * T] = fl[r];
+    for (uint32_t l = threadIdx.x + RL * T; l < d.nf; l += T) lin_f[l] = (l < d.nfo) ? d.fp + l : fs[l];
+}
+
 template <uint32_t T, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                     const uint32_t* __restrict__ vf_base,
@@ -318,25 +362,9 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanVVF   L = plan_vvf(b, d.nv, d.ne, d.nf, STABLE ? T / 64u : 1u);
-    {
-        // face linear ids are loaded ahead of the staging's LDS stores
-        constexpr uint32_t RL = 3;
-        const uint32_t*    fs = m.f_lin + d.f_off;
-        uint32_t           fl[RL];
-#pragma unroll
-        for (uint32_t r = 0; r < RL; ++r) {
-            const uint32_t l = threadIdx.x + r * T;
-            fl[r]            = (l < d.nfo) ? d.fp + l : ((l < d.nf) ? fs[l] : 0u);
-        }
-        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin_v, d.nv, d.nvo,
-                          d.vp, m.v_lin + d.v_off);
-#pragma unroll
-        for (uint32_t r = 0; r < RL; ++r)
-            if (threadIdx.x + r * T < d.nf) L.lin_f[threadIdx.x + r * T] = fl[r];
-        for (uint32_t l = threadIdx.x + RL * T; l < d.nf; l += T) L.lin_f[l] = (l < d.nfo) ? d.fp + l : fs[l];
-    }
     if constexpr (STABLE) {
+        const PlanVVF L = plan_vvf(b, d.nv, d.ne, d.nf, T / 64u);
+        stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
         {
             const uint32_t bse = vv_base[p];
             const uint32_t tot = sort_columns_stable<T>(
@@ -360,34 +388,32 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         }
         return;
     }
+    const PlanVVF16 L = plan_vvf16(b, d.nv, d.nvo, d.ne, d.nf);
+    stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
-    // VV: entry i = (edge i >> 1, endpoint i & 1); slot = edge << 16 | other endpoint
+    // VV: slot = entry index i (edge i >> 1, endpoint i & 1), so a row sorted
+    // by slot lists its edges in ascending local id
     fill_columns<T, true>(
-        2 * d.ne, d.nvo,
-        [&](uint32_t i) {
-            const uint32_t x = L.ev[i >> 1];
-            return (i & 1u) ? hi16(x) : lo16(x);
-        },
-        [&](uint32_t i) {
-            const uint32_t x = L.ev[i >> 1];
-            return ((i >> 1) << 16) | ((i & 1u) ? lo16(x) : hi16(x));
-        },
-        L.off, L.slots, L.wave);
+        2 * d.ne, d.nvo, [&](uint32_t i) { return ev_end(L.ev, i); }, [&](uint32_t i) { return i; }, L.off,
+        L.slots, L.wave);
     {
         const uint32_t bse = vv_base[p];
         for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
             const uint32_t o = L.off[v], k = L.off[v + 1] - o;
             vv_row[d.vp + v] = static_cast<int32_t>(bse + o);
-            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = L.lin_v[lo16(s)]; });
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); });
         }
         if (threadIdx.x == 0) vv_row[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
         __syncthreads();
-        copy_out<T>(vv_col + bse, L.slots, L.off[d.nvo]);
+        int32_t* dst = vv_col + bse;
+        for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T)
+            dst[k] = static_cast<int32_t>(L.lin_v[ev_other(L.ev, L.slots[k])]);
     }
     __syncthreads();  // the VV slots and offsets are read out before VF reuses them
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
+    // VF: slot = local face
     fill_columns<T, true>(
         3 * d.nf, d.nvo, [&](uint32_t i) { return face_vertex(L.fe, L.ev, i / 3u, i % 3u); },
         [&](uint32_t i) { return i / 3u; }, L.off, L.slots, L.wave);
@@ -396,11 +422,12 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
             const uint32_t o = L.off[v], k = L.off[v + 1] - o;
             vf_row[d.vp + v] = static_cast<int32_t>(bse + o);
-            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = L.lin_f[s]; });
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); });
         }
         if (threadIdx.x == 0) vf_row[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
         __syncthreads();
-        copy_out<T>(vf_col + bse, L.slots, L.off[d.nvo]);
+        int32_t* dst = vf_col + bse;
+        for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) dst[k] = static_cast<int32_t>(L.lin_f[L.slots[k]]);
     }
 }
 

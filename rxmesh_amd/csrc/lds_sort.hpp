@@ -61,8 +61,8 @@ __device__ __forceinline__ void cswap(uint32_t& a, uint32_t& b)
 // sorted in registers with Batcher's odd-even merge network (19
 // comparators); longer rows fall back to an insertion sort in LDS. The
 // callback out(j, slot) receives the sorted entries.
-template <typename Out>
-__device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
+template <typename SlotT, typename Out>
+__device__ __forceinline__ void sort_row(SlotT* slots, uint32_t k, Out out)
 {
     if (k <= 8u) {
         uint32_t a[8];
@@ -79,7 +79,7 @@ __device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
             if (j < k) out(j, a[j]);
     } else {
         for (uint32_t i = 1; i < k; ++i) {
-            const uint32_t x = slots[i];
+            const SlotT x = slots[i];
             uint32_t       j = i;
             while (j > 0 && slots[j - 1] > x) {
                 slots[j] = slots[j - 1];
@@ -103,9 +103,10 @@ __device__ __forceinline__ void sort_row(uint32_t* slots, uint32_t k, Out out)
 // computed once and held in registers between the two passes.
 // ZEROED: the caller already cleared off[0..ncols] before a barrier (saves
 // this function's clearing pass and its barrier).
-template <uint32_t T, bool ZEROED = false, uint32_t R = 12, typename KeyF, typename PackF>
+// SlotT is uint32_t, or uint16_t where the slot value fits 16 bits (less LDS).
+template <uint32_t T, bool ZEROED = false, uint32_t R = 12, typename KeyF, typename PackF, typename SlotT>
 __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF key, PackF pack, uint32_t* off,
-                                             uint32_t* slots, uint32_t* s_wave)
+                                             SlotT* slots, uint32_t* s_wave)
 {
     if constexpr (!ZEROED) {
         for (uint32_t c = threadIdx.x; c <= ncols; c += T) off[c] = 0;
@@ -125,7 +126,7 @@ __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF ke
         scan_excl<T>(off + 1, ncols, s_wave);
 #pragma unroll
         for (uint32_t r = 0; r < R; ++r)
-            if (kr[r] < ncols) slots[atomicAdd(&off[kr[r] + 1u], 1u)] = pr[r];
+            if (kr[r] < ncols) slots[atomicAdd(&off[kr[r] + 1u], 1u)] = static_cast<SlotT>(pr[r]);
     } else {
         for (uint32_t i = threadIdx.x; i < n; i += T) {
             const uint32_t k = key(i);
@@ -134,7 +135,7 @@ __device__ __forceinline__ void fill_columns(uint32_t n, uint32_t ncols, KeyF ke
         scan_excl<T>(off + 1, ncols, s_wave);
         for (uint32_t i = threadIdx.x; i < n; i += T) {
             const uint32_t k = key(i);
-            if (k < ncols) slots[atomicAdd(&off[k + 1u], 1u)] = pack(i);
+            if (k < ncols) slots[atomicAdd(&off[k + 1u], 1u)] = static_cast<SlotT>(pack(i));
         }
     }
     __syncthreads();
