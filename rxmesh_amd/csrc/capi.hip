@@ -548,6 +548,18 @@ int smooth_once(rxm_mesh* m, float lambda, const float* src, float* dst, void* s
                           : smooth_once_t<false>(m, lambda, src, dst, stream);
 }
 
+// The connectivity step of the build: on the device (edges, EV and the VF /
+// EF CSRs by stable radix sorts) unless the mesh is host-only; both give the
+// same arrays.
+int supporting(HostMesh& h, const uint32_t* fv, uint32_t nF, uint32_t nV_hint, uint32_t flags)
+{
+    if (flags & RXM_HOST_ONLY) return build_supporting(h, fv, nF, nV_hint);
+    int rc = supporting_prepare(h, fv, nF, nV_hint);
+    if (rc != RXM_OK) return rc;
+    rc = build_supporting_device(h.fv.data(), nF, h.nV, h.ev, h.fe_g, h.vf_off, h.vf_val, h.ef_off, h.ef_val, h.nE);
+    return rc != RXM_OK ? rc : supporting_finish(h);
+}
+
 bool valid_op(int op)
 {
     return op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF || op == RXM_OP_FV ||
@@ -566,7 +578,7 @@ rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_
     rxm_mesh* m = new (std::nothrow) rxm_mesh();
     if (!m) return RXM_ERR_OUT_OF_MEMORY;
     const auto t0 = std::chrono::steady_clock::now();
-    int rc = build_supporting(m->h, fv, num_faces, num_vertices);
+    int rc = supporting(m->h, fv, num_faces, num_vertices, flags);
     const auto t1 = std::chrono::steady_clock::now();
     if (rc == RXM_OK) {
         if (patch_file && patch_file[0]) {
@@ -602,7 +614,7 @@ rxm_status rxm_mesh_create_with_patches(const uint32_t* fv, uint32_t num_faces, 
     *out = nullptr;
     rxm_mesh* m = new (std::nothrow) rxm_mesh();
     if (!m) return RXM_ERR_OUT_OF_MEMORY;
-    int rc = build_supporting(m->h, fv, num_faces, num_vertices);
+    int rc = supporting(m->h, fv, num_faces, num_vertices, flags);
     if (rc == RXM_OK) rc = set_patches(m->h, face_patch, num_patches, nullptr, nullptr);
     if (rc != RXM_OK) {
         delete m;
@@ -624,7 +636,7 @@ rxm_status rxm_mesh_create_batch(const uint32_t* const* fvs, const uint32_t* num
     for (uint32_t k = 0; k < count && rc == RXM_OK; ++k) {
         parts[k].reset(new (std::nothrow) HostMesh());
         if (!parts[k]) return RXM_ERR_OUT_OF_MEMORY;
-        rc = build_supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0);
+        rc = supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0, flags);
         if (rc == RXM_OK)
             rc = run_patcher(*parts[k], patch_size, (flags & RXM_PATCHER_KMEANS) ? kPatcherKMeans : kPatcherBisect);
         if (rc == RXM_OK) rc = build_patches(*parts[k]);
@@ -882,7 +894,7 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
         Bump bs(nullptr), ba(nullptr);
         plan_q2(bs, kind, h.max_nv, h.max_ne, h.max_nf, T / 64u);
         plan_q2(ba, kind, h.max_nv, h.max_ne, h.max_nf, 1u);
-        if (m->query_stable != 0 && bs.bytes() <= kMaxLdsBytes)
+        if (m->query_stable == 1 && bs.bytes() <= kMaxLdsBytes)
             return launch_patches_xcd(k_stable, T, bs.bytes(), stream, m, m->dm, base, row_ptr, col);
         return launch_patches_xcd(k_atomic, T, ba.bytes(), stream, m, m->dm, base, row_ptr, col);
     };
@@ -922,9 +934,14 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     Bump               bs(nullptr), ba(nullptr);  // stable / atomic plans, as in rxm_query_csr
     plan_vvf(bs, m->h.max_nv, m->h.max_ne, m->h.max_nf, T / 64u);
     plan_vvf16(ba, m->h.max_nv, m->h.max_nvo, m->h.max_ne, m->h.max_nf);
-    if (m->query_stable != 0 && bs.bytes() <= kMaxLdsBytes)
+    if (m->query_stable == 1 && bs.bytes() <= kMaxLdsBytes)
         return launch_patches_xcd(k_query2_vv_vf<T, true>, T, bs.bytes(), stream, m, m->dm, vv_base, vf_base,
                                   vv_row_ptr, vv_col, vf_row_ptr, vf_col);
+    Bump bc(nullptr);  // one counting sort for both relations
+    plan_vvfc(bc, m->h.max_nv, m->h.max_nvo, m->h.max_ne, m->h.max_nf);
+    if (m->query_stable == 2 && bc.bytes() <= kMaxLdsBytes)
+        return launch_patches_xcd(k_query2_vv_vf_c<T>, T, bc.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
+                                  vv_col, vf_row_ptr, vf_col);
     return launch_patches_xcd(k_query2_vv_vf<T, false>, T, ba.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                               vv_col, vf_row_ptr, vf_col);
 }
@@ -1137,7 +1154,9 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
         m->ring_threads = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
-    if (kernel == 10 && (threads == 0 || threads == 1)) {  // query transposes: 1 stable counting sort, 0 atomic
+    // query transposes: 0 LDS-atomic counting sort, 1 atomic-free stable sort,
+    // 2 as 0 with the fused VV + VF launch sorting both in one pass
+    if (kernel == 10 && threads >= 0 && threads <= 2) {
         m->query_stable = static_cast<uint32_t>(threads);
         return RXM_OK;
     }

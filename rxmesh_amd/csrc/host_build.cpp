@@ -67,9 +67,10 @@ void exclusive_scan(const std::vector<uint32_t>& cnt, std::vector<uint32_t>& off
 // order of first appearance over the half-edges (f, i) -> (fv[i], fv[i+1]);
 // instead of a hash map, half-edges are bucketed by their smaller endpoint
 // and each bucket's groups are resolved to their first half-edge.
-int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint)
+int supporting_prepare(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint)
 {
     if (fv == nullptr || nF == 0) return kErrInvalidArg;
+    if (3ull * nF >= kInvalid32) return kErrInvalidArg;
     m.nF = nF;
     m.fv.assign(fv, fv + 3ull * nF);
     uint32_t vmax = 0;
@@ -80,8 +81,55 @@ int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_h
         const uint32_t* t = &m.fv[3ull * f];
         if (t[0] == t[1] || t[1] == t[2] || t[0] == t[2]) return kErrInvalidArg;
     }
+    return kOk;
+}
+
+int supporting_finish(HostMesh& m)
+{
+    const uint64_t nH = 3ull * m.nF;
+    const uint32_t nE = m.nE;
+    if (m.ev.size() != 2ull * nE || m.fe_g.size() != nH || m.ef_off.size() != nE + 1ull ||
+        m.vf_off.size() != m.nV + 1ull || m.ef_val.size() != nH || m.vf_val.size() != nH)
+        return kErrInvalidArg;
+    // consistent orientation: of the half-edges of an edge, only its first
+    // (whose direction EV keeps) starts at ev[2e]
+    std::vector<uint8_t> fwd(nE, 0);
+    m.oriented = true;
+    for (uint64_t h = 0; h < nH; ++h) {
+        const uint32_t e = m.fe_g[h];
+        if (m.fv[h] == m.ev[2ull * e] && fwd[e]++ > 0) m.oriented = false;
+    }
+    // statistics (rxmesh.cpp:589-660)
+    m.closed = true;
+    m.manifold = true;
+    m.max_edge_faces = 0;
+    std::vector<uint32_t> val(m.nV, 0);
+    for (uint32_t e = 0; e < nE; ++e) {
+        const uint32_t c = m.ef_off[e + 1] - m.ef_off[e];
+        if (c == 1) m.closed = false;
+        if (c > 2) m.manifold = false;
+        m.max_edge_faces = std::max(m.max_edge_faces, c);
+        val[m.ev[2ull * e]]++;
+        val[m.ev[2ull * e + 1]]++;
+    }
+    m.max_valence = *std::max_element(val.begin(), val.end());
+    m.max_face_adj = 0;
+    for (uint32_t f = 0; f < m.nF; ++f) {
+        uint32_t s = 0;
+        for (int i = 0; i < 3; ++i) {
+            const uint32_t e = m.fe_g[3ull * f + i];
+            s += m.ef_off[e + 1] - m.ef_off[e] - 1;
+        }
+        m.max_face_adj = std::max(m.max_face_adj, s);
+    }
+    return kOk;
+}
+
+int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint)
+{
+    int rc = supporting_prepare(m, fv, nF, nV_hint);
+    if (rc != kOk) return rc;
     const uint64_t nH = 3ull * nF;
-    if (nH >= kInvalid32) return kErrInvalidArg;
 
     // vertex -> faces, face order
     std::vector<uint32_t> cnt(m.nV, 0);
@@ -146,32 +194,7 @@ int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_h
         std::vector<uint32_t> cur(m.ef_off.begin(), m.ef_off.end() - 1);
         for (uint64_t h = 0; h < nH; ++h) m.ef_val[cur[m.fe_g[h]]++] = static_cast<uint32_t>(h / 3);
     }
-    // consistent orientation: the half-edges sharing an edge with its first
-    // half-edge run the other way
-    m.oriented = true;
-    for (uint64_t h = 0; h < nH; ++h)
-        if (leader[h] != h && he_a(h) == m.ev[2ull * m.fe_g[h]]) m.oriented = false;
-    // statistics (rxmesh.cpp:589-660)
-    m.closed = true;
-    m.manifold = true;
-    m.max_edge_faces = 0;
-    std::vector<uint32_t> val(m.nV, 0);
-    for (uint32_t e = 0; e < nE; ++e) {
-        uint32_t c = ecnt[e];
-        if (c == 1) m.closed = false;
-        if (c > 2) m.manifold = false;
-        m.max_edge_faces = std::max(m.max_edge_faces, c);
-        val[m.ev[2ull * e]]++;
-        val[m.ev[2ull * e + 1]]++;
-    }
-    m.max_valence = *std::max_element(val.begin(), val.end());
-    m.max_face_adj = 0;
-    for (uint32_t f = 0; f < nF; ++f) {
-        uint32_t s = 0;
-        for (int i = 0; i < 3; ++i) s += ecnt[m.fe_g[3ull * f + i]] - 1;
-        m.max_face_adj = std::max(m.max_face_adj, s);
-    }
-    return kOk;
+    return supporting_finish(m);
 }
 
 // ---------------------------------------------------------------------------

@@ -72,6 +72,17 @@ enum Status : int
 
 // Step a1: edges, incidence, statistics. nV_hint = 0 infers nV = max id + 1.
 int build_supporting(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint);
+// The two ends of build_supporting around the edge / VF / EF construction, for
+// a caller that builds those elsewhere (on the device): prepare validates fv
+// and sets fv, nF, nV; finish checks the arrays' sizes and derives the
+// orientation flag and the statistics from ev, fe_g, ef_off and vf_off.
+int supporting_prepare(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV_hint);
+int supporting_finish(HostMesh& m);
+// The edge ids (first appearance), EV, fe_g and the VF / EF CSRs built on the
+// current device by stable radix sorts (gpu_build.hip); fv is host memory.
+int build_supporting_device(const uint32_t* fv, uint32_t nF, uint32_t nV, std::vector<uint32_t>& ev,
+                            std::vector<uint32_t>& fe, std::vector<uint32_t>& vf_off, std::vector<uint32_t>& vf_val,
+                            std::vector<uint32_t>& ef_off, std::vector<uint32_t>& ef_val, uint32_t& nE);
 
 // Step a2: deterministic patcher (fills face_patch / patches_val / patches_offset).
 //   kPatcherBisect: recursive proportional bisection into about

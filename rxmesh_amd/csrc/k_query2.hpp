@@ -431,6 +431,75 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
     }
 }
 
+// The fused VV + VF launch with one counting sort over the combined column
+// space: columns [0, nvo) are the VV rows, [nvo, 2 nvo) the VF rows; entries
+// [0, 2 ne) are the edge endpoints (slot = entry index), [2 ne, 2 ne + 3 nf)
+// the face corners (slot = local face). One histogram, one scan and one fill
+// serve both relations, and their rows are sorted and copied out under the
+// same barriers; the slot array holds both (more LDS than k_query2_vv_vf's
+// atomic form). Same CSRs as k_query2_vv_vf.
+struct PlanVVFC
+{
+    uint32_t *ev, *off, *wave, *lin_v, *lin_f;
+    uint16_t *fe, *slots;
+};
+__host__ __device__ inline PlanVVFC plan_vvfc(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf)
+{
+    PlanVVFC p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
+    p.off   = b.alloc<uint32_t>(2 * nvo + 1);
+    p.slots = b.alloc<uint16_t>(r4(2 * ne + 3 * nf));
+    p.wave  = b.alloc<uint32_t>(20);
+    p.lin_v = b.alloc<uint32_t>(nv);
+    p.lin_f = b.alloc<uint32_t>(nf);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query2_vv_vf_c(DevMesh m, const uint32_t* __restrict__ vv_base,
+                                                      const uint32_t* __restrict__ vf_base,
+                                                      int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
+                                                      int32_t* __restrict__ vf_row, int32_t* __restrict__ vf_col)
+{
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanVVFC  L = plan_vvfc(b, d.nv, d.nvo, d.ne, d.nf);
+    stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
+    const uint32_t nvo = d.nvo, nev = 2 * d.ne;
+    for (uint32_t c = threadIdx.x; c <= 2 * nvo; c += T) L.off[c] = 0;
+    __syncthreads();
+    fill_columns<T, true, 16>(
+        nev + 3 * d.nf, 2 * nvo,
+        [&](uint32_t i) {
+            if (i < nev) return ev_end(L.ev, i);
+            const uint32_t j = i - nev, v = face_vertex(L.fe, L.ev, j / 3u, j % 3u);
+            return (v < nvo) ? nvo + v : 2 * nvo;
+        },
+        [&](uint32_t i) { return (i < nev) ? i : (i - nev) / 3u; }, L.off, L.slots, L.wave);
+    const uint32_t bvv = vv_base[p], bvf = vf_base[p], fo = L.off[nvo];
+    for (uint32_t r = threadIdx.x; r < 2 * nvo; r += T) {
+        const uint32_t o = L.off[r], k = L.off[r + 1] - o;
+        if (r < nvo) vv_row[d.vp + r] = static_cast<int32_t>(bvv + o);
+        else vf_row[d.vp + r - nvo] = static_cast<int32_t>(bvf + o - fo);
+        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); });
+    }
+    if (threadIdx.x == 0) {
+        vv_row[d.vp + nvo] = static_cast<int32_t>(bvv + fo);
+        vf_row[d.vp + nvo] = static_cast<int32_t>(bvf + L.off[2 * nvo] - fo);
+    }
+    __syncthreads();
+    const uint32_t tot = L.off[2 * nvo];
+    for (uint32_t k = threadIdx.x; k < tot; k += T) {
+        const uint32_t sl = L.slots[k];
+        if (k < fo) vv_col[bvv + k] = static_cast<int32_t>(L.lin_v[ev_other(L.ev, sl)]);
+        else vf_col[bvf + k - fo] = static_cast<int32_t>(L.lin_f[sl]);
+    }
+}
+
 // EF (IS_FF == false) or FF. STABLE as in k_query2_vx.
 template <uint32_t T, bool IS_FF, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __restrict__ base,

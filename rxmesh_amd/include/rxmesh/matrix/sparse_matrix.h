@@ -16,6 +16,7 @@
 #include <stdexcept>
 #include <vector>
 
+#include "rxmesh/matrix/dense_matrix.h"
 #include "rxmesh/rxmesh_static.h"
 
 namespace rxmesh {

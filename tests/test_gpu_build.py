@@ -67,3 +67,26 @@ def test_device_edges_reject_bad_faces():
     with pytest.raises(rx.RXMError) as e:
         rx.build_edges_device(torch.from_numpy(F).cuda(), 3)  # degenerate face
     assert e.value.code == 1
+
+
+@pytest.mark.parametrize("name", ["plane.obj", "cube.obj", "sphere3.obj", "bunnyhead.obj", "ico_perm", "torus",
+                                  "disk", "c1", "flipped", "fin", "ico_odd"])
+def test_mesh_create_device_connectivity_equals_host_build(name):
+    """rxm_mesh_create builds the edges, EV and the VF / EF CSRs on the
+    device; every array the build exports, the mesh statistics and the
+    patch layout equal the host-only build's."""
+    import rxmesh_amd as rx
+    F = np.ascontiguousarray(_faces(name), dtype=np.uint32)
+    print("case", name, len(F), flush=True)
+    d = rx.Mesh(F, patch_size=64)
+    h = rx.Mesh(F, patch_size=64, host_only=True)
+    for key in ("num_vertices", "num_edges", "num_faces", "num_patches", "is_closed", "is_edge_manifold",
+                "is_consistently_oriented", "max_valence", "max_edge_incident_faces", "max_face_adjacent_faces"):
+        if key in h.info:
+            assert d.info[key] == h.info[key], key
+    for arr in rx.ARR:
+        if arr.startswith("vv2"):
+            continue
+        assert np.array_equal(d.array(arr), h.array(arr)), arr
+    d.close()
+    h.close()
