@@ -153,11 +153,13 @@ def batch_bench(count, steps, reps, patch_size=None):
                                                               lam, eta)
         log("autotuned patch size %d: %s" % (patch_size, ps_times))
     t_setup = time.perf_counter()
-    parts = []
-    for k, (_, f) in enumerate(geo):
-        parts.append(rx.Mesh(f, patch_size=patch_size))
-        if (k + 1) % 64 == 0:
-            log("built %d meshes" % (k + 1))
+    # the plain meshes are built concurrently (the C-ABI releases the GIL), largest first
+    from concurrent.futures import ThreadPoolExecutor
+    order = sorted(range(count), key=lambda k: -len(geo[k][1]))
+    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
+        built = dict(zip(order, pool.map(lambda k: rx.Mesh(geo[k][1], patch_size=patch_size), order)))
+    parts = [built[k] for k in range(count)]
+    log("built %d meshes" % count)
     union = rx.Mesh(None, parts=parts)
     log("union built: %d patches" % union.P)
     xs = [torch.from_numpy(np.ascontiguousarray(p[m.array("v_l2g")].T.astype(np.float32))).cuda()

@@ -10,6 +10,8 @@
 #include <mutex>
 #include <new>
 #include <string>
+#include <thread>
+#include <atomic>
 #include <vector>
 
 #include "../../include/rxmesh_amd.h"
@@ -631,16 +633,39 @@ rxm_status rxm_mesh_create_batch(const uint32_t* const* fvs, const uint32_t* num
     if (!out || !fvs || !num_faces || count == 0) return RXM_ERR_INVALID_ARG;
     *out = nullptr;
     std::vector<std::unique_ptr<HostMesh>> parts(count);
-    int rc = RXM_OK;
     auto t0 = std::chrono::steady_clock::now();
-    for (uint32_t k = 0; k < count && rc == RXM_OK; ++k) {
-        parts[k].reset(new (std::nothrow) HostMesh());
-        if (!parts[k]) return RXM_ERR_OUT_OF_MEMORY;
-        rc = supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0, flags);
-        if (rc == RXM_OK)
-            rc = run_patcher(*parts[k], patch_size, (flags & RXM_PATCHER_KMEANS) ? kPatcherKMeans : kPatcherBisect);
-        if (rc == RXM_OK) rc = build_patches(*parts[k]);
+    // the meshes are built concurrently, largest first, each worker taking
+    // the next one; the first error in mesh order is returned
+    std::vector<uint32_t> order(count);
+    for (uint32_t k = 0; k < count; ++k) order[k] = k;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return num_faces[a] > num_faces[b]; });
+    std::vector<int>      status(count, RXM_OK);
+    std::atomic<uint32_t> next{0};
+    auto work = [&] {
+        for (uint32_t i; (i = next.fetch_add(1)) < count;) {
+            const uint32_t k = order[i];
+            parts[k].reset(new (std::nothrow) HostMesh());
+            if (!parts[k]) {
+                status[k] = RXM_ERR_OUT_OF_MEMORY;
+                continue;
+            }
+            int rc = supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0, flags);
+            if (rc == RXM_OK)
+                rc = run_patcher(*parts[k], patch_size,
+                                 (flags & RXM_PATCHER_KMEANS) ? kPatcherKMeans : kPatcherBisect);
+            if (rc == RXM_OK) rc = build_patches(*parts[k]);
+            status[k] = rc;
+        }
+    };
+    {
+        const unsigned workers = std::max(1u, std::min<unsigned>(num_threads(), count));
+        std::vector<std::thread> pool;
+        for (unsigned w = 1; w < workers; ++w) pool.emplace_back(work);
+        work();
+        for (auto& th : pool) th.join();
     }
+    int rc = RXM_OK;
+    for (uint32_t k = 0; k < count && rc == RXM_OK; ++k) rc = status[k];
     if (rc != RXM_OK) return rc;
     rxm_mesh* m = new (std::nothrow) rxm_mesh();
     if (!m) return RXM_ERR_OUT_OF_MEMORY;
@@ -933,7 +958,8 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     constexpr uint32_t T = kQueryThreads;
     Bump               bs(nullptr), ba(nullptr);  // stable / atomic plans, as in rxm_query_csr
     plan_vvf(bs, m->h.max_nv, m->h.max_ne, m->h.max_nf, T / 64u);
-    plan_vvf16(ba, m->h.max_nv, m->h.max_nvo, m->h.max_ne, m->h.max_nf);
+    const bool lazy_f = m->query_stable == 3;
+    plan_vvf16(ba, m->h.max_nv, m->h.max_nvo, m->h.max_ne, lazy_f ? 0u : m->h.max_nf, m->h.max_nf);
     if (m->query_stable == 1 && bs.bytes() <= kMaxLdsBytes)
         return launch_patches_xcd(k_query2_vv_vf<T, true>, T, bs.bytes(), stream, m, m->dm, vv_base, vf_base,
                                   vv_row_ptr, vv_col, vf_row_ptr, vf_col);
@@ -942,6 +968,9 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     if (m->query_stable == 2 && bc.bytes() <= kMaxLdsBytes)
         return launch_patches_xcd(k_query2_vv_vf_c<T>, T, bc.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                                   vv_col, vf_row_ptr, vf_col);
+    if (lazy_f)
+        return launch_patches_xcd(k_query2_vv_vf<T, false, true>, T, ba.bytes(), stream, m, m->dm, vv_base, vf_base,
+                                  vv_row_ptr, vv_col, vf_row_ptr, vf_col);
     return launch_patches_xcd(k_query2_vv_vf<T, false>, T, ba.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                               vv_col, vf_row_ptr, vf_col);
 }
@@ -1155,8 +1184,9 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
         return RXM_OK;
     }
     // query transposes: 0 LDS-atomic counting sort, 1 atomic-free stable sort,
-    // 2 as 0 with the fused VV + VF launch sorting both in one pass
-    if (kernel == 10 && threads >= 0 && threads <= 2) {
+    // 2 as 0 with the fused VV + VF launch sorting both in one pass, 3 as 0
+    // with the fused launch reading ribbon face ids at copy-out
+    if (kernel == 10 && threads >= 0 && threads <= 3) {
         m->query_stable = static_cast<uint32_t>(threads);
         return RXM_OK;
     }

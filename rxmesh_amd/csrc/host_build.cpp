@@ -343,7 +343,11 @@ class Bisector
         for (auto& c : components(all)) {
             std::sort(c.begin(), c.end());
             const uint32_t k = static_cast<uint32_t>((c.size() + cap - 1) / cap);
-            for (const auto& leaf : run(std::move(c), k, cap)) {
+            // on a graph relabelled in BFS order a component is a run of ids
+            // that the relabelling BFS swept from its smallest caller id, so
+            // the root split's first BFS would end at the run's last id
+            const uint32_t far = orig_ ? c.back() : kInvalid32;
+            for (const auto& leaf : run(std::move(c), k, cap, far)) {
                 for (uint32_t f : leaf) fp[f] = next;
                 ++next;
             }
@@ -354,13 +358,15 @@ class Bisector
    private:
     static constexpr size_t kParallelMin = size_t(1) << 15;
 
-    Leaves run(std::vector<uint32_t> faces, uint32_t k, uint32_t cap)
+    // far: the face a BFS from the set's smallest caller id reaches last, when
+    // already known (kInvalid32 otherwise)
+    Leaves run(std::vector<uint32_t> faces, uint32_t k, uint32_t cap, uint32_t far = kInvalid32)
     {
         if (k <= 1 && faces.size() <= cap) return Leaves{std::move(faces)};
         if (k <= 1) k = 2;
         std::vector<uint32_t> A, B;
         uint32_t              kA, kB;
-        split(std::move(faces), k, cap, A, B, kA, kB);
+        split(std::move(faces), k, cap, A, B, kA, kB, far);
         Leaves left, right;
         if (A.size() + B.size() >= kParallelMin && workers_.fetch_add(1) < num_threads()) {
             std::thread th([&] { left = run(std::move(A), kA, cap); });
@@ -377,7 +383,7 @@ class Bisector
     }
 
     void split(std::vector<uint32_t> faces, uint32_t k, uint32_t cap, std::vector<uint32_t>& A,
-               std::vector<uint32_t>& B, uint32_t& kA, uint32_t& kB)
+               std::vector<uint32_t>& B, uint32_t& kA, uint32_t& kB, uint32_t far = kInvalid32)
     {
         const uint32_t n  = static_cast<uint32_t>(faces.size());
         const uint32_t k1 = k / 2, k2 = k - k1;
@@ -390,7 +396,7 @@ class Bisector
         uint32_t              start = faces[0];  // the set's smallest caller id
         for (uint32_t f : faces)
             if (oid(f) < oid(start)) start = f;
-        const uint32_t a = farthest(faces, start, tag, q);
+        const uint32_t a = (far != kInvalid32) ? far : farthest(faces, start, tag, q);
         std::vector<uint32_t> d[3];
         const uint32_t        b = farthest(faces, a, tag, q);
         d[0]                    = dists(faces);

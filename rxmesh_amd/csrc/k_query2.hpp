@@ -316,7 +316,9 @@ struct PlanVVF16
     uint32_t *ev, *off, *wave, *lin_v, *lin_f;
     uint16_t *fe, *slots;
 };
-__host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf)
+// nf_lin: faces whose linear ids are staged (0 when they are not)
+__host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf_lin,
+                                                uint32_t nf)
 {
     PlanVVF16 p{};
     p.ev    = b.alloc<uint32_t>(r4(ne));
@@ -325,7 +327,7 @@ __host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t n
     p.slots = b.alloc<uint16_t>(r4(2 * ne > 3 * nf ? 2 * ne : 3 * nf));
     p.wave  = b.alloc<uint32_t>(20);
     p.lin_v = b.alloc<uint32_t>(nv);
-    p.lin_f = b.alloc<uint32_t>(nf);
+    p.lin_f = nf_lin ? b.alloc<uint32_t>(nf_lin) : nullptr;
     return p;
 }
 
@@ -351,7 +353,11 @@ __device__ __forceinline__ void stage_vv_vf(const DevMesh& m, const PatchDesc& d
     for (uint32_t l = threadIdx.x + RL * T; l < d.nf; l += T) lin_f[l] = (l < d.nfo) ? d.fp + l : fs[l];
 }
 
-template <uint32_t T, bool STABLE = false>
+// LAZY_F (LDS-atomic form only): the face linear ids are not staged; the VF
+// copy-out computes an owned face's id and reads a ribbon face's from the
+// owner table, which leaves the workgroup ~3 KB less LDS (one more
+// workgroup per CU on 512-face patches).
+template <uint32_t T, bool STABLE = false, bool LAZY_F = false>
 __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                     const uint32_t* __restrict__ vf_base,
                                                     int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
@@ -388,8 +394,12 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         }
         return;
     }
-    const PlanVVF16 L = plan_vvf16(b, d.nv, d.nvo, d.ne, d.nf);
-    stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
+    const PlanVVF16 L = plan_vvf16(b, d.nv, d.nvo, d.ne, LAZY_F ? 0u : d.nf, d.nf);
+    if constexpr (LAZY_F)
+        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin_v, d.nv, d.nvo, d.vp,
+                          m.v_lin + d.v_off);
+    else
+        stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
     // VV: slot = entry index i (edge i >> 1, endpoint i & 1), so a row sorted
@@ -427,7 +437,15 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         if (threadIdx.x == 0) vf_row[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
         __syncthreads();
         int32_t* dst = vf_col + bse;
-        for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) dst[k] = static_cast<int32_t>(L.lin_f[L.slots[k]]);
+        if constexpr (LAZY_F) {
+            const uint32_t* fl = m.f_lin + d.f_off;
+            for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) {
+                const uint32_t f = L.slots[k];
+                dst[k]           = static_cast<int32_t>(f < d.nfo ? d.fp + f : fl[f]);
+            }
+        } else {
+            for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) dst[k] = static_cast<int32_t>(L.lin_f[L.slots[k]]);
+        }
     }
 }
 
@@ -475,9 +493,10 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf_c(DevMesh m, const uint32_t*
     fill_columns<T, true, 16>(
         nev + 3 * d.nf, 2 * nvo,
         [&](uint32_t i) {
-            if (i < nev) return ev_end(L.ev, i);
-            const uint32_t j = i - nev, v = face_vertex(L.fe, L.ev, j / 3u, j % 3u);
-            return (v < nvo) ? nvo + v : 2 * nvo;
+            // ribbon vertices (local id >= nvo) own no row in either half: the
+            // key 2 nvo drops the entry
+            const uint32_t v = (i < nev) ? ev_end(L.ev, i) : face_vertex(L.fe, L.ev, (i - nev) / 3u, (i - nev) % 3u);
+            return (v < nvo) ? ((i < nev) ? v : nvo + v) : 2 * nvo;
         },
         [&](uint32_t i) { return (i < nev) ? i : (i - nev) / 3u; }, L.off, L.slots, L.wave);
     const uint32_t bvv = vv_base[p], bvf = vf_base[p], fo = L.off[nvo];

@@ -283,3 +283,24 @@ def test_vv2_plan_rows_are_owner_rows(case):
             lin = g2l[ext[l]]
             want = l2g[ocol[orp[lin]:orp[lin + 1]]]
             assert np.array_equal(ext[row], want)
+
+
+def test_concurrent_builds_equal_serial_builds():
+    """Meshes built from several threads at once (the C-ABI releases the GIL)
+    export the same arrays as the same meshes built one by one, and a batch
+    built with its meshes in parallel equals the union of the serial ones."""
+    from concurrent.futures import ThreadPoolExecutor
+    meshes = [mg.icosphere(4, seed=k, sigma=0.01)[1] for k in range(3)] + [mg.grid(40 + 7 * k, seed=k)[1]
+                                                                          for k in range(3)]
+    meshes += [mg.permute_mesh(*mg.config_mesh("c1"), seed=5)[1]]
+    serial = [rx.Mesh(f, patch_size=256, host_only=True) for f in meshes]
+    with ThreadPoolExecutor(max_workers=4) as pool:
+        conc = list(pool.map(lambda f: rx.Mesh(f, patch_size=256, host_only=True), meshes))
+    names = [a for a in rx.ARR if not a.startswith("vv2")]
+    for a, b in zip(serial, conc):
+        for n in names:
+            assert np.array_equal(a.array(n), b.array(n)), n
+    batch = rx.Mesh(None, batch=meshes, patch_size=256, host_only=True)
+    union = rx.Mesh(None, parts=serial, host_only=True)
+    for n in names:
+        assert np.array_equal(batch.array(n), union.array(n)), n
