@@ -43,7 +43,7 @@ def parse():
                          "query_evd, query_vv_vf, arap_weights, arap_rotations, arap_energy_grad, copy")
     ap.add_argument("--reps", type=int, default=20)
     ap.add_argument("--query-sort", type=int, default=None,
-                    help="--profile: counting-sort form of the queries (0 LDS atomics, 1 atomic-free; default: library's)")
+                    help="--profile: counting-sort form of the queries (0 LDS atomics, 1 atomic-free, 2 one pass for VV + VF; default: library's)")
     ap.add_argument("--smooth-threads", type=int, default=None, help="smoothing workgroup size (default: autotuned)")
     ap.add_argument("--smooth-layout", type=int, default=None, help="smoothing LDS layout 0/1 (default: autotuned)")
     ap.add_argument("--smooth-fused", type=int, default=None,
@@ -71,19 +71,38 @@ def measured_traffic(workload, patch_size, kernel):
     return None
 
 
-def event_time_ms(fn, reps):
-    """Average duration of fn() over reps launches, HIP events on the
-    current stream (the stream every rxmesh_amd launch uses here)."""
+def event_time_ms(fn, reps, groups=5):
+    """Duration of one fn() call on the device: the reps calls are queued in
+    `groups` runs with a HIP event (on the current stream, the stream every
+    rxmesh_amd launch uses here) between runs; the figure is the median over
+    runs of a run's time / its calls. The host queues the calls ahead of the
+    device, so a run's interval is device time; the median keeps a host stall
+    (the Python collector, a descheduled thread) that leaves the device idle
+    inside one run out of the figure, and an event per run rather than per
+    call keeps the events' own cost out of it. The collector is off while
+    the calls are queued."""
+    import gc
+    import numpy as np
     import torch
     s = torch.cuda.current_stream()
-    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    groups = max(1, min(groups, reps))
+    per = max(1, reps // groups)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(groups + 1)]
     fn()
-    a.record(s)
-    for _ in range(reps):
-        fn()
-    b.record(s)
-    b.synchronize()
-    return a.elapsed_time(b) / reps
+    torch.cuda.synchronize()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        ev[0].record(s)
+        for g in range(groups):
+            for _ in range(per):
+                fn()
+            ev[g + 1].record(s)
+        ev[-1].synchronize()
+    finally:
+        if was:
+            gc.enable()
+    return float(np.median([ev[g].elapsed_time(ev[g + 1]) / per for g in range(groups)]))
 
 
 def cpu_baseline(reps, m, fv, pos, iters, gpu_entries, build_s):

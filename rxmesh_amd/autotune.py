@@ -153,9 +153,9 @@ def tune_smooth_symdir(mesh, Mr, x, tmp, lam, eta, steps=2, reps=3, use_cache=Tr
     return best, timings
 
 
-# transposes: 0 LDS-atomic histogram + row sort, 1 atomic-free ballot/mbcnt sort, 2 as 0 with VV and VF
-# sorted in one pass in the fused launch, 3 as 0 with the fused launch reading ribbon face ids at copy-out
-QUERY_SORTS = (0, 1, 2, 3)
+# transposes: 0 LDS-atomic histogram + row sort, 1 atomic-free ballot/mbcnt sort, 2 as 0 with the fused
+# VV + VF launch in one counting sort over both relations (the library's default)
+QUERY_SORTS = (0, 1, 2)
 
 
 def tune_query_sort(mesh, reps=20):

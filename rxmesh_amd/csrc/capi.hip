@@ -92,7 +92,9 @@ struct rxm_mesh
     uint32_t        grad_queue     = 0;        // 1: persistent work-queue launch
     uint32_t*       d_queue        = nullptr;  // work-queue ticket counter
     bool            system_runs_ok = false;    // check_system_runs passed
-    uint32_t        query_stable   = 0;        // 1: atomic-free stable counting sort in the queries, 0: LDS atomics + row sort
+    uint32_t        query_stable   = 2;        // transposes: 0 LDS atomics + row sort, 1 atomic-free stable sort, 2 as 0 with the fused VV + VF launch in one pass
+    uint32_t        vvf_nslots     = 0;        // largest VV + VF entry count of a patch (0: not computed yet)
+    uint32_t        max_nrv = 0, max_nrf = 0;  // largest ribbon vertex / face count of a patch
     int             num_cus        = 0;
     // batch (disjoint union of meshes): per-mesh id / patch offsets
     std::vector<uint32_t> b_v, b_e, b_f, b_p;
@@ -956,20 +958,29 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     if (!rc) rc = device_base(m, RXM_OP_VF, &vf_base);
     if (rc) return rc;
     constexpr uint32_t T = kQueryThreads;
-    Bump               bs(nullptr), ba(nullptr);  // stable / atomic plans, as in rxm_query_csr
+    if (m->query_stable == 2) {  // one counting sort for both relations
+        if (!m->vvf_nslots) {
+            const auto& vb = op_base(m, RXM_OP_VV);
+            const auto& fb = op_base(m, RXM_OP_VF);
+            for (uint32_t p = 0; p < m->h.P; ++p) {
+                m->vvf_nslots = std::max(m->vvf_nslots, vb[p + 1] - vb[p] + fb[p + 1] - fb[p]);
+                m->max_nrv    = std::max(m->max_nrv, m->h.nv[p] - m->h.nvo[p]);
+                m->max_nrf    = std::max(m->max_nrf, m->h.nf[p] - m->h.nfo[p]);
+            }
+            m->vvf_nslots = std::max(m->vvf_nslots, 1u);
+        }
+        Bump b1(nullptr);
+        plan_vvf1(b1, m->h.max_nvo, m->h.max_ne, m->vvf_nslots, m->max_nrv, m->max_nrf);
+        // VV slots hold entry indices 2 e + end in 16 bits
+        if (b1.bytes() <= kMaxLdsBytes && 2u * m->h.max_ne <= 0x10000u)
+            return launch_patches_xcd(k_query2_vv_vf1<T>, T, b1.bytes(), stream, m, m->dm, m->vvf_nslots, m->max_nrv,
+                                      m->max_nrf, vv_base, vf_base, vv_row_ptr, vv_col, vf_row_ptr, vf_col);
+    }
+    Bump bs(nullptr), ba(nullptr);  // stable / atomic plans, as in rxm_query_csr
     plan_vvf(bs, m->h.max_nv, m->h.max_ne, m->h.max_nf, T / 64u);
-    const bool lazy_f = m->query_stable == 3;
-    plan_vvf16(ba, m->h.max_nv, m->h.max_nvo, m->h.max_ne, lazy_f ? 0u : m->h.max_nf, m->h.max_nf);
+    plan_vvf16(ba, m->h.max_nv, m->h.max_nvo, m->h.max_ne, m->h.max_nf);
     if (m->query_stable == 1 && bs.bytes() <= kMaxLdsBytes)
         return launch_patches_xcd(k_query2_vv_vf<T, true>, T, bs.bytes(), stream, m, m->dm, vv_base, vf_base,
-                                  vv_row_ptr, vv_col, vf_row_ptr, vf_col);
-    Bump bc(nullptr);  // one counting sort for both relations
-    plan_vvfc(bc, m->h.max_nv, m->h.max_nvo, m->h.max_ne, m->h.max_nf);
-    if (m->query_stable == 2 && bc.bytes() <= kMaxLdsBytes)
-        return launch_patches_xcd(k_query2_vv_vf_c<T>, T, bc.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
-                                  vv_col, vf_row_ptr, vf_col);
-    if (lazy_f)
-        return launch_patches_xcd(k_query2_vv_vf<T, false, true>, T, ba.bytes(), stream, m, m->dm, vv_base, vf_base,
                                   vv_row_ptr, vv_col, vf_row_ptr, vf_col);
     return launch_patches_xcd(k_query2_vv_vf<T, false>, T, ba.bytes(), stream, m, m->dm, vv_base, vf_base, vv_row_ptr,
                               vv_col, vf_row_ptr, vf_col);
@@ -1183,10 +1194,10 @@ rxm_status rxm_set_launch_threads(rxm_mesh* m, int kernel, int threads)
         m->ring_threads = static_cast<uint32_t>(threads);
         return RXM_OK;
     }
-    // query transposes: 0 LDS-atomic counting sort, 1 atomic-free stable sort,
-    // 2 as 0 with the fused VV + VF launch sorting both in one pass, 3 as 0
-    // with the fused launch reading ribbon face ids at copy-out
-    if (kernel == 10 && threads >= 0 && threads <= 3) {
+    // query transposes: 0 LDS-atomic counting sort + row sort, 1 atomic-free
+    // stable sort, 2 (default) as 0 with the fused VV + VF launch in one
+    // counting sort over both relations (k_query2_vv_vf1)
+    if (kernel == 10 && threads >= 0 && threads <= 2) {
         m->query_stable = static_cast<uint32_t>(threads);
         return RXM_OK;
     }

@@ -11,6 +11,8 @@
 #include <memory>
 #include <numeric>
 #include <thread>
+#include <chrono>
+#include <cstdio>
 
 namespace rxm {
 
@@ -660,7 +662,10 @@ int run_patcher(HostMesh& m, uint32_t patch_size, int mode)
         fp.assign(nF, 0);
         return set_patches(m, fp.data(), 1, nullptr, nullptr);
     }
+    auto clk = [] { return std::chrono::steady_clock::now(); };
+    auto t0  = clk();
     FaceGraph g = face_graph(m);
+    auto t1 = clk();
     // BFS order over all components from face 0
     std::vector<uint32_t> order, rank(nF, kInvalid32);
     order.reserve(nF);
@@ -703,11 +708,18 @@ int run_patcher(HostMesh& m, uint32_t patch_size, int mode)
             uint32_t       k = gr.off[i];
             for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) gr.val[k++] = rank[g.val[j]];
         }, 8192);
+        auto t2 = clk();
         std::vector<uint32_t> fpr(nF, kInvalid32);
         Bisector bs(gr, nF, &order);
         nP = bs.run_all(nF, patch_size, fpr);
+        auto t3 = clk();
         fp.assign(nF, kInvalid32);
         for (uint32_t i = 0; i < nF; ++i) fp[order[i]] = fpr[i];
+        if (std::getenv("RXM_TIMING")) {
+            auto s_ = [](auto a, auto b) { return std::chrono::duration<double>(b - a).count(); };
+            std::fprintf(stderr, "patcher: face graph %.3f s, relabel %.3f s, bisection %.3f s\n", s_(t0, t1),
+                         s_(t1, t2), s_(t2, t3));
+        }
     }
     // recursive bisection of oversize patches
     std::vector<std::vector<uint32_t>> members(nP);

@@ -316,9 +316,7 @@ struct PlanVVF16
     uint32_t *ev, *off, *wave, *lin_v, *lin_f;
     uint16_t *fe, *slots;
 };
-// nf_lin: faces whose linear ids are staged (0 when they are not)
-__host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf_lin,
-                                                uint32_t nf)
+__host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf)
 {
     PlanVVF16 p{};
     p.ev    = b.alloc<uint32_t>(r4(ne));
@@ -327,7 +325,7 @@ __host__ __device__ inline PlanVVF16 plan_vvf16(Bump& b, uint32_t nv, uint32_t n
     p.slots = b.alloc<uint16_t>(r4(2 * ne > 3 * nf ? 2 * ne : 3 * nf));
     p.wave  = b.alloc<uint32_t>(20);
     p.lin_v = b.alloc<uint32_t>(nv);
-    p.lin_f = nf_lin ? b.alloc<uint32_t>(nf_lin) : nullptr;
+    p.lin_f = b.alloc<uint32_t>(nf);
     return p;
 }
 
@@ -353,11 +351,7 @@ __device__ __forceinline__ void stage_vv_vf(const DevMesh& m, const PatchDesc& d
     for (uint32_t l = threadIdx.x + RL * T; l < d.nf; l += T) lin_f[l] = (l < d.nfo) ? d.fp + l : fs[l];
 }
 
-// LAZY_F (LDS-atomic form only): the face linear ids are not staged; the VF
-// copy-out computes an owned face's id and reads a ribbon face's from the
-// owner table, which leaves the workgroup ~3 KB less LDS (one more
-// workgroup per CU on 512-face patches).
-template <uint32_t T, bool STABLE = false, bool LAZY_F = false>
+template <uint32_t T, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                     const uint32_t* __restrict__ vf_base,
                                                     int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
@@ -394,12 +388,8 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         }
         return;
     }
-    const PlanVVF16 L = plan_vvf16(b, d.nv, d.nvo, d.ne, LAZY_F ? 0u : d.nf, d.nf);
-    if constexpr (LAZY_F)
-        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, L.fe, m.fe + 3u * d.f_off, d.nf, L.lin_v, d.nv, d.nvo, d.vp,
-                          m.v_lin + d.v_off);
-    else
-        stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
+    const PlanVVF16 L = plan_vvf16(b, d.nv, d.nvo, d.ne, d.nf);
+    stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
     for (uint32_t c = threadIdx.x; c <= d.nvo; c += T) L.off[c] = 0;
     __syncthreads();
     // VV: slot = entry index i (edge i >> 1, endpoint i & 1), so a row sorted
@@ -437,85 +427,229 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf(DevMesh m, const uint32_t* _
         if (threadIdx.x == 0) vf_row[d.vp + d.nvo] = static_cast<int32_t>(bse + L.off[d.nvo]);
         __syncthreads();
         int32_t* dst = vf_col + bse;
-        if constexpr (LAZY_F) {
-            const uint32_t* fl = m.f_lin + d.f_off;
-            for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) {
-                const uint32_t f = L.slots[k];
-                dst[k]           = static_cast<int32_t>(f < d.nfo ? d.fp + f : fl[f]);
-            }
-        } else {
-            for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) dst[k] = static_cast<int32_t>(L.lin_f[L.slots[k]]);
-        }
+        for (uint32_t k = threadIdx.x; k < L.off[d.nvo]; k += T) dst[k] = static_cast<int32_t>(L.lin_f[L.slots[k]]);
     }
 }
 
-// The fused VV + VF launch with one counting sort over the combined column
-// space: columns [0, nvo) are the VV rows, [nvo, 2 nvo) the VF rows; entries
-// [0, 2 ne) are the edge endpoints (slot = entry index), [2 ne, 2 ne + 3 nf)
-// the face corners (slot = local face). One histogram, one scan and one fill
-// serve both relations, and their rows are sorted and copied out under the
-// same barriers; the slot array holds both (more LDS than k_query2_vv_vf's
-// atomic form). Same CSRs as k_query2_vv_vf.
-struct PlanVVFC
+// ---- fused VV + VF in one counting sort ------------------------------------
+// Sorting networks for rows of up to 4, 6 and 8 entries (5, 12 and 19
+// comparators; each sorts every 0/1 input, which by the 0-1 principle makes
+// it a sorting network).
+template <uint32_t N>
+__device__ __forceinline__ void net_sort(uint32_t (&a)[8])
 {
-    uint32_t *ev, *off, *wave, *lin_v, *lin_f;
-    uint16_t *fe, *slots;
+    if constexpr (N == 4) {
+        cswap(a[0], a[1]); cswap(a[2], a[3]);
+        cswap(a[0], a[2]); cswap(a[1], a[3]);
+        cswap(a[1], a[2]);
+    } else if constexpr (N == 6) {
+        cswap(a[0], a[5]); cswap(a[1], a[3]); cswap(a[2], a[4]);
+        cswap(a[1], a[2]); cswap(a[3], a[4]);
+        cswap(a[0], a[3]); cswap(a[2], a[5]);
+        cswap(a[0], a[1]); cswap(a[2], a[3]); cswap(a[4], a[5]);
+        cswap(a[1], a[2]); cswap(a[3], a[4]);
+    } else {
+        cswap(a[0], a[1]); cswap(a[2], a[3]); cswap(a[4], a[5]); cswap(a[6], a[7]);
+        cswap(a[0], a[2]); cswap(a[1], a[3]); cswap(a[4], a[6]); cswap(a[5], a[7]);
+        cswap(a[1], a[2]); cswap(a[5], a[6]);
+        cswap(a[0], a[4]); cswap(a[1], a[5]); cswap(a[2], a[6]); cswap(a[3], a[7]);
+        cswap(a[2], a[4]); cswap(a[3], a[5]);
+        cswap(a[1], a[2]); cswap(a[3], a[4]); cswap(a[5], a[6]);
+    }
+}
+
+// LDS carve of the one-pass form: EV, the column starts of both relations
+// (VV rows are columns [0, nvo), VF rows [nvo, 2 nvo)), u16 slots for the
+// owned rows' entries (nslots: the largest VV + VF entry count of a patch)
+// and the linear ids of the ribbon (nrv vertices, nrf faces at most; owned
+// ids follow from the prefixes). FE is not staged.
+struct PlanVVF1
+{
+    uint32_t *ev, *off, *wave, *rv, *rf;
+    uint16_t* slots;
 };
-__host__ __device__ inline PlanVVFC plan_vvfc(Bump& b, uint32_t nv, uint32_t nvo, uint32_t ne, uint32_t nf)
+__host__ __device__ inline PlanVVF1 plan_vvf1(Bump& b, uint32_t nvo, uint32_t ne, uint32_t nslots, uint32_t nrv,
+                                              uint32_t nrf)
 {
-    PlanVVFC p{};
+    PlanVVF1 p{};
     p.ev    = b.alloc<uint32_t>(r4(ne));
-    p.fe    = b.alloc<uint16_t>(r4(3 * nf));
     p.off   = b.alloc<uint32_t>(2 * nvo + 1);
-    p.slots = b.alloc<uint16_t>(r4(2 * ne + 3 * nf));
+    p.slots = b.alloc<uint16_t>(nslots);
     p.wave  = b.alloc<uint32_t>(20);
-    p.lin_v = b.alloc<uint32_t>(nv);
-    p.lin_f = b.alloc<uint32_t>(nf);
+    p.rv    = b.alloc<uint32_t>(nrv);
+    p.rf    = b.alloc<uint32_t>(nrf);
     return p;
 }
 
+// The fused VV + VF launch in one counting sort (the default form):
+//  - keys per edge (both endpoints of one EV word) and per face (three
+//    corners), not per entry; each thread's FE words come from HBM straight
+//    into registers with the staging's loads, and its keys stay in registers
+//    from the histogram to the fill;
+//  - one histogram, one scan and one fill over the combined column space
+//    (VV rows then VF rows), so the VF slots follow the VV slots;
+//  - each owned row sorted in registers by the smallest network that covers
+//    the longest row of its wave (a wave-uniform choice), written back to
+//    LDS, then one coalesced pass writes both CSRs, turning entries into
+//    linear ids (VV slot = entry index 2 e + end: below 2^16 needs ne <=
+//    2^15; VF slot = local face).
+// Needs ne <= RE T and nf <= RF T for the register path; larger patches take
+// plain loops. Same CSRs as k_query2_vv_vf.
 template <uint32_t T>
-__global__ __launch_bounds__(T) void k_query2_vv_vf_c(DevMesh m, const uint32_t* __restrict__ vv_base,
-                                                      const uint32_t* __restrict__ vf_base,
-                                                      int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
-                                                      int32_t* __restrict__ vf_row, int32_t* __restrict__ vf_col)
+__global__ __launch_bounds__(T) void k_query2_vv_vf1(DevMesh m, uint32_t nslots, uint32_t nrv, uint32_t nrf,
+                                                     const uint32_t* __restrict__ vv_base,
+                                                     const uint32_t* __restrict__ vf_base,
+                                                     int32_t* __restrict__ vv_row, int32_t* __restrict__ vv_col,
+                                                     int32_t* __restrict__ vf_row, int32_t* __restrict__ vf_col)
 {
+    constexpr uint32_t RE = (1536u + T - 1u) / T;  // edges held in registers
+    constexpr uint32_t RF = (1024u + T - 1u) / T;  // faces held in registers
     extern __shared__ uint4 smem[];
     const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
     if (p >= m.P) return;
     const PatchDesc d = load_desc(m.patch, p);
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanVVFC  L = plan_vvfc(b, d.nv, d.nvo, d.ne, d.nf);
-    stage_vv_vf<T>(m, d, L.ev, L.fe, L.lin_v, L.lin_f);
-    const uint32_t nvo = d.nvo, nev = 2 * d.ne;
-    for (uint32_t c = threadIdx.x; c <= 2 * nvo; c += T) L.off[c] = 0;
-    __syncthreads();
-    fill_columns<T, true, 16>(
-        nev + 3 * d.nf, 2 * nvo,
-        [&](uint32_t i) {
-            // ribbon vertices (local id >= nvo) own no row in either half: the
-            // key 2 nvo drops the entry
-            const uint32_t v = (i < nev) ? ev_end(L.ev, i) : face_vertex(L.fe, L.ev, (i - nev) / 3u, (i - nev) % 3u);
-            return (v < nvo) ? ((i < nev) ? v : nvo + v) : 2 * nvo;
-        },
-        [&](uint32_t i) { return (i < nev) ? i : (i - nev) / 3u; }, L.off, L.slots, L.wave);
-    const uint32_t bvv = vv_base[p], bvf = vf_base[p], fo = L.off[nvo];
-    for (uint32_t r = threadIdx.x; r < 2 * nvo; r += T) {
-        const uint32_t o = L.off[r], k = L.off[r + 1] - o;
-        if (r < nvo) vv_row[d.vp + r] = static_cast<int32_t>(bvv + o);
-        else vf_row[d.vp + r - nvo] = static_cast<int32_t>(bvf + o - fo);
-        sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); });
+    const PlanVVF1  L    = plan_vvf1(b, d.nvo, d.ne, nslots, nrv, nrf);
+    const uint32_t  nvo  = d.nvo, t = threadIdx.x;
+    const uint32_t* vl   = m.v_lin + d.v_off;
+    const uint32_t* fl   = m.f_lin + d.f_off;
+    const uint16_t* fs   = m.fe + 3u * d.f_off;
+    const bool      held = d.ne <= RE * T && d.nf <= RF * T;
+    // global loads first: this thread's FE words, the ribbon ids, EV
+    uint32_t fw[RF][3];
+    if (held) {
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r) {
+            const uint32_t f = t + r * T;
+#pragma unroll
+            for (uint32_t j = 0; j < 3; ++j) fw[r][j] = (f < d.nf) ? fs[3 * f + j] : 0u;
+        }
     }
-    if (threadIdx.x == 0) {
-        vv_row[d.vp + nvo] = static_cast<int32_t>(bvv + fo);
-        vf_row[d.vp + nvo] = static_cast<int32_t>(bvf + L.off[2 * nvo] - fo);
+    const uint32_t nr = d.nv - nvo, mr = d.nf - d.nfo;
+    const uint32_t x0 = (t < nr) ? vl[nvo + t] : 0u, y0 = (t < mr) ? fl[d.nfo + t] : 0u;
+    stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, nullptr, 0, 0, 0, nullptr);
+    if (t < nr) L.rv[t] = x0;
+    if (t < mr) L.rf[t] = y0;
+    for (uint32_t i = t + T; i < nr; i += T) L.rv[i] = vl[nvo + i];
+    for (uint32_t i = t + T; i < mr; i += T) L.rf[i] = fl[d.nfo + i];
+    for (uint32_t c = t; c <= 2 * nvo; c += T) L.off[c] = 0;
+    __syncthreads();
+    // VV entries of edge e are its endpoints (column = local vertex), VF
+    // entries of face f its corners (column = nvo + local vertex); entries
+    // of ribbon vertices (>= nvo) have no row and are dropped
+    auto fe_end = [&](uint32_t x) {  // local vertex an FE word starts at
+        const uint32_t e = L.ev[x >> 1];
+        return (x & 1u) ? hi16(e) : lo16(e);
+    };
+    uint32_t xe[RE], kf[RF][3];
+    if (held) {
+#pragma unroll
+        for (uint32_t r = 0; r < RE; ++r) {
+            const uint32_t e = t + r * T;
+            const uint32_t x = L.ev[min(e, d.ne - 1u)];
+            xe[r]            = (e < d.ne) ? x : 0xFFFFFFFFu;
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+            for (uint32_t j = 0; j < 3; ++j)  // the FE word of a face past nf is 0: a valid word
+                kf[r][j] = (t + r * T < d.nf) ? fe_end(fw[r][j]) : 0xFFFFu;
+#pragma unroll
+        for (uint32_t r = 0; r < RE; ++r) {
+            if (lo16(xe[r]) < nvo) atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u);
+            if (hi16(xe[r]) < nvo) atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+            for (uint32_t j = 0; j < 3; ++j)
+                if (kf[r][j] < nvo) atomicAdd(&L.off[nvo + kf[r][j] + 1u], 1u);
+    } else {
+        for (uint32_t e = t; e < d.ne; e += T) {
+            const uint32_t x = L.ev[e];
+            if (lo16(x) < nvo) atomicAdd(&L.off[lo16(x) + 1u], 1u);
+            if (hi16(x) < nvo) atomicAdd(&L.off[hi16(x) + 1u], 1u);
+        }
+        for (uint32_t f = t; f < d.nf; f += T)
+            for (uint32_t j = 0; j < 3; ++j) {
+                const uint32_t v = fe_end(fs[3 * f + j]);
+                if (v < nvo) atomicAdd(&L.off[nvo + v + 1u], 1u);
+            }
+    }
+    scan_excl<T>(L.off + 1, 2 * nvo, L.wave);
+    // fill: off[c + 1] advances from the start of column c to that of c + 1
+    if (held) {
+#pragma unroll
+        for (uint32_t r = 0; r < RE; ++r) {
+            const uint32_t i = 2u * (t + r * T);
+            if (lo16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i);
+            if (hi16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i + 1u);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+            for (uint32_t j = 0; j < 3; ++j)
+                if (kf[r][j] < nvo)
+                    L.slots[atomicAdd(&L.off[nvo + kf[r][j] + 1u], 1u)] = static_cast<uint16_t>(t + r * T);
+    } else {
+        for (uint32_t e = t; e < d.ne; e += T) {
+            const uint32_t x = L.ev[e];
+            if (lo16(x) < nvo) L.slots[atomicAdd(&L.off[lo16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e);
+            if (hi16(x) < nvo) L.slots[atomicAdd(&L.off[hi16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e + 1u);
+        }
+        for (uint32_t f = t; f < d.nf; f += T)
+            for (uint32_t j = 0; j < 3; ++j) {
+                const uint32_t v = fe_end(fs[3 * f + j]);
+                if (v < nvo) L.slots[atomicAdd(&L.off[nvo + v + 1u], 1u)] = static_cast<uint16_t>(f);
+            }
     }
     __syncthreads();
-    const uint32_t tot = L.off[2 * nvo];
-    for (uint32_t k = threadIdx.x; k < tot; k += T) {
-        const uint32_t sl = L.slots[k];
-        if (k < fo) vv_col[bvv + k] = static_cast<int32_t>(L.lin_v[ev_other(L.ev, sl)]);
-        else vf_col[bvf + k - fo] = static_cast<int32_t>(L.lin_f[sl]);
+    const uint32_t nvv = L.off[nvo], ntot = L.off[2 * nvo];
+    const uint32_t bvv = vv_base[p], bvf = vf_base[p] - nvv;  // VF slots follow the VV slots (32-bit: may wrap)
+    if (t == 0) {
+        vv_row[d.vp + nvo] = static_cast<int32_t>(bvv + nvv);
+        vf_row[d.vp + nvo] = static_cast<int32_t>(bvf + ntot);
+    }
+    for (uint32_t c = t; c < 2 * nvo; c += T) {
+        const bool     isvv = c < nvo;
+        const uint32_t o = L.off[c], k = L.off[c + 1] - o;
+        (isvv ? vv_row : vf_row)[d.vp + (isvv ? c : c - nvo)] = static_cast<int32_t>((isvv ? bvv : bvf) + o);
+        auto     put = [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); };
+        uint32_t a[8];
+        if (!__builtin_amdgcn_ballot_w64(k > 4u)) {
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) a[j] = (j < k) ? L.slots[o + j] : 0xFFFFFFFFu;
+            net_sort<4>(a);
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                if (j < k) put(j, a[j]);
+        } else if (!__builtin_amdgcn_ballot_w64(k > 6u)) {
+#pragma unroll
+            for (uint32_t j = 0; j < 6; ++j) a[j] = (j < k) ? L.slots[o + j] : 0xFFFFFFFFu;
+            net_sort<6>(a);
+#pragma unroll
+            for (uint32_t j = 0; j < 6; ++j)
+                if (j < k) put(j, a[j]);
+        } else if (k <= 8u) {
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) a[j] = (j < k) ? L.slots[o + j] : 0xFFFFFFFFu;
+            net_sort<8>(a);
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (j < k) put(j, a[j]);
+        } else {
+            sort_row(L.slots + o, k, put);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < ntot; i += T) {
+        const uint32_t s = L.slots[i];
+        if (i < nvv) {
+            const uint32_t w = ev_other(L.ev, s);
+            vv_col[bvv + i]  = static_cast<int32_t>((w < nvo) ? d.vp + w : L.rv[w - nvo]);
+        } else {
+            vf_col[bvf + i] = static_cast<int32_t>((s < d.nfo) ? d.fp + s : L.rf[s - d.nfo]);
+        }
     }
 }
 

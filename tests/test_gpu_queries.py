@@ -359,19 +359,19 @@ def test_stable_and_atomic_transposes_agree(name):
     """Knob 10 switches the transposed queries between the LDS atomic
     histogram + row sort (0), the atomic-free stable counting sort (1:
     ballot / mbcnt ranks, rows in entry order) and, for the fused VV + VF
-    launch, one atomic counting sort over both relations (2) or ribbon face
-    ids read at copy-out (3): identical CSRs,
-    bit for bit, for every transposed op and for the fused launch."""
+    launch, one counting sort over both relations with per-edge / per-face
+    keys and rows sorted in registers (2, the default): identical CSRs, bit
+    for bit, for every transposed op and for the fused launch."""
     import torch
     V, F, m, o = case(name)
     got = {}
-    for knob in (0, 1, 2, 3):
+    for knob in (0, 1, 2):
         m.set_launch_threads(10, knob)
         got[knob] = {op: tuple(t.cpu() for t in m.query(op)) for op in ("VV", "VE", "VF", "EF", "FF")}
         (a, b), (c, d) = m.query_vv_vf()
         got[knob]["VVF"] = (a.cpu(), b.cpu(), c.cpu(), d.cpu())
-    m.set_launch_threads(10, 0)
-    for knob in (1, 2, 3):
+    m.set_launch_threads(10, 2)
+    for knob in (1, 2):
         for op in got[0]:
             for u, v in zip(got[0][op], got[knob][op]):
                 assert torch.equal(u, v), (knob, op)
