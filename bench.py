@@ -2,12 +2,14 @@
 
 Default workload (one "step"): on the icosphere subdivided 9 times
 (2,621,442 V / 5,242,880 F, radial noise N(0, 0.005), vertex and face ids
-randomly permuted, patch size 512):
-  VV query materialised as CSR + VF query materialised as CSR +
-  10 Jacobi iterations of uniform Laplacian smoothing (lambda 0.5, fp32)
-  consuming the VV CSR of this step.
-value = VV/VF entries delivered per second (Gentries/s), where a smoothing
-iteration consumes the 2E VV entries of its one-rings.
+randomly permuted, patch size 512), the VV and VF queries materialised as
+int32 CSRs over linear ids by one launch (rxm_query_vv_vf).
+value = VV + VF entries written per second (Gentries/s) by that launch,
+inputs resident in HBM. Beside it: the query + smoothing step (the same
+query, then 10 Jacobi iterations of uniform Laplacian smoothing, lambda 0.5,
+fp32, over the engine's patch-local VV plan), the eight queries one by one,
+per-kernel timings with their roofline, the CPU port of the same work on the
+host, the other workloads and the 256-mesh batch.
 
 Prints ONE JSON line. --gpus N > 1 is rejected (single-GPU build).
 """
@@ -586,8 +588,9 @@ def main():
                      "traffic": (measured_traffic(args.workload, args.patch_size, "query_vv_vf")
                                  if args.patcher == "bisect" else None),
                      "algorithmic_bytes": db},
-        # the query + smoothing step of the north-star target: the query launch, then `it` fp32
-        # Jacobi iterations (lambda 0.5) that read the engine's static patch-local VV plan
+        # the query + smoothing step of the north-star target: the VV + VF query launch, then `it`
+        # fp32 Jacobi iterations (lambda 0.5) that read the engine's static patch-local VV plan
+        # (ms_separate_queries: VV and VF queried by separate launches, then the smoothing)
         "step_query_smooth": {"smooth_iters": it, "ms": round(ms_smooth, 4),
                               "ms_separate_queries": round(ms_separate, 4),
                               "algorithmic_bytes": step_bytes,
