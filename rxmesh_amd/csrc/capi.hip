@@ -82,7 +82,7 @@ struct rxm_mesh
     const uint32_t* d_cot_closed = nullptr;
     const uint32_t* d_cot_plist  = nullptr;
     uint32_t        cot_nplist   = 0;
-    const uint32_t* d_cot_ent    = nullptr;  // patch-local VV edge pattern (build_vv_edge_pattern)
+    const uint16_t* d_cot_ent    = nullptr;  // patch-local VV edge pattern (build_vv_edge_pattern)
     const uint32_t* d_cot_rs     = nullptr;
     uint32_t        smooth_threads = 128;
     uint32_t        ring_threads   = 256;      // oriented VV, VE
@@ -435,8 +435,9 @@ int ensure_cot_plan(rxm_mesh* m)
 {
     if (m->d_cot_closed) return RXM_OK;
     const HostMesh&       h = m->h;
-    std::vector<uint32_t> ent, rs;
-    build_vv_edge_pattern(h, op_base(m, RXM_OP_VV), ent, rs);
+    std::vector<uint16_t> ent;
+    std::vector<uint32_t> rs;
+    if (!build_vv_edge_pattern(h, op_base(m, RXM_OP_VV), ent, rs)) return RXM_ERR_PATCH_TOO_BIG;
     int rc0 = upload(m, ent, m->d_cot_ent);
     if (!rc0) rc0 = upload(m, rs, m->d_cot_rs);
     if (rc0) return rc0;
@@ -1062,7 +1063,7 @@ rxm_status rxm_arap_rotations(rxm_mesh* m, const float* rest, const float* x, co
     const int       rc      = arap_rows(m, &vv_base, &lds, false);
     if (rc) return rc;
     return launch_patches_xcd(k_arap_rotations<512>, 512, lds, stream, m, m->dm, vv_base,
-                              static_cast<const uint32_t*>(m->d_cot_rs), static_cast<const uint32_t*>(m->d_cot_ent),
+                              static_cast<const uint32_t*>(m->d_cot_rs), m->d_cot_ent,
                               rest, x, w, R);
 }
 
@@ -1081,7 +1082,7 @@ rxm_status rxm_arap_energy_grad(rxm_mesh* m, const float* rest, const float* x, 
         m->device_bytes += sizeof(float) * m->h.P;
     }
     rc = launch_patches_xcd(k_arap_energy_grad<512>, 512, lds, stream, m, m->dm, vv_base,
-                            static_cast<const uint32_t*>(m->d_cot_rs), static_cast<const uint32_t*>(m->d_cot_ent),
+                            static_cast<const uint32_t*>(m->d_cot_rs), m->d_cot_ent,
                             rest, x, w, R, m->d_partial, grad);
     if (rc) return rc;
     hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(1024), 0, static_cast<hipStream_t>(stream), m->h.P,

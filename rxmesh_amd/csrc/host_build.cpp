@@ -1038,9 +1038,11 @@ void closed_fan_vertices(const HostMesh& m, std::vector<uint8_t>& closed)
 
 // ---------------------------------------------------------------------------
 // Patch-local VV edge pattern (host_build.h).
-void build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint32_t>& ent,
+bool build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint16_t>& ent,
                            std::vector<uint32_t>& rs)
 {
+    for (uint32_t p = 0; p < m.P; ++p)
+        if (m.ne[p] >= 0x8000u) return false;
     ent.assign(vv_base[m.P], 0);
     rs.assign(m.nV + 1, 0);
     rs[m.nV] = vv_base[m.P];
@@ -1055,13 +1057,14 @@ void build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_ba
         }
         for (uint32_t l = 0; l < nvo; ++l) cnt[l + 1] += cnt[l];
         for (uint32_t l = 0; l < nvo; ++l) rs[vp + l] = vv_base[p] + cnt[l];
-        uint32_t* out = ent.data() + vv_base[p];
+        uint16_t* out = ent.data() + vv_base[p];
         for (uint32_t e = 0; e < ne; ++e) {
             const uint32_t a = ev[e] & 0xFFFFu, b = ev[e] >> 16;
-            if (a < nvo) out[cnt[a]++] = e | (a << 16);
-            if (b < nvo) out[cnt[b]++] = e | (b << 16);
+            if (a < nvo) out[cnt[a]++] = static_cast<uint16_t>(2u * e);
+            if (b < nvo) out[cnt[b]++] = static_cast<uint16_t>(2u * e + 1u);
         }
     }, 16);
+    return true;
 }
 
 // ---------------------------------------------------------------------------

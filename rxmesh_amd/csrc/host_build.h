@@ -169,10 +169,12 @@ void closed_fan_vertices(const HostMesh& m, std::vector<uint8_t>& closed);
 
 // Patch-local pattern of the VV query: for every patch p and owned vertex l,
 // the patch's local edges incident to l in ascending local edge id (the VV
-// query's row order), as ent[vv_base[p] + k] = edge | (l << 16); rs[lin] is
-// the first entry of linear vertex lin, rs[nV] the total. vv_base is the VV
-// query's per-patch entry base (P + 1).
-void build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint32_t>& ent,
+// query's row order), as ent[vv_base[p] + k] = 2 edge + (l is the edge's
+// second endpoint): the index, in FE's encoding, of the half-edge leaving l.
+// rs[lin] is the first entry of linear vertex lin, rs[nV] the total. vv_base
+// is the VV query's per-patch entry base (P + 1). Returns false when a patch
+// has 2^15 edges or more (the entries hold 16 bits).
+bool build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint16_t>& ent,
                            std::vector<uint32_t>& rs);
 
 unsigned num_threads();

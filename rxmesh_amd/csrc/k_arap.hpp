@@ -192,7 +192,8 @@ __device__ inline void best_rotation(const float (&S)[3][3], float (&R)[3][3])
 // every local edge.
 struct PlanAR
 {
-    uint32_t *ev, *rs, *ent;
+    uint32_t *ev, *rs;
+    uint16_t* ent;
     float *   px, *py, *pz, *xx, *xy, *xz, *w, *rot;
 };
 __host__ __device__ inline PlanAR plan_ar(Bump& b, uint32_t nv, uint32_t ne, uint32_t nvo, uint32_t nent, bool rot)
@@ -200,7 +201,7 @@ __host__ __device__ inline PlanAR plan_ar(Bump& b, uint32_t nv, uint32_t ne, uin
     PlanAR p{};
     p.ev  = b.alloc<uint32_t>(r4(ne));
     p.rs  = b.alloc<uint32_t>(nvo + 1);
-    p.ent = b.alloc<uint32_t>(nent);
+    p.ent = b.alloc<uint16_t>(nent);
     p.px  = b.alloc<float>(nv);
     p.py  = b.alloc<float>(nv);
     p.pz  = b.alloc<float>(nv);
@@ -214,14 +215,14 @@ __host__ __device__ inline PlanAR plan_ar(Bump& b, uint32_t nv, uint32_t ne, uin
 
 template <uint32_t T>
 __device__ __forceinline__ void stage_arap(const DevMesh& m, const PatchDesc& d, const PlanAR& L,
-                                           const uint32_t* __restrict__ rs, const uint32_t* __restrict__ ent,
+                                           const uint32_t* __restrict__ rs, const uint16_t* __restrict__ ent,
                                            uint32_t vb, uint32_t nent, const float* __restrict__ rest,
                                            const float* __restrict__ x, const float* __restrict__ w,
                                            const float* __restrict__ R)
 {
     constexpr uint32_t RB = (512u + T - 1u) / T;
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_words<T, 8>(L.ent, ent + vb, nent);
+    stage_halves<T, 8>(L.ent, ent + vb, nent);
     stage_words<T, 2>(L.rs, rs + d.vp, d.nvo + 1, vb);
     stage_positions_batched<T, RB, false>(L.px, L.py, L.pz, nullptr, rest, m.nV, d, m.v_lin);
     stage_positions_batched<T, RB, false>(L.xx, L.xy, L.xz, nullptr, x, m.nV, d, m.v_lin);
@@ -248,7 +249,7 @@ __device__ __forceinline__ void stage_arap(const DevMesh& m, const PatchDesc& d,
 template <uint32_t T>
 __global__ __launch_bounds__(T) void k_arap_rotations(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                       const uint32_t* __restrict__ rs,
-                                                      const uint32_t* __restrict__ ent,
+                                                      const uint16_t* __restrict__ ent,
                                                       const float* __restrict__ rest, const float* __restrict__ x,
                                                       const float* __restrict__ w, float* __restrict__ R)
 {
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(T) void k_arap_rotations(DevMesh m, const uint32_t*
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         float S[3][3] = {{0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}, {0.f, 0.f, 0.f}};
         for (uint32_t k = L.rs[v]; k < L.rs[v + 1]; ++k) {
-            const uint32_t e = L.ent[k] & 0xFFFFu, y = L.ev[e];
+            const uint32_t e = L.ent[k] >> 1, y = L.ev[e];
             const uint32_t j = (lo16(y) == v) ? hi16(y) : lo16(y);
             const float    wt = L.w[e];
             const float    a[3] = {L.px[v] - L.px[j], L.py[v] - L.py[j], L.pz[v] - L.pz[j]};
@@ -302,7 +303,7 @@ __device__ __forceinline__ f3 arap_residual(const PlanAR& L, uint32_t i, uint32_
 template <uint32_t T>
 __global__ __launch_bounds__(T) void k_arap_energy_grad(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                         const uint32_t* __restrict__ rs,
-                                                        const uint32_t* __restrict__ ent,
+                                                        const uint16_t* __restrict__ ent,
                                                         const float* __restrict__ rest, const float* __restrict__ x,
                                                         const float* __restrict__ w, const float* __restrict__ R,
                                                         float* __restrict__ partial, float* __restrict__ grad)
@@ -327,7 +328,7 @@ __global__ __launch_bounds__(T) void k_arap_energy_grad(DevMesh m, const uint32_
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         float gx = 0.f, gy = 0.f, gz = 0.f;
         for (uint32_t k = L.rs[v]; k < L.rs[v + 1]; ++k) {
-            const uint32_t e = L.ent[k] & 0xFFFFu, y = L.ev[e];
+            const uint32_t e = L.ent[k] >> 1, y = L.ev[e];
             const f3       r = arap_residual(L, lo16(y), hi16(y));
             const float    s = (lo16(y) == v ? 2.f : -2.f) * L.w[e];
             gx += s * r.x;

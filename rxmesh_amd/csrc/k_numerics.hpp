@@ -128,18 +128,20 @@ __device__ __forceinline__ FaceCot face_cot(f3 P0, f3 P1, f3 P2)
 struct PlanCOT
 {
     float4*   pos;
-    float4*   he;
-    uint32_t *lin, *ev, *rs, *ent;
+    float2 *  cot, *vor;  // per edge: the two half-edges' opposite cotangents / start Voronoi terms
+    uint32_t *lin, *ev, *rs;
+    uint16_t* ent;
 };
 __host__ __device__ inline PlanCOT plan_cot(Bump& b, uint32_t nv, uint32_t ne, uint32_t nvo, uint32_t nent)
 {
     PlanCOT p{};
     p.pos = b.alloc<float4>(nv);
-    p.he  = b.alloc<float4>(ne);
+    p.cot = b.alloc<float2>(ne);
+    p.vor = b.alloc<float2>(ne);
     p.lin = b.alloc<uint32_t>(nv);
     p.ev  = b.alloc<uint32_t>(r4(ne));
     p.rs  = b.alloc<uint32_t>(nvo + 1);
-    p.ent = b.alloc<uint32_t>(nent);
+    p.ent = b.alloc<uint16_t>(nent);
     return p;
 }
 
@@ -160,6 +162,26 @@ __device__ __forceinline__ void stage_words(uint32_t* __restrict__ dst, const ui
         for (uint32_t r = 0; r < R; ++r) {
             const uint32_t i = b0 + r * T + threadIdx.x;
             if (i < n) dst[i] = x[r] - sub;
+        }
+    }
+}
+
+// Copy n u16 global -> LDS (any alignment), R loads of a thread in flight
+// before its stores.
+template <uint32_t T, uint32_t R>
+__device__ __forceinline__ void stage_halves(uint16_t* __restrict__ dst, const uint16_t* __restrict__ src, uint32_t n)
+{
+    for (uint32_t b0 = 0; b0 < n; b0 += R * T) {
+        uint16_t x[R];
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t i = b0 + r * T + threadIdx.x;
+            x[r]             = (i < n) ? src[i] : uint16_t(0);
+        }
+#pragma unroll
+        for (uint32_t r = 0; r < R; ++r) {
+            const uint32_t i = b0 + r * T + threadIdx.x;
+            if (i < n) dst[i] = x[r];
         }
     }
 }
@@ -226,7 +248,7 @@ __device__ __forceinline__ void stage_vertices(const DevMesh& m, const PatchDesc
 template <uint32_t T>
 __global__ __launch_bounds__(T) void k_cot_assemble(DevMesh m, const uint32_t* __restrict__ vv_base,
                                                     const uint32_t* __restrict__ rs,
-                                                    const uint32_t* __restrict__ ent, const float* __restrict__ pos,
+                                                    const uint16_t* __restrict__ ent, const float* __restrict__ pos,
                                                     float dt, int32_t* __restrict__ row_ptr,
                                                     int32_t* __restrict__ col, float* __restrict__ val,
                                                     float* __restrict__ mass)
@@ -239,19 +261,29 @@ __global__ __launch_bounds__(T) void k_cot_assemble(DevMesh m, const uint32_t* _
     const uint32_t  bse  = vb + d.vp;  // first entry of the patch's rows
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanCOT   L = plan_cot(b, d.nv, d.ne, d.nvo, nent);
+    // the FE words of this thread's faces, loaded with the staging's loads
+    constexpr uint32_t RF = (1024u + T - 1u) / T;
+    const uint16_t*    fe = m.fe + 3u * d.f_off;
+    uint32_t           fw[RF][3];
+#pragma unroll
+    for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j) fw[r][j] = (threadIdx.x + r * T < d.nf) ? fe[3 * (threadIdx.x + r * T) + j] : 0u;
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_words<T, 8>(L.ent, ent + vb, nent);
+    stage_halves<T, 8>(L.ent, ent + vb, nent);
     stage_words<T, 2>(L.rs, rs + d.vp, d.nvo + 1, vb);
     stage_vertices<T>(m, d, pos, L.pos, L.lin);
-    for (uint32_t i = threadIdx.x; i < d.ne; i += T) L.he[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (uint32_t i = threadIdx.x; i < d.ne; i += T) {
+        L.cot[i] = make_float2(0.f, 0.f);
+        L.vor[i] = make_float2(0.f, 0.f);
+    }
     __syncthreads();
-    // per face (FE read straight from the slab): the terms of its half-edges
-    const uint16_t* fe = m.fe + 3u * d.f_off;
-    float2*         hh = reinterpret_cast<float2*>(L.he);  // hh[2e + dir]
-    for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
-        uint32_t x[3], a[3];
-#pragma unroll
-        for (int j = 0; j < 3; ++j) x[j] = fe[3 * f + j];
+    // per face: the terms of its half-edges, stored at the half-edge's FE
+    // word 2 e + dir (float half dir of edge e's pair)
+    float* hc = reinterpret_cast<float*>(L.cot);
+    float* hv = reinterpret_cast<float*>(L.vor);
+    auto   face = [&](const uint32_t (&x)[3]) {
+        uint32_t a[3];
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
             const uint32_t e = L.ev[x[j] >> 1];
@@ -261,7 +293,17 @@ __global__ __launch_bounds__(T) void k_cot_assemble(DevMesh m, const uint32_t* _
         const FaceCot c  = face_cot(f3{q0.x, q0.y, q0.z}, f3{q1.x, q1.y, q1.z}, f3{q2.x, q2.y, q2.z});
         // slot j runs a_j -> a_{j+1}: opposite corner a_{j+2}, start a_j
 #pragma unroll
-        for (int j = 0; j < 3; ++j) hh[x[j]] = make_float2(c.cot[(j + 2) % 3], c.vor[j]);
+        for (int j = 0; j < 3; ++j) {
+            hc[x[j]] = c.cot[(j + 2) % 3];
+            hv[x[j]] = c.vor[j];
+        }
+    };
+#pragma unroll
+    for (uint32_t r = 0; r < RF; ++r)
+        if (threadIdx.x + r * T < d.nf) face(fw[r]);
+    for (uint32_t f = threadIdx.x + RF * T; f < d.nf; f += T) {
+        const uint32_t x[3] = {fe[3 * f], fe[3 * f + 1], fe[3 * f + 2]};
+        face(x);
     }
     __syncthreads();
     // rows: diagonal and mass (entries in pattern order)
@@ -269,11 +311,11 @@ __global__ __launch_bounds__(T) void k_cot_assemble(DevMesh m, const uint32_t* _
         const uint32_t k0 = L.rs[v], k1 = L.rs[v + 1];
         float          sum_w = 0.f, vw = 0.f;
         for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t e = L.ent[k] & 0xFFFFu;
-            const float4   h = L.he[e];
-            const float    w = h.x + h.z;
+            const uint32_t u = L.ent[k];  // the half-edge leaving v
+            const float2   c = L.cot[u >> 1];
+            const float    w = c.x + c.y;
             sum_w += ((w >= 0.f) ? w : 0.f) * dt;
-            vw += (lo16(L.ev[e]) == v) ? h.y : h.w;
+            vw += hv[u];
         }
         const float    ms = 1.0f / (0.5f / vw);
         const uint32_t o  = bse + k0 + v;
@@ -285,11 +327,11 @@ __global__ __launch_bounds__(T) void k_cot_assemble(DevMesh m, const uint32_t* _
     if (threadIdx.x == 0) row_ptr[d.vp + d.nvo] = static_cast<int32_t>(bse + nent + d.nvo);
     // off-diagonal entries, contiguous over the patch's output range
     for (uint32_t k = threadIdx.x; k < nent; k += T) {
-        const uint32_t s = L.ent[k], e = s & 0xFFFFu, v = s >> 16, x = L.ev[e];
-        const float4   h = L.he[e];
-        const float    w = h.x + h.z;
+        const uint32_t u = L.ent[k], e = u >> 1, x = L.ev[e], v = (u & 1u) ? hi16(x) : lo16(x);
+        const float2   c = L.cot[e];
+        const float    w = c.x + c.y;
         const uint32_t o = bse + k + v + 1;
-        col[o]           = static_cast<int32_t>(L.lin[(lo16(x) == v) ? hi16(x) : lo16(x)]);
+        col[o]           = static_cast<int32_t>(L.lin[(u & 1u) ? lo16(x) : hi16(x)]);
         val[o]           = -(((w >= 0.f) ? w : 0.f) * dt);
     }
 }
@@ -624,7 +666,8 @@ struct PlanMF
 {
     float4*   pos;
     float4*   he;
-    uint32_t *lin, *ev, *rs, *ent;
+    uint32_t *lin, *ev, *rs;
+    uint16_t* ent;
     float *   vx, *vy, *vz, *red;
 };
 __host__ __device__ inline PlanMF plan_mf(Bump& b, uint32_t nv, uint32_t ne, uint32_t nvo, uint32_t nent,
@@ -638,7 +681,7 @@ __host__ __device__ inline PlanMF plan_mf(Bump& b, uint32_t nv, uint32_t ne, uin
     }
     p.ev  = b.alloc<uint32_t>(r4(ne));
     p.rs  = b.alloc<uint32_t>(nvo + 1);
-    p.ent = b.alloc<uint32_t>(nent);
+    p.ent = b.alloc<uint16_t>(nent);
     p.vx  = b.alloc<float>(nv);
     p.vy  = b.alloc<float>(nv);
     p.vz  = b.alloc<float>(nv);
@@ -664,7 +707,7 @@ __host__ __device__ inline PlanMF plan_mf(Bump& b, uint32_t nv, uint32_t ne, uin
 // written for the owned rows and <y, p> summed per workgroup into partial[p].
 template <uint32_t T, int MODE, bool UNIFORM, bool PRECOND>
 __global__ __launch_bounds__(T) void k_mf_apply(DevMesh m, const uint32_t* __restrict__ vv_base,
-                                                const uint32_t* __restrict__ rs, const uint32_t* __restrict__ ent,
+                                                const uint32_t* __restrict__ rs, const uint16_t* __restrict__ ent,
                                                 const float* __restrict__ coords, float dt,
                                                 const float* __restrict__ x, const float* __restrict__ r,
                                                 float* __restrict__ diag, float* __restrict__ mass,
@@ -681,7 +724,7 @@ __global__ __launch_bounds__(T) void k_mf_apply(DevMesh m, const uint32_t* __res
     Bump            b(reinterpret_cast<char*>(smem));
     const PlanMF    L = plan_mf(b, d.nv, d.ne, d.nvo, nent, UNIFORM, T);
     stage_u32(L.ev, m.ev + d.e_off, d.ne);
-    stage_words<T, 8>(L.ent, ent + vb, nent);
+    stage_halves<T, 8>(L.ent, ent + vb, nent);
     stage_words<T, 2>(L.rs, rs + d.vp, d.nvo + 1, vb);
     if constexpr (!UNIFORM) {
         stage_vertices<T>(m, d, coords, L.pos, L.lin);
@@ -732,7 +775,7 @@ __global__ __launch_bounds__(T) void k_mf_apply(DevMesh m, const uint32_t* __res
         const uint32_t k0 = L.rs[v], k1 = L.rs[v + 1];
         float          sum_w = 0.f, vw = 0.f, ax = 0.f, ay = 0.f, az = 0.f;
         for (uint32_t k = k0; k < k1; ++k) {
-            const uint32_t e = L.ent[k] & 0xFFFFu, xe = L.ev[e];
+            const uint32_t e = L.ent[k] >> 1, xe = L.ev[e];
             const uint32_t q = (lo16(xe) == v) ? hi16(xe) : lo16(xe);
             float          w = 1.f;
             if constexpr (!UNIFORM) {
