@@ -258,10 +258,11 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *             rxm_query_vv_vf): 0 = LDS-atomic histogram and slot fill
  *             followed by a sort of every row; 1 = atomic-free stable
  *             counting sort (wave ballots + mbcnt ranks + one block scan;
- *             rows come out in order, no row sort); 2 (default) = as 0, with
- *             rxm_query_vv_vf sorting both relations in one counting sort
- *             (keys per edge and per face, rows sorted in registers, one
- *             coalesced write of both CSRs).
+ *             rows come out in order, no row sort); 2 (default) = one pass
+ *             per launch for VV / VE / VF / EF and for rxm_query_vv_vf (both
+ *             relations in one counting sort): keys per edge or per face
+ *             held in registers, rows sorted in registers, one coalesced
+ *             write; FF as 0.
  * Settings of kernels 0-4 and 6-10 give bit-identical results; kernel 5 changes
  * the summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);

@@ -94,7 +94,7 @@ struct rxm_mesh
     bool            system_runs_ok = false;    // check_system_runs passed
     uint32_t        query_stable   = 2;        // transposes: 0 LDS atomics + row sort, 1 atomic-free stable sort, 2 as 0 with the fused VV + VF launch in one pass
     uint32_t        vvf_nslots     = 0;        // largest VV + VF entry count of a patch (0: not computed yet)
-    uint32_t        max_nrv = 0, max_nrf = 0;  // largest ribbon vertex / face count of a patch
+    uint32_t        max_nrv = 0, max_nre = 0, max_nrf = 0;  // largest ribbon vertex / edge / face count of a patch
     int             num_cus        = 0;
     // batch (disjoint union of meshes): per-mesh id / patch offsets
     std::vector<uint32_t> b_v, b_e, b_f, b_p;
@@ -238,6 +238,23 @@ uint32_t op_max_patch(rxm_mesh* m, int op)
     uint32_t    mx = 0;
     for (uint32_t p = 0; p < m->h.P; ++p) mx = std::max(mx, b[p + 1] - b[p]);
     return m->h_maxrow[op] = mx;
+}
+
+// Sizes of the one-pass query carves, computed once per mesh: the largest
+// VV + VF entry count of a patch and its largest ribbon vertex, edge and
+// face counts.
+void vvf_slots(rxm_mesh* m)
+{
+    if (m->vvf_nslots) return;
+    const auto& vb = op_base(m, RXM_OP_VV);
+    const auto& fb = op_base(m, RXM_OP_VF);
+    for (uint32_t p = 0; p < m->h.P; ++p) {
+        m->vvf_nslots = std::max(m->vvf_nslots, vb[p + 1] - vb[p] + fb[p + 1] - fb[p]);
+        m->max_nrv    = std::max(m->max_nrv, m->h.nv[p] - m->h.nvo[p]);
+        m->max_nre    = std::max(m->max_nre, m->h.ne[p] - m->h.neo[p]);
+        m->max_nrf    = std::max(m->max_nrf, m->h.nf[p] - m->h.nfo[p]);
+    }
+    m->vvf_nslots = std::max(m->vvf_nslots, 1u);
 }
 
 // The patch-local MCF and CG kernels stage patch p's run of the cotangent
@@ -914,6 +931,29 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
                                      : go(k_query_vx_oriented<512, false>, 512);
         return (op == RXM_OP_VV) ? go(k_query_vx_oriented<256, true>, 256) : go(k_query_vx_oriented<256, false>, 256);
     }
+    // VV / VE / VF / EF in the one-pass form (knob 10 = 2, the default) where
+    // the patches fit its carve (VV / VE slots are entry indices 2 e + end in
+    // 16 bits)
+    if (m->query_stable == 2 && (op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF || op == RXM_OP_EF)) {
+        vvf_slots(m);
+        const bool     ef     = op == RXM_OP_EF;
+        const uint32_t nrid   = (op == RXM_OP_VV) ? m->max_nrv : ((op == RXM_OP_VE) ? m->max_nre : m->max_nrf);
+        const uint32_t nslots = std::max(op_max_patch(m, op), 1u);
+        Bump           bx(nullptr);
+        plan_x1(bx, ef ? h.max_neo : h.max_nvo, ef ? 0u : h.max_ne, nslots, nrid);
+        const bool entry16 = op == RXM_OP_VF || ef || 2u * h.max_ne <= 0x10000u;
+        if (bx.bytes() <= kMaxLdsBytes && entry16) {
+            auto go = [&](auto kernel) {
+                return launch_patches_xcd(kernel, T, bx.bytes(), stream, m, m->dm, nslots, nrid, base, row_ptr, col);
+            };
+            switch (op) {
+                case RXM_OP_VV: return go(k_query2_x1<T, kX1VV>);
+                case RXM_OP_VE: return go(k_query2_x1<T, kX1VE>);
+                case RXM_OP_VF: return go(k_query2_x1<T, kX1VF>);
+                default: return go(k_query2_x1<T, kX1EF>);
+            }
+        }
+    }
     // transposed relations: atomic-free stable counting sort, or the LDS
     // atomic histogram + row sort (knob 10); identical rows. The stable
     // sort's (key, wave) table takes T / 64 times the histogram's LDS, so a
@@ -960,16 +1000,7 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
     if (rc) return rc;
     constexpr uint32_t T = kQueryThreads;
     if (m->query_stable == 2) {  // one counting sort for both relations
-        if (!m->vvf_nslots) {
-            const auto& vb = op_base(m, RXM_OP_VV);
-            const auto& fb = op_base(m, RXM_OP_VF);
-            for (uint32_t p = 0; p < m->h.P; ++p) {
-                m->vvf_nslots = std::max(m->vvf_nslots, vb[p + 1] - vb[p] + fb[p + 1] - fb[p]);
-                m->max_nrv    = std::max(m->max_nrv, m->h.nv[p] - m->h.nvo[p]);
-                m->max_nrf    = std::max(m->max_nrf, m->h.nf[p] - m->h.nfo[p]);
-            }
-            m->vvf_nslots = std::max(m->vvf_nslots, 1u);
-        }
+        vvf_slots(m);
         Bump b1(nullptr);
         plan_vvf1(b1, m->h.max_nvo, m->h.max_ne, m->vvf_nslots, m->max_nrv, m->max_nrf);
         // VV slots hold entry indices 2 e + end in 16 bits

@@ -287,8 +287,8 @@ __global__ __launch_bounds__(T) void k_query2_vf(DevMesh m, const uint32_t* __re
 // share their staged topology (EV, FE and both linear-id tables arrive in
 // one round trip), then the VV transpose and the VF transpose run in turn
 // in one slot array; each CSR is written exactly as k_query2_vx<T, true> and
-// k_query2_vf<T> write it. (One counting sort over a combined column space
-// for both relations was measured slower: 118 against 84 us on C2s.)
+// k_query2_vf<T> write it. The default form, k_query2_vv_vf1, sorts both
+// relations in one counting sort instead.
 struct PlanVVF
 {
     uint32_t *ev, *off, *slots, *wave, *lin_v, *lin_f;
@@ -650,6 +650,214 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf1(DevMesh m, uint32_t nslots,
         } else {
             vf_col[bvf + i] = static_cast<int32_t>((s < d.nfo) ? d.fp + s : L.rf[s - d.nfo]);
         }
+    }
+}
+
+// ---- one relation in one counting sort ---------------------------------------
+// VV, VE, VF or EF in the form of k_query2_vv_vf1: keys per edge (VV / VE:
+// both endpoints of one EV word) or per face (VF: three corners; EF: three
+// edges; FE words loaded into registers), held in registers from histogram
+// to fill; only the ribbon's linear ids of the output's kind are staged
+// (owned ids follow from the prefixes); each row sorted in registers by the
+// smallest network that covers the longest row of its wave; one coalesced
+// write. Columns are the owned vertices (VV / VE / VF) or edges (EF). EF
+// needs no EV: an FE word names its edge. Same CSR as k_query2_vx /
+// k_query2_vf / k_query2_ef.
+enum : uint32_t { kX1VV = 0, kX1VE = 1, kX1VF = 2, kX1EF = 3 };
+
+struct PlanX1
+{
+    uint32_t *ev, *off, *wave, *rid;
+    uint16_t* slots;
+};
+// ncol: the largest column count of a patch (owned vertices, or owned edges
+// for EF); ne: its largest edge count (0 for EF: no EV); nslots: the largest
+// entry count; nrid: the largest ribbon count of the output's element kind
+__host__ __device__ inline PlanX1 plan_x1(Bump& b, uint32_t ncol, uint32_t ne, uint32_t nslots, uint32_t nrid)
+{
+    PlanX1 p{};
+    p.ev    = b.alloc<uint32_t>(r4(ne));
+    p.off   = b.alloc<uint32_t>(ncol + 1);
+    p.slots = b.alloc<uint16_t>(nslots);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.rid   = b.alloc<uint32_t>(nrid);
+    return p;
+}
+
+template <uint32_t T, uint32_t REL>
+__global__ __launch_bounds__(T) void k_query2_x1(DevMesh m, uint32_t nslots, uint32_t nrid,
+                                                 const uint32_t* __restrict__ base, int32_t* __restrict__ row_ptr,
+                                                 int32_t* __restrict__ col)
+{
+    constexpr bool     EFK = REL == kX1EF;          // EF: keys are the faces' edges
+    constexpr bool     FK  = REL == kX1VF || EFK;  // keys per face
+    constexpr uint32_t RE = (1536u + T - 1u) / T;  // edges held in registers
+    constexpr uint32_t RF = (1024u + T - 1u) / T;  // faces held in registers
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const uint32_t  ncol = EFK ? d.neo : d.nvo, rp0 = EFK ? d.ep : d.vp;  // columns, their linear-id prefix
+    const PlanX1    L    = plan_x1(b, ncol, EFK ? 0u : d.ne, nslots, nrid);
+    const uint32_t  nvo  = d.nvo, t = threadIdx.x;
+    const uint16_t* fs   = m.fe + 3u * d.f_off;
+    const bool      held = FK ? d.nf <= RF * T : d.ne <= RE * T;
+    // global loads first: this thread's FE words (VF), the ribbon ids, EV
+    uint32_t fw[RF][3];
+    if constexpr (FK) {
+        if (held) {
+#pragma unroll
+            for (uint32_t r = 0; r < RF; ++r) {
+                const uint32_t f = t + r * T;
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j) fw[r][j] = (f < d.nf) ? fs[3 * f + j] : 0u;
+            }
+        }
+    }
+    // ribbon ids of the output's elements: vertices (VV), edges (VE), faces (VF / EF)
+    const uint32_t* rs = (REL == kX1VV) ? m.v_lin + d.v_off + nvo
+                                        : ((REL == kX1VE) ? m.e_lin + d.e_off + d.neo : m.f_lin + d.f_off + d.nfo);
+    const uint32_t  nr = (REL == kX1VV) ? d.nv - nvo : ((REL == kX1VE) ? d.ne - d.neo : d.nf - d.nfo);
+    const uint32_t  x0 = (t < nr) ? rs[t] : 0u;
+    if constexpr (!EFK)
+        stage_topology<T>(L.ev, m.ev + d.e_off, d.ne, nullptr, nullptr, 0, nullptr, 0, 0, 0, nullptr);
+    if (t < nr) L.rid[t] = x0;
+    for (uint32_t i = t + T; i < nr; i += T) L.rid[i] = rs[i];
+    for (uint32_t c = t; c <= ncol; c += T) L.off[c] = 0;
+    __syncthreads();
+    // the column of an FE word: the local vertex it starts at (VF), its edge (EF)
+    auto fe_end = [&](uint32_t x) {
+        if constexpr (EFK) {
+            return x >> 1;
+        } else {
+            const uint32_t e = L.ev[x >> 1];
+            return (x & 1u) ? hi16(e) : lo16(e);
+        }
+    };
+    uint32_t xe[FK ? 1u : RE], kf[FK ? RF : 1u][3];
+    if constexpr (FK) {
+        if (held) {
+#pragma unroll
+            for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j)  // the FE word of a face past nf is 0: a valid word
+                    kf[r][j] = (t + r * T < d.nf) ? fe_end(fw[r][j]) : 0xFFFFu;
+#pragma unroll
+            for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j)
+                    if (kf[r][j] < ncol) atomicAdd(&L.off[kf[r][j] + 1u], 1u);
+        } else {
+            for (uint32_t f = t; f < d.nf; f += T)
+                for (uint32_t j = 0; j < 3; ++j) {
+                    const uint32_t v = fe_end(fs[3 * f + j]);
+                    if (v < ncol) atomicAdd(&L.off[v + 1u], 1u);
+                }
+        }
+    } else {
+        if (held) {
+#pragma unroll
+            for (uint32_t r = 0; r < RE; ++r) {
+                const uint32_t e = t + r * T;
+                const uint32_t x = L.ev[min(e, d.ne - 1u)];
+                xe[r]            = (e < d.ne) ? x : 0xFFFFFFFFu;
+            }
+#pragma unroll
+            for (uint32_t r = 0; r < RE; ++r) {
+                if (lo16(xe[r]) < nvo) atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u);
+                if (hi16(xe[r]) < nvo) atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u);
+            }
+        } else {
+            for (uint32_t e = t; e < d.ne; e += T) {
+                const uint32_t x = L.ev[e];
+                if (lo16(x) < nvo) atomicAdd(&L.off[lo16(x) + 1u], 1u);
+                if (hi16(x) < nvo) atomicAdd(&L.off[hi16(x) + 1u], 1u);
+            }
+        }
+    }
+    scan_excl<T>(L.off + 1, ncol, L.wave);
+    // fill: off[c + 1] advances from the start of column c to that of c + 1;
+    // slot = entry index 2 e + end (VV / VE) or local face (VF / EF)
+    if constexpr (FK) {
+        if (held) {
+#pragma unroll
+            for (uint32_t r = 0; r < RF; ++r)
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j)
+                    if (kf[r][j] < ncol)
+                        L.slots[atomicAdd(&L.off[kf[r][j] + 1u], 1u)] = static_cast<uint16_t>(t + r * T);
+        } else {
+            for (uint32_t f = t; f < d.nf; f += T)
+                for (uint32_t j = 0; j < 3; ++j) {
+                    const uint32_t v = fe_end(fs[3 * f + j]);
+                    if (v < ncol) L.slots[atomicAdd(&L.off[v + 1u], 1u)] = static_cast<uint16_t>(f);
+                }
+        }
+    } else {
+        if (held) {
+#pragma unroll
+            for (uint32_t r = 0; r < RE; ++r) {
+                const uint32_t i = 2u * (t + r * T);
+                if (lo16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i);
+                if (hi16(xe[r]) < nvo)
+                    L.slots[atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i + 1u);
+            }
+        } else {
+            for (uint32_t e = t; e < d.ne; e += T) {
+                const uint32_t x = L.ev[e];
+                if (lo16(x) < nvo) L.slots[atomicAdd(&L.off[lo16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e);
+                if (hi16(x) < nvo) L.slots[atomicAdd(&L.off[hi16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e + 1u);
+            }
+        }
+    }
+    __syncthreads();
+    const uint32_t ntot = L.off[ncol], bse = base[p];
+    if (t == 0) row_ptr[rp0 + ncol] = static_cast<int32_t>(bse + ntot);
+    for (uint32_t v = t; v < ncol; v += T) {
+        const uint32_t o = L.off[v], k = L.off[v + 1] - o;
+        row_ptr[rp0 + v] = static_cast<int32_t>(bse + o);
+        auto     put = [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); };
+        uint32_t a[8];
+        if (!__builtin_amdgcn_ballot_w64(k > 4u)) {
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j) a[j] = (j < k) ? L.slots[o + j] : 0xFFFFFFFFu;
+            net_sort<4>(a);
+#pragma unroll
+            for (uint32_t j = 0; j < 4; ++j)
+                if (j < k) put(j, a[j]);
+        } else if (!__builtin_amdgcn_ballot_w64(k > 6u)) {
+#pragma unroll
+            for (uint32_t j = 0; j < 6; ++j) a[j] = (j < k) ? L.slots[o + j] : 0xFFFFFFFFu;
+            net_sort<6>(a);
+#pragma unroll
+            for (uint32_t j = 0; j < 6; ++j)
+                if (j < k) put(j, a[j]);
+        } else if (k <= 8u) {
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j) a[j] = (j < k) ? L.slots[o + j] : 0xFFFFFFFFu;
+            net_sort<8>(a);
+#pragma unroll
+            for (uint32_t j = 0; j < 8; ++j)
+                if (j < k) put(j, a[j]);
+        } else {
+            sort_row(L.slots + o, k, put);
+        }
+    }
+    __syncthreads();
+    for (uint32_t i = t; i < ntot; i += T) {
+        const uint32_t s = L.slots[i];
+        uint32_t       l;
+        if constexpr (REL == kX1VV) {
+            const uint32_t w = ev_other(L.ev, s);
+            l                = (w < nvo) ? d.vp + w : L.rid[w - nvo];
+        } else if constexpr (REL == kX1VE) {
+            const uint32_t e = s >> 1;
+            l                = (e < d.neo) ? d.ep + e : L.rid[e - d.neo];
+        } else {
+            l = (s < d.nfo) ? d.fp + s : L.rid[s - d.nfo];
+        }
+        col[bse + i] = static_cast<int32_t>(l);
     }
 }
 
