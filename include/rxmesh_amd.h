@@ -259,10 +259,10 @@ rxm_status rxm_vv_plan_build(rxm_mesh* mesh, void* stream);
  *             followed by a sort of every row; 1 = atomic-free stable
  *             counting sort (wave ballots + mbcnt ranks + one block scan;
  *             rows come out in order, no row sort); 2 (default) = one pass
- *             per launch for VV / VE / VF / EF and for rxm_query_vv_vf (both
- *             relations in one counting sort): keys per edge or per face
- *             held in registers, rows sorted in registers, one coalesced
- *             write; FF as 0.
+ *             per launch for VV / VE / VF / EF / FF and for rxm_query_vv_vf
+ *             (both relations in one counting sort): keys per edge or per
+ *             face held in registers, rows sorted in registers, one
+ *             coalesced write.
  * Settings of kernels 0-4 and 6-10 give bit-identical results; kernel 5 changes
  * the summation order of a row (fp32 results within rounding). */
 rxm_status rxm_set_launch_threads(rxm_mesh* mesh, int kernel, int threads);

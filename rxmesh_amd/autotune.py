@@ -154,7 +154,7 @@ def tune_smooth_symdir(mesh, Mr, x, tmp, lam, eta, steps=2, reps=3, use_cache=Tr
 
 
 # transposes: 0 LDS-atomic histogram + row sort, 1 atomic-free ballot/mbcnt sort, 2 the one-pass form
-# (VV / VE / VF / EF each in one pass, the fused VV + VF launch in one counting sort; the default)
+# (VV / VE / VF / EF / FF each in one pass, the fused VV + VF launch in one counting sort; the default)
 QUERY_SORTS = (0, 1, 2)
 
 

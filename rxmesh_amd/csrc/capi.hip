@@ -954,6 +954,15 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
             }
         }
     }
+    if (m->query_stable == 2 && op == RXM_OP_FF) {  // FF in one pass
+        vvf_slots(m);
+        const uint32_t nout = std::max(op_max_patch(m, op), 1u);
+        Bump           bf(nullptr);
+        plan_ff1(bf, h.max_ne, h.max_nf, h.max_nfo, nout, m->max_nrf);
+        if (bf.bytes() <= kMaxLdsBytes)
+            return launch_patches_xcd(k_query2_ff1<T>, T, bf.bytes(), stream, m, m->dm, nout, m->max_nrf, base, row_ptr,
+                                      col);
+    }
     // transposed relations: atomic-free stable counting sort, or the LDS
     // atomic histogram + row sort (knob 10); identical rows. The stable
     // sort's (key, wave) table takes T / 64 times the histogram's LDS, so a

@@ -861,6 +861,147 @@ __global__ __launch_bounds__(T) void k_query2_x1(DevMesh m, uint32_t nslots, uin
     }
 }
 
+// FF in one pass: the EF counting sort over every local edge (keys = the
+// edges of the faces' FE words, held in registers), the edge rows sorted,
+// then each owned face lists the faces of its three edges but itself (edges
+// in FE order, faces ascending: the rows of k_query2_ef<T, true>) into LDS,
+// and one coalesced pass writes the CSR, turning local faces into linear
+// ids (ribbon faces' ids staged).
+struct PlanFF1
+{
+    uint32_t *off, *wave, *cur, *rid;
+    uint16_t *slots, *out;
+};
+// nout: the largest FF entry count of a patch; nrid: its largest ribbon face count
+__host__ __device__ inline PlanFF1 plan_ff1(Bump& b, uint32_t ne, uint32_t nf, uint32_t nfo, uint32_t nout,
+                                            uint32_t nrid)
+{
+    PlanFF1 p{};
+    p.off   = b.alloc<uint32_t>(ne + 1);
+    p.slots = b.alloc<uint16_t>(3 * nf);
+    p.wave  = b.alloc<uint32_t>(20);
+    p.cur   = b.alloc<uint32_t>(nfo + 1);
+    p.out   = b.alloc<uint16_t>(nout);
+    p.rid   = b.alloc<uint32_t>(nrid);
+    return p;
+}
+
+template <uint32_t T>
+__global__ __launch_bounds__(T) void k_query2_ff1(DevMesh m, uint32_t nout, uint32_t nrid,
+                                                  const uint32_t* __restrict__ base, int32_t* __restrict__ row_ptr,
+                                                  int32_t* __restrict__ col)
+{
+    constexpr uint32_t RF = (1024u + T - 1u) / T;  // faces held in registers
+    extern __shared__ uint4 smem[];
+    const uint32_t  p = patch_of_block(m.P);  // XCD-contiguous patch ranges
+    if (p >= m.P) return;
+    const PatchDesc d = load_desc(m.patch, p);
+    Bump            b(reinterpret_cast<char*>(smem));
+    const PlanFF1   L    = plan_ff1(b, d.ne, d.nf, d.nfo, nout, nrid);
+    const uint32_t  t    = threadIdx.x;
+    const uint16_t* fs   = m.fe + 3u * d.f_off;
+    const uint32_t* fl   = m.f_lin + d.f_off + d.nfo;
+    const bool      held = d.nf <= RF * T;
+    // global loads first: this thread's FE words, the ribbon faces' ids
+    uint32_t fw[RF][3];
+    if (held) {
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r) {
+            const uint32_t f = t + r * T;
+#pragma unroll
+            for (uint32_t j = 0; j < 3; ++j) fw[r][j] = (f < d.nf) ? fs[3 * f + j] : 0u;
+        }
+    }
+    const uint32_t nr = d.nf - d.nfo;
+    const uint32_t x0 = (t < nr) ? fl[t] : 0u;
+    for (uint32_t c = t; c <= d.ne; c += T) L.off[c] = 0;
+    if (t < nr) L.rid[t] = x0;
+    for (uint32_t i = t + T; i < nr; i += T) L.rid[i] = fl[i];
+    __syncthreads();
+    // EF histogram over every local face's three edges (column = local edge)
+    if (held) {
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r)
+            if (t + r * T < d.nf)
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j) atomicAdd(&L.off[(fw[r][j] >> 1) + 1u], 1u);
+    } else {
+        for (uint32_t f = t; f < d.nf; f += T)
+            for (uint32_t j = 0; j < 3; ++j) atomicAdd(&L.off[(fs[3 * f + j] >> 1) + 1u], 1u);
+    }
+    scan_excl<T>(L.off + 1, d.ne, L.wave);
+    if (held) {
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r)
+            if (t + r * T < d.nf)
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j)
+                    L.slots[atomicAdd(&L.off[(fw[r][j] >> 1) + 1u], 1u)] = static_cast<uint16_t>(t + r * T);
+    } else {
+        for (uint32_t f = t; f < d.nf; f += T)
+            for (uint32_t j = 0; j < 3; ++j)
+                L.slots[atomicAdd(&L.off[(fs[3 * f + j] >> 1) + 1u], 1u)] = static_cast<uint16_t>(f);
+    }
+    __syncthreads();
+    // every edge row ascending (two faces on a manifold edge: one compare)
+    for (uint32_t e = t; e < d.ne; e += T) {
+        const uint32_t o = L.off[e], k = L.off[e + 1] - o;
+        if (k == 2u) {
+            const uint16_t a0 = L.slots[o], a1 = L.slots[o + 1];
+            if (a1 < a0) {
+                L.slots[o]     = a1;
+                L.slots[o + 1] = a0;
+            }
+        } else if (k > 2u) {
+            sort_row(L.slots + o, k, [&](uint32_t j, uint32_t s) { L.slots[o + j] = static_cast<uint16_t>(s); });
+        }
+    }
+    __syncthreads();
+    // owned face f (edges e[0..2] in FE order): its FF row length, then its
+    // row into LDS at its offset; the held faces' edges come from registers
+    auto each_owned = [&](auto fn) {
+#pragma unroll
+        for (uint32_t r = 0; r < RF; ++r) {
+            const uint32_t f = t + r * T;
+            if (f < d.nfo) {
+                uint32_t e[3];
+#pragma unroll
+                for (uint32_t j = 0; j < 3; ++j) e[j] = (held ? fw[r][j] : uint32_t(fs[3 * f + j])) >> 1;
+                fn(f, e);
+            }
+        }
+        for (uint32_t f = t + RF * T; f < d.nfo; f += T) {
+            uint32_t e[3];
+            for (uint32_t j = 0; j < 3; ++j) e[j] = uint32_t(fs[3 * f + j]) >> 1;
+            fn(f, e);
+        }
+    };
+    each_owned([&](uint32_t f, const uint32_t (&e)[3]) {
+        uint32_t c = 0;
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j) c += L.off[e[j] + 1] - L.off[e[j]] - 1u;
+        L.cur[f] = c;
+    });
+    const uint32_t total = scan_excl<T>(L.cur, d.nfo, L.wave);
+    const uint32_t bse   = base[p];
+    if (t == 0) row_ptr[d.fp + d.nfo] = static_cast<int32_t>(bse + total);
+    each_owned([&](uint32_t f, const uint32_t (&e)[3]) {
+        uint32_t k        = L.cur[f];
+        row_ptr[d.fp + f] = static_cast<int32_t>(bse + k);
+#pragma unroll
+        for (uint32_t j = 0; j < 3; ++j)
+            for (uint32_t s = L.off[e[j]]; s < L.off[e[j] + 1]; ++s) {
+                const uint32_t g = L.slots[s];
+                if (g != f) L.out[k++] = static_cast<uint16_t>(g);
+            }
+    });
+    __syncthreads();
+    for (uint32_t i = t; i < total; i += T) {
+        const uint32_t g = L.out[i];
+        col[bse + i]     = static_cast<int32_t>((g < d.nfo) ? d.fp + g : L.rid[g - d.nfo]);
+    }
+}
+
 // EF (IS_FF == false) or FF. STABLE as in k_query2_vx.
 template <uint32_t T, bool IS_FF, bool STABLE = false>
 __global__ __launch_bounds__(T) void k_query2_ef(DevMesh m, const uint32_t* __restrict__ base,

@@ -360,8 +360,8 @@ def test_stable_and_atomic_transposes_agree(name):
     histogram + row sort (0), the atomic-free stable counting sort (1:
     ballot / mbcnt ranks, rows in entry order) and the one-pass form (2, the
     default: per-edge / per-face keys held in registers, rows sorted in
-    registers; VV, VE, VF and EF by their own launch, VV + VF in one counting
-    sort by the fused launch; FF as 0): identical CSRs, bit for bit, for every
+    registers; VV, VE, VF, EF and FF by their own launch, VV + VF in one
+    counting sort by the fused launch): identical CSRs, bit for bit, for every
     transposed op and for the fused launch."""
     import torch
     V, F, m, o = case(name)
