@@ -1063,8 +1063,8 @@ rxm_status rxm_gaussian_curvature(rxm_mesh* m, const float* pos, float* gc, floa
     const int rc = ensure_face_plan(m, stream);
     if (rc) return rc;
     Bump b(nullptr);
-    plan_vnp(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
-    return launch_patches_xcd(k_gaussian_curvature<256>, 256, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
+    plan_gc(b, m->h.max_nv, m->h.max_nvo, m->h.max_nf, m->fp_max_nent);
+    return launch_patches_xcd(k_gaussian_curvature<512>, 512, b.bytes(), stream, m, m->dm, m->d_fp_fbase,
                               m->d_fp_rbase, m->d_fp_nbase, m->d_fp_fvl, m->d_fp_roff, m->d_fp_vf, pos, gc,
                               area_mixed);
 }

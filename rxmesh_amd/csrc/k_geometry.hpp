@@ -715,6 +715,32 @@ __device__ __forceinline__ float corner_angle(float s, float c)
     return a;
 }
 
+// LDS carve of the Gaussian-curvature kernel: the face plan (FV table, VF
+// rows), positions, and per face corner the angle and the mixed-area share
+// (3 floats each per face).
+struct PlanGC
+{
+    uint16_t *fvl, *ro, *vf;
+    float *   x, *y, *z, *ang, *amx;
+};
+__host__ __device__ inline PlanGC plan_gc(Bump& b, uint32_t nv, uint32_t nvo, uint32_t nf, uint32_t nent)
+{
+    PlanGC p{};
+    p.fvl = b.alloc<uint16_t>((3 * nf + 7u) & ~7u);
+    p.ro  = b.alloc<uint16_t>((nvo + 1u + 7u) & ~7u);
+    p.vf  = b.alloc<uint16_t>((nent + 7u) & ~7u);
+    p.x   = b.alloc<float>(nv);
+    p.y   = b.alloc<float>(nv);
+    p.z   = b.alloc<float>(nv);
+    p.ang = b.alloc<float>(3 * nf);
+    p.amx = b.alloc<float>(3 * nf);
+    return p;
+}
+
+// Face pass: every local face evaluates its triangle once (cross product,
+// corner dots, squared edge lengths) and stores each corner's angle and
+// mixed-area share; vertex pass: each owned vertex sums its VF row's corner
+// terms in ascending local face order.
 template <uint32_t T>
 __global__ __launch_bounds__(T) void k_gaussian_curvature(DevMesh m, const uint32_t* __restrict__ fbase,
                                                           const uint32_t* __restrict__ rbase,
@@ -731,52 +757,54 @@ __global__ __launch_bounds__(T) void k_gaussian_curvature(DevMesh m, const uint3
     const PatchDesc d   = load_desc(m.patch, p);
     const uint32_t  nb0 = nbase[p], nent = nbase[p + 1] - nb0;
     Bump            b(reinterpret_cast<char*>(smem));
-    const PlanVNP   L = plan_vnp(b, d.nv, d.nvo, d.nf, nent);
+    const PlanGC    L = plan_gc(b, d.nv, d.nvo, d.nf, nent);
     const uint32_t  N = m.nV;
     stage_u16x8<T>(L.fvl, fvl + fbase[p], 3 * d.nf);
     stage_u16x8<T>(L.ro, roff + rbase[p], d.nvo + 1);
     stage_u16x8<T>(L.vf, vf + nb0, nent);
-    stage_positions_batched<T, 512u / T, false>(L.x, L.y, L.z, nullptr, pos, N, d, m.v_lin);
+    stage_positions_batched<T, (512u + T - 1u) / T, false>(L.x, L.y, L.z, nullptr, pos, N, d, m.v_lin);
+    __syncthreads();
+    for (uint32_t f = threadIdx.x; f < d.nf; f += T) {
+        float px[3], py[3], pz[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t q = L.fvl[3 * f + j];
+            px[j]            = L.x[q];
+            py[j]            = L.y[q];
+            pz[j]            = L.z[q];
+        }
+        const float e1x = px[1] - px[0], e1y = py[1] - py[0], e1z = pz[1] - pz[0];
+        const float e2x = px[2] - px[0], e2y = py[2] - py[0], e2z = pz[2] - pz[0];
+        const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
+        const float s = sqrtf(cx * cx + cy * cy + cz * cz);
+        float       c[3], l[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const int   j1 = (j + 1) % 3, j2 = (j + 2) % 3;
+            const float ax = px[j1] - px[j], ay = py[j1] - py[j], az = pz[j1] - pz[j];
+            const float bx = px[j2] - px[j], by = py[j2] - py[j], bz = pz[j2] - pz[j];
+            c[j]           = ax * bx + ay * by + az * bz;
+            l[j]           = ax * ax + ay * ay + az * az;
+        }
+        // an angle exceeds pi / 2 exactly when its corner dot product is negative
+        const bool  ob = (c[0] < 0.f) || (c[1] < 0.f) || (c[2] < 0.f);
+        const float rs = __builtin_amdgcn_rcpf(s);  // cot = dot / |cross| for the non-obtuse case
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int i1 = (i + 1) % 3, i2 = (i + 2) % 3;
+            L.ang[3 * f + i] = corner_angle(s, c[i]);
+            L.amx[3 * f + i] = ob ? ((c[i] < 0.f) ? 0.25f * s : 0.125f * s)
+                                  : 0.125f * (l[i2] * (c[i1] * rs) + l[i] * (c[i2] * rs));
+        }
+    }
     __syncthreads();
     const float two_pi = 6.28318530717958647692f;
     for (uint32_t v = threadIdx.x; v < d.nvo; v += T) {
         float g = two_pi, am = 0.f;
         for (uint32_t k = L.ro[v]; k < L.ro[v + 1]; ++k) {
-            const uint32_t sl = L.vf[k], f = sl >> 2, i = sl & 3u;
-            uint32_t       q[3];
-            float          px[3], py[3], pz[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                q[j]  = L.fvl[3 * f + j];
-                px[j] = L.x[q[j]];
-                py[j] = L.y[q[j]];
-                pz[j] = L.z[q[j]];
-            }
-            const float e1x = px[1] - px[0], e1y = py[1] - py[0], e1z = pz[1] - pz[0];
-            const float e2x = px[2] - px[0], e2y = py[2] - py[0], e2z = pz[2] - pz[0];
-            const float cx = e1y * e2z - e1z * e2y, cy = e1z * e2x - e1x * e2z, cz = e1x * e2y - e1y * e2x;
-            const float s = sqrtf(cx * cx + cy * cy + cz * cz);
-            float       c[3], l[3];
-#pragma unroll
-            for (int j = 0; j < 3; ++j) {
-                const int   j1 = (j + 1) % 3, j2 = (j + 2) % 3;
-                const float ax = px[j1] - px[j], ay = py[j1] - py[j], az = pz[j1] - pz[j];
-                const float bx = px[j2] - px[j], by = py[j2] - py[j], bz = pz[j2] - pz[j];
-                c[j]           = ax * bx + ay * by + az * bz;
-                l[j]           = ax * ax + ay * ay + az * az;
-            }
-            // this vertex is corner i of face f; an angle exceeds pi / 2
-            // exactly when its corner dot product is negative, so only this
-            // corner's angle needs atan2 (the reference evaluates all three)
-            const uint32_t i1 = (i + 1u) % 3u, i2 = (i + 2u) % 3u;
-            const float    ri = corner_angle(s, c[i]);
-            const bool     ob = (c[0] < 0.f) || (c[1] < 0.f) || (c[2] < 0.f);
-            if (ob) am += (c[i] < 0.f) ? 0.25f * s : 0.125f * s;
-            else {
-                const float rs = __builtin_amdgcn_rcpf(s);  // cot = dot / |cross| for both other corners
-                am += 0.125f * (l[i2] * (c[i1] * rs) + l[i] * (c[i2] * rs));
-            }
-            g -= ri;
+            const uint32_t sl = L.vf[k], fc = 3u * (sl >> 2) + (sl & 3u);  // corner (sl & 3) of face sl >> 2
+            am += L.amx[fc];
+            g -= L.ang[fc];
         }
         const uint32_t gg = d.vp + v;
         gc[gg]            = g / am;
