@@ -1263,7 +1263,10 @@ rxm_status rxm_laplacian_smooth_csr(rxm_mesh* m, const int32_t* row_ptr, const i
     if (!m || !row_ptr || !col || !pos || !out || iters < 1 || pos == out) return RXM_ERR_INVALID_ARG;
     if (iters > 1 && (!tmp || tmp == out || tmp == pos)) return RXM_ERR_INVALID_ARG;
     if (!m->on_device) return RXM_ERR_UNSUPPORTED;
-    constexpr uint32_t LANES = 8;
+    // one lane per row: rows hold about six entries, so wider rows idle lanes
+    // and pay a shuffle reduction; one lane also sums in row order, the
+    // order the patch-local smoothing uses
+    constexpr uint32_t LANES = 1;
     const uint32_t     n     = m->h.nV;
     const uint64_t     thr   = uint64_t(n) * LANES;
     const float*       src   = pos;

@@ -915,7 +915,8 @@ __global__ __launch_bounds__(1024) void k_cg_scalar(uint32_t n, const float* __r
 
 // ---- uniform smoothing over a materialised VV CSR ---------------------------
 // One Jacobi step x' = x + lambda (mean of row neighbours - x) over the CSR of
-// a VV query (linear ids). LANES lanes share a row; fixed xor-tree reduction.
+// a VV query (linear ids). LANES lanes share a row; fixed xor-tree reduction
+// (LANES = 1: the row summed in order).
 template <uint32_t LANES>
 __global__ __launch_bounds__(256) void k_smooth_csr(uint32_t n, const int32_t* __restrict__ row_ptr,
                                                     const int32_t* __restrict__ col, float lambda,

@@ -341,6 +341,9 @@ def test_smoothing_on_vv_csr_vs_fp64(name):
     err = np.linalg.norm(got - ref, axis=1) / np.maximum(np.linalg.norm(ref, axis=1), 1e-30)
     assert err.max() <= 1e-5, err.max()
     assert torch.equal(out, m.smooth_csr(vv, x, 0.5, 10))
+    # one lane per row sums each row in CSR order, the patch-local smoothing's
+    # neighbour order: the two paths agree bit for bit
+    assert torch.equal(out, m.smooth(x, 0.5, 10))
 
 
 def test_permutation_round_trip():
