@@ -174,11 +174,20 @@ def batch_bench(count, steps, reps, patch_size=None):
                                                               lam, eta)
         log("autotuned patch size %d: %s" % (patch_size, ps_times))
     t_setup = time.perf_counter()
-    # the plain meshes are built concurrently (the C-ABI releases the GIL), largest first
+    # the plain meshes: those above an even share of the faces first, one at a time with every
+    # host thread, then the rest concurrently (the C-ABI releases the GIL), largest first, one
+    # host thread each (serial_build: no nested thread pools)
     from concurrent.futures import ThreadPoolExecutor
+    workers = min(16, os.cpu_count() or 1)
     order = sorted(range(count), key=lambda k: -len(geo[k][1]))
-    with ThreadPoolExecutor(max_workers=min(8, os.cpu_count() or 1)) as pool:
-        built = dict(zip(order, pool.map(lambda k: rx.Mesh(geo[k][1], patch_size=patch_size), order)))
+    total = sum(len(f) for _, f in geo)
+    nbig = 0
+    while nbig < count and workers > 1 and len(geo[order[nbig]][1]) * workers > total:
+        nbig += 1
+    built = {k: rx.Mesh(geo[k][1], patch_size=patch_size) for k in order[:nbig]}
+    with ThreadPoolExecutor(max_workers=workers) as pool:
+        built.update(zip(order[nbig:], pool.map(lambda k: rx.Mesh(geo[k][1], patch_size=patch_size,
+                                                                  serial_build=True), order[nbig:])))
     parts = [built[k] for k in range(count)]
     log("built %d meshes" % count)
     union = rx.Mesh(None, parts=parts)

@@ -60,8 +60,10 @@ enum {
 /* rxm_mesh_create flags. The default patcher is a recursive proportional
  * bisection (about ceil(F / patch_size) connected patches, the fewest the cap
  * allows, with the smallest ribbons); RXM_PATCHER_KMEANS selects the k-means
- * patcher instead (patches ~0.7 of the cap). */
-enum { RXM_HOST_ONLY = 1, RXM_PATCHER_KMEANS = 2 };
+ * patcher instead (patches ~0.7 of the cap). RXM_BUILD_SERIAL runs the host
+ * side of the build on the calling thread only (for callers that build many
+ * meshes from their own threads at once; the result is the same). */
+enum { RXM_HOST_ONLY = 1, RXM_PATCHER_KMEANS = 2, RXM_BUILD_SERIAL = 4 };
 
 /* vertex normal weighting */
 enum { RXM_NORMALS_MAX = 0, RXM_NORMALS_AREA = 1 };

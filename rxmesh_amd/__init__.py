@@ -24,6 +24,7 @@ ARR = {"ev": 0, "face_patch": 1, "vertex_patch": 2, "edge_patch": 3, "patches_va
        "fe_local": 20, "vv2_roff": 21, "vv2_nbr": 22, "vv2_xlin": 23, "vv2_offsets": 24}
 RXM_HOST_ONLY = 1
 RXM_PATCHER_KMEANS = 2
+RXM_BUILD_SERIAL = 4
 
 
 class RXMError(RuntimeError):
@@ -198,14 +199,15 @@ class Mesh:
     (include/rxmesh/rxmesh_static.h:61-100)."""
 
     def __init__(self, fv, num_vertices=0, patch_size=512, patch_file=None, face_patch=None,
-                 host_only=False, batch=None, parts=None, patcher="bisect"):
+                 host_only=False, batch=None, parts=None, patcher="bisect", serial_build=False):
         """fv: F x 3 triangles. Alternatives: batch = list of fv arrays (their
         disjoint union, built here), parts = list of built Mesh objects
         (their union, reusing the host build)."""
         h = ctypes.c_void_p()
         if patcher not in ("kmeans", "bisect"):
             raise ValueError("patcher must be 'kmeans' or 'bisect'")
-        flags = (RXM_HOST_ONLY if host_only else 0) | (RXM_PATCHER_KMEANS if patcher == "kmeans" else 0)
+        flags = ((RXM_HOST_ONLY if host_only else 0) | (RXM_PATCHER_KMEANS if patcher == "kmeans" else 0)
+                 | (RXM_BUILD_SERIAL if serial_build else 0))
         self._fv = None
         if parts is not None:
             self._parts = list(parts)

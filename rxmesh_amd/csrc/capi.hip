@@ -599,7 +599,8 @@ rxm_status rxm_mesh_create(const uint32_t* fv, uint32_t num_faces, uint32_t num_
     *out = nullptr;
     rxm_mesh* m = new (std::nothrow) rxm_mesh();
     if (!m) return RXM_ERR_OUT_OF_MEMORY;
-    const auto t0 = std::chrono::steady_clock::now();
+    ThreadLimit serial((flags & RXM_BUILD_SERIAL) ? 1u : 0u);  // 0: keep the calling thread's cap
+    const auto  t0 = std::chrono::steady_clock::now();
     int rc = supporting(m->h, fv, num_faces, num_vertices, flags);
     const auto t1 = std::chrono::steady_clock::now();
     if (rc == RXM_OK) {
@@ -660,25 +661,35 @@ rxm_status rxm_mesh_create_batch(const uint32_t* const* fvs, const uint32_t* num
     for (uint32_t k = 0; k < count; ++k) order[k] = k;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return num_faces[a] > num_faces[b]; });
     std::vector<int>      status(count, RXM_OK);
-    std::atomic<uint32_t> next{0};
+    // a mesh above an even share of the faces would alone outlast the
+    // others' concurrent builds: such meshes are built first, one at a time,
+    // with every host thread; the rest concurrently, one thread each
+    const unsigned workers = std::max(1u, std::min<unsigned>(num_threads(), count));
+    uint64_t       total   = 0;
+    for (uint32_t k = 0; k < count; ++k) total += num_faces[k];
+    uint32_t nbig = 0;
+    while (nbig < count && workers > 1 && uint64_t(num_faces[order[nbig]]) * workers > total) ++nbig;
+    std::atomic<uint32_t> next{nbig};
+    auto build_one = [&](uint32_t k) {
+        parts[k].reset(new (std::nothrow) HostMesh());
+        if (!parts[k]) {
+            status[k] = RXM_ERR_OUT_OF_MEMORY;
+            return;
+        }
+        int rc = supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0, flags);
+        if (rc == RXM_OK)
+            rc = run_patcher(*parts[k], patch_size, (flags & RXM_PATCHER_KMEANS) ? kPatcherKMeans : kPatcherBisect);
+        if (rc == RXM_OK) rc = build_patches(*parts[k]);
+        status[k] = rc;
+    };
+    for (uint32_t i = 0; i < nbig; ++i) build_one(order[i]);
     auto work = [&] {
+        ThreadLimit serial(1);  // one mesh per thread: its host build runs serially
         for (uint32_t i; (i = next.fetch_add(1)) < count;) {
-            const uint32_t k = order[i];
-            parts[k].reset(new (std::nothrow) HostMesh());
-            if (!parts[k]) {
-                status[k] = RXM_ERR_OUT_OF_MEMORY;
-                continue;
-            }
-            int rc = supporting(*parts[k], fvs[k], num_faces[k], num_vertices ? num_vertices[k] : 0, flags);
-            if (rc == RXM_OK)
-                rc = run_patcher(*parts[k], patch_size,
-                                 (flags & RXM_PATCHER_KMEANS) ? kPatcherKMeans : kPatcherBisect);
-            if (rc == RXM_OK) rc = build_patches(*parts[k]);
-            status[k] = rc;
+            build_one(order[i]);
         }
     };
     {
-        const unsigned workers = std::max(1u, std::min<unsigned>(num_threads(), count));
         std::vector<std::thread> pool;
         for (unsigned w = 1; w < workers; ++w) pool.emplace_back(work);
         work();
@@ -720,8 +731,13 @@ rxm_status rxm_mesh_batch_from(const rxm_mesh* const* meshes, uint32_t count, ui
     }
     rxm_mesh* m = new (std::nothrow) rxm_mesh();
     if (!m) return RXM_ERR_OUT_OF_MEMORY;
-    int rc = concat_meshes(ptrs, m->h, m->b_v, m->b_e, m->b_f, m->b_p);
+    const auto t0 = std::chrono::steady_clock::now();
+    int        rc = concat_meshes(ptrs, m->h, m->b_v, m->b_e, m->b_f, m->b_p);
+    const auto t1 = std::chrono::steady_clock::now();
     if (rc == RXM_OK && !(flags & RXM_HOST_ONLY)) rc = upload_mesh(m);
+    if (std::getenv("RXM_TIMING"))
+        fprintf(stderr, "batch_from: concat %.3f s, upload %.3f s\n", std::chrono::duration<double>(t1 - t0).count(),
+                std::chrono::duration<double>(std::chrono::steady_clock::now() - t1).count());
     if (rc != RXM_OK) {
         rxm_mesh_destroy(m);
         return rc;

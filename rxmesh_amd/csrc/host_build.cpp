@@ -16,6 +16,10 @@
 
 namespace rxm {
 
+namespace {
+thread_local unsigned tl_limit = 0;  // 0: no cap on this thread
+}
+
 unsigned num_threads()
 {
     static unsigned n = [] {
@@ -25,8 +29,14 @@ unsigned num_threads()
         if (t > 64) t = 64;
         return t;
     }();
-    return n;
+    return tl_limit ? std::min(n, tl_limit) : n;
 }
+
+ThreadLimit::ThreadLimit(unsigned k) : prev(tl_limit)
+{
+    if (k) tl_limit = k;
+}
+ThreadLimit::~ThreadLimit() { tl_limit = prev; }
 
 namespace {
 
@@ -41,8 +51,10 @@ void parallel_for(size_t n, F&& f, size_t chunk = 256)
     }
     std::atomic<size_t>      next{0};
     std::vector<std::thread> pool;
+    const unsigned           lim = tl_limit;
     for (unsigned t = 0; t < T; ++t)
-        pool.emplace_back([&] {
+        pool.emplace_back([&, lim] {
+            ThreadLimit cap(lim);
             size_t s;
             while ((s = next.fetch_add(chunk)) < n) {
                 size_t e = std::min(n, s + chunk);
@@ -370,8 +382,12 @@ class Bisector
         uint32_t              kA, kB;
         split(std::move(faces), k, cap, A, B, kA, kB, far);
         Leaves left, right;
-        if (A.size() + B.size() >= kParallelMin && workers_.fetch_add(1) < num_threads()) {
-            std::thread th([&] { left = run(std::move(A), kA, cap); });
+        if (A.size() + B.size() >= kParallelMin && workers_.fetch_add(1) + 1 < num_threads()) {
+            const unsigned lim = tl_limit;
+            std::thread th([&, lim] {
+                ThreadLimit inherit(lim);
+                left = run(std::move(A), kA, cap);
+            });
             right = run(std::move(B), kB, cap);
             th.join();
             workers_.fetch_sub(1);
@@ -433,7 +449,7 @@ class Bisector
             }
             cuts[c] = cut;
         };
-        if (n >= kParallelMin * 8) {  // large sets: the three candidates on their own threads
+        if (n >= kParallelMin * 8 && num_threads() >= 3) {  // large sets: the three candidates on their own threads
             std::thread t1(evaluate, 1), t2(evaluate, 2);
             evaluate(0);
             t1.join();
@@ -1296,19 +1312,24 @@ int concat_meshes(const std::vector<const HostMesh*>& parts, HostMesh& o, std::v
     o.nV = vb[K]; o.nE = eb[K]; o.nF = fb[K]; o.P = pb[K];
     o.patch_size = parts[0]->patch_size;
     o.closed = true; o.manifold = true; o.oriented = true;
-    o.fv.resize(3ull * o.nF); o.fe_g.resize(3ull * o.nF); o.ev.resize(2ull * o.nE);
-    o.vf_off.assign(o.nV + 1, 0); o.vf_val.resize(3ull * o.nF);
-    o.ef_off.assign(o.nE + 1, 0); o.ef_val.resize(3ull * o.nF);
-    o.face_patch.resize(o.nF); o.patches_val.resize(o.nF); o.patches_offset.resize(o.P);
-    o.ribbon_val.resize(rb[K]); o.ribbon_offset.resize(o.P);
-    o.vertex_patch.resize(o.nV); o.edge_patch.resize(o.nE);
-    for (auto* v : {&o.nv, &o.ne, &o.nf, &o.nvo, &o.neo, &o.nfo}) v->resize(o.P);
-    for (auto* v : {&o.v_off, &o.e_off, &o.f_off, &o.vp, &o.ep, &o.fp}) v->assign(o.P + 1, 0);
-    o.ltog_v.resize(vs[K]); o.v_lin.resize(vs[K]); o.v_own.resize(vs[K]);
-    o.ltog_e.resize(es[K]); o.e_lin.resize(es[K]); o.e_own.resize(es[K]); o.ev_local.resize(es[K]);
-    o.ltog_f.resize(fs[K]); o.f_lin.resize(fs[K]); o.f_own.resize(fs[K]); o.fe_local.resize(3ull * fs[K]);
-    o.v_l2g.resize(o.nV); o.v_g2l.resize(o.nV); o.e_l2g.resize(o.nE); o.e_g2l.resize(o.nE);
-    o.f_l2g.resize(o.nF); o.f_g2l.resize(o.nF);
+    // the output arrays (several GB for large batches) are sized on all host
+    // threads at once, one array per thread: the zeroing and the first-touch
+    // page faults of a serial resize would dominate the concatenation
+    std::vector<std::pair<std::vector<uint32_t>*, uint64_t>> sz = {
+        {&o.fv, 3ull * o.nF},      {&o.fe_g, 3ull * o.nF},    {&o.ev, 2ull * o.nE},      {&o.vf_off, o.nV + 1ull},
+        {&o.vf_val, 3ull * o.nF},  {&o.ef_off, o.nE + 1ull},  {&o.ef_val, 3ull * o.nF},  {&o.face_patch, o.nF},
+        {&o.patches_val, o.nF},    {&o.patches_offset, o.P},  {&o.ribbon_val, rb[K]},    {&o.ribbon_offset, o.P},
+        {&o.vertex_patch, o.nV},   {&o.edge_patch, o.nE},     {&o.ltog_v, vs[K]},        {&o.v_lin, vs[K]},
+        {&o.v_own, vs[K]},         {&o.ltog_e, es[K]},        {&o.e_lin, es[K]},         {&o.e_own, es[K]},
+        {&o.ev_local, es[K]},      {&o.ltog_f, fs[K]},        {&o.f_lin, fs[K]},         {&o.f_own, fs[K]},
+        {&o.v_l2g, o.nV},          {&o.v_g2l, o.nV},          {&o.e_l2g, o.nE},          {&o.e_g2l, o.nE},
+        {&o.f_l2g, o.nF},          {&o.f_g2l, o.nF}};
+    for (auto* v : {&o.nv, &o.ne, &o.nf, &o.nvo, &o.neo, &o.nfo}) sz.push_back({v, o.P});
+    for (auto* v : {&o.v_off, &o.e_off, &o.f_off, &o.vp, &o.ep, &o.fp}) sz.push_back({v, o.P + 1ull});
+    parallel_for(sz.size() + 1, [&](size_t i) {
+        if (i < sz.size()) sz[i].first->assign(sz[i].second, 0u);
+        else o.fe_local.assign(3ull * fs[K], uint16_t(0));
+    }, 1);
     for (size_t k = 0; k < K; ++k) {
         const HostMesh& m = *parts[k];
         o.closed = o.closed && m.closed;

@@ -177,6 +177,15 @@ void closed_fan_vertices(const HostMesh& m, std::vector<uint8_t>& closed);
 bool build_vv_edge_pattern(const HostMesh& m, const std::vector<uint32_t>& vv_base, std::vector<uint16_t>& ent,
                            std::vector<uint32_t>& rs);
 
+// Host threads a build may use: RXM_NUM_THREADS or the hardware's count,
+// capped on the calling thread by a ThreadLimit in scope (k = 0 keeps the
+// current cap; threads a build spawns inherit the cap).
 unsigned num_threads();
+struct ThreadLimit
+{
+    explicit ThreadLimit(unsigned k);
+    ~ThreadLimit();
+    unsigned prev;
+};
 
 }  // namespace rxm

@@ -286,16 +286,18 @@ def test_vv2_plan_rows_are_owner_rows(case):
 
 
 def test_concurrent_builds_equal_serial_builds():
-    """Meshes built from several threads at once (the C-ABI releases the GIL)
-    export the same arrays as the same meshes built one by one, and a batch
-    built with its meshes in parallel equals the union of the serial ones."""
+    """Meshes built from several threads at once (the C-ABI releases the GIL),
+    each with its host build on its own thread only (serial_build), export the
+    same arrays as the same meshes built one by one with every host thread,
+    and a batch built with its meshes in parallel (one thread each) equals the
+    union of the serial ones."""
     from concurrent.futures import ThreadPoolExecutor
     meshes = [mg.icosphere(4, seed=k, sigma=0.01)[1] for k in range(3)] + [mg.grid(40 + 7 * k, seed=k)[1]
                                                                           for k in range(3)]
     meshes += [mg.permute_mesh(*mg.config_mesh("c1"), seed=5)[1]]
     serial = [rx.Mesh(f, patch_size=256, host_only=True) for f in meshes]
     with ThreadPoolExecutor(max_workers=4) as pool:
-        conc = list(pool.map(lambda f: rx.Mesh(f, patch_size=256, host_only=True), meshes))
+        conc = list(pool.map(lambda f: rx.Mesh(f, patch_size=256, host_only=True, serial_build=True), meshes))
     names = [a for a in rx.ARR if not a.startswith("vv2")]
     for a, b in zip(serial, conc):
         for n in names:
