@@ -94,9 +94,10 @@ def test_mesh_create_device_connectivity_equals_host_build(name):
 
 def test_concurrent_device_builds_equal_serial_builds():
     """Meshes built on the device path from several threads at once (each
-    build has its own stream and buffers; the C-ABI releases the GIL) and a
-    batch whose meshes are built in parallel export the same arrays as the
-    same meshes built one by one."""
+    build has its own stream and buffers; the C-ABI releases the GIL; host
+    side on the calling thread only, serial_build) and a batch whose meshes
+    are built in parallel export the same arrays as the same meshes built one
+    by one with every host thread."""
     from concurrent.futures import ThreadPoolExecutor
     import rxmesh_amd as rx
     meshes = [mg.icosphere(4, seed=k, sigma=0.01)[1] for k in range(3)] + [mg.grid(40 + 7 * k, seed=k)[1]
@@ -105,7 +106,7 @@ def test_concurrent_device_builds_equal_serial_builds():
     meshes = [np.ascontiguousarray(f, dtype=np.uint32) for f in meshes]
     serial = [rx.Mesh(f, patch_size=256) for f in meshes]
     with ThreadPoolExecutor(max_workers=4) as pool:
-        conc = list(pool.map(lambda f: rx.Mesh(f, patch_size=256), meshes))
+        conc = list(pool.map(lambda f: rx.Mesh(f, patch_size=256, serial_build=True), meshes))
     names = [a for a in rx.ARR if not a.startswith("vv2")]
     for a, b in zip(serial, conc):
         for n in names:
