@@ -36,7 +36,7 @@ def parse():
     ap.add_argument("--smooth-iters", type=int, default=10)
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-configs", action="store_true", help="skip the c0 / c2s / c2t / c3 workloads")
+    ap.add_argument("--no-configs", action="store_true", help="skip the c0 / c1 / c2s / c2t / c3 workloads")
     ap.add_argument("--cpu-reps", type=int, default=5, help="CPU baseline runs per thread count (median)")
     ap.add_argument("--profile", default=None,
                     help="run only this kernel --reps times (for rocprofv3): query_vv, query_vf, "
@@ -298,6 +298,8 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
     the launch stream against the kernel's algorithmic bytes (fp32 = 4 B,
     int32 index = 4 B, patch-local u16 = 2 B):
       c0: area-weighted vertex normals + VV query on the jittered 128 x 128 grid;
+      c1: the full query suite (VV, VE, VF, EV, EF, FV, FE, FF) and 10
+          uniform smoothing iterations on the permuted icosphere L6;
       c2s / c2t: cotangent CSR assembly (A = M + dt L, oriented VV rows) and
           explicit mean-curvature flow x <- x + (M x - A x) / M, 50 SpMV-3
           iterations, dt = 1e-5, on the icosphere L9 and the 2048 x 1024 torus;
@@ -329,6 +331,30 @@ def configs_bench(reps, c2s_mesh=None, c2s_pos=None):
                          "query_vv": _entry(t_q, 4 * E + 8 * E + 4 * (V + 1), 2 * E, "Gentries_s")}
     m.close()
     log("c0 done")
+
+    # c1: every query (topology read + int32 CSR written, as in the main table) and the smoothing
+    pos, fv = mg.config_mesh("c1")
+    m = rx.Mesh(fv, patch_size=512)
+    V, E, F = m.nV, m.nE, m.nF
+    x = lin(m, pos)
+    topo = {"VV": 4 * E, "VE": 4 * E, "VF": 4 * E + 6 * F, "FV": 4 * E + 6 * F, "FE": 6 * F, "FF": 6 * F,
+            "EV": 4 * E, "EF": 6 * F}
+    c1 = {"V": V, "F": F, "patches": m.P}
+    for op in ("VV", "VE", "VF", "EV", "EF", "FV", "FE", "FF"):
+        nnz, nsrc = m.query_nnz(op), m.num_src(op)
+        rp_q = torch.empty(nsrc + 1, dtype=torch.int32, device="cuda")
+        col_q = torch.empty(max(nnz, 1), dtype=torch.int32, device="cuda")
+        t = event_time_ms(lambda: m.query(op, row_ptr=rp_q, col=col_q), reps)
+        c1["query_" + op.lower()] = _entry(t, topo[op] + 4 * nnz + 4 * (nsrc + 1), nnz, "Gentries_s")
+    y, tmp = torch.empty_like(x), torch.empty_like(x)
+    from rxmesh_amd import autotune
+    cfg, _ = autotune.tune_smooth(m, x, y, use_cache=False, iters=10)  # launch form for 10 iterations
+    m.smooth(x, 0.5, 10, out=y, tmp=tmp)
+    t_s = event_time_ms(lambda: m.smooth(x, 0.5, 10, out=y, tmp=tmp), reps)
+    c1["smooth_10_iterations"] = dict(_entry(t_s, 10 * (24 * V + 4 * E), 10 * 2 * E, "Gentries_s"), config=cfg)
+    out["c1_icosphere_L6"] = c1
+    m.close()
+    log("c1 done")
 
     def mcf(m, x, name):
         V, E, F = m.nV, m.nE, m.nF

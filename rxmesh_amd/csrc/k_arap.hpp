@@ -75,14 +75,17 @@ __global__ __launch_bounds__(T) void k_arap_weights(DevMesh m, const float* __re
 }
 
 // ---- best rotation -------------------------------------------------------------
-// One Jacobi rotation of the symmetric 4x4 a in the (P, Q) plane, zeroing
-// a[P][Q]; v accumulates the eigenvectors as columns. The rotation only has
-// to be orthogonal, not exact (an angle off by an ulp leaves a tiny a[P][Q]
-// that later sweeps remove): c = rsqrt(1 + t^2) and s = t c keep c^2 + s^2 = 1
-// to an ulp, so the hardware reciprocal and reciprocal square root serve.
+// One Jacobi rotation of the symmetric 4x4 a in the (P, Q) plane (P < Q),
+// zeroing a[P][Q]; v accumulates the eigenvectors as columns. Only the upper
+// triangle of a is kept: a_PP -= t a_PQ, a_QQ += t a_PQ, a_PQ = 0, and the
+// two other rows' (P, Q) pairs rotate. The rotation only has to be
+// orthogonal, not exact (an angle off by an ulp leaves a tiny a[P][Q] that
+// later sweeps remove): c = rsqrt(1 + t^2) and s = t c keep c^2 + s^2 = 1 to
+// an ulp, so the hardware reciprocal and reciprocal square root serve.
 template <int P, int Q>
 __device__ __forceinline__ void jacobi4(float (&a)[4][4], float (&v)[4][4])
 {
+    static_assert(P < Q, "upper triangle");
     const float apq = a[P][Q];
     if (fabsf(apq) < 1e-30f) return;  // negligible in the unit-scaled matrix; keeps rcp away from denormals
     const float theta = 0.5f * (a[Q][Q] - a[P][P]) * __builtin_amdgcn_rcpf(apq);
@@ -91,17 +94,18 @@ __device__ __forceinline__ void jacobi4(float (&a)[4][4], float (&v)[4][4])
     const float den = (at < 1e18f) ? at + (at * at + 1.f) * __builtin_amdgcn_rsqf(at * at + 1.f) : 2.f * at;
     const float t   = copysignf(__builtin_amdgcn_rcpf(den), theta);
     const float c = __builtin_amdgcn_rsqf(t * t + 1.f), s = t * c;
+    a[P][P] -= t * apq;
+    a[Q][Q] += t * apq;
+    a[P][Q] = 0.f;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const float akp = a[k][P], akq = a[k][Q];
-        a[k][P]         = c * akp - s * akq;
-        a[k][Q]         = s * akp + c * akq;
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const float apk = a[P][k], aqk = a[Q][k];
-        a[P][k]         = c * apk - s * aqk;
-        a[Q][k]         = s * apk + c * aqk;
+        if (k == P || k == Q) continue;
+        // the upper-triangle entries (k, P) and (k, Q)
+        float& akp = (k < P) ? a[k][P] : a[P][k];
+        float& akq = (k < Q) ? a[k][Q] : a[Q][k];
+        const float xp = akp, xq = akq;
+        akp              = c * xp - s * xq;
+        akq              = s * xp + c * xq;
     }
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
