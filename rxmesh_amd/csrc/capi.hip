@@ -948,8 +948,8 @@ rxm_status rxm_query_csr(rxm_mesh* m, int op, int oriented, int32_t* row_ptr, in
         return (op == RXM_OP_VV) ? go(k_query_vx_oriented<256, true>, 256) : go(k_query_vx_oriented<256, false>, 256);
     }
     // VV / VE / VF / EF in the one-pass form (knob 10 = 2, the default) where
-    // the patches fit its carve (VV / VE slots are entry indices 2 e + end in
-    // 16 bits)
+    // the patches fit its carve (VV / VE slots are EV's u16 indices 2 e + 1 -
+    // end, 16 bits)
     if (m->query_stable == 2 && (op == RXM_OP_VV || op == RXM_OP_VE || op == RXM_OP_VF || op == RXM_OP_EF)) {
         vvf_slots(m);
         const bool     ef     = op == RXM_OP_EF;
@@ -1028,7 +1028,7 @@ static int query_vv_vf_launch(rxm_mesh* m, int32_t* vv_row_ptr, int32_t* vv_col,
         vvf_slots(m);
         Bump b1(nullptr);
         plan_vvf1(b1, m->h.max_nvo, m->h.max_ne, m->vvf_nslots, m->max_nrv, m->max_nrf);
-        // VV slots hold entry indices 2 e + end in 16 bits
+        // VV slots hold EV u16 indices (2 e + 1 - end) in 16 bits
         if (b1.bytes() <= kMaxLdsBytes && 2u * m->h.max_ne <= 0x10000u)
             return launch_patches_xcd(k_query2_vv_vf1<T>, T, b1.bytes(), stream, m, m->dm, m->vvf_nslots, m->max_nrv,
                                       m->max_nrf, vv_base, vf_base, vv_row_ptr, vv_col, vf_row_ptr, vf_col);

@@ -476,8 +476,8 @@ __host__ __device__ inline PlanVVF1 plan_vvf1(Bump& b, uint32_t nvo, uint32_t ne
     p.off   = b.alloc<uint32_t>(2 * nvo + 1);
     p.slots = b.alloc<uint16_t>(nslots);
     p.wave  = b.alloc<uint32_t>(20);
-    p.rv    = b.alloc<uint32_t>(nrv);
-    p.rf    = b.alloc<uint32_t>(nrf);
+    p.rv    = b.alloc<uint32_t>(nrv ? nrv : 1u);  // at least one: the column pass reads clamped indices
+    p.rf    = b.alloc<uint32_t>(nrf ? nrf : 1u);
     return p;
 }
 
@@ -491,8 +491,8 @@ __host__ __device__ inline PlanVVF1 plan_vvf1(Bump& b, uint32_t nvo, uint32_t ne
 //  - each owned row sorted in registers by the smallest network that covers
 //    the longest row of its wave (a wave-uniform choice), written back to
 //    LDS, then one coalesced pass writes both CSRs, turning entries into
-//    linear ids (VV slot = entry index 2 e + end: below 2^16 needs ne <=
-//    2^15; VF slot = local face).
+//    linear ids (VV slot = EV's u16 index of the other endpoint, 2 e + 1 -
+//    end: below 2^16 needs ne <= 2^15; VF slot = local face).
 // Needs ne <= RE T and nf <= RF T for the register path; larger patches take
 // plain loops. Same CSRs as k_query2_vv_vf.
 template <uint32_t T>
@@ -577,13 +577,15 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf1(DevMesh m, uint32_t nslots,
             }
     }
     scan_excl<T>(L.off + 1, 2 * nvo, L.wave);
-    // fill: off[c + 1] advances from the start of column c to that of c + 1
+    // fill: off[c + 1] advances from the start of column c to that of c + 1;
+    // a VV slot is the u16 index in EV of the row's other endpoint (2 e + 1
+    // for the row of lo16, 2 e for that of hi16), which sorts like e
     if (held) {
 #pragma unroll
         for (uint32_t r = 0; r < RE; ++r) {
             const uint32_t i = 2u * (t + r * T);
-            if (lo16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i);
-            if (hi16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i + 1u);
+            if (lo16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i + 1u);
+            if (hi16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i);
         }
 #pragma unroll
         for (uint32_t r = 0; r < RF; ++r)
@@ -594,8 +596,8 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf1(DevMesh m, uint32_t nslots,
     } else {
         for (uint32_t e = t; e < d.ne; e += T) {
             const uint32_t x = L.ev[e];
-            if (lo16(x) < nvo) L.slots[atomicAdd(&L.off[lo16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e);
-            if (hi16(x) < nvo) L.slots[atomicAdd(&L.off[hi16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e + 1u);
+            if (lo16(x) < nvo) L.slots[atomicAdd(&L.off[lo16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e + 1u);
+            if (hi16(x) < nvo) L.slots[atomicAdd(&L.off[hi16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e);
         }
         for (uint32_t f = t; f < d.nf; f += T)
             for (uint32_t j = 0; j < 3; ++j) {
@@ -642,14 +644,18 @@ __global__ __launch_bounds__(T) void k_query2_vv_vf1(DevMesh m, uint32_t nslots,
         }
     }
     __syncthreads();
-    for (uint32_t i = t; i < ntot; i += T) {
+    // columns: a ribbon id read at a clamped index (in range for every
+    // entry), selected where the element is not owned; no branch per entry
+    const uint16_t* ev16 = reinterpret_cast<const uint16_t*>(L.ev);
+    for (uint32_t i = t; i < nvv; i += T) {
+        const uint32_t w = ev16[L.slots[i]];
+        const uint32_t r = L.rv[max(w, nvo) - nvo];
+        vv_col[bvv + i]  = static_cast<int32_t>((w < nvo) ? d.vp + w : r);
+    }
+    for (uint32_t i = nvv + t; i < ntot; i += T) {  // bvf + i: 32-bit sum, back in range
         const uint32_t s = L.slots[i];
-        if (i < nvv) {
-            const uint32_t w = ev_other(L.ev, s);
-            vv_col[bvv + i]  = static_cast<int32_t>((w < nvo) ? d.vp + w : L.rv[w - nvo]);
-        } else {
-            vf_col[bvf + i] = static_cast<int32_t>((s < d.nfo) ? d.fp + s : L.rf[s - d.nfo]);
-        }
+        const uint32_t r = L.rf[max(s, d.nfo) - d.nfo];
+        vf_col[bvf + i]  = static_cast<int32_t>((s < d.nfo) ? d.fp + s : r);
     }
 }
 
@@ -680,7 +686,7 @@ __host__ __device__ inline PlanX1 plan_x1(Bump& b, uint32_t ncol, uint32_t ne, u
     p.off   = b.alloc<uint32_t>(ncol + 1);
     p.slots = b.alloc<uint16_t>(nslots);
     p.wave  = b.alloc<uint32_t>(20);
-    p.rid   = b.alloc<uint32_t>(nrid);
+    p.rid   = b.alloc<uint32_t>(nrid ? nrid : 1u);  // at least one: the column pass reads clamped indices
     return p;
 }
 
@@ -778,7 +784,8 @@ __global__ __launch_bounds__(T) void k_query2_x1(DevMesh m, uint32_t nslots, uin
     }
     scan_excl<T>(L.off + 1, ncol, L.wave);
     // fill: off[c + 1] advances from the start of column c to that of c + 1;
-    // slot = entry index 2 e + end (VV / VE) or local face (VF / EF)
+    // slot = EV's u16 index of the row's other endpoint, 2 e + 1 - end (VV /
+    // VE: sorts like e), or local face (VF / EF)
     if constexpr (FK) {
         if (held) {
 #pragma unroll
@@ -799,15 +806,15 @@ __global__ __launch_bounds__(T) void k_query2_x1(DevMesh m, uint32_t nslots, uin
 #pragma unroll
             for (uint32_t r = 0; r < RE; ++r) {
                 const uint32_t i = 2u * (t + r * T);
-                if (lo16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i);
-                if (hi16(xe[r]) < nvo)
-                    L.slots[atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i + 1u);
+                if (lo16(xe[r]) < nvo)
+                    L.slots[atomicAdd(&L.off[lo16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i + 1u);
+                if (hi16(xe[r]) < nvo) L.slots[atomicAdd(&L.off[hi16(xe[r]) + 1u], 1u)] = static_cast<uint16_t>(i);
             }
         } else {
             for (uint32_t e = t; e < d.ne; e += T) {
                 const uint32_t x = L.ev[e];
-                if (lo16(x) < nvo) L.slots[atomicAdd(&L.off[lo16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e);
-                if (hi16(x) < nvo) L.slots[atomicAdd(&L.off[hi16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e + 1u);
+                if (lo16(x) < nvo) L.slots[atomicAdd(&L.off[lo16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e + 1u);
+                if (hi16(x) < nvo) L.slots[atomicAdd(&L.off[hi16(x) + 1u], 1u)] = static_cast<uint16_t>(2u * e);
             }
         }
     }
@@ -845,19 +852,17 @@ __global__ __launch_bounds__(T) void k_query2_x1(DevMesh m, uint32_t nslots, uin
         }
     }
     __syncthreads();
+    // columns: local element w of the output's kind, its ribbon id read at a
+    // clamped index (in range for every entry) and selected where w is not
+    // owned; no branch per entry
+    const uint16_t* ev16 = reinterpret_cast<const uint16_t*>(L.ev);
+    const uint32_t  no   = (REL == kX1VV) ? nvo : ((REL == kX1VE) ? d.neo : d.nfo);
+    const uint32_t  pre  = (REL == kX1VV) ? d.vp : ((REL == kX1VE) ? d.ep : d.fp);
     for (uint32_t i = t; i < ntot; i += T) {
         const uint32_t s = L.slots[i];
-        uint32_t       l;
-        if constexpr (REL == kX1VV) {
-            const uint32_t w = ev_other(L.ev, s);
-            l                = (w < nvo) ? d.vp + w : L.rid[w - nvo];
-        } else if constexpr (REL == kX1VE) {
-            const uint32_t e = s >> 1;
-            l                = (e < d.neo) ? d.ep + e : L.rid[e - d.neo];
-        } else {
-            l = (s < d.nfo) ? d.fp + s : L.rid[s - d.nfo];
-        }
-        col[bse + i] = static_cast<int32_t>(l);
+        const uint32_t w = (REL == kX1VV) ? static_cast<uint32_t>(ev16[s]) : ((REL == kX1VE) ? s >> 1 : s);
+        const uint32_t r = L.rid[max(w, no) - no];
+        col[bse + i]     = static_cast<int32_t>((w < no) ? pre + w : r);
     }
 }
 
