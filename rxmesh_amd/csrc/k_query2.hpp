@@ -887,7 +887,7 @@ __host__ __device__ inline PlanFF1 plan_ff1(Bump& b, uint32_t ne, uint32_t nf, u
     p.wave  = b.alloc<uint32_t>(20);
     p.cur   = b.alloc<uint32_t>(nfo + 1);
     p.out   = b.alloc<uint16_t>(nout);
-    p.rid   = b.alloc<uint32_t>(nrid);
+    p.rid   = b.alloc<uint32_t>(nrid ? nrid : 1u);  // at least one: the column pass reads clamped indices
     return p;
 }
 
@@ -1001,9 +1001,10 @@ __global__ __launch_bounds__(T) void k_query2_ff1(DevMesh m, uint32_t nout, uint
             }
     });
     __syncthreads();
-    for (uint32_t i = t; i < total; i += T) {
+    for (uint32_t i = t; i < total; i += T) {  // ribbon id at a clamped index, selected: no branch
         const uint32_t g = L.out[i];
-        col[bse + i]     = static_cast<int32_t>((g < d.nfo) ? d.fp + g : L.rid[g - d.nfo]);
+        const uint32_t r = L.rid[max(g, d.nfo) - d.nfo];
+        col[bse + i]     = static_cast<int32_t>((g < d.nfo) ? d.fp + g : r);
     }
 }
 
