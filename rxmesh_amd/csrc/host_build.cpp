@@ -98,6 +98,25 @@ int supporting_prepare(HostMesh& m, const uint32_t* fv, uint32_t nF, uint32_t nV
     return kOk;
 }
 
+namespace {
+
+// f(begin, end) over contiguous chunks of [0, n) on the build's threads
+template <class F>
+void parallel_chunks(size_t n, F&& f)
+{
+    const size_t nc = std::max<size_t>(1, std::min<size_t>(n / 65536 + 1, size_t(num_threads()) * 4));
+    parallel_for(nc, [&](size_t c) { f(n * c / nc, n * (c + 1) / nc); }, 1);
+}
+
+void atomic_max(std::atomic<uint32_t>& a, uint32_t x)
+{
+    uint32_t cur = a.load(std::memory_order_relaxed);
+    while (cur < x && !a.compare_exchange_weak(cur, x, std::memory_order_relaxed)) {
+    }
+}
+
+}  // namespace
+
 int supporting_finish(HostMesh& m)
 {
     const uint64_t nH = 3ull * m.nF;
@@ -107,35 +126,60 @@ int supporting_finish(HostMesh& m)
         return kErrInvalidArg;
     // consistent orientation: of the half-edges of an edge, only its first
     // (whose direction EV keeps) starts at ev[2e]
-    std::vector<uint8_t> fwd(nE, 0);
-    m.oriented = true;
-    for (uint64_t h = 0; h < nH; ++h) {
-        const uint32_t e = m.fe_g[h];
-        if (m.fv[h] == m.ev[2ull * e] && fwd[e]++ > 0) m.oriented = false;
+    {
+        std::unique_ptr<std::atomic<uint8_t>[]> fwd(new std::atomic<uint8_t>[nE ? nE : 1u]());
+        std::atomic<bool>                       oriented{true};
+        parallel_chunks(nH, [&](size_t a, size_t b) {
+            bool ok = true;
+            for (size_t h = a; h < b; ++h) {
+                const uint32_t e = m.fe_g[h];
+                if (m.fv[h] == m.ev[2ull * e] && fwd[e].fetch_add(1, std::memory_order_relaxed) > 0) ok = false;
+            }
+            if (!ok) oriented.store(false, std::memory_order_relaxed);
+        });
+        m.oriented = oriented.load();
     }
     // statistics (rxmesh.cpp:589-660)
-    m.closed = true;
-    m.manifold = true;
-    m.max_edge_faces = 0;
-    std::vector<uint32_t> val(m.nV, 0);
-    for (uint32_t e = 0; e < nE; ++e) {
-        const uint32_t c = m.ef_off[e + 1] - m.ef_off[e];
-        if (c == 1) m.closed = false;
-        if (c > 2) m.manifold = false;
-        m.max_edge_faces = std::max(m.max_edge_faces, c);
-        val[m.ev[2ull * e]]++;
-        val[m.ev[2ull * e + 1]]++;
-    }
-    m.max_valence = *std::max_element(val.begin(), val.end());
-    m.max_face_adj = 0;
-    for (uint32_t f = 0; f < m.nF; ++f) {
-        uint32_t s = 0;
-        for (int i = 0; i < 3; ++i) {
-            const uint32_t e = m.fe_g[3ull * f + i];
-            s += m.ef_off[e + 1] - m.ef_off[e] - 1;
+    std::unique_ptr<std::atomic<uint32_t>[]> val(new std::atomic<uint32_t>[m.nV ? m.nV : 1u]());
+    std::atomic<bool>                        closed{true}, manifold{true};
+    std::atomic<uint32_t>                    mef{0}, mval{0}, mfa{0};
+    parallel_chunks(nE, [&](size_t a, size_t b) {
+        bool     cl = true, mf = true;
+        uint32_t mx = 0;
+        for (size_t e = a; e < b; ++e) {
+            const uint32_t c = m.ef_off[e + 1] - m.ef_off[e];
+            if (c == 1) cl = false;
+            if (c > 2) mf = false;
+            mx = std::max(mx, c);
+            val[m.ev[2 * e]].fetch_add(1, std::memory_order_relaxed);
+            val[m.ev[2 * e + 1]].fetch_add(1, std::memory_order_relaxed);
         }
-        m.max_face_adj = std::max(m.max_face_adj, s);
-    }
+        if (!cl) closed.store(false, std::memory_order_relaxed);
+        if (!mf) manifold.store(false, std::memory_order_relaxed);
+        atomic_max(mef, mx);
+    });
+    parallel_chunks(m.nV, [&](size_t a, size_t b) {
+        uint32_t mx = 0;
+        for (size_t v = a; v < b; ++v) mx = std::max(mx, val[v].load(std::memory_order_relaxed));
+        atomic_max(mval, mx);
+    });
+    parallel_chunks(m.nF, [&](size_t a, size_t b) {
+        uint32_t mx = 0;
+        for (size_t f = a; f < b; ++f) {
+            uint32_t s = 0;
+            for (int i = 0; i < 3; ++i) {
+                const uint32_t e = m.fe_g[3 * f + i];
+                s += m.ef_off[e + 1] - m.ef_off[e] - 1;
+            }
+            mx = std::max(mx, s);
+        }
+        atomic_max(mfa, mx);
+    });
+    m.closed         = closed.load();
+    m.manifold       = manifold.load();
+    m.max_edge_faces = mef.load();
+    m.max_valence    = mval.load();
+    m.max_face_adj   = mfa.load();
     return kOk;
 }
 
