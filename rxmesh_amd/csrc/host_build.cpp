@@ -271,6 +271,142 @@ struct FaceGraph
     std::vector<uint32_t> off, val;
 };
 
+// ---- level-synchronous BFS with the serial FIFO order ------------------------
+// Threads of a BFS team meet at a barrier three times per level; a waiting
+// thread spins briefly, then yields its core.
+class SpinBarrier
+{
+   public:
+    explicit SpinBarrier(unsigned n) : n_(n) {}
+    void wait()
+    {
+        const unsigned g = gen_.load(std::memory_order_acquire);
+        if (count_.fetch_add(1, std::memory_order_acq_rel) + 1 == n_) {
+            count_.store(0, std::memory_order_relaxed);
+            gen_.fetch_add(1, std::memory_order_release);
+            return;
+        }
+        for (unsigned spins = 0; gen_.load(std::memory_order_acquire) == g; ++spins)
+            if (spins > 256) std::this_thread::yield();
+    }
+
+   private:
+    const unsigned        n_;
+    std::atomic<unsigned> count_{0}, gen_{0};
+};
+
+// Threads a BFS over n faces may use: large sets only (a level costs three
+// barriers), at most 16, shared with the `busy` other tasks in flight.
+constexpr size_t kTeamMin = size_t(1) << 20;
+unsigned bfs_team(size_t n, unsigned busy = 0)
+{
+    if (n < kTeamMin) return 1;
+    return std::max(1u, std::min(16u, num_threads() / (busy + 1)));
+}
+
+// BFS from src over the faces f with ok(f) and dist[f] == kInvalid32: appends
+// them to q in the order of the serial FIFO BFS and sets dist (0 at src).
+// With nthr >= 2 each level runs on a team of threads with that same order:
+// the serial BFS pops the level's faces in q order and pushes each one's
+// unvisited neighbours in adjacency order, so a face reached at level L + 1
+// belongs to the earliest level-L face in q that has it as a neighbour
+// (claimed by an atomic max of (epoch, ~position): stale claims of older
+// epochs lose, and no reset is needed), and the next level is the parents'
+// claimed neighbours in (parent position, adjacency) order, placed by a
+// prefix sum of per-thread counts. `cap` bounds the faces reached.
+template <class Ok>
+void fifo_bfs(const FaceGraph& g, uint32_t src, Ok ok, uint32_t* dist, std::atomic<uint64_t>* claim, uint32_t epoch,
+              std::vector<uint32_t>& q, size_t cap, unsigned nthr)
+{
+    const size_t base = q.size();
+    dist[src]         = 0;
+    if (nthr < 2) {
+        q.push_back(src);
+        for (size_t h = base; h < q.size(); ++h) {
+            const uint32_t f = q[h];
+            for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+                const uint32_t o = g.val[j];
+                if (ok(o) && dist[o] == kInvalid32) {
+                    dist[o] = dist[f] + 1;
+                    q.push_back(o);
+                }
+            }
+        }
+        return;
+    }
+    q.resize(base + cap);
+    q[base] = src;
+    std::vector<size_t> cnt(nthr, 0);
+    size_t              last = base + 1;
+    SpinBarrier         bar(nthr);
+    auto key = [&](size_t i) { return (uint64_t(epoch) << 32) | (0xFFFFFFFFu - static_cast<uint32_t>(i)); };
+    // o at slot j of f's adjacency is f's to emit: claimed by f, first slot naming o
+    auto mine = [&](uint32_t f, uint32_t j, uint32_t o, uint64_t k) {
+        if (claim[o].load(std::memory_order_relaxed) != k) return false;
+        for (uint32_t i = g.off[f]; i < j; ++i)
+            if (g.val[i] == o) return false;
+        return true;
+    };
+    auto body = [&](unsigned t) {
+        size_t   lo = base, hi = base + 1;
+        uint32_t lev = 0;
+        while (lo < hi) {
+            const size_t a = lo + (hi - lo) * t / nthr, b = lo + (hi - lo) * (t + 1) / nthr;
+            for (size_t i = a; i < b; ++i) {
+                const uint32_t f = q[i];
+                const uint64_t k = key(i);
+                for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+                    const uint32_t o = g.val[j];
+                    if (!ok(o) || dist[o] != kInvalid32) continue;
+                    uint64_t cur = claim[o].load(std::memory_order_relaxed);
+                    while (cur < k && !claim[o].compare_exchange_weak(cur, k, std::memory_order_relaxed)) {
+                    }
+                }
+            }
+            bar.wait();
+            size_t c = 0;
+            for (size_t i = a; i < b; ++i) {
+                const uint32_t f = q[i];
+                const uint64_t k = key(i);
+                for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) c += mine(f, j, g.val[j], k);
+            }
+            cnt[t] = c;
+            bar.wait();
+            size_t w = hi, tot = 0;
+            for (unsigned u = 0; u < nthr; ++u) {
+                if (u < t) w += cnt[u];
+                tot += cnt[u];
+            }
+            for (size_t i = a; i < b; ++i) {
+                const uint32_t f = q[i];
+                const uint64_t k = key(i);
+                for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
+                    const uint32_t o = g.val[j];
+                    if (mine(f, j, o, k)) {
+                        q[w++]  = o;
+                        dist[o] = lev + 1;
+                    }
+                }
+            }
+            bar.wait();
+            lo = hi;
+            hi += tot;
+            ++lev;
+        }
+        if (t == 0) last = hi;
+    };
+    std::vector<std::thread> team;
+    const unsigned           lim = tl_limit;
+    for (unsigned t = 1; t < nthr; ++t)
+        team.emplace_back([&, t, lim] {
+            ThreadLimit inherit(lim);
+            body(t);
+        });
+    body(0);
+    for (auto& th : team) th.join();
+    q.resize(last);
+}
+
 FaceGraph face_graph(const HostMesh& m)
 {
     FaceGraph g;
@@ -509,9 +645,36 @@ class Bisector
         // instead of sorting it
         for (uint32_t f : faces) side_[f] = cside_[best][f];
         auto collect = [&] {
-            A.clear();
-            B.clear();
-            for (uint32_t f : faces) (side_[f] == 1u ? A : B).push_back(f);
+            if (n < kTeamMin) {
+                A.clear();
+                B.clear();
+                for (uint32_t f : faces) (side_[f] == 1u ? A : B).push_back(f);
+                return;
+            }
+            // large sets: per-chunk counts, then each chunk writes its run of A
+            // and of B (the same ascending order as the serial pass)
+            const size_t          nc = std::min<size_t>(64, n / 65536 + 1);
+            std::vector<uint32_t> ca(nc + 1, 0), cb(nc + 1, 0);
+            parallel_for(nc, [&](size_t c) {
+                uint32_t k = 0;
+                for (size_t i = n * c / nc; i < n * (c + 1) / nc; ++i) k += side_[faces[i]] == 1u;
+                ca[c + 1] = k;
+                cb[c + 1] = static_cast<uint32_t>(n * (c + 1) / nc - n * c / nc) - k;
+            }, 1);
+            for (size_t c = 0; c < nc; ++c) {
+                ca[c + 1] += ca[c];
+                cb[c + 1] += cb[c];
+            }
+            A.resize(ca[nc]);
+            B.resize(cb[nc]);
+            parallel_for(nc, [&](size_t c) {
+                uint32_t ia = ca[c], ib = cb[c];
+                for (size_t i = n * c / nc; i < n * (c + 1) / nc; ++i) {
+                    const uint32_t f = faces[i];
+                    if (side_[f] == 1u) A[ia++] = f;
+                    else B[ib++] = f;
+                }
+            }, 1);
         };
         collect();
         // keep both halves connected: pieces cut off from the larger
@@ -726,23 +889,19 @@ int run_patcher(HostMesh& m, uint32_t patch_size, int mode)
     auto t0  = clk();
     FaceGraph g = face_graph(m);
     auto t1 = clk();
-    // BFS order over all components from face 0
+    // BFS order over all components from face 0 (on a team of threads for a
+    // large mesh, in the same order)
     std::vector<uint32_t> order, rank(nF, kInvalid32);
-    order.reserve(nF);
-    for (uint32_t s = 0, head = 0; s < nF; ++s) {
-        if (rank[s] != kInvalid32) continue;
-        rank[s] = static_cast<uint32_t>(order.size());
-        order.push_back(s);
-        while (head < order.size()) {
-            uint32_t f = order[head++];
-            for (uint32_t j = g.off[f]; j < g.off[f + 1]; ++j) {
-                uint32_t n = g.val[j];
-                if (rank[n] == kInvalid32) {
-                    rank[n] = static_cast<uint32_t>(order.size());
-                    order.push_back(n);
-                }
-            }
+    {
+        std::vector<uint32_t>                    lvl(nF, kInvalid32);
+        std::unique_ptr<std::atomic<uint64_t>[]> claim(new std::atomic<uint64_t>[nF]());
+        order.reserve(nF);
+        for (uint32_t s = 0; s < nF; ++s) {
+            if (lvl[s] != kInvalid32) continue;
+            const size_t left = nF - order.size();
+            fifo_bfs(g, s, [](uint32_t) { return true; }, lvl.data(), claim.get(), 1u, order, left, bfs_team(left));
         }
+        parallel_for(nF, [&](size_t i) { rank[order[i]] = static_cast<uint32_t>(i); }, size_t(1) << 16);
     }
     uint32_t nP = 0;
     if (mode == kPatcherKMeans) {
