@@ -679,10 +679,32 @@ class Bisector
         collect();
         // keep both halves connected: pieces cut off from the larger
         // component of one half border the other half, so they move there
-        move_cut_off(A, 2u);
-        collect();
-        move_cut_off(B, 1u);
-        collect();
+        if (n >= kTeamMin && num_threads() >= 2) {
+            // both halves' components at once (disjoint sets: own tags and
+            // scratch); B's again only where A gave pieces away
+            Leaves         ca, cb;
+            const unsigned lim = tl_limit;
+            std::thread    th([&, lim] {
+                ThreadLimit inherit(lim);
+                cb = components(B);
+            });
+            ca = components(A);
+            th.join();
+            for (size_t i = 1; i < ca.size(); ++i)
+                for (uint32_t f : ca[i]) side_[f] = 2u;
+            if (ca.size() > 1) {
+                collect();
+                cb = components(B);
+            }
+            for (size_t i = 1; i < cb.size(); ++i)
+                for (uint32_t f : cb[i]) side_[f] = 1u;
+            collect();
+        } else {
+            move_cut_off(A, 2u);
+            collect();
+            move_cut_off(B, 1u);
+            collect();
+        }
         for (uint32_t f : faces) side_[f] = 0;
         std::vector<uint32_t>().swap(faces);
         kA = std::max<uint32_t>(k1, static_cast<uint32_t>((A.size() + cap - 1) / cap));
