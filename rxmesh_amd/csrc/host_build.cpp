@@ -591,23 +591,44 @@ class Bisector
         // one whose split cuts the fewest face adjacencies wins (keeps cuts
         // short on meshes with handles, e.g. around a torus tube)
         std::vector<uint32_t> q;
-        uint32_t              start = faces[0];  // the set's smallest caller id
-        for (uint32_t f : faces)
-            if (oid(f) < oid(start)) start = f;
+        uint32_t start = faces[0];  // the set's smallest caller id
+        {
+            const size_t          nc = chunks(n);
+            std::vector<uint32_t> cs(nc, faces[0]);
+            each_chunk(n, nc, [&](size_t c, size_t a, size_t b) {
+                for (size_t i = a; i < b; ++i)
+                    if (oid(faces[i]) < oid(cs[c])) cs[c] = faces[i];
+            });
+            for (uint32_t f : cs)
+                if (oid(f) < oid(start)) start = f;
+        }
         const uint32_t a = (far != kInvalid32) ? far : farthest(faces, start, tag, q);
         std::vector<uint32_t> d[3];
         const uint32_t        b = farthest(faces, a, tag, q);
         d[0]                    = dists(faces);
         bfs(faces, b, tag, q);
         d[1]           = dists(faces);
+        // the face farthest from both (largest min distance, ties: smallest
+        // caller id); per chunk, then the chunks in order by the same rule
         uint32_t c     = start;
         uint32_t cbest = 0;
-        for (uint32_t i = 0; i < n; ++i) {
-            const uint32_t m = std::min(d[0][i], d[1][i]);
-            if (m != kInvalid32 && (m > cbest || (m == cbest && m > 0 && oid(faces[i]) < oid(c)))) {
-                cbest = m;
-                c     = faces[i];
-            }
+        {
+            const size_t          nc = chunks(n);
+            std::vector<uint32_t> cc(nc, start), cm(nc, 0);
+            each_chunk(n, nc, [&](size_t k, size_t a, size_t b) {
+                for (size_t i = a; i < b; ++i) {
+                    const uint32_t m = std::min(d[0][i], d[1][i]);
+                    if (m != kInvalid32 && (m > cm[k] || (m == cm[k] && m > 0 && oid(faces[i]) < oid(cc[k])))) {
+                        cm[k] = m;
+                        cc[k] = faces[i];
+                    }
+                }
+            });
+            for (size_t k = 0; k < nc; ++k)
+                if (cm[k] > cbest || (cm[k] == cbest && cm[k] > 0 && oid(cc[k]) < oid(c))) {
+                    cbest = cm[k];
+                    c     = cc[k];
+                }
         }
         bfs(faces, c, tag, q);
         d[2] = dists(faces);
@@ -643,7 +664,7 @@ class Bisector
         // sides of the best cut (1 = A, 2 = B); each half is rebuilt in
         // ascending face order by one pass over `faces` (sorted on entry)
         // instead of sorting it
-        for (uint32_t f : faces) side_[f] = cside_[best][f];
+        each(n, [&](size_t i) { side_[faces[i]] = cside_[best][faces[i]]; });
         auto collect = [&] {
             if (n < kTeamMin) {
                 A.clear();
@@ -705,7 +726,7 @@ class Bisector
             move_cut_off(B, 1u);
             collect();
         }
-        for (uint32_t f : faces) side_[f] = 0;
+        each(n, [&](size_t i) { side_[faces[i]] = 0; });
         std::vector<uint32_t>().swap(faces);
         kA = std::max<uint32_t>(k1, static_cast<uint32_t>((A.size() + cap - 1) / cap));
         kB = std::max<uint32_t>(k2, static_cast<uint32_t>((B.size() + cap - 1) / cap));
@@ -766,10 +787,29 @@ class Bisector
         for (size_t t = 0; t < tie.size(); ++t) side[tie[t].second] = (t < take) ? 1u : 2u;
     }
 
+    // loops over a face set: on all threads for sets above 2^20 faces
+    static size_t chunks(size_t n) { return n >= kTeamMin ? std::min<size_t>(64, n / 65536 + 1) : 1; }
+    template <class F>
+    static void each_chunk(size_t n, size_t nc, F&& f)
+    {
+        if (nc == 1) {
+            f(size_t(0), size_t(0), n);
+            return;
+        }
+        parallel_for(nc, [&](size_t c) { f(c, n * c / nc, n * (c + 1) / nc); }, 1);
+    }
+    template <class F>
+    static void each(size_t n, F&& f)
+    {
+        each_chunk(n, chunks(n), [&](size_t, size_t a, size_t b) {
+            for (size_t i = a; i < b; ++i) f(i);
+        });
+    }
+
     uint32_t mark(const std::vector<uint32_t>& faces)
     {
         const uint32_t tag = next_tag_.fetch_add(1);
-        for (uint32_t f : faces) tag_[f].store(tag, std::memory_order_relaxed);
+        each(faces.size(), [&](size_t i) { tag_[faces[i]].store(tag, std::memory_order_relaxed); });
         return tag;
     }
 
@@ -849,7 +889,7 @@ class Bisector
     std::vector<uint32_t> dists(const std::vector<uint32_t>& faces) const
     {
         std::vector<uint32_t> r(faces.size());
-        for (size_t i = 0; i < faces.size(); ++i) r[i] = dist_[faces[i]];
+        each(faces.size(), [&](size_t i) { r[i] = dist_[faces[i]]; });
         return r;
     }
 
